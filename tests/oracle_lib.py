@@ -1,0 +1,238 @@
+"""ctypes bindings for the CPU oracle (oracle/liboracle_awcm.so) and for the
+reference build (oracle/_ref/libawcm_ref.so), which export the same entry
+points under the prefixes ``orc`` and ``ref``.
+
+Test infrastructure only: the product never imports this module.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle_awcm.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libawcm_ref.so")
+
+_dp = C.POINTER(C.c_double)
+_u8p = C.POINTER(C.c_uint8)
+_i32p = C.POINTER(C.c_int32)
+_ip = C.POINTER(C.c_int)
+
+
+def dptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def u8ptr(a):
+    return a.ctypes.data_as(_u8p)
+
+
+def i32ptr(a):
+    return a.ctypes.data_as(_i32p)
+
+
+_libs = {}
+
+
+def load(prefix="orc"):
+    """Load the oracle (prefix 'orc') or the reference build (prefix 'ref')."""
+    if prefix in _libs:
+        return _libs[prefix]
+    path = ORACLE_SO if prefix == "orc" else REF_SO
+    if not os.path.exists(path):
+        raise FileNotFoundError(path + " is not built (make -C oracle)")
+    lib = C.CDLL(path)
+    p = prefix
+    V = C.c_void_p
+    sig = {
+        f"{p}1_create": (V, [C.c_int, C.c_int, C.c_double, C.c_double]),
+        f"{p}1_destroy": (None, [V]),
+        f"{p}1_nx": (C.c_int, [V]),
+        f"{p}1_set_q": (None, [V, _dp]),
+        f"{p}1_get_q": (None, [V, _dp]),
+        f"{p}1_get_masks": (None, [V, _u8p, _i32p]),
+        f"{p}1_set_mask0": (None, [V, _u8p]),
+        f"{p}1_prepare": (None, [V, C.c_double]),
+        f"{p}1_step": (C.c_double, [V, C.c_double, C.c_double, _dp, C.c_int, _ip, _dp]),
+        f"{p}2_nx": (C.c_int, [V]),
+        f"{p}2_ny": (C.c_int, [V]),
+        f"{p}2_destroy": (None, [V]),
+        f"{p}2_set_q": (None, [V, _dp]),
+        f"{p}2_get_q": (None, [V, _dp]),
+        f"{p}2_get_masks": (None, [V, _u8p, _i32p]),
+        f"{p}2_set_mask0": (None, [V, _u8p]),
+        f"{p}2_prepare": (None, [V, C.c_double]),
+        f"{p}2_step": (C.c_double, [V, C.c_double, C.c_double, _dp, C.c_int, _ip, _dp]),
+        f"{p}_flux1": (None, [C.c_double, _dp, C.c_double, _dp]),
+        f"{p}_flux2": (None, [C.c_double, C.c_int, _dp, C.c_double, _dp]),
+        f"{p}_wenoz_left": (C.c_double, [C.c_double] * 5),
+        f"{p}_wenoz_right": (C.c_double, [C.c_double] * 5),
+        f"{p}_weights": (None, [_dp]),
+    }
+    if prefix == "orc":
+        sig["orc2_create"] = (V, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int])
+        sig["orc_ic1"] = (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, _dp, _dp,
+                                    C.c_double, C.c_double, _dp, _dp, _dp])
+        sig["orc_ic2"] = (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                    C.c_double, C.c_double, _dp, C.c_double, _dp, _dp, _dp])
+        sig["orc1_oob"] = (C.c_long, [V])
+        sig["orc2_oob"] = (C.c_long, [V])
+    else:
+        sig["ref2_create"] = (V, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                  C.c_double, C.c_double, C.c_int])
+        sig["ref_ic1"] = (C.c_int, [C.c_int, C.c_int, _dp, _dp, _dp])
+        sig["ref_ic2"] = (C.c_int, [C.c_int, C.c_int, _dp, _dp, _dp])
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _libs[prefix] = lib
+    return lib
+
+
+def have_ref():
+    return os.path.exists(REF_SO)
+
+
+# ---------------------------------------------------------------------------
+# Problem set-ups (own restatement; see oracle/awcm_oracle.cpp orc_ic1/orc_ic2)
+SOD_L = (1.0, 0.0, 1.0)
+SOD_R = (0.125, 0.0, 0.1)
+SHU_L = (3.857143, 2.629369, 10.33333)
+# 15-digit four-quadrant states (rho, u, v, p), quadrants 1..4 (PROBLEM_QUADRANTS.h:43-73)
+QUAD_STATES = np.array([
+    [1.5, 0.0, 0.0, 1.5],
+    [0.532258064516129, 1.206045378311055, 0.0, 0.3],
+    [0.137992831541219, 1.206045378311055, 1.206045378311055, 0.029032258064516],
+    [0.532258064516129, 0.0, 1.206045378311055, 0.3],
+], dtype=np.float64)
+
+
+def ic1(kind, res, xl, xu, split, sl, sr, amp=0.2, gamma=1.4):
+    lib = load("orc")
+    q = np.zeros((res + 6) * 3)
+    dx = np.zeros(1)
+    x0 = np.zeros(1)
+    a = np.array(sl, dtype=np.float64)
+    b = np.array(sr, dtype=np.float64)
+    lib.orc_ic1(kind, res, xl, xu, split, dptr(a), dptr(b), amp, gamma, dptr(q), dptr(dx), dptr(x0))
+    return q, float(dx[0]), float(x0[0])
+
+
+def ic2(kind, nx, ny, xl, xu, yl, yu, s, states=None, gamma=1.4):
+    lib = load("orc")
+    q = np.zeros((nx + 6) * (ny + 6) * 4)
+    dx = np.zeros(1)
+    dy = np.zeros(1)
+    st = np.ascontiguousarray(states if states is not None else QUAD_STATES, dtype=np.float64)
+    lib.orc_ic2(kind, nx, ny, xl, xu, yl, yu, s, dptr(st), gamma, dptr(q), dptr(dx), dptr(dy))
+    return q, float(dx[0]), float(dy[0])
+
+
+def print_times(t_final, num_print):
+    """arr_printing_times of the drivers (main_1d_euler.cpp:43-48)."""
+    inc = t_final / (num_print - 1)
+    return np.array([inc * p for p in range(num_print)], dtype=np.float64)
+
+
+class Solver1D:
+    """One 1-D AWCM solve on the oracle ('orc') or the reference build ('ref')."""
+
+    def __init__(self, prefix, num_levels, nx_coarse, gamma, dx):
+        self.lib = load(prefix)
+        self.p = prefix
+        self.h = getattr(self.lib, f"{prefix}1_create")(num_levels, nx_coarse, gamma, dx)
+        self.nx = getattr(self.lib, f"{prefix}1_nx")(self.h)
+        self.t = np.zeros(1)
+        self.cnt = C.c_int(1)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            getattr(self.lib, f"{self.p}1_destroy")(self.h)
+            self.h = None
+
+    def set_q(self, q):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        assert q.size == (self.nx + 6) * 3
+        getattr(self.lib, f"{self.p}1_set_q")(self.h, dptr(q))
+
+    def get_q(self):
+        q = np.zeros((self.nx + 6) * 3)
+        getattr(self.lib, f"{self.p}1_get_q")(self.h, dptr(q))
+        return q
+
+    def set_mask0(self, m0):
+        m0 = np.ascontiguousarray(m0, dtype=np.uint8)
+        getattr(self.lib, f"{self.p}1_set_mask0")(self.h, u8ptr(m0))
+
+    def masks(self):
+        m = np.zeros(5 * (self.nx + 6), dtype=np.uint8)
+        lv = np.zeros(4 * self.nx, dtype=np.int32)
+        getattr(self.lib, f"{self.p}1_get_masks")(self.h, u8ptr(m), i32ptr(lv))
+        return m.reshape(5, self.nx + 6), lv.reshape(4, self.nx)
+
+    def prepare(self, tol):
+        getattr(self.lib, f"{self.p}1_prepare")(self.h, tol)
+
+    def step(self, tol, cfl, pts):
+        pts = np.ascontiguousarray(pts, dtype=np.float64)
+        return getattr(self.lib, f"{self.p}1_step")(self.h, tol, cfl, dptr(pts), len(pts),
+                                                       C.byref(self.cnt), dptr(self.t))
+
+
+class Solver2D:
+    """One 2-D AWCM solve on the oracle ('orc') or the reference build ('ref').
+
+    bc_kind 0: zero-order extrapolation (quadrants); 1: double Mach reflection.
+    """
+
+    def __init__(self, prefix, num_levels, nxc, nyc, gamma, bc_kind, xl, xu, yl, yu):
+        self.lib = load(prefix)
+        self.p = prefix
+        if prefix == "orc":
+            nx = (nxc - 1) * 2 ** (num_levels - 1) + 1
+            dx = (xu - xl) / (nx - 1)
+            self.h = self.lib.orc2_create(num_levels, nxc, nyc, gamma, dx, xl, bc_kind)
+        else:
+            self.h = self.lib.ref2_create(num_levels, nxc, nyc, gamma, xl, xu, yl, yu, bc_kind)
+        self.nx = getattr(self.lib, f"{prefix}2_nx")(self.h)
+        self.ny = getattr(self.lib, f"{prefix}2_ny")(self.h)
+        self.t = np.zeros(1)
+        self.cnt = C.c_int(1)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            getattr(self.lib, f"{self.p}2_destroy")(self.h)
+            self.h = None
+
+    @property
+    def npts(self):
+        return (self.nx + 6) * (self.ny + 6)
+
+    def set_q(self, q):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        assert q.size == self.npts * 4
+        getattr(self.lib, f"{self.p}2_set_q")(self.h, dptr(q))
+
+    def get_q(self):
+        q = np.zeros(self.npts * 4)
+        getattr(self.lib, f"{self.p}2_get_q")(self.h, dptr(q))
+        return q
+
+    def set_mask0(self, m0):
+        m0 = np.ascontiguousarray(m0, dtype=np.uint8)
+        getattr(self.lib, f"{self.p}2_set_mask0")(self.h, u8ptr(m0))
+
+    def masks(self):
+        m = np.zeros(5 * self.npts, dtype=np.uint8)
+        lv = np.zeros(4 * self.nx * self.ny, dtype=np.int32)
+        getattr(self.lib, f"{self.p}2_get_masks")(self.h, u8ptr(m), i32ptr(lv))
+        return m.reshape(5, self.ny + 6, self.nx + 6), lv.reshape(4, self.ny, self.nx)
+
+    def prepare(self, tol):
+        getattr(self.lib, f"{self.p}2_prepare")(self.h, tol)
+
+    def step(self, tol, cfl, pts):
+        pts = np.ascontiguousarray(pts, dtype=np.float64)
+        return getattr(self.lib, f"{self.p}2_step")(self.h, tol, cfl, dptr(pts), len(pts),
+                                                       C.byref(self.cnt), dptr(self.t))
