@@ -1,0 +1,1019 @@
+// 2-D AWCM compressible-Euler solver on gfx950: kernels and the host-side
+// sequence of one step.
+//
+// Device layout (SoA, HBM):
+//   state   4 planes (rho, rho u, rho v, E) of P*(Ny+6) doubles, P = Nx+6,
+//           index (i+3) + P*(j+3) -- the reference's ghosted linear index
+//           (ARRAY_2D.h:112), so every stencil address is the reference's.
+//   masks   uint8, same ghosted index; mask_temp, mask0..3, a snapshot and the
+//           interpolation work mask.
+//   levels  uint8, index i + Nx*j (no ghosts).
+//   lists   per stage: compacted finest-level points and coarse points,
+//           packed (j << 16) | i.
+//
+// Kernel map (reference function -> kernels):
+//   ForwardWaveletTransformationEuler (WAVELET_2D.h:478-624)   k2_fwt_d3, k2_fwt_d12, k2_fwt_c per level
+//   BandingMask + AddAdjacentZone (AWCM_MANAGER_2D.h:14-61)      k2_band_snap, k2_adjacent
+//   CheckReconstruction (AWCM_MANAGER_2D.h:63-158)               k2_check_d3, k2_check_d12 per level
+//   ComputeLevel + mask copy (AWCM_MANAGER_2D.h:215-241)         k2_level_copy
+//   AddPointForDifferentiation (AWCM_MANAGER_2D.h:160-213)       k2_addpoint
+//   InverseWaveletTransformationEuler (WAVELET_2D.h:626-782)     k2_iwt_c, k2_iwt_d12, k2_iwt_d3 per level
+//   InterpolateEuler (WAVELET_2D.h:784-907)                      k2_interp_tmp, k2_check_*, k2_interp_d12, k2_interp_d3
+//   CFL / alpha reductions (main_2d_euler(dmach).cpp:61-104)     k2_alpha (+ dt in k2_bc)
+//   ghost fill (PROBLEM_QUADRANTS.h:123-203, PROBLEM_DOUBLEMACH.h:106-245)   k2_bc
+//   stage loops (main_2d_euler(dmach).cpp:106-322)               k2_stage_weno<S>, k2_stage_coarse<S>
+//   RK combine + RemoveResidueEuler (:325-334, AWCM_MANAGER_2D.h:267-280)    k2_combine
+// Every kernel returns at once when the step is skipped (t >= t_final), so
+// many steps can be queued or graph-replayed without host round trips.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "awcm_capi.h"
+#include "awcm_common.h"
+#include "solver2d.h"
+
+namespace awcm {
+namespace {
+
+constexpr int G = kGhost;
+constexpr double kFluxGamma = 1.4;  // WENOZ_EULER_SOLVER_2D is default-constructed in the drivers
+
+__device__ __forceinline__ long long gidx(const Dims2& D, int i, int j) {
+    return (long long)(i + G) + (long long)D.P * (j + G);
+}
+
+__device__ __forceinline__ bool lat_point(const Lat& g, int t, int& i, int& j) {
+    if (t >= g.ni * g.nj) return false;
+    const int a = t % g.ni, b = t / g.ni;
+    i = g.is + a * g.dx;
+    j = g.js + b * g.dy;
+    return true;
+}
+
+__device__ __forceinline__ void mset(const Dims2& D, uint8_t* m, int i, int j, Scal2* sc) {
+    if (i < -G || i > D.Nx - 1 + G || j < -G || j > D.Ny - 1 + G) { sc->flag |= 1; return; }
+    m[gidx(D, i, j)] = 1;
+}
+
+// Clamped 4-point stencil start of a detail (predict) point: WAVELET_2D.h:506-509
+__device__ __forceinline__ void pstart(int i, int d, int iend_lim, int& is, int& k) {
+    const int dh = d / 2;
+    is = i - d - dh;
+    k = 1;
+    if (is < 0) is = 0, k = 0;
+    else if (i + d + dh > iend_lim) is = iend_lim - 3 * d, k = 2;
+}
+// Clamped stencil start of a coarse (update) point: WAVELET_2D.h:595-598
+__device__ __forceinline__ void ustart(int i, int d, int iend_lim, int& is, int& k) {
+    const int dh = d / 2;
+    is = i - d - dh;
+    k = 1;
+    if (is < 0) is = dh, k = 0;
+    else if (i + d + dh > iend_lim) is = iend_lim - 3 * d - dh, k = 2;
+}
+
+// ---------------------------------------------------------------- begin ----
+__global__ void k2_begin(Dims2 D, Scal2* sc, double* alpha, int force) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int na = 3 * (D.Nx + D.Ny);
+    for (int k = t; k < na; k += gridDim.x * blockDim.x) alpha[k] = 0.0;
+    if (t == 0) {
+        sc->skip = (!force && !(sc->t < sc->t_final)) ? 1 : 0;
+        sc->vmax = 0ull;
+        for (int k = 0; k < 3; k++) { sc->cntW[k] = 0; sc->cntC[k] = 0; }
+    }
+}
+
+__global__ void k2_copy(const uint8_t* __restrict__ src, uint8_t* __restrict__ dst, long long n, const Scal2* sc) {
+    if (sc->skip) return;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) dst[t] = src[t];
+}
+
+// ------------------------------------------------------------ forward FWT ----
+__global__ void k2_fwt_d3(Dims2 D, Lat g, int d, Wts W, double tol, double* __restrict__ rho, uint8_t* __restrict__ mask,
+                          const Scal2* sc) {
+    if (sc->skip) return;
+    int i, j;
+    if (!lat_point(g, blockIdx.x * blockDim.x + threadIdx.x, i, j)) return;
+    const long long p = gidx(D, i, j);
+    if (!mask[p]) return;
+    int is, js, k1, k2;
+    pstart(i, d, D.Nx - 1, is, k1);
+    pstart(j, d, D.Ny - 1, js, k2);
+    double v = rho[p];
+    for (int c = 0; c < 4; c++) v -= rho[gidx(D, is + c * d, j)] * W.w[c][k1 + 1];
+    for (int c = 0; c < 4; c++) v -= rho[gidx(D, i, js + c * d)] * W.w[c][k2 + 1];
+    for (int c2 = 0; c2 < 4; c2++)
+        for (int c1 = 0; c1 < 4; c1++)
+            v += rho[gidx(D, is + c1 * d, js + c2 * d)] * W.w[c1][k1 + 1] * W.w[c2][k2 + 1];
+    v *= 0.25;
+    rho[p] = v;
+    if (t_abs(v) < tol) mask[p] = 0;
+}
+
+__global__ void k2_fwt_d12(Dims2 D, Lat g1, Lat g2, int d, Wts W, double tol, double* __restrict__ rho,
+                           uint8_t* __restrict__ mask, const Scal2* sc) {
+    if (sc->skip) return;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n1 = g1.ni * g1.nj;
+    int i, j;
+    const bool d1 = t < n1;
+    if (d1) { if (!lat_point(g1, t, i, j)) return; }
+    else if (!lat_point(g2, t - n1, i, j)) return;
+    const long long p = gidx(D, i, j);
+    if (!mask[p]) return;
+    double v = rho[p];
+    int s, k;
+    if (d1) {
+        pstart(i, d, D.Nx - 1, s, k);
+        for (int c = 0; c < 4; c++) v -= rho[gidx(D, s + c * d, j)] * W.w[c][k + 1];
+    } else {
+        pstart(j, d, D.Ny - 1, s, k);
+        for (int c = 0; c < 4; c++) v -= rho[gidx(D, i, s + c * d)] * W.w[c][k + 1];
+    }
+    v *= 0.5;
+    rho[p] = v;
+    if (t_abs(v) < tol) mask[p] = 0;
+}
+
+__global__ void k2_fwt_c(Dims2 D, Lat g, int d, Wts W, double* __restrict__ rho, const uint8_t* __restrict__ mask,
+                         const Scal2* sc) {
+    if (sc->skip) return;
+    int i, j;
+    if (!lat_point(g, blockIdx.x * blockDim.x + threadIdx.x, i, j)) return;
+    const long long p = gidx(D, i, j);
+    if (!mask[p]) return;
+    int is, js, k1, k2;
+    ustart(i, d, D.Nx - 1, is, k1);
+    ustart(j, d, D.Ny - 1, js, k2);
+    double v = rho[p];
+    for (int c = 0; c < 4; c++) v += rho[gidx(D, is + c * d, j)] * W.w[c][k1 + 1];
+    for (int c = 0; c < 4; c++) v += rho[gidx(D, i, js + c * d)] * W.w[c][k2 + 1];
+    for (int c2 = 0; c2 < 4; c2++)
+        for (int c1 = 0; c1 < 4; c1++)
+            v += rho[gidx(D, is + c1 * d, js + c2 * d)] * W.w[c1][k1 + 1] * W.w[c2][k2 + 1];
+    rho[p] = v;
+}
+
+// ------------------------------------------------------------ inverse FWT ----
+__global__ void k2_iwt_c(Dims2 D, Lat g, int d, Wts W, double* __restrict__ rho, const uint8_t* __restrict__ mask,
+                         const Scal2* sc) {
+    if (sc->skip) return;
+    int i, j;
+    if (!lat_point(g, blockIdx.x * blockDim.x + threadIdx.x, i, j)) return;
+    const long long p = gidx(D, i, j);
+    if (!mask[p]) { rho[p] = 0; return; }
+    int is, js, k1, k2;
+    ustart(i, d, D.Nx - 1, is, k1);
+    ustart(j, d, D.Ny - 1, js, k2);
+    double v = rho[p];
+    for (int c = 0; c < 4; c++) v -= rho[gidx(D, is + c * d, j)] * W.w[c][k1 + 1];
+    for (int c = 0; c < 4; c++) v -= rho[gidx(D, i, js + c * d)] * W.w[c][k2 + 1];
+    for (int c2 = 0; c2 < 4; c2++)
+        for (int c1 = 0; c1 < 4; c1++)
+            v -= rho[gidx(D, is + c1 * d, js + c2 * d)] * W.w[c1][k1 + 1] * W.w[c2][k2 + 1];
+    rho[p] = v;
+}
+
+// d2 then d1 in the reference; the two sub-bands read only level-lev points, so
+// one launch covers both.
+__global__ void k2_iwt_d12(Dims2 D, Lat g1, Lat g2, int d, Wts W, double* __restrict__ rho,
+                           const uint8_t* __restrict__ mask, const Scal2* sc) {
+    if (sc->skip) return;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n1 = g1.ni * g1.nj;
+    int i, j;
+    const bool d1 = t < n1;
+    if (d1) { if (!lat_point(g1, t, i, j)) return; }
+    else if (!lat_point(g2, t - n1, i, j)) return;
+    const long long p = gidx(D, i, j);
+    if (!mask[p]) { rho[p] = 0; return; }
+    double v = rho[p];
+    v *= 2.0;
+    int s, k;
+    if (d1) {
+        pstart(i, d, D.Nx - 1, s, k);
+        for (int c = 0; c < 4; c++) v += rho[gidx(D, s + c * d, j)] * W.w[c][k + 1];
+    } else {
+        pstart(j, d, D.Ny - 1, s, k);
+        for (int c = 0; c < 4; c++) v += rho[gidx(D, i, s + c * d)] * W.w[c][k + 1];
+    }
+    rho[p] = v;
+}
+
+__global__ void k2_iwt_d3(Dims2 D, Lat g, int d, Wts W, double* __restrict__ rho, const uint8_t* __restrict__ mask,
+                          const Scal2* sc) {
+    if (sc->skip) return;
+    int i, j;
+    if (!lat_point(g, blockIdx.x * blockDim.x + threadIdx.x, i, j)) return;
+    const long long p = gidx(D, i, j);
+    if (!mask[p]) { rho[p] = 0; return; }
+    int is, js, k1, k2;
+    pstart(i, d, D.Nx - 1, is, k1);
+    pstart(j, d, D.Ny - 1, js, k2);
+    double v = rho[p];
+    v *= 4.0;
+    for (int c = 0; c < 4; c++) v += rho[gidx(D, is + c * d, j)] * W.w[c][k1 + 1];
+    for (int c = 0; c < 4; c++) v += rho[gidx(D, i, js + c * d)] * W.w[c][k2 + 1];
+    for (int c2 = 0; c2 < 4; c2++)
+        for (int c1 = 0; c1 < 4; c1++)
+            v -= rho[gidx(D, is + c1 * d, js + c2 * d)] * W.w[c1][k1 + 1] * W.w[c2][k2 + 1];
+    rho[p] = v;
+}
+
+// ------------------------------------------------------ interpolation ------
+__global__ void k2_interp_tmp(Dims2 D, const uint8_t* __restrict__ to, const uint8_t* __restrict__ from,
+                              uint8_t* __restrict__ tmp, const Scal2* sc) {
+    if (sc->skip) return;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long n = (long long)D.P * (D.Ny + 2 * G);
+    if (t >= n) return;
+    const int i = int(t % D.P) - G, j = int(t / D.P) - G;
+    uint8_t v = to[t];
+    if (i >= 0 && i <= D.Nx - 1 && j >= 0 && j <= D.Ny - 1 && from[t]) v = 0;
+    tmp[t] = v;
+}
+
+__global__ void k2_interp_d12(Dims2 D, Lat g1, Lat g2, int d, Wts W, double* __restrict__ Q, const uint8_t* __restrict__ tmp,
+                              const uint8_t* __restrict__ from, const Scal2* sc) {
+    if (sc->skip) return;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n1 = g1.ni * g1.nj;
+    int i, j;
+    const bool d1 = t < n1;
+    if (d1) { if (!lat_point(g1, t, i, j)) return; }
+    else if (!lat_point(g2, t - n1, i, j)) return;
+    const long long p = gidx(D, i, j);
+    if (tmp[p] == 0 || from[p] == 1) return;
+    int s, k;
+    if (d1) pstart(i, d, D.Nx - 1, s, k);
+    else pstart(j, d, D.Ny - 1, s, k);
+    for (int c4 = 1; c4 < 4; c4++) {
+        double* q = Q + c4 * D.plane;
+        double v = 0.0;
+        if (d1) for (int c = 0; c < 4; c++) v += q[gidx(D, s + c * d, j)] * W.w[c][k + 1];
+        else for (int c = 0; c < 4; c++) v += q[gidx(D, i, s + c * d)] * W.w[c][k + 1];
+        q[p] = v;
+    }
+}
+
+__global__ void k2_interp_d3(Dims2 D, Lat g, int d, Wts W, double* __restrict__ Q, const uint8_t* __restrict__ tmp,
+                             const uint8_t* __restrict__ from, const Scal2* sc) {
+    if (sc->skip) return;
+    int i, j;
+    if (!lat_point(g, blockIdx.x * blockDim.x + threadIdx.x, i, j)) return;
+    const long long p = gidx(D, i, j);
+    if (tmp[p] == 0 || from[p] == 1) return;
+    int is, js, k1, k2;
+    pstart(i, d, D.Nx - 1, is, k1);
+    pstart(j, d, D.Ny - 1, js, k2);
+    for (int c4 = 1; c4 < 4; c4++) {
+        double* q = Q + c4 * D.plane;
+        double v = 0.0;
+        for (int c = 0; c < 4; c++) v += q[gidx(D, is + c * d, j)] * W.w[c][k1 + 1];
+        for (int c = 0; c < 4; c++) v += q[gidx(D, i, js + c * d)] * W.w[c][k2 + 1];
+        for (int c2 = 0; c2 < 4; c2++)
+            for (int c1 = 0; c1 < 4; c1++)
+                v -= q[gidx(D, is + c1 * d, js + c2 * d)] * W.w[c1][k1 + 1] * W.w[c2][k2 + 1];
+        q[p] = v;
+    }
+}
+
+// --------------------------------------------------------- mask algebra ----
+__global__ void k2_check_d3(Dims2 D, Lat g, int d, uint8_t* __restrict__ mask, Scal2* sc) {
+    if (sc->skip) return;
+    int i, j;
+    if (!lat_point(g, blockIdx.x * blockDim.x + threadIdx.x, i, j)) return;
+    if (!mask[gidx(D, i, j)]) return;
+    int is, js, k1, k2;
+    pstart(i, d, D.Nx - 1, is, k1);
+    pstart(j, d, D.Ny - 1, js, k2);
+    for (int c = 0; c < 4; c++) mset(D, mask, is + c * d, j, sc);
+    for (int c = 0; c < 4; c++) mset(D, mask, i, js + c * d, sc);
+    for (int c2 = 0; c2 < 4; c2++)
+        for (int c1 = 0; c1 < 4; c1++) mset(D, mask, is + c1 * d, js + c2 * d, sc);
+}
+
+__global__ void k2_check_d12(Dims2 D, Lat g1, Lat g2, int d, uint8_t* __restrict__ mask, Scal2* sc) {
+    if (sc->skip) return;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n1 = g1.ni * g1.nj;
+    int i, j;
+    const bool d1 = t < n1;
+    if (d1) { if (!lat_point(g1, t, i, j)) return; }
+    else if (!lat_point(g2, t - n1, i, j)) return;
+    if (!mask[gidx(D, i, j)]) return;
+    int s, k;
+    if (d1) {
+        pstart(i, d, D.Nx - 1, s, k);
+        for (int c = 0; c < 4; c++) mset(D, mask, s + c * d, j, sc);
+    } else {
+        pstart(j, d, D.Ny - 1, s, k);
+        for (int c = 0; c < 4; c++) mset(D, mask, i, s + c * d, sc);
+    }
+}
+
+// BandingMask (PROBLEM_SHOCK_VORTEX_INTERACTION.h:205-213, 2-wide band) on mask0,
+// then the AddAdjacentZone snapshot.
+__global__ void k2_band_snap(Dims2 D, uint8_t* __restrict__ m0, uint8_t* __restrict__ snap, const Scal2* sc) {
+    if (sc->skip) return;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long n = (long long)D.P * (D.Ny + 2 * G);
+    if (t >= n) return;
+    const int i = int(t % D.P) - G, j = int(t / D.P) - G;
+    uint8_t v = m0[t];
+    if (i >= 0 && i <= D.Nx - 1 && j >= 0 && j <= D.Ny - 1 &&
+        !(2 <= i && i <= D.Nx - 1 - 2 && j >= 2 && j <= D.Ny - 1 - 2)) {
+        v = 1;
+        m0[t] = 1;
+    }
+    snap[t] = v;
+}
+
+// Sub-band of an interior point: level lev (0..L-2) and mu (1: odd i, 2: odd j,
+// 3: both), or -1 for coarsest-grid and boundary points.  Equivalent to
+// membership of the subgrid(lev, mu) lattices of MRA_GRID_UNIFORM_2D.h:114-130.
+__device__ __forceinline__ int classify(int i, int j, const Dims2& D, int& mu) {
+    if (i <= 0 || j <= 0 || i >= D.Nx - 1 || j >= D.Ny - 1) return -1;
+    const int a = __builtin_ctz(i), b = __builtin_ctz(j);
+    const int s = a < b ? a : b;
+    const int lev = D.L - 2 - s;
+    if (lev < 0) return -1;
+    mu = (a == s && b == s) ? 3 : (a == s ? 1 : 2);
+    return lev;
+}
+
+// AWCM_MANAGER_2D::AddAdjacentZone (AWCM_MANAGER_2D.h:14-61), all levels at once
+// from the snapshot.
+__global__ void k2_adjacent(Dims2 D, const uint8_t* __restrict__ snap, uint8_t* __restrict__ mask, Scal2* sc) {
+    if (sc->skip) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= D.Nx) return;
+    int mu;
+    const int lev = classify(i, j, D, mu);
+    if (lev < 0 || !snap[gidx(D, i, j)]) return;
+    if (lev == D.maxlev - 1) {
+        mset(D, mask, i - 1, j, sc); mset(D, mask, i + 1, j, sc);
+        mset(D, mask, i, j - 1, sc); mset(D, mask, i, j + 1, sc);
+    } else {
+        const int m = 1 << (D.L - 1 - lev);
+        const int dq = m / 4, dh = m / 2;
+        mset(D, mask, i - dq, j, sc); mset(D, mask, i + dq, j, sc);
+        mset(D, mask, i, j - dq, sc); mset(D, mask, i, j + dq, sc);
+        mset(D, mask, i - dh, j, sc); mset(D, mask, i + dh, j, sc);
+        mset(D, mask, i, j - dh, sc); mset(D, mask, i, j + dh, sc);
+    }
+}
+
+// mask_{k+1} = mask_k (ghosts included) and ComputeLevel(level_k, mask_k)
+// (AWCM_MANAGER_2D.h:215-241).  A point with no active neighbour keeps its
+// previous level, as in the reference.
+__global__ void k2_level_copy(Dims2 D, const uint8_t* __restrict__ src, uint8_t* __restrict__ dst,
+                              uint8_t* __restrict__ lvl, Scal2* sc) {
+    if (sc->skip) return;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long n = (long long)D.P * (D.Ny + 2 * G);
+    if (t >= n) return;
+    const uint8_t v = src[t];
+    dst[t] = v;
+    const int i = int(t % D.P) - G, j = int(t / D.P) - G;
+    if (i < 0 || i > D.Nx - 1 || j < 0 || j > D.Ny - 1) return;
+    const long long li = (long long)i + (long long)D.Nx * j;
+    if (!v) { lvl[li] = 0; return; }
+    int d = 1;
+    for (int lev = D.maxlev; lev >= 0; lev--) {
+        if (i - d < -G || i + d > D.Nx - 1 + G || j - d < -G || j + d > D.Ny - 1 + G) { sc->flag |= 1; return; }
+        if (src[t - d] || src[t + d] || src[t - (long long)d * D.P] || src[t + (long long)d * D.P]) {
+            lvl[li] = (uint8_t)lev;
+            return;
+        }
+        d *= 2;
+    }
+}
+
+// AWCM_MANAGER_2D::AddPointForDifferentiation (AWCM_MANAGER_2D.h:160-213).  The
+// reference snapshots mask_{k+1} right after copying it from mask_k, so mask_k
+// is the snapshot.
+__global__ void k2_addpoint(Dims2 D, const uint8_t* __restrict__ snap, const uint8_t* __restrict__ lvl,
+                            uint8_t* __restrict__ mask, Scal2* sc) {
+    if (sc->skip) return;
+    const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x, j = 1 + blockIdx.y;
+    if (i > D.Nx - 2 || j > D.Ny - 2) return;
+    if (!snap[gidx(D, i, j)]) return;
+    const int lev = lvl[(long long)i + (long long)D.Nx * j];
+    const int d = 1 << (D.L - 1 - lev);  // grid[lev].dx
+    const int NX = D.Nx - 1, NY = D.Ny - 1;
+    if (lev == D.maxlev) {
+        for (int c = -3; c <= 3; c++) { mset(D, mask, i + c, j, sc); mset(D, mask, i, j + c, sc); }
+        return;
+    }
+    int lo, hi;
+    if (i - 2 * d < 0) lo = -1, hi = 2;
+    else if (i + 2 * d > NX) lo = -2, hi = 1;
+    else lo = -2, hi = 2;
+    for (int c = lo; c <= hi; c++) mset(D, mask, i + d * c, j, sc);
+    if (j - 2 * d < 0) lo = -1, hi = 2;
+    else if (j + 2 * d > NY) lo = -2, hi = 1;
+    else lo = -2, hi = 2;
+    for (int c = lo; c <= hi; c++) mset(D, mask, i, j + d * c, sc);
+}
+
+// --------------------------------------------------------- compaction ------
+// Stage work lists: for k = 0..2 the points of mask_k at the finest level
+// (WENO-Z faces) and at coarser levels (central differences).  One atomic per
+// list per block; list order is irrelevant to the result.
+constexpr int kCompactThreads = 256, kCompactPer = 8;
+
+__global__ void __launch_bounds__(kCompactThreads)
+k2_compact(Dims2 D, const uint8_t* m0, const uint8_t* m1, const uint8_t* m2, const uint8_t* l0, const uint8_t* l1,
+           const uint8_t* l2, uint32_t* lists, Scal2* sc) {
+    if (sc->skip) return;
+    __shared__ int wtot[6][kCompactThreads / 64];
+    __shared__ int base[6];
+    const long long N = (long long)D.Nx * D.Ny;
+    const uint8_t* mk[3] = {m0, m1, m2};
+    const uint8_t* lk[3] = {l0, l1, l2};
+    uint32_t flags[kCompactPer];
+    int cnt[6] = {0, 0, 0, 0, 0, 0};
+    const long long b0 = (long long)blockIdx.x * kCompactThreads * kCompactPer;
+    for (int it = 0; it < kCompactPer; it++) {
+        const long long li = b0 + (long long)it * kCompactThreads + threadIdx.x;
+        uint32_t f = 0;
+        if (li < N) {
+            const int i = int(li % D.Nx), j = int(li / D.Nx);
+            const long long p = gidx(D, i, j);
+            for (int k = 0; k < 3; k++) {
+                if (!mk[k][p]) continue;
+                const int l = (lk[k][li] == D.maxlev) ? 0 : 1;
+                f |= 1u << (2 * k + l);
+                cnt[2 * k + l]++;
+            }
+        }
+        flags[it] = f;
+    }
+    // block exclusive scan of the six counts
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int incl[6];
+    for (int q = 0; q < 6; q++) {
+        int v = cnt[q];
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(v, o, 64);
+            if (lane >= o) v += y;
+        }
+        incl[q] = v;
+        if (lane == 63) wtot[q][wid] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        int s = 0;
+        for (int w = 0; w < kCompactThreads / 64; w++) { const int x = wtot[threadIdx.x][w]; wtot[threadIdx.x][w] = s; s += x; }
+        int* ctr = (threadIdx.x & 1) ? &sc->cntC[threadIdx.x >> 1] : &sc->cntW[threadIdx.x >> 1];
+        base[threadIdx.x] = s ? atomicAdd(ctr, s) : 0;
+    }
+    __syncthreads();
+    int pos[6];
+    for (int q = 0; q < 6; q++) pos[q] = base[q] + wtot[q][wid] + incl[q] - cnt[q];
+    for (int it = 0; it < kCompactPer; it++) {
+        const uint32_t f = flags[it];
+        if (!f) continue;
+        const long long li = b0 + (long long)it * kCompactThreads + threadIdx.x;
+        const uint32_t packed = (uint32_t(li / D.Nx) << 16) | uint32_t(li % D.Nx);
+        for (int q = 0; q < 6; q++)
+            if (f & (1u << q)) lists[(long long)q * N + pos[q]++] = packed;
+    }
+}
+
+// ---------------------------------------------------- CFL and LF alphas ----
+// alpha_x(j) = max_i SRx, alpha_y(i) = max_j SRy over `mask` (exact maxima), and
+// for stage 1 also max(SRx + SRy) for the CFL step (main_2d_euler(dmach).cpp:61-104).
+constexpr int kAlphaRows = 32;
+
+__global__ void __launch_bounds__(256) k2_alpha(Dims2 D, const uint8_t* __restrict__ mask, const double* __restrict__ Q,
+                                                double* ax, double* ay, double gamma, int with_vmax, Scal2* sc) {
+    if (sc->skip) return;
+    __shared__ double colr[4][64];
+    __shared__ double vr[4];
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int j0 = blockIdx.y * kAlphaRows + threadIdx.y;
+    double cm = 0, vm = 0;
+    for (int r = 0; r < kAlphaRows / 4; r++) {
+        const int j = j0 + 4 * r;
+        double sx = 0;
+        if (i < D.Nx && j < D.Ny) {
+            const long long p = gidx(D, i, j);
+            if (mask[p]) {
+                const double q[4] = {Q[p], Q[p + D.plane], Q[p + 2 * D.plane], Q[p + 3 * D.plane]};
+                sx = spec_rad2x(gamma, q);
+                const double sy = spec_rad2y(gamma, q);
+                cm = t_max(cm, sy);
+                vm = t_max(vm, sx + sy);
+            }
+        }
+        const double rm = wave_max(sx);
+        if (threadIdx.x == 0 && j < D.Ny && !(rm <= 0)) atomic_max_pos((unsigned long long*)&ax[j], rm);
+    }
+    colr[threadIdx.y][threadIdx.x] = cm;
+    const double wv = wave_max(vm);
+    if (threadIdx.x == 0) vr[threadIdx.y] = wv;
+    __syncthreads();
+    if (threadIdx.y == 0) {
+        const double c = t_max(t_max(colr[0][threadIdx.x], colr[1][threadIdx.x]), t_max(colr[2][threadIdx.x], colr[3][threadIdx.x]));
+        if (i < D.Nx && !(c <= 0)) atomic_max_pos((unsigned long long*)&ay[i], c);
+        if (with_vmax && threadIdx.x == 0) {
+            const double v = t_max(t_max(vr[0], vr[1]), t_max(vr[2], vr[3]));
+            if (!(v <= 0)) atomic_max_pos(&sc->vmax, v);
+        }
+    }
+}
+
+// ------------------------------------------------------------ ghost fill ----
+__device__ __forceinline__ void put4(double* Q, long long plane, long long p, const double* v) {
+    Q[p] = v[0]; Q[p + plane] = v[1]; Q[p + 2 * plane] = v[2]; Q[p + 3 * plane] = v[3];
+}
+__device__ __forceinline__ void get4(const double* Q, long long plane, long long p, double* v) {
+    v[0] = Q[p]; v[1] = Q[p + plane]; v[2] = Q[p + 2 * plane]; v[3] = Q[p + 3 * plane];
+}
+
+// Ghost fill of one field at time t; with compute_dt, block 0 first turns the
+// CFL maximum into dt with the output-time clip (main_2d_euler(dmach).cpp:76-83).
+__global__ void k2_bc(Dims2 D, double* Q, int kind, double x0, double dx, int compute_dt, const double* pt, Scal2* sc) {
+    if (sc->skip) return;
+    if (compute_dt && blockIdx.x == 0 && threadIdx.x == 0) {
+        const double vmax = __longlong_as_double((long long)sc->vmax);
+        double dt = sc->cfl * dx / vmax;
+        int dp = 0;
+        if (sc->cnt < sc->n_print && sc->t + dt > pt[sc->cnt]) { dt = pt[sc->cnt] - sc->t; dp = 1; }
+        sc->dt = dt;
+        sc->do_print = dp;
+        if (!(vmax > 0) || !(vmax < INFINITY)) sc->flag |= 2;
+    }
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nl = G * D.Ny, nb = G * D.Nx;
+    const double T = sc->t;
+    int side, i, j;
+    if (t < nl) { side = 0; j = t / G; i = -1 - t % G; }
+    else if (t < 2 * nl) { side = 1; j = (t - nl) / G; i = D.Nx + (t - nl) % G; }
+    else if (t < 2 * nl + nb) { side = 2; i = (t - 2 * nl) / G; j = -1 - (t - 2 * nl) % G; }
+    else if (t < 2 * nl + 2 * nb) { side = 3; i = (t - 2 * nl - nb) / G; j = D.Ny + (t - 2 * nl - nb) % G; }
+    else return;
+    const long long p = gidx(D, i, j);
+    double v[4];
+    if (kind == AWCM_BC_TRANSMISSIVE) {
+        const int si = side == 0 ? 0 : side == 1 ? D.Nx - 1 : i;
+        const int sj = side == 2 ? 0 : side == 3 ? D.Ny - 1 : j;
+        get4(Q, D.plane, gidx(D, si, sj), v);
+        put4(Q, D.plane, p, v);
+        return;
+    }
+    // double Mach reflection (EOS at gamma = 1.4, as BOUNDARY_MANAGER_DMACH)
+    const double sqrt3 = sqrt(3.0);
+    if (side == 0) {
+        const double rho = 8, u = 8.25 * sqrt(3.0) * 0.5, vv = -8.25 * 0.5, pr = 116.5;
+        v[0] = rho; v[1] = rho * u; v[2] = rho * vv; v[3] = eos_E2(1.4, rho, u, vv, pr);
+    } else if (side == 1) {
+        get4(Q, D.plane, gidx(D, D.Nx - 1, j), v);
+    } else if (side == 2) {
+        const double x = x0 + (i - 0) * dx;
+        if (x > 1.0 / 6.0) {
+            get4(Q, D.plane, gidx(D, i, 0), v);
+            v[2] = -v[2];
+        } else {
+            v[0] = 8; v[1] = 8 * 8.25 * sqrt3 * 0.5; v[2] = -8 * 8.25 * 0.5;
+            v[3] = eos_E2(1.4, 8, 8.25 * sqrt3 * 0.5, -8.25 * 0.5, 116.5);
+        }
+    } else {
+        const double x = x0 + (i - 0) * dx;
+        if (x < 1.0 / 6.0 + (1 + 20 * T) / sqrt3) {
+            v[0] = 8; v[1] = 8 * 8.25 * sqrt3 * 0.5; v[2] = -8 * 8.25 * 0.5;
+            v[3] = eos_E2(1.4, 8, 8.25 * sqrt3 * 0.5, -8.25 * 0.5, 116.5);
+        } else {
+            v[0] = 1.4; v[1] = 1.4 * 0; v[2] = 1.4 * 0; v[3] = eos_E2(1.4, 1.4, 0, 0, 1);
+        }
+    }
+    put4(Q, D.plane, p, v);
+}
+
+// ------------------------------------------------------------ RK stages ----
+// S = 1: Q2 = Q - dt/h r(Q);  S = 2: Q1 = 0.75 Q + 0.25 (Q2 - dt/h r(Q2));
+// S = 3: Q0 = Q1 - dt/h r(Q1)  (main_2d_euler(dmach).cpp:162, 242, 321)
+template <int S>
+__device__ __forceinline__ void stage_out(const Dims2& D, long long p, const double* r, double h, double dt,
+                                          const double* Qa, const double* Qs, double* Qo) {
+    for (int k = 0; k < 4; k++) {
+        const long long o = p + k * D.plane;
+        if (S == 1) Qo[o] = Qs[o] - dt / h * r[k];
+        else if (S == 2) Qo[o] = 0.75 * Qa[o] + 0.25 * (Qs[o] - dt / h * r[k]);
+        else Qo[o] = Qs[o] - dt / h * r[k];
+    }
+}
+
+// Finest-level points: WENO-Z/Roe flux differences in x and y.
+template <int S>
+__global__ void __launch_bounds__(128) k2_stage_weno(Dims2 D, const uint32_t* __restrict__ list, int which,
+                                                     const double* __restrict__ Qa, const double* __restrict__ Qs,
+                                                     double* __restrict__ Qo, const double* __restrict__ ax,
+                                                     const double* __restrict__ ay, double gdx, Scal2* sc) {
+    if (sc->skip) return;
+    const int n = sc->cntW[which];
+    const double dt = sc->dt;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        const uint32_t e = list[t];
+        const int i = int(e & 0xffffu), j = int(e >> 16);
+        const long long p = gidx(D, i, j);
+        double s[7][4];
+        double fa[4], fb[4], r[4];
+        for (int q = 0; q < 7; q++) get4(Qs, D.plane, p + (q - 3), s[q]);
+        face_flux2<0>(kFluxGamma, s, ax[j], fa);
+        face_flux2<0>(kFluxGamma, s + 1, ax[j], fb);
+        for (int k = 0; k < 4; k++) r[k] = fb[k] - fa[k];
+        for (int q = 0; q < 7; q++) get4(Qs, D.plane, p + (long long)(q - 3) * D.P, s[q]);
+        face_flux2<1>(kFluxGamma, s, ay[i], fa);
+        face_flux2<1>(kFluxGamma, s + 1, ay[i], fb);
+        for (int k = 0; k < 4; k++) r[k] = r[k] + (fb[k] - fa[k]);
+        stage_out<S>(D, p, r, gdx * 1, dt, Qa, Qs, Qo);
+    }
+}
+
+// Coarser-level points: central differences of the physical flux at spacing d
+// with the one-sided variants of AWCM_2D.h:166-187.
+template <int S>
+__global__ void __launch_bounds__(256) k2_stage_coarse(Dims2 D, const uint32_t* __restrict__ list, int which,
+                                                       const uint8_t* __restrict__ lvl, const double* __restrict__ Qa,
+                                                       const double* __restrict__ Qs, double* __restrict__ Qo,
+                                                       double gamma, double gdx, Scal2* sc) {
+    if (sc->skip) return;
+    const int n = sc->cntC[which];
+    const double dt = sc->dt;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        const uint32_t e = list[t];
+        const int i = int(e & 0xffffu), j = int(e >> 16);
+        const long long p = gidx(D, i, j);
+        const int lev = lvl[(long long)i + (long long)D.Nx * j];
+        const int d = 1 << (D.L - 1 - lev);
+        // a closed mask keeps every used stencil point inside the ghost band
+        const bool okx = (i <= d) ? (i - d >= -G && i + 2 * d <= D.Nx - 1 + G)
+                                  : (i - 2 * d >= -G && i + ((i >= D.Nx - 1 - d) ? d : 2 * d) <= D.Nx - 1 + G);
+        const bool oky = (j <= d) ? (j - d >= -G && j + 2 * d <= D.Ny - 1 + G)
+                                  : (j - 2 * d >= -G && j + ((j >= D.Ny - 1 - d) ? d : 2 * d) <= D.Ny - 1 + G);
+        if (!okx || !oky) { sc->flag |= 1; continue; }
+        double q[4], fll[4], fl[4], fr[4], frr[4], dF[4], dG[4];
+        const long long sx = d, sy = (long long)d * D.P;
+        if (i <= d) {
+            get4(Qs, D.plane, p - sx, q); flux_f2<0>(gamma, q, fl);
+            get4(Qs, D.plane, p + sx, q); flux_f2<0>(gamma, q, fr);
+            get4(Qs, D.plane, p + 2 * sx, q); flux_f2<0>(gamma, q, frr);
+            for (int k = 0; k < 4; k++) dF[k] = kD1 * (fr[k] - fl[k]) + kD2 * (frr[k] - fl[k]);
+        } else if (i >= D.Nx - 1 - d) {
+            get4(Qs, D.plane, p - 2 * sx, q); flux_f2<0>(gamma, q, fll);
+            get4(Qs, D.plane, p - sx, q); flux_f2<0>(gamma, q, fl);
+            get4(Qs, D.plane, p + sx, q); flux_f2<0>(gamma, q, fr);
+            for (int k = 0; k < 4; k++) dF[k] = kD1 * (fr[k] - fl[k]) + kD2 * (fr[k] - fll[k]);
+        } else {
+            get4(Qs, D.plane, p - 2 * sx, q); flux_f2<0>(gamma, q, fll);
+            get4(Qs, D.plane, p - sx, q); flux_f2<0>(gamma, q, fl);
+            get4(Qs, D.plane, p + sx, q); flux_f2<0>(gamma, q, fr);
+            get4(Qs, D.plane, p + 2 * sx, q); flux_f2<0>(gamma, q, frr);
+            for (int k = 0; k < 4; k++) dF[k] = kD1 * (fr[k] - fl[k]) + kD2 * (frr[k] - fll[k]);
+        }
+        if (j <= d) {
+            get4(Qs, D.plane, p - sy, q); flux_f2<1>(gamma, q, fl);
+            get4(Qs, D.plane, p + sy, q); flux_f2<1>(gamma, q, fr);
+            get4(Qs, D.plane, p + 2 * sy, q); flux_f2<1>(gamma, q, frr);
+            for (int k = 0; k < 4; k++) dG[k] = kD1 * (fr[k] - fl[k]) + kD2 * (frr[k] - fl[k]);
+        } else if (j >= D.Ny - 1 - d) {
+            get4(Qs, D.plane, p - 2 * sy, q); flux_f2<1>(gamma, q, fll);
+            get4(Qs, D.plane, p - sy, q); flux_f2<1>(gamma, q, fl);
+            get4(Qs, D.plane, p + sy, q); flux_f2<1>(gamma, q, fr);
+            for (int k = 0; k < 4; k++) dG[k] = kD1 * (fr[k] - fl[k]) + kD2 * (fr[k] - fll[k]);
+        } else {
+            get4(Qs, D.plane, p - 2 * sy, q); flux_f2<1>(gamma, q, fll);
+            get4(Qs, D.plane, p - sy, q); flux_f2<1>(gamma, q, fl);
+            get4(Qs, D.plane, p + sy, q); flux_f2<1>(gamma, q, fr);
+            get4(Qs, D.plane, p + 2 * sy, q); flux_f2<1>(gamma, q, frr);
+            for (int k = 0; k < 4; k++) dG[k] = kD1 * (fr[k] - fl[k]) + kD2 * (frr[k] - fll[k]);
+        }
+        double r[4];
+        for (int k = 0; k < 4; k++) r[k] = dF[k] + dG[k];
+        stage_out<S>(D, p, r, gdx * d, dt, Qa, Qs, Qo);
+    }
+}
+
+// RK combine on mask0 and RemoveResidueEuler on the interior; block 0 advances t.
+__global__ void k2_combine(Dims2 D, const uint8_t* __restrict__ m0, double* __restrict__ Q,
+                           const double* __restrict__ Q0, int do_combine, Scal2* sc) {
+    if (sc->skip) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (do_combine && i == 0 && j == 0) {
+        sc->t = sc->t + sc->dt;
+        if (sc->do_print) sc->cnt = sc->cnt + 1;
+        sc->steps += 1;
+        sc->sum_active0 += (long long)sc->cntW[0] + sc->cntC[0];
+    }
+    if (i >= D.Nx) return;
+    const long long p = gidx(D, i, j);
+    if (m0[p]) {
+        if (!do_combine) return;
+        for (int k = 0; k < 4; k++) {
+            const long long o = p + k * D.plane;
+            Q[o] = (Q[o] + 2.0 * Q0[o]) / 3.0;
+        }
+    } else if (i >= 1 && i <= D.Nx - 2 && j >= 1 && j <= D.Ny - 2) {
+        for (int k = 0; k < 4; k++) Q[p + k * D.plane] = 0.0;
+    }
+}
+
+// AoS (reference layout) <-> SoA planes
+__global__ void k2_aos_to_soa(const double* __restrict__ aos, double* __restrict__ soa, long long n, long long plane) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    for (int k = 0; k < 4; k++) soa[t + k * plane] = aos[t * 4 + k];
+}
+__global__ void k2_soa_to_aos(const double* __restrict__ soa, double* __restrict__ aos, long long n, long long plane) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    for (int k = 0; k < 4; k++) aos[t * 4 + k] = soa[t + k * plane];
+}
+__global__ void k2_widen_levels(const uint8_t* __restrict__ src, int32_t* __restrict__ dst, long long n) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) dst[t] = src[t];
+}
+
+inline unsigned nblk(long long n, int b) { return unsigned((n + b - 1) / b); }
+
+Lat mklat(int is, int js, int ie, int je, int ires, int jres) {
+    Lat g;
+    g.is = is; g.js = js; g.ie = ie; g.je = je;
+    g.dx = (ie - is) / (ires - 1);
+    g.dy = (je - js) / (jres - 1);
+    g.ni = (ie - is) / g.dx + 1;
+    g.nj = (je - js) / g.dy + 1;
+    return g;
+}
+
+}  // namespace
+
+// ============================================================== host side ====
+
+Solver2D::~Solver2D() {
+    if (stream) hipStreamSynchronize(stream);
+    for (void* p : allocs) hipFree(p);
+    if (stream) hipStreamDestroy(stream);
+}
+
+int Solver2D::dalloc(void* pp, size_t bytes) {
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) return AWCM_ENOMEM;
+    allocs.push_back(p);
+    *static_cast<void**>(pp) = p;
+    return AWCM_OK;
+}
+
+int Solver2D::init(int L, int nxc_, int nyc_, double g, double dx_, double x0_, int bck) {
+    if (L < 2 || L > 14 || nxc_ < 5 || nyc_ < 5) return AWCM_EINVAL;
+    if (!(dx_ > 0) || !(g > 1)) return AWCM_EINVAL;
+    if (bck != AWCM_BC_TRANSMISSIVE && bck != AWCM_BC_DOUBLE_MACH) return AWCM_EINVAL;
+    nxc = nxc_; nyc = nyc_; gamma = g; dx = dx_; x0 = x0_; bc_kind = bck;
+    D.L = L;
+    D.maxlev = L - 1;
+    D.Nx = (nxc - 1) * (1 << (L - 1)) + 1;
+    D.Ny = (nyc - 1) * (1 << (L - 1)) + 1;
+    if (D.Nx > 65535 || D.Ny > 65535) return AWCM_EINVAL;
+    D.P = D.Nx + 2 * G;
+    D.plane = (long long)D.P * (D.Ny + 2 * G);
+    // MRA_GRID_INT_2D::Initialize (MRA_GRID_UNIFORM_2D.h:87-134)
+    const int Nx = D.Nx, Ny = D.Ny;
+    for (int lev = L - 1; lev >= 0; lev--) {
+        const int m = 1 << (L - lev - 1);
+        if (lev == L - 1) {
+            grd[lev] = mklat(0, 0, Nx - 1, Ny - 1, (nxc - 1) * (1 << lev) + 1, (nyc - 1) * (1 << lev) + 1);
+            continue;
+        }
+        grd[lev] = mklat(m, m, (Nx - 1) - m, (Ny - 1) - m, (nxc - 1) * (1 << lev) + 1 - 2, (nyc - 1) * (1 << lev) + 1 - 2);
+        sub[lev][0] = mklat(int(0.5 * m), m, int(Nx - 0.5 * m - 1), Ny - m - 1, int(Nx / m), int(Ny / m - 1));
+        sub[lev][1] = mklat(m, int(0.5 * m), Nx - m - 1, int(Ny - 0.5 * m - 1), int(Nx / m - 1), int(Ny / m));
+        sub[lev][2] = mklat(int(0.5 * m), int(0.5 * m), int(Nx - 0.5 * m - 1), int(Ny - 0.5 * m - 1), int(Nx / m), int(Ny / m));
+    }
+    wavelet_weights(W.w);
+    if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) return AWCM_EDEVICE;
+    const size_t pb = sizeof(double) * 4 * size_t(D.plane);
+    const size_t mb = size_t(D.plane);
+    const size_t nl = size_t(Nx) * Ny;
+    int st;
+    if ((st = dalloc(&Q, pb)) || (st = dalloc(&Q0, pb)) || (st = dalloc(&Q1, pb)) || (st = dalloc(&Q2, pb))) return st;
+    if ((st = dalloc(&mt, mb)) || (st = dalloc(&snap, mb)) || (st = dalloc(&tmp, mb))) return st;
+    for (int k = 0; k < 4; k++) if ((st = dalloc(&m[k], mb)) || (st = dalloc(&lv[k], nl))) return st;
+    if ((st = dalloc(&alpha, sizeof(double) * 3 * (Nx + Ny)))) return st;
+    if ((st = dalloc(&lists, sizeof(uint32_t) * 6 * nl))) return st;
+    if ((st = dalloc(&pt, sizeof(double) * 4096))) return st;
+    if ((st = dalloc(&sc, sizeof(Scal2)))) return st;
+    hipMemset(Q, 0, pb); hipMemset(Q0, 0, pb); hipMemset(Q1, 0, pb); hipMemset(Q2, 0, pb);
+    // AWCM_2D ctor (AWCM_2D.h:26-41): masks false except mask0; levels = max level
+    hipMemset(mt, 0, mb); hipMemset(snap, 0, mb); hipMemset(tmp, 0, mb);
+    for (int k = 0; k < 4; k++) { hipMemset(m[k], k == 0 ? 1 : 0, mb); hipMemset(lv[k], D.maxlev, nl); }
+    hipMemset(alpha, 0, sizeof(double) * 3 * (Nx + Ny));
+    Scal2 s0;
+    std::memset(&s0, 0, sizeof(s0));
+    s0.cnt = 1;
+    if (hipMemcpy(sc, &s0, sizeof(s0), hipMemcpyHostToDevice) != hipSuccess) return AWCM_EDEVICE;
+    int dev = 0, ncu = 256;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    grid_blocks = ncu * 8;
+    return hipDeviceSynchronize() == hipSuccess ? AWCM_OK : AWCM_EDEVICE;
+}
+
+int Solver2D::upload_q(const double* q) {
+    const size_t n = size_t(D.plane);
+    if (hipMemcpyAsync(Q2, q, sizeof(double) * 4 * n, hipMemcpyHostToDevice, stream) != hipSuccess) return AWCM_EDEVICE;
+    hipLaunchKernelGGL(k2_aos_to_soa, dim3(nblk(n, 256)), dim3(256), 0, stream, Q2, Q, (long long)n, D.plane);
+    return hipStreamSynchronize(stream) == hipSuccess ? AWCM_OK : AWCM_EDEVICE;
+}
+
+int Solver2D::download_q(double* q) {
+    const size_t n = size_t(D.plane);
+    double* stage = nullptr;
+    if (hipMalloc(&stage, sizeof(double) * 4 * n) != hipSuccess) return AWCM_ENOMEM;
+    hipLaunchKernelGGL(k2_soa_to_aos, dim3(nblk(n, 256)), dim3(256), 0, stream, Q, stage, (long long)n, D.plane);
+    int st = hipMemcpyAsync(q, stage, sizeof(double) * 4 * n, hipMemcpyDeviceToHost, stream) == hipSuccess ? AWCM_OK : AWCM_EDEVICE;
+    if (hipStreamSynchronize(stream) != hipSuccess) st = AWCM_EDEVICE;
+    hipFree(stage);
+    return st;
+}
+
+int Solver2D::set_mask0(const uint8_t* m0) {
+    if (hipStreamSynchronize(stream) != hipSuccess) return AWCM_EDEVICE;
+    return hipMemcpy(m[0], m0, size_t(D.plane), hipMemcpyHostToDevice) == hipSuccess ? AWCM_OK : AWCM_EDEVICE;
+}
+
+int Solver2D::get_masks(uint8_t* masks, int32_t* levels) {
+    if (hipStreamSynchronize(stream) != hipSuccess) return AWCM_EDEVICE;
+    const size_t mb = size_t(D.plane), nl = size_t(D.Nx) * D.Ny;
+    if (masks) {
+        if (hipMemcpy(masks, mt, mb, hipMemcpyDeviceToHost) != hipSuccess) return AWCM_EDEVICE;
+        for (int k = 0; k < 4; k++)
+            if (hipMemcpy(masks + (k + 1) * mb, m[k], mb, hipMemcpyDeviceToHost) != hipSuccess) return AWCM_EDEVICE;
+    }
+    if (levels) {
+        int32_t* wide = nullptr;
+        if (hipMalloc(&wide, sizeof(int32_t) * nl) != hipSuccess) return AWCM_ENOMEM;
+        int st = AWCM_OK;
+        for (int k = 0; k < 4 && !st; k++) {
+            hipLaunchKernelGGL(k2_widen_levels, dim3(nblk(nl, 256)), dim3(256), 0, stream, lv[k], wide, (long long)nl);
+            if (hipMemcpyAsync(levels + k * nl, wide, sizeof(int32_t) * nl, hipMemcpyDeviceToHost, stream) != hipSuccess)
+                st = AWCM_EDEVICE;
+            if (hipStreamSynchronize(stream) != hipSuccess) st = AWCM_EDEVICE;
+        }
+        hipFree(wide);
+        if (st) return st;
+    }
+    return AWCM_OK;
+}
+
+int Solver2D::set_schedule(const double* p, int n) {
+    if (n < 0 || n > 4096 || (n > 0 && !p)) return AWCM_EINVAL;
+    if (hipStreamSynchronize(stream) != hipSuccess) return AWCM_EDEVICE;
+    if (n > 0 && hipMemcpy(pt, p, sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess) return AWCM_EDEVICE;
+    Scal2 s;
+    if (hipMemcpy(&s, sc, sizeof(s), hipMemcpyDeviceToHost) != hipSuccess) return AWCM_EDEVICE;
+    s.n_print = n;
+    return hipMemcpy(sc, &s, sizeof(s), hipMemcpyHostToDevice) == hipSuccess ? AWCM_OK : AWCM_EDEVICE;
+}
+
+int Solver2D::set_state(double t, int cnt, double t_final, double tol, double cfl) {
+    if (hipStreamSynchronize(stream) != hipSuccess) return AWCM_EDEVICE;
+    Scal2 s;
+    if (hipMemcpy(&s, sc, sizeof(s), hipMemcpyDeviceToHost) != hipSuccess) return AWCM_EDEVICE;
+    s.t = t; s.cnt = cnt; s.t_final = t_final; s.tol = tol; s.cfl = cfl;
+    s.steps = 0; s.sum_active0 = 0; s.flag = 0;
+    return hipMemcpy(sc, &s, sizeof(s), hipMemcpyHostToDevice) == hipSuccess ? AWCM_OK : AWCM_EDEVICE;
+}
+
+int Solver2D::get_state(double* t, int* cnt, int* steps, long long* updates, int* flag) {
+    if (hipStreamSynchronize(stream) != hipSuccess) return AWCM_EDEVICE;
+    Scal2 s;
+    if (hipMemcpy(&s, sc, sizeof(s), hipMemcpyDeviceToHost) != hipSuccess) return AWCM_EDEVICE;
+    if (t) *t = s.t;
+    if (cnt) *cnt = s.cnt;
+    if (steps) *steps = s.steps;
+    if (updates) *updates = s.sum_active0;
+    if (flag) *flag = s.flag;
+    return AWCM_OK;
+}
+
+#define AWCM_LAUNCH(...)                                             \
+    do {                                                             \
+        hipLaunchKernelGGL(__VA_ARGS__);                             \
+        if (hipGetLastError() != hipSuccess) return AWCM_EDEVICE;    \
+    } while (0)
+
+int Solver2D::enqueue_begin(bool force) {
+    AWCM_LAUNCH(k2_begin, dim3(nblk(3 * (D.Nx + D.Ny), 256)), dim3(256), 0, stream, D, sc, alpha, force ? 1 : 0);
+    return AWCM_OK;
+}
+
+// CheckReconstruction on `mask`: level by level, d3 then d1/d2.
+static int check_recon(Solver2D& s, uint8_t* mask) {
+    for (int lev = s.D.maxlev - 1; lev >= 0; lev--) {
+        const int d = 1 << (s.D.L - 1 - lev);
+        const Lat& g3 = s.sub[lev][2];
+        AWCM_LAUNCH(k2_check_d3, dim3(nblk((long long)g3.ni * g3.nj, 256)), dim3(256), 0, s.stream, s.D, g3, d, mask, s.sc);
+        const Lat& g1 = s.sub[lev][0];
+        const Lat& g2 = s.sub[lev][1];
+        AWCM_LAUNCH(k2_check_d12, dim3(nblk((long long)g1.ni * g1.nj + (long long)g2.ni * g2.nj, 256)), dim3(256), 0,
+                    s.stream, s.D, g1, g2, d, mask, s.sc);
+    }
+    return AWCM_OK;
+}
+
+int Solver2D::enqueue_prepare() {
+    const long long nm = D.plane;
+    // mask_temp = mask0
+    AWCM_LAUNCH(k2_copy, dim3(nblk(nm, 256)), dim3(256), 0, stream, m[0], mt, nm, sc);
+    // forward transform, finest detail level first
+    for (int lev = D.maxlev - 1; lev >= 0; lev--) {
+        const int d = 1 << (D.L - 1 - lev);
+        const Lat& g3 = sub[lev][2];
+        const Lat& g1 = sub[lev][0];
+        const Lat& g2 = sub[lev][1];
+        const Lat& gc = grd[lev];
+        AWCM_LAUNCH(k2_fwt_d3, dim3(nblk((long long)g3.ni * g3.nj, 256)), dim3(256), 0, stream, D, g3, d, W, tol_host, Q, m[0], sc);
+        AWCM_LAUNCH(k2_fwt_d12, dim3(nblk((long long)g1.ni * g1.nj + (long long)g2.ni * g2.nj, 256)), dim3(256), 0, stream, D,
+                    g1, g2, d, W, tol_host, Q, m[0], sc);
+        AWCM_LAUNCH(k2_fwt_c, dim3(nblk((long long)gc.ni * gc.nj, 256)), dim3(256), 0, stream, D, gc, d, W, Q, m[0], sc);
+    }
+    AWCM_LAUNCH(k2_band_snap, dim3(nblk(nm, 256)), dim3(256), 0, stream, D, m[0], snap, sc);
+    AWCM_LAUNCH(k2_adjacent, dim3(nblk(D.Nx, 256), D.Ny), dim3(256), 0, stream, D, snap, m[0], sc);
+    if (int st = check_recon(*this, m[0])) return st;
+    for (int k = 1; k <= 3; k++) {
+        AWCM_LAUNCH(k2_level_copy, dim3(nblk(nm, 256)), dim3(256), 0, stream, D, m[k - 1], m[k], lv[k - 1], sc);
+        AWCM_LAUNCH(k2_addpoint, dim3(nblk(D.Nx - 2, 256), D.Ny - 2), dim3(256), 0, stream, D, m[k - 1], lv[k - 1], m[k], sc);
+        if (int st = check_recon(*this, m[k])) return st;
+    }
+    // inverse transform, coarsest level first
+    for (int lev = 0; lev < D.maxlev; lev++) {
+        const int d = 1 << (D.L - 1 - lev);
+        const Lat& g3 = sub[lev][2];
+        const Lat& g1 = sub[lev][0];
+        const Lat& g2 = sub[lev][1];
+        const Lat& gc = grd[lev];
+        AWCM_LAUNCH(k2_iwt_c, dim3(nblk((long long)gc.ni * gc.nj, 256)), dim3(256), 0, stream, D, gc, d, W, Q, m[3], sc);
+        AWCM_LAUNCH(k2_iwt_d12, dim3(nblk((long long)g1.ni * g1.nj + (long long)g2.ni * g2.nj, 256)), dim3(256), 0, stream, D,
+                    g1, g2, d, W, Q, m[3], sc);
+        AWCM_LAUNCH(k2_iwt_d3, dim3(nblk((long long)g3.ni * g3.nj, 256)), dim3(256), 0, stream, D, g3, d, W, Q, m[3], sc);
+    }
+    // interpolation of momentum and energy at new points
+    AWCM_LAUNCH(k2_interp_tmp, dim3(nblk(nm, 256)), dim3(256), 0, stream, D, m[3], mt, tmp, sc);
+    if (int st = check_recon(*this, tmp)) return st;
+    for (int lev = 0; lev < D.maxlev; lev++) {
+        const int d = 1 << (D.L - 1 - lev);
+        const Lat& g3 = sub[lev][2];
+        const Lat& g1 = sub[lev][0];
+        const Lat& g2 = sub[lev][1];
+        AWCM_LAUNCH(k2_interp_d12, dim3(nblk((long long)g1.ni * g1.nj + (long long)g2.ni * g2.nj, 256)), dim3(256), 0, stream,
+                    D, g1, g2, d, W, Q, tmp, mt, sc);
+        AWCM_LAUNCH(k2_interp_d3, dim3(nblk((long long)g3.ni * g3.nj, 256)), dim3(256), 0, stream, D, g3, d, W, Q, tmp, mt, sc);
+    }
+    return AWCM_OK;
+}
+
+int Solver2D::enqueue_step() {
+    if (int st = enqueue_begin(false)) return st;
+    if (int st = enqueue_prepare()) return st;
+    const long long nl = (long long)D.Nx * D.Ny;
+    AWCM_LAUNCH(k2_compact, dim3(nblk(nl, kCompactThreads * kCompactPer)), dim3(kCompactThreads), 0, stream, D, m[0], m[1],
+                m[2], lv[0], lv[1], lv[2], lists, sc);
+    const dim3 ag(nblk(D.Nx, 64), nblk(D.Ny, kAlphaRows)), ab(64, 4);
+    const unsigned nbc = nblk(2 * G * (D.Nx + D.Ny), 256);
+    const unsigned gw = grid_blocks * 2, gc = grid_blocks;
+    // stage 1: alpha over mask3 of Q (+ CFL), ghosts of Q, mask2/level2 -> Q2
+    AWCM_LAUNCH(k2_alpha, ag, ab, 0, stream, D, m[3], Q, ax(0), ay(0), gamma, 1, sc);
+    AWCM_LAUNCH(k2_bc, dim3(nbc), dim3(256), 0, stream, D, Q, bc_kind, x0, dx, 1, pt, sc);
+    AWCM_LAUNCH(k2_stage_weno<1>, dim3(gw), dim3(128), 0, stream, D, list(2, 1), 2, Q, Q, Q2, ax(0), ay(0), dx, sc);
+    AWCM_LAUNCH(k2_stage_coarse<1>, dim3(gc), dim3(256), 0, stream, D, list(2, 0), 2, lv[2], Q, Q, Q2, gamma, dx, sc);
+    // stage 2: ghosts of Q2, alpha over mask2 of Q2, mask1/level1 -> Q1
+    AWCM_LAUNCH(k2_bc, dim3(nbc), dim3(256), 0, stream, D, Q2, bc_kind, x0, dx, 0, pt, sc);
+    AWCM_LAUNCH(k2_alpha, ag, ab, 0, stream, D, m[2], Q2, ax(1), ay(1), gamma, 0, sc);
+    AWCM_LAUNCH(k2_stage_weno<2>, dim3(gw), dim3(128), 0, stream, D, list(1, 1), 1, Q, Q2, Q1, ax(1), ay(1), dx, sc);
+    AWCM_LAUNCH(k2_stage_coarse<2>, dim3(gc), dim3(256), 0, stream, D, list(1, 0), 1, lv[1], Q, Q2, Q1, gamma, dx, sc);
+    // stage 3: ghosts of Q1, alpha over mask1 of Q1, mask0/level0 -> Q0
+    AWCM_LAUNCH(k2_bc, dim3(nbc), dim3(256), 0, stream, D, Q1, bc_kind, x0, dx, 0, pt, sc);
+    AWCM_LAUNCH(k2_alpha, ag, ab, 0, stream, D, m[1], Q1, ax(2), ay(2), gamma, 0, sc);
+    AWCM_LAUNCH(k2_stage_weno<3>, dim3(gw), dim3(128), 0, stream, D, list(0, 1), 0, Q1, Q1, Q0, ax(2), ay(2), dx, sc);
+    AWCM_LAUNCH(k2_stage_coarse<3>, dim3(gc), dim3(256), 0, stream, D, list(0, 0), 0, lv[0], Q1, Q1, Q0, gamma, dx, sc);
+    AWCM_LAUNCH(k2_combine, dim3(nblk(D.Nx, 256), D.Ny), dim3(256), 0, stream, D, m[0], Q, Q0, 1, sc);
+    return AWCM_OK;
+}
+
+int Solver2D::enqueue_post() {
+    if (int st = enqueue_begin(true)) return st;
+    AWCM_LAUNCH(k2_combine, dim3(nblk(D.Nx, 256), D.Ny), dim3(256), 0, stream, D, m[0], Q, Q0, 0, sc);
+    return AWCM_OK;
+}
+
+}  // namespace awcm

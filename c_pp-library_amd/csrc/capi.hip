@@ -1,0 +1,206 @@
+// C ABI entry points of the solver handle (include/awcm_capi.h).  A 1-D handle
+// is a batch of one instance; a 2-D handle owns a Solver2D.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "awcm_capi.h"
+#include "runtime.h"
+
+using awcm::Batch1D;
+using awcm::Solver2D;
+
+namespace awcm {
+
+int set_schedule1(Batch1D& B, const double* pt, int npt) {
+    if (npt < 0 || (npt > 0 && !pt)) return AWCM_EINVAL;
+    if (hipStreamSynchronize(B.stream) != hipSuccess) return AWCM_EDEVICE;
+    if (npt > B.n_print) {
+        double* p = nullptr;
+        if (B.dalloc(&p, sizeof(double) * npt)) return AWCM_ENOMEM;
+        B.d_pt = p;
+    }
+    if (npt > 0 && hipMemcpy(B.d_pt, pt, sizeof(double) * npt, hipMemcpyHostToDevice) != hipSuccess) return AWCM_EDEVICE;
+    B.n_print = std::max(B.n_print, npt);
+    for (auto& in : B.h) { in.n_print = npt; in.print_times = B.d_pt; }
+    return AWCM_OK;
+}
+
+}  // namespace awcm
+
+namespace {
+
+int flag_status(int flag) {
+    if (flag & 2) return AWCM_ENONFINITE;
+    if (flag & 1) return AWCM_ERANGE;
+    return AWCM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int awcm_create_1d(int num_levels, int nx_coarse, double gamma, double dx, awcm_solver** out) {
+    if (!out) return AWCM_EINVAL;
+    *out = nullptr;
+    auto* h = new (std::nothrow) awcm_solver();
+    if (!h) return AWCM_ENOMEM;
+    h->dim = 1;
+    awcm_instance_1d p;
+    p.num_levels = num_levels;
+    p.nx_coarse = nx_coarse;
+    p.gamma = gamma;
+    p.dx = dx;
+    p.tol = 0;
+    p.cfl = 0.5;
+    p.t_final = 0;
+    const int st = h->b1.init(1, &p, nullptr, 0);
+    if (st) { delete h; return st; }
+    *out = h;
+    return AWCM_OK;
+}
+
+int awcm_create_2d(int num_levels, int nx_coarse, int ny_coarse, double gamma, double dx, double x0, int bc_kind,
+                   awcm_solver** out) {
+    if (!out) return AWCM_EINVAL;
+    *out = nullptr;
+    auto* h = new (std::nothrow) awcm_solver();
+    if (!h) return AWCM_ENOMEM;
+    h->dim = 2;
+    h->s2 = new (std::nothrow) Solver2D();
+    if (!h->s2) { delete h; return AWCM_ENOMEM; }
+    const int st = h->s2->init(num_levels, nx_coarse, ny_coarse, gamma, dx, x0, bc_kind);
+    if (st) { delete h; return st; }
+    *out = h;
+    return AWCM_OK;
+}
+
+void awcm_destroy(awcm_solver* h) { delete h; }
+
+int awcm_dims(const awcm_solver* h, int* nx, int* ny) {
+    if (!h) return AWCM_EINVAL;
+    if (h->dim == 1) {
+        if (nx) *nx = h->b1.h[0].Nx;
+        if (ny) *ny = 1;
+    } else {
+        if (nx) *nx = h->s2->D.Nx;
+        if (ny) *ny = h->s2->D.Ny;
+    }
+    return AWCM_OK;
+}
+
+int awcm_upload_q(awcm_solver* h, const double* q) {
+    if (!h || !q) return AWCM_EINVAL;
+    return h->dim == 1 ? awcm::upload_q1(h->b1, 0, q) : h->s2->upload_q(q);
+}
+
+int awcm_download_q(awcm_solver* h, double* q) {
+    if (!h || !q) return AWCM_EINVAL;
+    return h->dim == 1 ? awcm::download_q1(h->b1, 0, q) : h->s2->download_q(q);
+}
+
+int awcm_set_mask0(awcm_solver* h, const uint8_t* mask0) {
+    if (!h || !mask0) return AWCM_EINVAL;
+    return h->dim == 1 ? awcm::set_mask0_1(h->b1, 0, mask0) : h->s2->set_mask0(mask0);
+}
+
+int awcm_get_masks(awcm_solver* h, uint8_t* masks, int32_t* levels) {
+    if (!h) return AWCM_EINVAL;
+    return h->dim == 1 ? awcm::get_masks1(h->b1, 0, masks, levels) : h->s2->get_masks(masks, levels);
+}
+
+int awcm_set_schedule(awcm_solver* h, const double* print_times, int n_print) {
+    if (!h) return AWCM_EINVAL;
+    return h->dim == 1 ? awcm::set_schedule1(h->b1, print_times, n_print) : h->s2->set_schedule(print_times, n_print);
+}
+
+int awcm_prepare_euler(awcm_solver* h, double tol) {
+    if (!h) return AWCM_EINVAL;
+    if (h->dim == 1) {
+        Batch1D& B = h->b1;
+        B.h[0].tol = tol;
+        int st = B.push();
+        if (!st) st = B.launch(0, 0, 0, 1);
+        if (!st) st = B.pull();
+        if (!st) st = B.check_status();
+        return st;
+    }
+    Solver2D& S = *h->s2;
+    S.tol_host = tol;
+    int st = S.enqueue_begin(true);
+    if (!st) st = S.enqueue_prepare();
+    int flag = 0;
+    if (!st) st = S.get_state(nullptr, nullptr, nullptr, nullptr, &flag);
+    return st ? st : flag_status(flag);
+}
+
+int awcm_post_process_euler(awcm_solver* h) {
+    if (!h) return AWCM_EINVAL;
+    if (h->dim == 1) {
+        Batch1D& B = h->b1;
+        int st = B.push();
+        if (!st) st = B.launch(2, 0, 0, 1);
+        if (!st) st = B.pull();
+        return st;
+    }
+    int st = h->s2->enqueue_post();
+    if (!st) st = hipStreamSynchronize(h->s2->stream) == hipSuccess ? AWCM_OK : AWCM_EDEVICE;
+    return st;
+}
+
+int awcm_run(awcm_solver* h, double tol, double cfl, double t_final, int max_steps, double* t, int* cnt,
+             int* steps_done, long long* updates) {
+    if (!h || !t || !cnt || max_steps < 0 || !(cfl > 0)) return AWCM_EINVAL;
+    if (h->dim == 1) {
+        Batch1D& B = h->b1;
+        awcm::Inst1D& in = B.h[0];
+        in.tol = tol; in.cfl = cfl; in.t_final = t_final; in.t = *t; in.cnt = *cnt;
+        in.steps = 0; in.sum_active0 = 0; in.status = 0;
+        int st = B.push();
+        if (!st) st = B.launch(1, max_steps, 0, 1);
+        if (!st) st = B.pull();
+        if (!st) {
+            *t = in.t; *cnt = in.cnt;
+            if (steps_done) *steps_done = in.steps;
+            if (updates) *updates = in.sum_active0;
+            st = B.check_status();
+        }
+        return st;
+    }
+    Solver2D& S = *h->s2;
+    S.tol_host = tol;
+    int st = S.set_state(*t, *cnt, t_final, tol, cfl);
+    // Queue the steps in chunks; once t >= t_final the queued steps are no-ops.
+    const int chunk = 32;
+    int done = 0;
+    while (!st && done < max_steps) {
+        const int k = std::min(chunk, max_steps - done);
+        for (int s = 0; s < k && !st; s++) st = S.enqueue_step();
+        done += k;
+        double tt = 0;
+        int flag = 0;
+        if (!st) st = S.get_state(&tt, nullptr, nullptr, nullptr, &flag);
+        if (!st) st = flag_status(flag);
+        if (!st && !(tt < t_final)) break;
+    }
+    if (!st) {
+        int nsteps = 0;
+        long long upd = 0;
+        st = S.get_state(t, cnt, &nsteps, &upd, nullptr);
+        if (steps_done) *steps_done = nsteps;
+        if (updates) *updates = upd;
+    }
+    return st;
+}
+
+int awcm_synchronize(awcm_solver* h) {
+    if (!h) return AWCM_EINVAL;
+    hipStream_t s = h->dim == 1 ? h->b1.stream : h->s2->stream;
+    return hipStreamSynchronize(s) == hipSuccess ? AWCM_OK : AWCM_EDEVICE;
+}
+
+}  // extern "C"

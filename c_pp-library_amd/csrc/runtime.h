@@ -1,0 +1,58 @@
+// Host-side runtime objects behind the C ABI handles.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "awcm_capi.h"
+#include "awcm_internal.h"
+#include "solver2d.h"
+
+namespace awcm {
+
+// N independent 1-D solves, one workgroup each (one instance = plain solver).
+struct Batch1D {
+    int n = 0;
+    int max_nx = 0;
+    int n_print = 0;
+    int threads = 256;
+    int workgroups = 0;
+    std::vector<Inst1D> h;        // host mirror of the descriptors
+    Inst1D* d_inst = nullptr;
+    int* d_order = nullptr;
+    int* d_identity = nullptr;
+    int* d_qctr = nullptr;
+    double* d_pt = nullptr;
+    hipStream_t stream = nullptr;
+    std::vector<void*> allocs;
+
+    ~Batch1D();
+    int init(int n, const awcm_instance_1d* params, const double* pt, int npt);
+    int dalloc(void* pp, size_t bytes);
+    int push();
+    int pull();
+    int launch(int mode, int max_steps, int first, int count);
+    const int* d_order_for(int first, int count);
+    int check_status();
+};
+
+int upload_q1(Batch1D& B, int idx, const double* q);
+int download_q1(Batch1D& B, int idx, double* q);
+int get_masks1(Batch1D& B, int idx, uint8_t* masks, int32_t* levels);
+int set_mask0_1(Batch1D& B, int idx, const uint8_t* m0);
+int set_schedule1(Batch1D& B, const double* pt, int npt);
+
+}  // namespace awcm
+
+struct awcm_batch {
+    awcm::Batch1D b1;
+};
+
+struct awcm_solver {
+    int dim = 1;
+    awcm::Batch1D b1;       // dim 1
+    awcm::Solver2D* s2 = nullptr;  // dim 2
+    ~awcm_solver() { delete s2; }
+};

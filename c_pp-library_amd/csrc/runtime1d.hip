@@ -1,0 +1,263 @@
+// Host runtime of the 1-D solver and of the batched 1-D form behind the C ABI.
+// A plain 1-D solver handle is a batch of one instance.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <numeric>
+#include <vector>
+
+#include "awcm_capi.h"
+#include "awcm_internal.h"
+#include "runtime.h"
+
+namespace awcm {
+
+__global__ void k1_solve(Inst1D* insts, int n_inst, const int* order, int* qctr, int mode, int max_steps, int nxp);
+
+namespace {
+
+constexpr int G = 3;
+
+size_t lds_bytes(int nxp) { return size_t(11) * nxp; }
+
+}  // namespace
+
+Batch1D::~Batch1D() {
+    for (void* p : allocs) hipFree(p);
+    if (stream) hipStreamDestroy(stream);
+}
+
+int Batch1D::init(int n_, const awcm_instance_1d* params, const double* pt, int npt) {
+    n = n_;
+    if (n <= 0 || !params) return AWCM_EINVAL;
+    if (npt < 0 || (npt > 0 && !pt)) return AWCM_EINVAL;
+    if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) return AWCM_EDEVICE;
+    h.resize(n);
+    max_nx = 0;
+    for (int k = 0; k < n; k++) {
+        const awcm_instance_1d& p = params[k];
+        if (p.num_levels < 2 || p.num_levels > 14 || p.nx_coarse < 6) return AWCM_EINVAL;
+        if (!(p.dx > 0) || !(p.gamma > 1) || !(p.cfl > 0)) return AWCM_EINVAL;
+        Inst1D& in = h[k];
+        std::memset(&in, 0, sizeof(in));
+        in.L = p.num_levels;
+        in.nxc = p.nx_coarse;
+        in.Nx = (p.nx_coarse - 1) * (1 << (p.num_levels - 1)) + 1;
+        in.gamma = p.gamma;
+        in.dx = p.dx;
+        in.tol = p.tol;
+        in.cfl = p.cfl;
+        in.t_final = p.t_final;
+        in.cnt = 1;
+        max_nx = std::max(max_nx, in.Nx);
+    }
+    // device buffers
+    int st = dalloc(&d_pt, sizeof(double) * std::max(npt, 1));
+    if (st) return st;
+    if (npt > 0 && hipMemcpy(d_pt, pt, sizeof(double) * npt, hipMemcpyHostToDevice) != hipSuccess) return AWCM_EDEVICE;
+    n_print = npt;
+    for (int k = 0; k < n; k++) {
+        Inst1D& in = h[k];
+        const int NP = in.Nx + 2 * G;
+        in.n_print = npt;
+        in.print_times = d_pt;
+        if ((st = dalloc(&in.Q, sizeof(double) * 3 * NP))) return st;
+        if ((st = dalloc(&in.Qc, sizeof(double) * 3 * NP))) return st;
+        if ((st = dalloc(&in.Qt, sizeof(double) * 3 * NP))) return st;
+        if ((st = dalloc(&in.F, sizeof(double) * 3 * (in.Nx + 1)))) return st;
+        if ((st = dalloc(&in.masks, size_t(5) * NP))) return st;
+        if ((st = dalloc(&in.levels, size_t(4) * in.Nx))) return st;
+        hipMemset(in.Q, 0, sizeof(double) * 3 * NP);
+        hipMemset(in.Qc, 0, sizeof(double) * 3 * NP);
+        hipMemset(in.Qt, 0, sizeof(double) * 3 * NP);
+        // AWCM_1D ctor (AWCM_1D.h:23-34): all masks false except mask0, levels 0
+        hipMemset(in.masks, 0, size_t(5) * NP);
+        hipMemset(in.masks + NP, 1, NP);
+        hipMemset(in.levels, 0, size_t(4) * in.Nx);
+    }
+    if ((st = dalloc(&d_inst, sizeof(Inst1D) * n))) return st;
+    if ((st = dalloc(&d_order, sizeof(int) * n))) return st;
+    if ((st = dalloc(&d_qctr, sizeof(int)))) return st;
+    // largest instance first (longest-processing-time order for the work queue)
+    std::vector<int> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return h[a].Nx > h[b].Nx; });
+    if (hipMemcpy(d_order, order.data(), sizeof(int) * n, hipMemcpyHostToDevice) != hipSuccess) return AWCM_EDEVICE;
+    if (hipDeviceSynchronize() != hipSuccess) return AWCM_EDEVICE;
+    threads = max_nx >= 1024 ? 1024 : 256;
+    workgroups = 0;
+    return AWCM_OK;
+}
+
+int Batch1D::dalloc(void* pp, size_t bytes) {
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) return AWCM_ENOMEM;
+    allocs.push_back(p);
+    *static_cast<void**>(pp) = p;
+    return AWCM_OK;
+}
+
+int Batch1D::push() {
+    return hipMemcpyAsync(d_inst, h.data(), sizeof(Inst1D) * n, hipMemcpyHostToDevice, stream) == hipSuccess
+               ? AWCM_OK : AWCM_EDEVICE;
+}
+
+int Batch1D::pull() {
+    if (hipMemcpyAsync(h.data(), d_inst, sizeof(Inst1D) * n, hipMemcpyDeviceToHost, stream) != hipSuccess)
+        return AWCM_EDEVICE;
+    return hipStreamSynchronize(stream) == hipSuccess ? AWCM_OK : AWCM_EDEVICE;
+}
+
+int Batch1D::launch(int mode, int max_steps, int first, int count) {
+    const int nxp = ((max_nx + 2 * G) + 15) / 16 * 16;
+    if (hipMemsetAsync(d_qctr, 0, sizeof(int), stream) != hipSuccess) return AWCM_EDEVICE;
+    const int wgs = workgroups > 0 ? std::min(workgroups, count) : count;
+    hipLaunchKernelGGL(k1_solve, dim3(wgs), dim3(threads), lds_bytes(nxp), stream, d_inst + first, count,
+                       d_order_for(first, count), d_qctr, mode, max_steps, nxp);
+    return hipGetLastError() == hipSuccess ? AWCM_OK : AWCM_EDEVICE;
+}
+
+// For a launch over a sub-range (sequential mode) the order is the identity.
+const int* Batch1D::d_order_for(int first, int count) {
+    if (first == 0 && count == n) return d_order;
+    if (!d_identity) {
+        std::vector<int> id(n, 0);
+        if (dalloc(&d_identity, sizeof(int) * n)) return d_order;
+        hipMemcpy(d_identity, id.data(), sizeof(int) * n, hipMemcpyHostToDevice);
+    }
+    return d_identity;
+}
+
+int Batch1D::check_status() {
+    for (int k = 0; k < n; k++)
+        if (h[k].status) return h[k].status == -5 ? AWCM_ERANGE : AWCM_EDEVICE;
+    return AWCM_OK;
+}
+
+}  // namespace awcm
+
+using awcm::Batch1D;
+
+extern "C" {
+
+int awcm_batch_create_1d(int n, const awcm_instance_1d* params, const double* print_times, int n_print,
+                         awcm_batch** out) {
+    if (!out) return AWCM_EINVAL;
+    *out = nullptr;
+    auto* b = new (std::nothrow) awcm_batch();
+    if (!b) return AWCM_ENOMEM;
+    const int st = b->b1.init(n, params, print_times, n_print);
+    if (st) { delete b; return st; }
+    *out = b;
+    return AWCM_OK;
+}
+
+void awcm_batch_destroy(awcm_batch* b) { delete b; }
+
+int awcm_batch_upload_q(awcm_batch* b, int idx, const double* q) {
+    if (!b || !q || idx < 0 || idx >= b->b1.n) return AWCM_EINVAL;
+    return awcm::upload_q1(b->b1, idx, q);
+}
+
+int awcm_batch_download_q(awcm_batch* b, int idx, double* q) {
+    if (!b || !q || idx < 0 || idx >= b->b1.n) return AWCM_EINVAL;
+    return awcm::download_q1(b->b1, idx, q);
+}
+
+int awcm_batch_get_masks(awcm_batch* b, int idx, uint8_t* masks, int32_t* levels) {
+    if (!b || idx < 0 || idx >= b->b1.n) return AWCM_EINVAL;
+    return awcm::get_masks1(b->b1, idx, masks, levels);
+}
+
+int awcm_batch_set_launch(awcm_batch* b, int threads, int workgroups) {
+    if (!b || threads < 64 || threads > 1024 || threads % 64 || workgroups < 0) return AWCM_EINVAL;
+    b->b1.threads = threads;
+    b->b1.workgroups = workgroups;
+    return AWCM_OK;
+}
+
+int awcm_batch_run(awcm_batch* b, int max_steps) {
+    if (!b || max_steps < 0) return AWCM_EINVAL;
+    Batch1D& B = b->b1;
+    int st = B.push();
+    if (!st) st = B.launch(1, max_steps, 0, B.n);
+    if (!st) st = B.pull();
+    if (!st) st = B.check_status();
+    return st;
+}
+
+int awcm_batch_run_sequential(awcm_batch* b, int max_steps) {
+    if (!b || max_steps < 0) return AWCM_EINVAL;
+    Batch1D& B = b->b1;
+    int st = B.push();
+    for (int k = 0; k < B.n && !st; k++) st = B.launch(1, max_steps, k, 1);
+    if (!st) st = B.pull();
+    if (!st) st = B.check_status();
+    return st;
+}
+
+int awcm_batch_state(awcm_batch* b, int idx, double* t, int* cnt, int* steps, long long* updates, int* status) {
+    if (!b || idx < 0 || idx >= b->b1.n) return AWCM_EINVAL;
+    const awcm::Inst1D& in = b->b1.h[idx];
+    if (t) *t = in.t;
+    if (cnt) *cnt = in.cnt;
+    if (steps) *steps = in.steps;
+    if (updates) *updates = in.sum_active0;
+    if (status) *status = in.status;
+    return AWCM_OK;
+}
+
+int awcm_batch_synchronize(awcm_batch* b) {
+    if (!b) return AWCM_EINVAL;
+    return hipStreamSynchronize(b->b1.stream) == hipSuccess ? AWCM_OK : AWCM_EDEVICE;
+}
+
+}  // extern "C"
+
+namespace awcm {
+
+int upload_q1(Batch1D& B, int idx, const double* q) {
+    Inst1D& in = B.h[idx];
+    const int NP = in.Nx + 2 * 3;
+    std::vector<double> soa(size_t(3) * NP);
+    for (int i = 0; i < NP; i++)
+        for (int c = 0; c < 3; c++) soa[size_t(c) * NP + i] = q[size_t(i) * 3 + c];
+    if (hipMemcpy(in.Q, soa.data(), sizeof(double) * soa.size(), hipMemcpyHostToDevice) != hipSuccess) return AWCM_EDEVICE;
+    return AWCM_OK;
+}
+
+int download_q1(Batch1D& B, int idx, double* q) {
+    Inst1D& in = B.h[idx];
+    const int NP = in.Nx + 2 * 3;
+    std::vector<double> soa(size_t(3) * NP);
+    if (hipStreamSynchronize(B.stream) != hipSuccess) return AWCM_EDEVICE;
+    if (hipMemcpy(soa.data(), in.Q, sizeof(double) * soa.size(), hipMemcpyDeviceToHost) != hipSuccess) return AWCM_EDEVICE;
+    for (int i = 0; i < NP; i++)
+        for (int c = 0; c < 3; c++) q[size_t(i) * 3 + c] = soa[size_t(c) * NP + i];
+    return AWCM_OK;
+}
+
+int get_masks1(Batch1D& B, int idx, uint8_t* masks, int32_t* levels) {
+    Inst1D& in = B.h[idx];
+    const int NP = in.Nx + 2 * 3;
+    if (hipStreamSynchronize(B.stream) != hipSuccess) return AWCM_EDEVICE;
+    if (masks && hipMemcpy(masks, in.masks, size_t(5) * NP, hipMemcpyDeviceToHost) != hipSuccess) return AWCM_EDEVICE;
+    if (levels) {
+        std::vector<uint8_t> lv(size_t(4) * in.Nx);
+        if (hipMemcpy(lv.data(), in.levels, lv.size(), hipMemcpyDeviceToHost) != hipSuccess) return AWCM_EDEVICE;
+        for (size_t k = 0; k < lv.size(); k++) levels[k] = lv[k];
+    }
+    return AWCM_OK;
+}
+
+int set_mask0_1(Batch1D& B, int idx, const uint8_t* m0) {
+    Inst1D& in = B.h[idx];
+    const int NP = in.Nx + 2 * 3;
+    if (hipStreamSynchronize(B.stream) != hipSuccess) return AWCM_EDEVICE;
+    return hipMemcpy(in.masks + NP, m0, NP, hipMemcpyHostToDevice) == hipSuccess ? AWCM_OK : AWCM_EDEVICE;
+}
+
+}  // namespace awcm

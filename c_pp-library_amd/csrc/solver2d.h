@@ -1,0 +1,78 @@
+// Host object of one 2-D AWCM solve: device buffers, geometry and the kernel
+// sequence of one step (see awcm2d.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+namespace awcm {
+
+// An integer lattice of the MRA grid (GRID_INT_2D, MRA_GRID_UNIFORM_2D.h:10-60).
+struct Lat {
+    int is, js, ie, je, dx, dy, ni, nj;
+};
+
+struct Dims2 {
+    int Nx, Ny, P, L, maxlev;
+    long long plane;  // doubles per state plane, P * (Ny + 6)
+};
+
+struct Wts {
+    double w[4][5];  // wavelet weight(i, j) at [i][j + 1]
+};
+
+// Device scalars of the time loop (one allocation).
+struct Scal2 {
+    unsigned long long vmax;  // bits of max(SRx + SRy) over mask3
+    double t, dt, t_final, tol, cfl;
+    long long sum_active0;
+    int cnt, n_print, steps, skip, do_print, flag;
+    int cntW[3], cntC[3];     // stage work-list lengths (mask0, mask1, mask2)
+};
+
+struct Solver2D {
+    Dims2 D{};
+    int nxc = 0, nyc = 0, bc_kind = 0;
+    double gamma = 1.4, dx = 0, x0 = 0;
+    double tol_host = 0;  // wavelet threshold of the queued steps
+    Lat sub[16][3]{};
+    Lat grd[16]{};
+    Wts W{};
+    hipStream_t stream = nullptr;
+    std::vector<void*> allocs;
+
+    double *Q = nullptr, *Q0 = nullptr, *Q1 = nullptr, *Q2 = nullptr;  // 4 planes each
+    uint8_t *mt = nullptr, *m[4] = {}, *snap = nullptr, *tmp = nullptr;
+    uint8_t* lv[4] = {};
+    double* alpha = nullptr;     // 3 sets of (Ny alpha_x, Nx alpha_y)
+    uint32_t* lists = nullptr;   // 6 lists of Nx*Ny packed (j << 16 | i)
+    double* pt = nullptr;        // print times
+    Scal2* sc = nullptr;
+    int grid_blocks = 0;         // persistent grid for list kernels
+    int weno_threads = 128;
+
+    ~Solver2D();
+    int init(int L, int nxc, int nyc, double gamma, double dx, double x0, int bc_kind);
+    int dalloc(void* pp, size_t bytes);
+    uint32_t* list(int k, int weno) const { return lists + size_t(2 * k + (weno ? 0 : 1)) * size_t(D.Nx) * D.Ny; }
+    double* ax(int s) const { return alpha + size_t(s) * (D.Nx + D.Ny); }
+    double* ay(int s) const { return alpha + size_t(s) * (D.Nx + D.Ny) + D.Ny; }
+
+    int upload_q(const double* q);
+    int download_q(double* q);
+    int set_mask0(const uint8_t* m0);
+    int get_masks(uint8_t* masks, int32_t* levels);
+    int set_schedule(const double* p, int n);
+    int set_state(double t, int cnt, double t_final, double tol, double cfl);
+    int get_state(double* t, int* cnt, int* steps, long long* updates, int* flag);
+
+    // kernel sequences (enqueued on `stream`, no host synchronisation)
+    int enqueue_begin(bool force);
+    int enqueue_prepare();
+    int enqueue_step();       // begin + prepare + dt + 3 stages + combine
+    int enqueue_post();
+};
+
+}  // namespace awcm

@@ -1,0 +1,229 @@
+"""Python binding of the C ABI in include/awcm_capi.h (libawcm_hip.so).
+
+Loads the in-tree HIP library and raises if it is missing: there is no CPU
+fallback behind these classes.  Layouts are the reference's (AoS, ghost 3).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libawcm_hip.so")
+
+AWCM_OK = 0
+ERRORS = {-1: "EINVAL", -2: "EDEVICE", -3: "ENOMEM", -4: "ENONFINITE", -5: "ERANGE", -6: "ESTATE"}
+BC_TRANSMISSIVE = 0
+BC_DOUBLE_MACH = 1
+
+
+class AwcmError(RuntimeError):
+    def __init__(self, code, what):
+        super().__init__(f"{what} failed: {ERRORS.get(code, code)}")
+        self.code = code
+
+
+class Instance1D(C.Structure):
+    _fields_ = [("num_levels", C.c_int), ("nx_coarse", C.c_int), ("gamma", C.c_double), ("dx", C.c_double),
+                ("tol", C.c_double), ("cfl", C.c_double), ("t_final", C.c_double)]
+
+
+_dp = C.POINTER(C.c_double)
+_u8p = C.POINTER(C.c_uint8)
+_i32p = C.POINTER(C.c_int32)
+_lib = None
+
+
+def lib():
+    """The loaded HIP library (raises FileNotFoundError if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError(f"{LIB_PATH} is not built; run python c_pp-library_amd/build.py")
+    L = C.CDLL(LIB_PATH)
+    V = C.c_void_p
+    I = C.c_int
+    D = C.c_double
+    sig = {
+        "awcm_create_1d": [I, I, D, D, C.POINTER(V)],
+        "awcm_create_2d": [I, I, I, D, D, D, I, C.POINTER(V)],
+        "awcm_dims": [V, C.POINTER(I), C.POINTER(I)],
+        "awcm_upload_q": [V, _dp],
+        "awcm_download_q": [V, _dp],
+        "awcm_set_mask0": [V, _u8p],
+        "awcm_get_masks": [V, _u8p, _i32p],
+        "awcm_prepare_euler": [V, D],
+        "awcm_post_process_euler": [V],
+        "awcm_set_schedule": [V, _dp, I],
+        "awcm_run": [V, D, D, D, I, _dp, C.POINTER(I), C.POINTER(I), C.POINTER(C.c_longlong)],
+        "awcm_synchronize": [V],
+        "awcm_batch_create_1d": [I, C.POINTER(Instance1D), _dp, I, C.POINTER(V)],
+        "awcm_batch_upload_q": [V, I, _dp],
+        "awcm_batch_download_q": [V, I, _dp],
+        "awcm_batch_get_masks": [V, I, _u8p, _i32p],
+        "awcm_batch_set_launch": [V, I, I],
+        "awcm_batch_run": [V, I],
+        "awcm_batch_run_sequential": [V, I],
+        "awcm_batch_state": [V, I, _dp, C.POINTER(I), C.POINTER(I), C.POINTER(C.c_longlong), C.POINTER(I)],
+        "awcm_batch_synchronize": [V],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    L.awcm_destroy.argtypes = [V]
+    L.awcm_destroy.restype = None
+    L.awcm_batch_destroy.argtypes = [V]
+    L.awcm_batch_destroy.restype = None
+    _lib = L
+    return L
+
+
+def _check(code, what):
+    if code != AWCM_OK:
+        raise AwcmError(code, what)
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+class Solver:
+    """One AWCM solve on the GPU (dim 1 or 2), mirroring AWCM_1D / AWCM_2D."""
+
+    def __init__(self, dim, num_levels, nx_coarse, gamma, dx, ny_coarse=0, x0=0.0, bc_kind=BC_TRANSMISSIVE):
+        L = lib()
+        self.h = C.c_void_p()
+        if dim == 1:
+            _check(L.awcm_create_1d(num_levels, nx_coarse, gamma, dx, C.byref(self.h)), "awcm_create_1d")
+        else:
+            _check(L.awcm_create_2d(num_levels, nx_coarse, ny_coarse, gamma, dx, x0, bc_kind, C.byref(self.h)),
+                   "awcm_create_2d")
+        self.dim = dim
+        nx, ny = C.c_int(), C.c_int()
+        _check(L.awcm_dims(self.h, C.byref(nx), C.byref(ny)), "awcm_dims")
+        self.nx, self.ny = nx.value, ny.value
+        self.t = 0.0
+        self.cnt = 1
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            lib().awcm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def npts(self):
+        return (self.nx + 6) if self.dim == 1 else (self.nx + 6) * (self.ny + 6)
+
+    @property
+    def ncomp(self):
+        return 3 if self.dim == 1 else 4
+
+    def upload_q(self, q):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        assert q.size == self.npts * self.ncomp
+        _check(lib().awcm_upload_q(self.h, _d(q)), "awcm_upload_q")
+
+    def download_q(self):
+        q = np.empty(self.npts * self.ncomp)
+        _check(lib().awcm_download_q(self.h, _d(q)), "awcm_download_q")
+        return q
+
+    def set_mask0(self, m0):
+        m0 = np.ascontiguousarray(m0, dtype=np.uint8)
+        assert m0.size == self.npts
+        _check(lib().awcm_set_mask0(self.h, m0.ctypes.data_as(_u8p)), "awcm_set_mask0")
+
+    def masks(self):
+        m = np.empty(5 * self.npts, dtype=np.uint8)
+        lv = np.empty(4 * self.nx * self.ny, dtype=np.int32)
+        _check(lib().awcm_get_masks(self.h, m.ctypes.data_as(_u8p), lv.ctypes.data_as(_i32p)), "awcm_get_masks")
+        if self.dim == 1:
+            return m.reshape(5, self.nx + 6), lv.reshape(4, self.nx)
+        return m.reshape(5, self.ny + 6, self.nx + 6), lv.reshape(4, self.ny, self.nx)
+
+    def prepare(self, tol):
+        _check(lib().awcm_prepare_euler(self.h, tol), "awcm_prepare_euler")
+
+    def post_process(self):
+        _check(lib().awcm_post_process_euler(self.h), "awcm_post_process_euler")
+
+    def set_schedule(self, print_times):
+        pt = np.ascontiguousarray(print_times, dtype=np.float64)
+        _check(lib().awcm_set_schedule(self.h, _d(pt), len(pt)), "awcm_set_schedule")
+
+    def run(self, tol, cfl, t_final, max_steps):
+        """Advance up to max_steps steps (stops at t_final); returns (steps, updates)."""
+        t = np.array([self.t])
+        cnt = C.c_int(self.cnt)
+        steps = C.c_int(0)
+        upd = C.c_longlong(0)
+        _check(lib().awcm_run(self.h, tol, cfl, t_final, max_steps, _d(t), C.byref(cnt), C.byref(steps),
+                              C.byref(upd)), "awcm_run")
+        self.t = float(t[0])
+        self.cnt = cnt.value
+        return steps.value, upd.value
+
+    def synchronize(self):
+        _check(lib().awcm_synchronize(self.h), "awcm_synchronize")
+
+
+class Batch1D:
+    """Many independent 1-D solves in one persistent launch."""
+
+    def __init__(self, params, print_times):
+        arr = (Instance1D * len(params))()
+        for k, p in enumerate(params):
+            arr[k] = Instance1D(*p)
+        pt = np.ascontiguousarray(print_times, dtype=np.float64)
+        self.h = C.c_void_p()
+        _check(lib().awcm_batch_create_1d(len(params), arr, _d(pt), len(pt), C.byref(self.h)), "awcm_batch_create_1d")
+        self.n = len(params)
+        self.nx = [(p[1] - 1) * 2 ** (p[0] - 1) + 1 for p in params]
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            lib().awcm_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def upload_q(self, idx, q):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        assert q.size == (self.nx[idx] + 6) * 3
+        _check(lib().awcm_batch_upload_q(self.h, idx, _d(q)), "awcm_batch_upload_q")
+
+    def download_q(self, idx):
+        q = np.empty((self.nx[idx] + 6) * 3)
+        _check(lib().awcm_batch_download_q(self.h, idx, _d(q)), "awcm_batch_download_q")
+        return q
+
+    def masks(self, idx):
+        n = self.nx[idx]
+        m = np.empty(5 * (n + 6), dtype=np.uint8)
+        lv = np.empty(4 * n, dtype=np.int32)
+        _check(lib().awcm_batch_get_masks(self.h, idx, m.ctypes.data_as(_u8p), lv.ctypes.data_as(_i32p)),
+               "awcm_batch_get_masks")
+        return m.reshape(5, n + 6), lv.reshape(4, n)
+
+    def set_launch(self, threads, workgroups=0):
+        _check(lib().awcm_batch_set_launch(self.h, threads, workgroups), "awcm_batch_set_launch")
+
+    def run(self, max_steps=1 << 30):
+        _check(lib().awcm_batch_run(self.h, max_steps), "awcm_batch_run")
+
+    def run_sequential(self, max_steps=1 << 30):
+        _check(lib().awcm_batch_run_sequential(self.h, max_steps), "awcm_batch_run_sequential")
+
+    def state(self, idx):
+        t = np.zeros(1)
+        cnt, steps, status = C.c_int(), C.c_int(), C.c_int()
+        upd = C.c_longlong()
+        _check(lib().awcm_batch_state(self.h, idx, _d(t), C.byref(cnt), C.byref(steps), C.byref(upd),
+                                      C.byref(status)), "awcm_batch_state")
+        return dict(t=float(t[0]), cnt=cnt.value, steps=steps.value, updates=upd.value, status=status.value)

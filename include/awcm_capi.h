@@ -1,0 +1,148 @@
+/*
+ * awcm_capi.h -- C ABI of the MI355X (gfx950) AWCM compressible-Euler solver.
+ *
+ * The reference (doseongju/C_pp-library) is a header-only C++ library with no
+ * FFI; its "API" is the class surface its drivers call.  Each entry point
+ * below replaces one reference call (file:line cited on the entry point), or
+ * a whole driver loop, with a bulk operation on device-resident state.  The
+ * C++ header API in c_pp-library_amd/include/awcm/ (AWCM_1D, AWCM_2D, ...)
+ * is built on these calls.
+ *
+ * Conventions
+ *   - Plain C types only; no HIP or torch types cross this boundary.
+ *   - Every call returns an int status: AWCM_OK (0) or a negative AWCM_E*.
+ *   - State layout at the boundary is the reference's (FIELD_UNIFORM_1D.h:51-55,
+ *     FIELD_UNIFORM_2D.h:42-58, ARRAY_2D.h:112): AoS, ghost width 3,
+ *       1-D: q[(i+3)*3 + c],                i = -3 .. Nx+2
+ *       2-D: q[((i+3) + (Nx+6)*(j+3))*4 + c]
+ *     masks are uint8 with the same ghosted indexing, levels int32 without
+ *     ghosts (lvl[i] / lvl[i + Nx*j]).
+ *   - A handle owns one HIP stream; calls on one handle are ordered and not
+ *     thread-safe; different handles are independent.
+ *   - FP64 throughout; results are bit-identical to the reference's on the
+ *     same inputs (same masks, levels and fields).
+ */
+#ifndef AWCM_CAPI_H
+#define AWCM_CAPI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    AWCM_OK = 0,
+    AWCM_EINVAL = -1,     /* bad argument (sizes, levels, null pointers)        */
+    AWCM_EDEVICE = -2,    /* HIP runtime / kernel launch failure                */
+    AWCM_ENOMEM = -3,     /* device allocation failed                           */
+    AWCM_ENONFINITE = -4, /* state not finite / non-positive (sqrt of negative) */
+    AWCM_ERANGE = -5,     /* a mask scatter left the ghosted domain             */
+    AWCM_ESTATE = -6      /* call not valid in the handle's current state       */
+};
+
+/* Ghost-fill kinds (the per-problem BOUNDARY_MANAGER classes). */
+enum {
+    AWCM_BC_TRANSMISSIVE = 0, /* zero-order extrapolation: PROBLEM_SHOCKTUBE.h:86-107,
+                                 PROBLEM_ENTROPY_INTERACTING.h:81-102,
+                                 PROBLEM_QUADRANTS.h:123-203                          */
+    AWCM_BC_DOUBLE_MACH = 1   /* BOUNDARY_MANAGER_DMACH::SetBoundaryCondition(t),
+                                 PROBLEM_DOUBLEMACH.h:106-245                          */
+};
+
+typedef struct awcm_solver awcm_solver;
+typedef struct awcm_batch awcm_batch;
+
+/* ---- solver handles ------------------------------------------------------ */
+
+/* MRA_GRID_INT_1D::Initialize (MRA_GRID_UNIFORM_1D.h:70-98) + AWCM_1D(mra)
+ * (AWCM_1D.h:21-41).  Nx = (nx_coarse-1)*2^(num_levels-1)+1.  dx is the
+ * physical spacing the driver takes from its GRID_UNIFORM_1D.  Requires
+ * 2 <= num_levels <= 14 and nx_coarse >= 6. */
+int awcm_create_1d(int num_levels, int nx_coarse, double gamma, double dx, awcm_solver** out);
+
+/* MRA_GRID_INT_2D::Initialize (MRA_GRID_UNIFORM_2D.h:87-134) + AWCM_2D(mra)
+ * (AWCM_2D.h:24-48).  dx is GRID_UNIFORM_2D::dx (dy must equal it, as the
+ * drivers assume); x0 is the x of column 0 (used by the double-Mach ghost
+ * fill).  Requires 2 <= num_levels <= 14, nx_coarse, ny_coarse >= 5. */
+int awcm_create_2d(int num_levels, int nx_coarse, int ny_coarse, double gamma, double dx, double x0,
+                   int bc_kind, awcm_solver** out);
+
+void awcm_destroy(awcm_solver* h);
+
+/* Finest-grid sizes (ny = 1 for 1-D). */
+int awcm_dims(const awcm_solver* h, int* nx, int* ny);
+
+/* Host <-> device copies of the conserved field (reference AoS layout). */
+int awcm_upload_q(awcm_solver* h, const double* q);
+int awcm_download_q(awcm_solver* h, double* q);
+
+/* Replace mask0 (the previous step's active set, ghosted uint8). */
+int awcm_set_mask0(awcm_solver* h, const uint8_t* mask0);
+
+/* Read the five masks (mask_temp, mask0..mask3; each ghosted) and the four
+ * level fields (level0..level3) -- the public members AWCM_1D/2D expose
+ * (AWCM_1D.h:12-13, AWCM_2D.h:15-16). */
+int awcm_get_masks(awcm_solver* h, uint8_t* masks, int32_t* levels);
+
+/* AWCM_1D::PrepareEuler (AWCM_1D.h:86-120) / AWCM_2D::PrepareEuler
+ * (AWCM_2D.h:114-147): forward transform, threshold, mask refinement, inverse
+ * transform and interpolation of the new points, on the device. */
+int awcm_prepare_euler(awcm_solver* h, double tol);
+
+/* AWCM_*::PostProcessEuler (AWCM_1D.h:122-125, AWCM_2D.h:149-152). */
+int awcm_post_process_euler(awcm_solver* h);
+
+/* Output-time schedule (arr_printing_times of the drivers,
+ * main_1d_euler.cpp:43-48): dt is clipped so steps land on these times. */
+int awcm_set_schedule(awcm_solver* h, const double* print_times, int n_print);
+
+/* Time loop of Wavelet/main_1d_euler.cpp:61-260 (1-D, WENO-Z flux pair) or
+ * main_2d_euler(dmach|quad).cpp:56-350 (2-D): up to max_steps iterations of
+ * PrepareEuler, CFL dt (clipped to the schedule), three SSP-RK3 stages with
+ * ghost fills, and PostProcessEuler, stopping once t >= t_final.
+ * t and cnt (the print counter, 1 on the first call) are in/out; steps_done
+ * and updates (sum over steps of |mask0|) may be NULL. */
+int awcm_run(awcm_solver* h, double tol, double cfl, double t_final, int max_steps, double* t, int* cnt,
+             int* steps_done, long long* updates);
+
+/* Block until the handle's queued device work is finished. */
+int awcm_synchronize(awcm_solver* h);
+
+/* ---- batched 1-D form (no reference counterpart: replaces N sequential
+ *      main_1d_euler.cpp runs) ------------------------------------------- */
+
+typedef struct {
+    int num_levels;
+    int nx_coarse;
+    double gamma;
+    double dx;
+    double tol;
+    double cfl;
+    double t_final;
+} awcm_instance_1d;
+
+/* n independent 1-D solves sharing one output-time schedule. */
+int awcm_batch_create_1d(int n, const awcm_instance_1d* params, const double* print_times, int n_print,
+                         awcm_batch** out);
+void awcm_batch_destroy(awcm_batch* b);
+int awcm_batch_upload_q(awcm_batch* b, int idx, const double* q);
+int awcm_batch_download_q(awcm_batch* b, int idx, double* q);
+int awcm_batch_get_masks(awcm_batch* b, int idx, uint8_t* masks, int32_t* levels);
+/* Launch geometry: threads per workgroup (64..1024, multiple of 64) and
+ * workgroups per launch (0 = one per instance). */
+int awcm_batch_set_launch(awcm_batch* b, int threads, int workgroups);
+/* Run every instance to its t_final (or max_steps steps) in one persistent
+ * launch; instances are dispatched largest-first from a device work queue. */
+int awcm_batch_run(awcm_batch* b, int max_steps);
+/* The same solves, one launch per instance, in index order (the baseline the
+ * batched form is measured against). */
+int awcm_batch_run_sequential(awcm_batch* b, int max_steps);
+int awcm_batch_state(awcm_batch* b, int idx, double* t, int* cnt, int* steps, long long* updates, int* status);
+int awcm_batch_synchronize(awcm_batch* b);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AWCM_CAPI_H */

@@ -1,0 +1,206 @@
+"""GPU parity: the HIP solver (through the C ABI) against the CPU oracle.
+
+Bar: masks and level fields bit-exact, fields bit-exact (the FP64 operation
+order is the reference's and no FMA contraction is allowed), dt and t equal.
+The per-field relative-L1 figure is also asserted against 1e-12 so a failure
+message shows how far off a field is.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _pyawcm():
+    import pyawcm
+    pyawcm.lib()  # raises if the HIP library is not built
+    return pyawcm
+
+
+def rel_l1(a, b):
+    den = np.abs(b).sum()
+    return float(np.abs(a - b).sum() / (den if den > 0 else 1.0))
+
+
+def assert_same(qg, qo, mg, mo, lg, lo, what):
+    assert np.array_equal(mg, mo), f"{what}: masks differ at {np.argwhere(mg != mo)[:5].tolist()}"
+    assert np.array_equal(lg, lo), f"{what}: levels differ at {np.argwhere(lg != lo)[:5].tolist()}"
+    assert rel_l1(qg, qo) <= 1e-12, f"{what}: rel L1 {rel_l1(qg, qo)}"
+    assert np.array_equal(qg.view(np.uint64), qo.view(np.uint64)), \
+        f"{what}: fields not bit-identical (max abs diff {np.abs(qg - qo).max()})"
+
+
+def sod_setup(L=7, nxc=17):
+    nx = (nxc - 1) * 2 ** (L - 1) + 1
+    q, dx, _ = O.ic1(0, nx, 0.0, 1.0, 0.5, O.SOD_L, O.SOD_R)
+    return q, dx
+
+
+def test_1d_prepare_matches_oracle():
+    P = _pyawcm()
+    q, dx = sod_setup()
+    g = P.Solver(1, 7, 17, 1.4, dx)
+    o = O.Solver1D("orc", 7, 17, 1.4, dx)
+    g.upload_q(q)
+    o.set_q(q)
+    g.prepare(1e-4)
+    o.prepare(1e-4)
+    mg, lg = g.masks()
+    mo, lo = o.masks()
+    assert_same(g.download_q(), o.get_q(), mg, mo, lg, lo, "1-D prepare")
+
+
+def test_1d_sod_steps_match_oracle():
+    P = _pyawcm()
+    q, dx = sod_setup()
+    pts = O.print_times(0.2, 41)
+    g = P.Solver(1, 7, 17, 1.4, dx)
+    g.set_schedule(pts)
+    o = O.Solver1D("orc", 7, 17, 1.4, dx)
+    g.upload_q(q)
+    o.set_q(q)
+    for n in range(12):
+        o.step(1e-4, 0.5, pts)
+        g.run(1e-4, 0.5, 0.2, 1)
+        assert g.t == o.t[0] and g.cnt == o.cnt.value, f"step {n}: t {g.t} vs {o.t[0]}"
+        mg, lg = g.masks()
+        mo, lo = o.masks()
+        assert_same(g.download_q(), o.get_q(), mg, mo, lg, lo, f"1-D step {n}")
+
+
+def test_1d_sod_full_run_matches_oracle():
+    P = _pyawcm()
+    q, dx = sod_setup()
+    pts = O.print_times(0.2, 41)
+    g = P.Solver(1, 7, 17, 1.4, dx)
+    g.set_schedule(pts)
+    g.upload_q(q)
+    steps, upd = g.run(1e-4, 0.5, 0.2, 100000)
+    o = O.Solver1D("orc", 7, 17, 1.4, dx)
+    o.set_q(q)
+    n = 0
+    while o.t[0] < 0.2:
+        o.step(1e-4, 0.5, pts)
+        n += 1
+    assert steps == n and g.t == o.t[0] == 0.2
+    assert upd > 0
+    mg, lg = g.masks()
+    mo, lo = o.masks()
+    assert_same(g.download_q(), o.get_q(), mg, mo, lg, lo, "1-D Sod t=0.2")
+
+
+def quad_setup(L, nxc):
+    nx = (nxc - 1) * 2 ** (L - 1) + 1
+    q, dx, _ = O.ic2(0, nx, nx, 0.0, 1.0, 0.0, 1.0, 0.5)
+    return q, dx
+
+
+def dmr_setup(L, nxc, nyc, rho_pre=1.4):
+    nx = (nxc - 1) * 2 ** (L - 1) + 1
+    ny = (nyc - 1) * 2 ** (L - 1) + 1
+    q, dx, _ = O.ic2(1, nx, ny, 0.0, 4.0, 0.0, 1.0, rho_pre)
+    return q, dx
+
+
+@pytest.mark.parametrize("case", ["quad", "dmr"])
+def test_2d_prepare_matches_oracle(case):
+    P = _pyawcm()
+    if case == "quad":
+        L, nxc, nyc, bc = 5, 9, 9, 0
+        q, dx = quad_setup(L, nxc)
+        xl, xu = 0.0, 1.0
+    else:
+        L, nxc, nyc, bc = 4, 33, 9, 1
+        q, dx = dmr_setup(L, nxc, nyc)
+        xl, xu = 0.0, 4.0
+    g = P.Solver(2, L, nxc, 1.4, dx, ny_coarse=nyc, x0=xl, bc_kind=bc)
+    o = O.Solver2D("orc", L, nxc, nyc, 1.4, bc, xl, xu, 0.0, 1.0)
+    g.upload_q(q)
+    o.set_q(q)
+    g.prepare(1e-3)
+    o.prepare(1e-3)
+    mg, lg = g.masks()
+    mo, lo = o.masks()
+    assert_same(g.download_q(), o.get_q(), mg, mo, lg, lo, f"2-D {case} prepare")
+
+
+@pytest.mark.parametrize("case", ["quad", "dmr"])
+def test_2d_steps_match_oracle(case):
+    P = _pyawcm()
+    if case == "quad":
+        L, nxc, nyc, bc, tf, npr = 5, 9, 9, 0, 0.25, 9
+        q, dx = quad_setup(L, nxc)
+        xl, xu = 0.0, 1.0
+    else:
+        L, nxc, nyc, bc, tf, npr = 4, 33, 9, 1, 0.2, 21
+        q, dx = dmr_setup(L, nxc, nyc)
+        xl, xu = 0.0, 4.0
+    pts = O.print_times(tf, npr)
+    g = P.Solver(2, L, nxc, 1.4, dx, ny_coarse=nyc, x0=xl, bc_kind=bc)
+    g.set_schedule(pts)
+    o = O.Solver2D("orc", L, nxc, nyc, 1.4, bc, xl, xu, 0.0, 1.0)
+    g.upload_q(q)
+    o.set_q(q)
+    for n in range(15):
+        o.step(1e-3, 0.4, pts)
+        g.run(1e-3, 0.4, tf, 1)
+        assert g.t == o.t[0] and g.cnt == o.cnt.value, f"step {n}: t {g.t} vs {o.t[0]}"
+        mg, lg = g.masks()
+        mo, lo = o.masks()
+        assert_same(g.download_q(), o.get_q(), mg, mo, lg, lo, f"2-D {case} step {n}")
+
+
+def test_2d_multistep_run_matches_oracle():
+    """Several steps queued in one awcm_run call (no host round trip per step)."""
+    P = _pyawcm()
+    L, nxc, nyc = 4, 33, 9
+    q, dx = dmr_setup(L, nxc, nyc)
+    pts = O.print_times(0.2, 21)
+    g = P.Solver(2, L, nxc, 1.4, dx, ny_coarse=nyc, x0=0.0, bc_kind=1)
+    g.set_schedule(pts)
+    g.upload_q(q)
+    steps, _ = g.run(1e-3, 0.4, 0.2, 40)
+    o = O.Solver2D("orc", L, nxc, nyc, 1.4, 1, 0.0, 4.0, 0.0, 1.0)
+    o.set_q(q)
+    for _ in range(steps):
+        o.step(1e-3, 0.4, pts)
+    assert steps == 40 and g.t == o.t[0]
+    mg, lg = g.masks()
+    mo, lo = o.masks()
+    assert_same(g.download_q(), o.get_q(), mg, mo, lg, lo, "2-D DMR 40 steps")
+
+
+def test_batch_matches_oracle():
+    P = _pyawcm()
+    rng = np.random.default_rng(7)
+    params, qs = [], []
+    for k in range(6):
+        L = 6 + k % 3
+        nx = 16 * 2 ** (L - 1) + 1
+        rl, rr = rng.uniform(0.1, 1.0, 2)
+        pl, pr = rng.uniform(0.05, 1.0, 2)
+        u = rng.uniform(-0.5, 0.5)
+        x0 = rng.uniform(0.3, 0.7)
+        q, dx, _ = O.ic1(0, nx, 0.0, 1.0, x0, (rl, u, pl), (rr, u, pr))
+        params.append((L, 17, 1.4, dx, 1e-3 if k % 2 else 1e-4, 0.5, 0.02))
+        qs.append((q, dx))
+    pts = O.print_times(0.02, 5)
+    b = P.Batch1D(params, pts)
+    for k, (q, _) in enumerate(qs):
+        b.upload_q(k, q)
+    b.run()
+    for k, (q, dx) in enumerate(qs):
+        L, nxc, g, dx_, tol, cfl, tf = params[k]
+        o = O.Solver1D("orc", L, nxc, g, dx_)
+        o.set_q(q)
+        n = 0
+        while o.t[0] < tf:
+            o.step(tol, cfl, pts)
+            n += 1
+        st = b.state(k)
+        assert st["status"] == 0 and st["steps"] == n and st["t"] == o.t[0]
+        mg, lg = b.masks(k)
+        mo, lo = o.masks()
+        assert_same(b.download_q(k), o.get_q(), mg, mo, lg, lo, f"batch instance {k}")
