@@ -1,0 +1,199 @@
+#!/usr/bin/env python3
+"""Headline benchmark: 2-D double Mach reflection AWCM forward pass on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): Mach-10 shock
+at 60 degrees on [0,4]x[0,1], finest grid 4097 x 1025 (7 levels, 65 x 17
+coarse), pre-shock state (1.4, 0, 0, 1), eps = 1e-3, WENO-Z5 + Roe, SSP-RK3 at
+CFL 0.4, output schedule of 21 times to t = 0.2, FP64.
+
+A step is one full AWCM time step (PrepareEuler + CFL dt + three SSP-RK3
+stages + post-processing) over the device-resident state.  The solution is
+first advanced to --advance (untimed) so the measured steps see a developed
+flow, then --warmup steps run untimed and --steps steps are timed with HIP
+events on the solver's stream (no host synchronisation inside the window).
+
+value = active-point RK3 updates per second = sum over the timed steps of
+|mask0| / device time.  The CPU baseline resumes the reference build (or the
+oracle port) from the exact state the timed window started from and times a
+bounded number of the same steps on the host cores.
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "c_pp-library_amd"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+METRIC = "active-point RK3 updates/s & %HBM peak; batched vs plain; adjoint/fwd cost"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+# DMR configuration
+LEVELS, NXC, NYC = 7, 65, 17
+RES = (NYC - 1) * 2 ** (LEVELS - 1) + 1  # 1025
+TOL, CFL, T_FINAL, NUM_PRINT, RHO_PRE = 1e-3, 0.4, 0.2, 21, 1.4
+
+# Algorithmic cost of the dominant kernel (finest-level WENO-Z/Roe stage update).
+# FP64 operations of one face flux as the reference writes it (+ - * / sqrt,
+# WENOZ.h:236-320 with ROESOLVER.h / EULER_EQUATION.h): 1202.  A finest-level
+# point needs one new x-face and one new y-face (faces are shared by
+# neighbours), so 2 x 1202 flops, plus the 12-op difference/update.
+FLOPS_PER_WENO_POINT = 2 * 1202 + 12
+# Bytes: read the point's state (32 B), write the stage output (32 B), read the
+# RK base for stage 2 (32 B, one stage in three), the 4 B list entry; stencil
+# neighbours are the neighbours' own reads (dense limit).
+BYTES_PER_WENO_POINT = 32 + 32 + 32.0 / 3 + 4
+
+
+def ensure_built():
+    import pyawcm
+    if not os.path.exists(pyawcm.LIB_PATH):
+        subprocess.run([sys.executable, os.path.join(ROOT, "c_pp-library_amd", "build.py"), "-q"], check=True)
+    return pyawcm
+
+
+def cpu_baseline(q, m0, t, cnt, budget_s):
+    """Time the same steps on the host from the GPU state (reference build if present)."""
+    try:
+        import oracle_lib as O
+    except Exception as e:  # pragma: no cover
+        return {"error": f"oracle unavailable: {e}"}
+    kind, prefix = ("reference", "ref") if O.have_ref() else ("port", "orc")
+    try:
+        s = O.Solver2D(prefix, LEVELS, NXC, NYC, 1.4, 1, 0.0, 4.0, 0.0, 1.0)
+    except FileNotFoundError as e:
+        return {"error": str(e)}
+    s.set_q(q)
+    s.set_mask0(m0)
+    s.t[0] = t
+    s.cnt.value = cnt
+    pts = O.print_times(T_FINAL, NUM_PRINT)
+    upd, elapsed, steps = 0, 0.0, 0
+    while elapsed < budget_s and steps < 50:
+        t0 = time.perf_counter()
+        s.step(TOL, CFL, pts)
+        elapsed += time.perf_counter() - t0
+        steps += 1
+        m, _ = s.masks()
+        upd += int(m[1, 3:-3, 3:-3].sum())
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": upd / elapsed, "unit": "updates/s", "cores": threads, "kind": kind,
+            "sample": f"{steps} DMR steps resumed from the GPU state at t={t:.6f} "
+                      f"({elapsed:.1f} s wall, OpenMP {threads} threads, {prefix} build)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--advance", type=float, default=0.02, help="untimed advance to this time first")
+    ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU-baseline work")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (this build runs every solve on a single MI355X); "
+              f"--gpus {args.gpus} is not supported", file=sys.stderr)
+        sys.exit(2)
+
+    P = ensure_built()
+    q0, dx = P.ic_double_mach_2d(RES, RHO_PRE)
+    S = P.Solver(2, LEVELS, NXC, 1.4, dx, ny_coarse=NYC, x0=0.0, bc_kind=P.BC_DOUBLE_MACH)
+    S.set_schedule(P.print_times(T_FINAL, NUM_PRINT))
+    S.upload_q(q0)
+    t_adv0 = time.perf_counter()
+    adv_steps = 0
+    while S.t < args.advance:
+        n, _ = S.run(TOL, CFL, args.advance, 256)
+        adv_steps += n
+        if n == 0:
+            break
+    adv_s = time.perf_counter() - t_adv0
+    S.run(TOL, CFL, T_FINAL, args.warmup)
+    S.synchronize()
+    # state at the start of the timed window: replayed for the per-kernel timing
+    # pass and handed to the CPU baseline
+    t_start, cnt_start = S.t, S.cnt
+    q_start = S.download_q()
+    m_start = S.masks()[0][1].copy()
+
+    # timed window: whole-step HIP graph replays, no host synchronisation
+    steps, updates, ms = S.run_timed(TOL, CFL, T_FINAL, args.steps)
+    t_end = S.t
+
+    # per-kernel HIP-event timing over the same window (state restored bit-exactly)
+    S.upload_q(q_start)
+    S.set_mask0(m_start)
+    S.t, S.cnt = t_start, cnt_start
+    S.profile(True)
+    steps_p, updates_p, ms_p = S.run_timed(TOL, CFL, T_FINAL, args.steps)
+    w_ms, w_launch, w_pts = S.profile_read(P.KERNEL_STAGE_WENO)
+    c_ms, c_launch, c_pts = S.profile_read(P.KERNEL_STAGE_COARSE)
+    p_ms, p_launch, _ = S.profile_read(P.KERNEL_PREPARE)
+    S.profile(False)
+    assert steps_p == steps and updates_p == updates and S.t == t_end, "profiled replay diverged"
+
+    value = updates / (ms / 1e3) if ms > 0 else 0.0
+    # dominant kernel: finest-level WENO-Z/Roe stage update
+    avg_launch_s = (w_ms / w_launch) / 1e3 if w_launch else float("nan")
+    pts_per_launch = w_pts / w_launch if w_launch else 0.0
+    flops = pts_per_launch * FLOPS_PER_WENO_POINT
+    achieved_tflops = flops / avg_launch_s / 1e12 if w_launch else 0.0
+    achieved_gbs = pts_per_launch * BYTES_PER_WENO_POINT / avg_launch_s / 1e9 if w_launch else 0.0
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "updates/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms / max(steps, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (double Mach reflection initial condition, advanced on the GPU)",
+        "config": {"workload": "2-D double Mach reflection AWCM, 4097x1025 finest (7 levels, 65x17 coarse), "
+                               "eps=1e-3, WENO-Z5+Roe, SSP-RK3 CFL 0.4, FP64",
+                   "t_start": t_start, "t_end": t_end, "eager_ms_per_step": ms_p / max(steps, 1), "advance_steps": adv_steps,
+                   "advance_wall_s": round(adv_s, 3), "updates": updates,
+                   "mean_active_points": updates / max(steps, 1)},
+        "roofline": {
+            "bound": "fp64-valu",
+            "kernel": "k2_stage_weno (finest-level WENO-Z/Roe stage update)",
+            "achieved": achieved_tflops,
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved_tflops / FP64_PEAK_TFLOPS,
+            "traffic": None,
+            "hbm_achieved_gbs": achieved_gbs,
+            "hbm_frac": achieved_gbs / HBM_PEAK_GBS,
+            "avg_launch_ms": avg_launch_s * 1e3,
+            "points_per_launch": pts_per_launch,
+            "flops_per_point": FLOPS_PER_WENO_POINT,
+        },
+        "breakdown_ms_per_step": {
+            "stage_weno": w_ms / max(steps, 1),
+            "stage_coarse": c_ms / max(steps, 1),
+            "prepare_euler": p_ms / max(steps, 1),
+        },
+    }
+    if not args.no_cpu:
+        cb = cpu_baseline(q_start, m_start, t_start, cnt_start, args.cpu_budget)
+        out["cpu_baseline"] = cb
+        if "value" in cb and cb["value"] > 0:
+            out["gpu_over_cpu"] = value / cb["value"]
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

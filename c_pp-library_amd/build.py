@@ -15,7 +15,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SRC = ["csrc/awcm1d.hip", "csrc/runtime1d.hip", "csrc/awcm2d.hip", "csrc/capi.hip"]
+SRC = ["csrc/awcm1d.hip", "csrc/runtime1d.hip", "csrc/awcm2d.hip", "csrc/capi.hip", "csrc/problems.cpp"]
 OUT = os.path.join(HERE, "libawcm_hip.so")
 ARCH = os.environ.get("AWCM_OFFLOAD_ARCH", "gfx950")
 
@@ -36,7 +36,13 @@ def build(verbose=False):
            "-o", OUT + ".tmp"] + [os.path.join(HERE, s) for s in SRC]
     if verbose:
         print(" ".join(cmd), flush=True)
-    subprocess.run(cmd, check=True, cwd=HERE)
+    r = subprocess.run(cmd, cwd=HERE, capture_output=True, text=True)
+    # the host pass does not know the packed-FP32 feature flag; that note is harmless
+    err = "\n".join(l for l in r.stderr.splitlines() if "packed-fp32-ops" not in l)
+    if err.strip():
+        print(err, file=sys.stderr)
+    if r.returncode != 0:
+        raise subprocess.CalledProcessError(r.returncode, cmd)
     os.replace(OUT + ".tmp", OUT)
     return OUT
 

@@ -30,6 +30,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -714,6 +715,8 @@ __global__ void k2_combine(Dims2 D, const uint8_t* __restrict__ m0, double* __re
         if (sc->do_print) sc->cnt = sc->cnt + 1;
         sc->steps += 1;
         sc->sum_active0 += (long long)sc->cntW[0] + sc->cntC[0];
+        sc->sumW += (long long)sc->cntW[0] + sc->cntW[1] + sc->cntW[2];
+        sc->sumC += (long long)sc->cntC[0] + sc->cntC[1] + sc->cntC[2];
     }
     if (i >= D.Nx) return;
     const long long p = gidx(D, i, j);
@@ -762,6 +765,9 @@ Lat mklat(int is, int js, int ie, int je, int ires, int jres) {
 
 Solver2D::~Solver2D() {
     if (stream) hipStreamSynchronize(stream);
+    if (gexec) hipGraphExecDestroy(gexec);
+    if (graph) hipGraphDestroy(graph);
+    for (hipEvent_t e : ev_pool) hipEventDestroy(e);
     for (void* p : allocs) hipFree(p);
     if (stream) hipStreamDestroy(stream);
 }
@@ -778,12 +784,13 @@ int Solver2D::init(int L, int nxc_, int nyc_, double g, double dx_, double x0_, 
     if (L < 2 || L > 14 || nxc_ < 5 || nyc_ < 5) return AWCM_EINVAL;
     if (!(dx_ > 0) || !(g > 1)) return AWCM_EINVAL;
     if (bck != AWCM_BC_TRANSMISSIVE && bck != AWCM_BC_DOUBLE_MACH) return AWCM_EINVAL;
+    if ((long long)(nxc_ - 1) * (1 << (L - 1)) + 1 > 65535 || (long long)(nyc_ - 1) * (1 << (L - 1)) + 1 > 65535)
+        return AWCM_EINVAL;
     nxc = nxc_; nyc = nyc_; gamma = g; dx = dx_; x0 = x0_; bc_kind = bck;
     D.L = L;
     D.maxlev = L - 1;
     D.Nx = (nxc - 1) * (1 << (L - 1)) + 1;
     D.Ny = (nyc - 1) * (1 << (L - 1)) + 1;
-    if (D.Nx > 65535 || D.Ny > 65535) return AWCM_EINVAL;
     D.P = D.Nx + 2 * G;
     D.plane = (long long)D.P * (D.Ny + 2 * G);
     // MRA_GRID_INT_2D::Initialize (MRA_GRID_UNIFORM_2D.h:87-134)
@@ -825,6 +832,7 @@ int Solver2D::init(int L, int nxc_, int nyc_, double g, double dx_, double x0_, 
     hipGetDevice(&dev);
     hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
     grid_blocks = ncu * 8;
+    if (const char* e = getenv("AWCM_NO_GRAPH")) use_graph = e[0] == '0';
     return hipDeviceSynchronize() == hipSuccess ? AWCM_OK : AWCM_EDEVICE;
 }
 
@@ -912,6 +920,22 @@ int Solver2D::get_state(double* t, int* cnt, int* steps, long long* updates, int
         if (hipGetLastError() != hipSuccess) return AWCM_EDEVICE;    \
     } while (0)
 
+hipEvent_t Solver2D::mark() {
+    if (ev_used == ev_pool.size()) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        ev_pool.push_back(e);
+    }
+    hipEvent_t e = ev_pool[ev_used++];
+    if (hipEventRecord(e, stream) != hipSuccess) return nullptr;
+    return e;
+}
+
+void Solver2D::prof_clear() {
+    for (auto& v : prof_ev) v.clear();
+    ev_used = 0;
+}
+
 int Solver2D::enqueue_begin(bool force) {
     AWCM_LAUNCH(k2_begin, dim3(nblk(3 * (D.Nx + D.Ny), 256)), dim3(256), 0, stream, D, sc, alpha, force ? 1 : 0);
     return AWCM_OK;
@@ -984,30 +1008,63 @@ int Solver2D::enqueue_prepare() {
 
 int Solver2D::enqueue_step() {
     if (int st = enqueue_begin(false)) return st;
+    hipEvent_t e0 = prof ? mark() : nullptr;
     if (int st = enqueue_prepare()) return st;
     const long long nl = (long long)D.Nx * D.Ny;
     AWCM_LAUNCH(k2_compact, dim3(nblk(nl, kCompactThreads * kCompactPer)), dim3(kCompactThreads), 0, stream, D, m[0], m[1],
                 m[2], lv[0], lv[1], lv[2], lists, sc);
+    if (prof) prof_ev[2].push_back({e0, mark()});
     const dim3 ag(nblk(D.Nx, 64), nblk(D.Ny, kAlphaRows)), ab(64, 4);
     const unsigned nbc = nblk(2 * G * (D.Nx + D.Ny), 256);
     const unsigned gw = grid_blocks * 2, gc = grid_blocks;
+    hipEvent_t pe = nullptr;
     // stage 1: alpha over mask3 of Q (+ CFL), ghosts of Q, mask2/level2 -> Q2
     AWCM_LAUNCH(k2_alpha, ag, ab, 0, stream, D, m[3], Q, ax(0), ay(0), gamma, 1, sc);
     AWCM_LAUNCH(k2_bc, dim3(nbc), dim3(256), 0, stream, D, Q, bc_kind, x0, dx, 1, pt, sc);
+    if (prof) pe = mark();
     AWCM_LAUNCH(k2_stage_weno<1>, dim3(gw), dim3(128), 0, stream, D, list(2, 1), 2, Q, Q, Q2, ax(0), ay(0), dx, sc);
+    if (prof) prof_ev[0].push_back({pe, mark()});
+    if (prof) pe = mark();
     AWCM_LAUNCH(k2_stage_coarse<1>, dim3(gc), dim3(256), 0, stream, D, list(2, 0), 2, lv[2], Q, Q, Q2, gamma, dx, sc);
+    if (prof) prof_ev[1].push_back({pe, mark()});
     // stage 2: ghosts of Q2, alpha over mask2 of Q2, mask1/level1 -> Q1
     AWCM_LAUNCH(k2_bc, dim3(nbc), dim3(256), 0, stream, D, Q2, bc_kind, x0, dx, 0, pt, sc);
     AWCM_LAUNCH(k2_alpha, ag, ab, 0, stream, D, m[2], Q2, ax(1), ay(1), gamma, 0, sc);
+    if (prof) pe = mark();
     AWCM_LAUNCH(k2_stage_weno<2>, dim3(gw), dim3(128), 0, stream, D, list(1, 1), 1, Q, Q2, Q1, ax(1), ay(1), dx, sc);
+    if (prof) prof_ev[0].push_back({pe, mark()});
+    if (prof) pe = mark();
     AWCM_LAUNCH(k2_stage_coarse<2>, dim3(gc), dim3(256), 0, stream, D, list(1, 0), 1, lv[1], Q, Q2, Q1, gamma, dx, sc);
+    if (prof) prof_ev[1].push_back({pe, mark()});
     // stage 3: ghosts of Q1, alpha over mask1 of Q1, mask0/level0 -> Q0
     AWCM_LAUNCH(k2_bc, dim3(nbc), dim3(256), 0, stream, D, Q1, bc_kind, x0, dx, 0, pt, sc);
     AWCM_LAUNCH(k2_alpha, ag, ab, 0, stream, D, m[1], Q1, ax(2), ay(2), gamma, 0, sc);
+    if (prof) pe = mark();
     AWCM_LAUNCH(k2_stage_weno<3>, dim3(gw), dim3(128), 0, stream, D, list(0, 1), 0, Q1, Q1, Q0, ax(2), ay(2), dx, sc);
+    if (prof) prof_ev[0].push_back({pe, mark()});
+    if (prof) pe = mark();
     AWCM_LAUNCH(k2_stage_coarse<3>, dim3(gc), dim3(256), 0, stream, D, list(0, 0), 0, lv[0], Q1, Q1, Q0, gamma, dx, sc);
+    if (prof) prof_ev[1].push_back({pe, mark()});
     AWCM_LAUNCH(k2_combine, dim3(nblk(D.Nx, 256), D.Ny), dim3(256), 0, stream, D, m[0], Q, Q0, 1, sc);
     return AWCM_OK;
+}
+
+int Solver2D::launch_step() {
+    if (!use_graph || prof) return enqueue_step();
+    if (!gexec || graph_tol != tol_host) {
+        if (gexec) { hipGraphExecDestroy(gexec); gexec = nullptr; }
+        if (graph) { hipGraphDestroy(graph); graph = nullptr; }
+        if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return AWCM_EDEVICE;
+        const int st = enqueue_step();
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(stream, &g);
+        if (st) { if (g) hipGraphDestroy(g); return st; }
+        if (e != hipSuccess || !g) return AWCM_EDEVICE;
+        graph = g;
+        if (hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0) != hipSuccess) { gexec = nullptr; return AWCM_EDEVICE; }
+        graph_tol = tol_host;
+    }
+    return hipGraphLaunch(gexec, stream) == hipSuccess ? AWCM_OK : AWCM_EDEVICE;
 }
 
 int Solver2D::enqueue_post() {

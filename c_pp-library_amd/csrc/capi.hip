@@ -179,7 +179,7 @@ int awcm_run(awcm_solver* h, double tol, double cfl, double t_final, int max_ste
     int done = 0;
     while (!st && done < max_steps) {
         const int k = std::min(chunk, max_steps - done);
-        for (int s = 0; s < k && !st; s++) st = S.enqueue_step();
+        for (int s = 0; s < k && !st; s++) st = S.launch_step();
         done += k;
         double tt = 0;
         int flag = 0;
@@ -195,6 +195,87 @@ int awcm_run(awcm_solver* h, double tol, double cfl, double t_final, int max_ste
         if (updates) *updates = upd;
     }
     return st;
+}
+
+int awcm_run_timed(awcm_solver* h, double tol, double cfl, double t_final, int n_steps, double* t, int* cnt,
+                   int* steps_done, long long* updates, double* elapsed_ms) {
+    if (!h || !t || !cnt || n_steps < 0 || !(cfl > 0)) return AWCM_EINVAL;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return AWCM_EDEVICE;
+    int st = AWCM_OK;
+    float ms = 0;
+    if (h->dim == 1) {
+        Batch1D& B = h->b1;
+        awcm::Inst1D& in = B.h[0];
+        in.tol = tol; in.cfl = cfl; in.t_final = t_final; in.t = *t; in.cnt = *cnt;
+        in.steps = 0; in.sum_active0 = 0; in.status = 0;
+        st = B.push();
+        if (!st && hipStreamSynchronize(B.stream) != hipSuccess) st = AWCM_EDEVICE;
+        if (!st && hipEventRecord(e0, B.stream) != hipSuccess) st = AWCM_EDEVICE;
+        if (!st) st = B.launch(1, n_steps, 0, 1);
+        if (!st && hipEventRecord(e1, B.stream) != hipSuccess) st = AWCM_EDEVICE;
+        if (!st) st = B.pull();
+        if (!st && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) st = AWCM_EDEVICE;
+        if (!st) {
+            *t = in.t; *cnt = in.cnt;
+            if (steps_done) *steps_done = in.steps;
+            if (updates) *updates = in.sum_active0;
+            st = B.check_status();
+        }
+    } else {
+        Solver2D& S = *h->s2;
+        S.tol_host = tol;
+        st = S.set_state(*t, *cnt, t_final, tol, cfl);
+        if (!st && hipEventRecord(e0, S.stream) != hipSuccess) st = AWCM_EDEVICE;
+        for (int s = 0; s < n_steps && !st; s++) st = S.launch_step();
+        if (!st && hipEventRecord(e1, S.stream) != hipSuccess) st = AWCM_EDEVICE;
+        int nsteps = 0, flag = 0;
+        long long upd = 0;
+        if (!st) st = S.get_state(t, cnt, &nsteps, &upd, &flag);
+        if (!st && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) st = AWCM_EDEVICE;
+        if (!st) {
+            if (steps_done) *steps_done = nsteps;
+            if (updates) *updates = upd;
+            st = flag_status(flag);
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (elapsed_ms) *elapsed_ms = ms;
+    return st;
+}
+
+int awcm_profile_enable(awcm_solver* h, int enable) {
+    if (!h) return AWCM_EINVAL;
+    if (h->dim != 2) return AWCM_ESTATE;
+    Solver2D& S = *h->s2;
+    if (hipStreamSynchronize(S.stream) != hipSuccess) return AWCM_EDEVICE;
+    S.prof = enable != 0;
+    S.prof_clear();
+    awcm::Scal2 sc;
+    if (hipMemcpy(&sc, S.sc, sizeof(sc), hipMemcpyDeviceToHost) != hipSuccess) return AWCM_EDEVICE;
+    sc.sumW = 0;
+    sc.sumC = 0;
+    return hipMemcpy(S.sc, &sc, sizeof(sc), hipMemcpyHostToDevice) == hipSuccess ? AWCM_OK : AWCM_EDEVICE;
+}
+
+int awcm_profile_read(awcm_solver* h, int kernel, double* total_ms, long long* launches, long long* points) {
+    if (!h || kernel < 0 || kernel > 2) return AWCM_EINVAL;
+    if (h->dim != 2) return AWCM_ESTATE;
+    Solver2D& S = *h->s2;
+    if (hipStreamSynchronize(S.stream) != hipSuccess) return AWCM_EDEVICE;
+    double tot = 0;
+    for (auto& pr : S.prof_ev[kernel]) {
+        float ms = 0;
+        if (!pr.first || !pr.second || hipEventElapsedTime(&ms, pr.first, pr.second) != hipSuccess) return AWCM_EDEVICE;
+        tot += ms;
+    }
+    awcm::Scal2 sc;
+    if (hipMemcpy(&sc, S.sc, sizeof(sc), hipMemcpyDeviceToHost) != hipSuccess) return AWCM_EDEVICE;
+    if (total_ms) *total_ms = tot;
+    if (launches) *launches = (long long)S.prof_ev[kernel].size();
+    if (points) *points = kernel == 0 ? sc.sumW : kernel == 1 ? sc.sumC : 0;
+    return AWCM_OK;
 }
 
 int awcm_synchronize(awcm_solver* h) {

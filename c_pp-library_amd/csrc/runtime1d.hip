@@ -34,13 +34,17 @@ int Batch1D::init(int n_, const awcm_instance_1d* params, const double* pt, int 
     n = n_;
     if (n <= 0 || !params) return AWCM_EINVAL;
     if (npt < 0 || (npt > 0 && !pt)) return AWCM_EINVAL;
+    for (int k = 0; k < n; k++) {
+        const awcm_instance_1d& p = params[k];
+        if (p.num_levels < 2 || p.num_levels > 14 || p.nx_coarse < 6) return AWCM_EINVAL;
+        if (!(p.dx > 0) || !(p.gamma > 1) || !(p.cfl > 0)) return AWCM_EINVAL;
+        if ((long long)(p.nx_coarse - 1) * (1 << (p.num_levels - 1)) + 1 > 65535) return AWCM_EINVAL;
+    }
     if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) return AWCM_EDEVICE;
     h.resize(n);
     max_nx = 0;
     for (int k = 0; k < n; k++) {
         const awcm_instance_1d& p = params[k];
-        if (p.num_levels < 2 || p.num_levels > 14 || p.nx_coarse < 6) return AWCM_EINVAL;
-        if (!(p.dx > 0) || !(p.gamma > 1) || !(p.cfl > 0)) return AWCM_EINVAL;
         Inst1D& in = h[k];
         std::memset(&in, 0, sizeof(in));
         in.L = p.num_levels;

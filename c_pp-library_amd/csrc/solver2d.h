@@ -30,6 +30,7 @@ struct Scal2 {
     long long sum_active0;
     int cnt, n_print, steps, skip, do_print, flag;
     int cntW[3], cntC[3];     // stage work-list lengths (mask0, mask1, mask2)
+    long long sumW, sumC;     // stage points processed, summed over steps
 };
 
 struct Solver2D {
@@ -52,6 +53,21 @@ struct Solver2D {
     Scal2* sc = nullptr;
     int grid_blocks = 0;         // persistent grid for list kernels
     int weno_threads = 128;
+
+    // whole-step HIP graph (captured once per threshold value, then replayed)
+    bool use_graph = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    double graph_tol = -1;
+    int launch_step();  // one step: graph replay, or eager when profiling
+
+    // per-kernel HIP-event profiling
+    bool prof = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_ev[3];
+    hipEvent_t mark();
+    void prof_clear();
 
     ~Solver2D();
     int init(int L, int nxc, int nyc, double gamma, double dx, double x0, int bc_kind);

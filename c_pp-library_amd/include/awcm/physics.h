@@ -16,6 +16,7 @@
 #include <cmath>
 
 #if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
 #define AWCM_HD __host__ __device__ __forceinline__
 #else
 #define AWCM_HD inline

@@ -67,6 +67,13 @@ def lib():
         "awcm_batch_run_sequential": [V, I],
         "awcm_batch_state": [V, I, _dp, C.POINTER(I), C.POINTER(I), C.POINTER(C.c_longlong), C.POINTER(I)],
         "awcm_batch_synchronize": [V],
+        "awcm_run_timed": [V, D, D, D, I, _dp, C.POINTER(I), C.POINTER(I), C.POINTER(C.c_longlong), _dp],
+        "awcm_profile_enable": [V, I],
+        "awcm_profile_read": [V, I, _dp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)],
+        "awcm_ic_riemann_1d": [I, D, D, D, _dp, _dp, D, _dp, _dp, _dp],
+        "awcm_ic_shu_osher_1d": [I, D, D, D, _dp, D, D, _dp, _dp, _dp],
+        "awcm_ic_quadrants_2d": [I, D, _dp, D, _dp, _dp],
+        "awcm_ic_double_mach_2d": [I, D, D, _dp, _dp],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -168,8 +175,87 @@ class Solver:
         self.cnt = cnt.value
         return steps.value, upd.value
 
+    def run_timed(self, tol, cfl, t_final, n_steps):
+        """Queue n_steps with no host sync; returns (steps, updates, device ms)."""
+        t = np.array([self.t])
+        cnt = C.c_int(self.cnt)
+        steps = C.c_int(0)
+        upd = C.c_longlong(0)
+        ms = np.zeros(1)
+        _check(lib().awcm_run_timed(self.h, tol, cfl, t_final, n_steps, _d(t), C.byref(cnt), C.byref(steps),
+                                    C.byref(upd), _d(ms)), "awcm_run_timed")
+        self.t = float(t[0])
+        self.cnt = cnt.value
+        return steps.value, upd.value, float(ms[0])
+
+    def profile(self, enable=True):
+        _check(lib().awcm_profile_enable(self.h, 1 if enable else 0), "awcm_profile_enable")
+
+    def profile_read(self, kernel):
+        """(device ms summed over launches, launches, points) of one kernel class."""
+        ms = np.zeros(1)
+        n, pts = C.c_longlong(), C.c_longlong()
+        _check(lib().awcm_profile_read(self.h, kernel, _d(ms), C.byref(n), C.byref(pts)), "awcm_profile_read")
+        return float(ms[0]), n.value, pts.value
+
     def synchronize(self):
         _check(lib().awcm_synchronize(self.h), "awcm_synchronize")
+
+
+KERNEL_STAGE_WENO, KERNEL_STAGE_COARSE, KERNEL_PREPARE = 0, 1, 2
+
+# 15-digit four-quadrant states (rho, u, v, p), quadrants NE, NW, SW, SE
+# (PROBLEM_QUADRANTS.h:43-73)
+QUAD_STATES = np.array([
+    [1.5, 0.0, 0.0, 1.5],
+    [0.532258064516129, 1.206045378311055, 0.0, 0.3],
+    [0.137992831541219, 1.206045378311055, 1.206045378311055, 0.029032258064516],
+    [0.532258064516129, 0.0, 1.206045378311055, 0.3],
+], dtype=np.float64)
+
+
+def ic_riemann_1d(res, xl, xu, split, left, right, gamma=1.4):
+    """SHOCKTUBE::Initialize on [xl, xu]; returns (q, dx, x0)."""
+    q = np.empty((res + 6) * 3)
+    a = np.ascontiguousarray(left, dtype=np.float64)
+    b = np.ascontiguousarray(right, dtype=np.float64)
+    dx, x0 = np.zeros(1), np.zeros(1)
+    _check(lib().awcm_ic_riemann_1d(res, xl, xu, split, _d(a), _d(b), gamma, _d(q), _d(dx), _d(x0)), "awcm_ic_riemann_1d")
+    return q, float(dx[0]), float(x0[0])
+
+
+def ic_shu_osher_1d(res, xl=-5.0, xu=5.0, split=-4.0, left=(3.857143, 2.629369, 10.33333), amp=0.2, gamma=1.4):
+    """ENTROPY_INTERACTING::Initialize; returns (q, dx, x0)."""
+    q = np.empty((res + 6) * 3)
+    a = np.ascontiguousarray(left, dtype=np.float64)
+    dx, x0 = np.zeros(1), np.zeros(1)
+    _check(lib().awcm_ic_shu_osher_1d(res, xl, xu, split, _d(a), amp, gamma, _d(q), _d(dx), _d(x0)),
+           "awcm_ic_shu_osher_1d")
+    return q, float(dx[0]), float(x0[0])
+
+
+def ic_quadrants_2d(res, split=0.5, states=QUAD_STATES, gamma=1.4):
+    """QUADRANTS::Initialize on [0,1]^2; returns (q, dx)."""
+    q = np.empty((res + 6) * (res + 6) * 4)
+    st = np.ascontiguousarray(states, dtype=np.float64)
+    dx = np.zeros(1)
+    _check(lib().awcm_ic_quadrants_2d(res, split, _d(st), gamma, _d(q), _d(dx)), "awcm_ic_quadrants_2d")
+    return q, float(dx[0])
+
+
+def ic_double_mach_2d(res, rho_pre=1.4, gamma=1.4):
+    """DOUBLEMACH::Initialize, ((res-1)*4+1) x res points; returns (q, dx)."""
+    nx = (res - 1) * 4 + 1
+    q = np.empty((nx + 6) * (res + 6) * 4)
+    dx = np.zeros(1)
+    _check(lib().awcm_ic_double_mach_2d(res, rho_pre, gamma, _d(q), _d(dx)), "awcm_ic_double_mach_2d")
+    return q, float(dx[0])
+
+
+def print_times(t_final, num_print):
+    """arr_printing_times of the drivers (main_1d_euler.cpp:43-48)."""
+    inc = t_final / (num_print - 1)
+    return np.array([inc * p for p in range(num_print)], dtype=np.float64)
 
 
 class Batch1D:

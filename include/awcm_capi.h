@@ -106,8 +106,46 @@ int awcm_set_schedule(awcm_solver* h, const double* print_times, int n_print);
 int awcm_run(awcm_solver* h, double tol, double cfl, double t_final, int max_steps, double* t, int* cnt,
              int* steps_done, long long* updates);
 
+/* As awcm_run, but all n_steps are queued with no host synchronisation in
+ * between (steps past t_final are no-ops on the device) and elapsed_ms is the
+ * device time between two HIP events recorded on the handle's stream around
+ * them.  Inputs must already be resident (awcm_upload_q). */
+int awcm_run_timed(awcm_solver* h, double tol, double cfl, double t_final, int n_steps, double* t, int* cnt,
+                   int* steps_done, long long* updates, double* elapsed_ms);
+
+/* Per-kernel HIP-event timing (2-D handles).  While enabled, every queued step
+ * records event pairs around the kernels of each class; awcm_profile_read
+ * returns the summed device time, the number of launches and the number of
+ * points those launches processed, then clears the counters. */
+enum {
+    AWCM_KERNEL_STAGE_WENO = 0,   /* finest-level WENO-Z/Roe stage updates     */
+    AWCM_KERNEL_STAGE_COARSE = 1, /* coarse-level central-difference updates    */
+    AWCM_KERNEL_PREPARE = 2       /* PrepareEuler + compaction (whole sequence) */
+};
+int awcm_profile_enable(awcm_solver* h, int enable);
+int awcm_profile_read(awcm_solver* h, int kernel, double* total_ms, long long* launches, long long* points);
+
 /* Block until the handle's queued device work is finished. */
 int awcm_synchronize(awcm_solver* h);
+
+/* ---- initial conditions (host side, reference layout) ------------------- */
+
+/* SHOCKTUBE::Initialize (PROBLEM_SHOCKTUBE.h:14-54) on [xl, xu]: cell-centred
+ * grid of res points, state `left` (rho,u,p) where x <= split, else `right`.
+ * q holds (res+6)*3 doubles; dx and x0 (x of point 0) are returned. */
+int awcm_ic_riemann_1d(int res, double xl, double xu, double split, const double* left, const double* right,
+                       double gamma, double* q, double* dx, double* x0);
+/* ENTROPY_INTERACTING::Initialize (PROBLEM_ENTROPY_INTERACTING.h:14-49):
+ * `left` where x < split, else rho = 1 + amp*sin(5x), u = 0, p = 1. */
+int awcm_ic_shu_osher_1d(int res, double xl, double xu, double split, const double* left, double amp,
+                         double gamma, double* q, double* dx, double* x0);
+/* QUADRANTS::Initialize (PROBLEM_QUADRANTS.h:15-81) on [0,1]^2, res x res
+ * points, split at (split, split); states[4*k..4*k+3] = (rho,u,v,p) of
+ * quadrant k+1 (NE, NW, SW, SE). */
+int awcm_ic_quadrants_2d(int res, double split, const double* states, double gamma, double* q, double* dx);
+/* DOUBLEMACH::Initialize (PROBLEM_DOUBLEMACH.h:13-63): ((res-1)*4+1) x res
+ * points on [0,4]x[0,1], pre-shock density rho_pre. */
+int awcm_ic_double_mach_2d(int res, double rho_pre, double gamma, double* q, double* dx);
 
 /* ---- batched 1-D form (no reference counterpart: replaces N sequential
  *      main_1d_euler.cpp runs) ------------------------------------------- */

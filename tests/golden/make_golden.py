@@ -1,0 +1,135 @@
+"""Write the golden vectors in tests/golden/ from the REFERENCE build.
+
+Runs only where oracle/_ref/libawcm_ref.so exists (built by `make -C oracle`
+from the reference's own headers).  Every vector below is produced by the
+reference's classes driven by oracle/ref_harness.cpp; the committed .npz files
+hold data only (inputs and expected outputs).
+
+    python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+import oracle_lib as O  # noqa: E402
+
+
+def kats(rng):
+    """Per-function known-answer vectors on seeded random inputs."""
+    R = O.load("ref")
+    out = {}
+    w = np.zeros(20)
+    R.ref_weights(O.dptr(w))
+    out["weights"] = w.reshape(4, 5)
+    wz_in = rng.normal(size=(64, 5))
+    out["wenoz_in"] = wz_in
+    out["wenoz_left"] = np.array([R.ref_wenoz_left(*r) for r in wz_in])
+    out["wenoz_right"] = np.array([R.ref_wenoz_right(*r) for r in wz_in])
+    # 1-D face fluxes on physical states (rho>0, p>0)
+    n = 64
+    s1 = np.zeros((n, 6, 3))
+    a1 = rng.uniform(0.5, 3.0, n)
+    for k in range(n):
+        rho = rng.uniform(0.1, 2.0, 6)
+        u = rng.uniform(-1.5, 1.5, 6)
+        p = rng.uniform(0.05, 2.0, 6)
+        s1[k, :, 0] = rho
+        s1[k, :, 1] = rho * u
+        s1[k, :, 2] = p / 0.3999999999999999 + 0.5 * rho * u * u
+    f1 = np.zeros((n, 3))
+    for k in range(n):
+        st = np.ascontiguousarray(s1[k]).ravel()
+        R.ref_flux1(1.4, O.dptr(st), a1[k], O.dptr(f1[k]))
+    out.update(flux1_states=s1, flux1_alpha=a1, flux1_out=f1)
+    s2 = np.zeros((n, 6, 4))
+    a2 = rng.uniform(0.5, 3.0, n)
+    for k in range(n):
+        rho = rng.uniform(0.1, 2.0, 6)
+        u = rng.uniform(-1.5, 1.5, 6)
+        v = rng.uniform(-1.5, 1.5, 6)
+        p = rng.uniform(0.05, 2.0, 6)
+        s2[k, :, 0] = rho
+        s2[k, :, 1] = rho * u
+        s2[k, :, 2] = rho * v
+        s2[k, :, 3] = p / 0.3999999999999999 + 0.5 * rho * (u * u + v * v)
+    f2 = np.zeros((n, 2, 4))
+    for k in range(n):
+        st = np.ascontiguousarray(s2[k]).ravel()
+        for d in range(2):
+            R.ref_flux2(1.4, d, O.dptr(st), a2[k], O.dptr(f2[k, d]))
+    out.update(flux2_states=s2, flux2_alpha=a2, flux2_out=f2)
+    return out
+
+
+def traj1(kind, L, nxc, tol, cfl, tf, npr, snaps):
+    nx = (nxc - 1) * 2 ** (L - 1) + 1
+    if kind == "sod":
+        q, dx, _ = O.ic1(0, nx, 0.0, 1.0, 0.5, O.SOD_L, O.SOD_R)
+    else:
+        q, dx, _ = O.ic1(1, nx, -5.0, 5.0, -4.0, O.SHU_L, O.SHU_L)
+    s = O.Solver1D("ref", L, nxc, 1.4, dx)
+    s.set_q(q)
+    pts = O.print_times(tf, npr)
+    out = dict(q0=q, dx=dx, tol=tol, cfl=cfl, levels=L, nx_coarse=nxc, print_times=pts, snaps=np.array(snaps))
+    n = 0
+    for target in snaps:
+        while n < target:
+            s.step(tol, cfl, pts)
+            n += 1
+        m, lv = s.masks()
+        out[f"q_{target}"] = s.get_q()
+        out[f"masks_{target}"] = m
+        out[f"levels_{target}"] = lv
+        out[f"t_{target}"] = s.t[0]
+        out[f"cnt_{target}"] = s.cnt.value
+    return out
+
+
+def traj2(kind, L, nxc, nyc, tol, cfl, tf, npr, snaps):
+    nx = (nxc - 1) * 2 ** (L - 1) + 1
+    ny = (nyc - 1) * 2 ** (L - 1) + 1
+    if kind == "quad":
+        q, dx, _ = O.ic2(0, nx, ny, 0.0, 1.0, 0.0, 1.0, 0.5)
+        bc, xu = 0, 1.0
+    else:
+        q, dx, _ = O.ic2(1, nx, ny, 0.0, 4.0, 0.0, 1.0, 1.4)
+        bc, xu = 1, 4.0
+    s = O.Solver2D("ref", L, nxc, nyc, 1.4, bc, 0.0, xu, 0.0, 1.0)
+    s.set_q(q)
+    pts = O.print_times(tf, npr)
+    out = dict(q0=q, dx=dx, tol=tol, cfl=cfl, levels=L, nx_coarse=nxc, ny_coarse=nyc, bc=bc, xu=xu,
+               print_times=pts, snaps=np.array(snaps))
+    n = 0
+    for target in snaps:
+        while n < target:
+            s.step(tol, cfl, pts)
+            n += 1
+        m, lv = s.masks()
+        out[f"q_{target}"] = s.get_q()
+        out[f"masks_{target}"] = m.astype(np.uint8)
+        out[f"levels_{target}"] = lv.astype(np.int8)
+        out[f"t_{target}"] = s.t[0]
+        out[f"cnt_{target}"] = s.cnt.value
+    return out
+
+
+def main():
+    if not O.have_ref():
+        sys.exit("oracle/_ref/libawcm_ref.so is not built (make -C oracle)")
+    rng = np.random.default_rng(20261203)
+    np.savez_compressed(os.path.join(HERE, "kat_functions.npz"), **kats(rng))
+    np.savez_compressed(os.path.join(HERE, "sod_1d_L6.npz"), **traj1("sod", 6, 9, 1e-4, 0.5, 0.2, 41, [1, 5, 40]))
+    np.savez_compressed(os.path.join(HERE, "shu_osher_1d_L6.npz"),
+                        **traj1("shu", 6, 9, 1e-4, 0.5, 1.8, 41, [1, 10, 30]))
+    np.savez_compressed(os.path.join(HERE, "quad_2d_L4.npz"), **traj2("quad", 4, 9, 9, 1e-3, 0.4, 0.25, 9, [1, 6]))
+    np.savez_compressed(os.path.join(HERE, "dmr_2d_L4.npz"), **traj2("dmr", 4, 17, 6, 1e-3, 0.4, 0.2, 21, [1, 6]))
+    for f in sorted(os.listdir(HERE)):
+        if f.endswith(".npz"):
+            print(f, os.path.getsize(os.path.join(HERE, f)), "bytes")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,89 @@
+"""C ABI checks that need no GPU: the library loads, exports every entry point
+include/awcm_capi.h declares, its host-side initial conditions are bit-identical
+to the oracle's, and invalid arguments are rejected before any device call."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "awcm_capi.h")
+
+
+def _lib():
+    import pyawcm
+    if not os.path.exists(pyawcm.LIB_PATH):
+        pytest.skip("HIP library not built (python c_pp-library_amd/build.py)")
+    return pyawcm
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:int|void)\s+(awcm_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_header_declares_entry_points():
+    syms = declared_symbols()
+    assert "awcm_prepare_euler" in syms and "awcm_run" in syms and "awcm_batch_run" in syms
+    assert len(syms) >= 25
+
+
+def test_library_exports_every_declared_symbol():
+    P = _lib()
+    lib = C.CDLL(P.LIB_PATH)
+    missing = [s for s in declared_symbols() if not hasattr(lib, s)]
+    assert not missing, f"not exported: {missing}"
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float64).view(np.uint64)
+
+
+def test_host_initial_conditions_match_oracle():
+    P = _lib()
+    q, dx, x0 = P.ic_riemann_1d(1025, 0.0, 1.0, 0.5, O.SOD_L, O.SOD_R)
+    qo, dxo, x0o = O.ic1(0, 1025, 0.0, 1.0, 0.5, O.SOD_L, O.SOD_R)
+    assert np.array_equal(bits(q), bits(qo)) and dx == dxo and x0 == x0o
+    q, dx, x0 = P.ic_shu_osher_1d(4097)
+    qo, dxo, x0o = O.ic1(1, 4097, -5.0, 5.0, -4.0, O.SHU_L, O.SHU_L)
+    assert np.array_equal(bits(q), bits(qo)) and dx == dxo and x0 == x0o
+    q, dx = P.ic_quadrants_2d(129, 0.5)
+    qo, dxo, _ = O.ic2(0, 129, 129, 0.0, 1.0, 0.0, 1.0, 0.5)
+    assert np.array_equal(bits(q), bits(qo)) and dx == dxo
+    q, dx = P.ic_double_mach_2d(65, 1.4)
+    qo, dxo, _ = O.ic2(1, 257, 65, 0.0, 4.0, 0.0, 1.0, 1.4)
+    assert np.array_equal(bits(q), bits(qo)) and dx == dxo
+
+
+@pytest.mark.skipif(not O.have_ref(), reason="reference build (oracle/_ref) not present")
+def test_host_initial_conditions_match_reference_problems():
+    P = _lib()
+    q, dx, x0 = P.ic_riemann_1d(257, 0.0, 10.0, 5.0, O.SOD_L, O.SOD_R)
+    qr = np.zeros(q.size)
+    dxr, x0r = np.zeros(1), np.zeros(1)
+    O.load("ref").ref_ic1(0, 257, O.dptr(qr), O.dptr(dxr), O.dptr(x0r))
+    assert np.array_equal(bits(q), bits(qr)) and dx == dxr[0]
+    q, dx = P.ic_double_mach_2d(33, 1.5)
+    qr = np.zeros(q.size)
+    dyr = np.zeros(1)
+    O.load("ref").ref_ic2(1, 33, O.dptr(qr), O.dptr(dxr), O.dptr(dyr))
+    assert np.array_equal(bits(q), bits(qr)) and dx == dxr[0]
+
+
+def test_invalid_arguments_rejected_before_device_work():
+    P = _lib()
+    lib = P.lib()
+    h = C.c_void_p()
+    assert lib.awcm_create_1d(1, 17, 1.4, 0.001, C.byref(h)) == -1      # too few levels
+    assert lib.awcm_create_1d(7, 5, 1.4, 0.001, C.byref(h)) == -1       # coarse grid too small
+    assert lib.awcm_create_1d(7, 17, 1.4, -1.0, C.byref(h)) == -1       # non-positive dx
+    assert lib.awcm_create_2d(7, 4, 17, 1.4, 0.001, 0.0, 1, C.byref(h)) == -1
+    assert lib.awcm_create_2d(7, 65, 17, 1.4, 0.001, 0.0, 9, C.byref(h)) == -1  # unknown BC kind
+    assert lib.awcm_create_2d(14, 65, 17, 1.4, 0.001, 0.0, 1, C.byref(h)) == -1  # grid too large
+    assert lib.awcm_run(None, 1e-3, 0.4, 0.2, 1, None, None, None, None) == -1
+    assert lib.awcm_ic_double_mach_2d(1, 1.4, 1.4, None, None) == -1

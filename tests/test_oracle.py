@@ -1,0 +1,365 @@
+"""Pins for the CPU oracle (runs without a GPU).
+
+* golden vectors made by the reference build (tests/golden/make_golden.py):
+  the oracle must reproduce them bit for bit;
+* the reference build itself (oracle/_ref), where present: per-function and
+  trajectory comparisons, initial conditions;
+* closed forms and invariants that a dropped term, a wrong sign or a
+  transposed index would break: dyadic wavelet weights, WENO-Z ideal-weight
+  limit and mirror symmetry, flux consistency F(Q,...,Q) = F(Q) (which needs
+  R L = I), nested masks, the always-active band, eps = 0 reducing the AWCM
+  step to a uniform-grid WENO-Z SSP-RK3 step, and Sod against the exact
+  Riemann solution.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+needs_ref = pytest.mark.skipif(not O.have_ref(), reason="reference build (oracle/_ref) not present")
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float64).view(np.uint64)
+
+
+def load(name):
+    return np.load(os.path.join(GOLD, name), allow_pickle=False)
+
+
+# ---------------------------------------------------------------- golden ----
+def test_oracle_matches_golden_kats():
+    g = load("kat_functions.npz")
+    L = O.load("orc")
+    w = np.zeros(20)
+    L.orc_weights(O.dptr(w))
+    assert np.array_equal(bits(w.reshape(4, 5)), bits(g["weights"]))
+    left = np.array([L.orc_wenoz_left(*r) for r in g["wenoz_in"]])
+    right = np.array([L.orc_wenoz_right(*r) for r in g["wenoz_in"]])
+    assert np.array_equal(bits(left), bits(g["wenoz_left"]))
+    assert np.array_equal(bits(right), bits(g["wenoz_right"]))
+    for k in range(len(g["flux1_alpha"])):
+        st = np.ascontiguousarray(g["flux1_states"][k]).ravel()
+        f = np.zeros(3)
+        L.orc_flux1(1.4, O.dptr(st), g["flux1_alpha"][k], O.dptr(f))
+        assert np.array_equal(bits(f), bits(g["flux1_out"][k]))
+    for k in range(len(g["flux2_alpha"])):
+        st = np.ascontiguousarray(g["flux2_states"][k]).ravel()
+        for d in range(2):
+            f = np.zeros(4)
+            L.orc_flux2(1.4, d, O.dptr(st), g["flux2_alpha"][k], O.dptr(f))
+            assert np.array_equal(bits(f), bits(g["flux2_out"][k, d]))
+
+
+@pytest.mark.parametrize("name", ["sod_1d_L6.npz", "shu_osher_1d_L6.npz"])
+def test_oracle_matches_golden_1d(name):
+    g = load(name)
+    s = O.Solver1D("orc", int(g["levels"]), int(g["nx_coarse"]), 1.4, float(g["dx"]))
+    s.set_q(g["q0"])
+    n = 0
+    for target in g["snaps"]:
+        while n < target:
+            s.step(float(g["tol"]), float(g["cfl"]), g["print_times"])
+            n += 1
+        m, lv = s.masks()
+        assert np.array_equal(m, g[f"masks_{target}"]), f"{name} masks at step {target}"
+        assert np.array_equal(lv, g[f"levels_{target}"]), f"{name} levels at step {target}"
+        assert np.array_equal(bits(s.get_q()), bits(g[f"q_{target}"])), f"{name} Q at step {target}"
+        assert s.t[0] == g[f"t_{target}"] and s.cnt.value == g[f"cnt_{target}"]
+
+
+@pytest.mark.parametrize("name", ["quad_2d_L4.npz", "dmr_2d_L4.npz"])
+def test_oracle_matches_golden_2d(name):
+    g = load(name)
+    s = O.Solver2D("orc", int(g["levels"]), int(g["nx_coarse"]), int(g["ny_coarse"]), 1.4, int(g["bc"]),
+                   0.0, float(g["xu"]), 0.0, 1.0)
+    s.set_q(g["q0"])
+    n = 0
+    for target in g["snaps"]:
+        while n < target:
+            s.step(float(g["tol"]), float(g["cfl"]), g["print_times"])
+            n += 1
+        m, lv = s.masks()
+        assert np.array_equal(m, g[f"masks_{target}"].reshape(m.shape)), f"{name} masks at step {target}"
+        assert np.array_equal(lv, g[f"levels_{target}"].reshape(lv.shape)), f"{name} levels at step {target}"
+        assert np.array_equal(bits(s.get_q()), bits(g[f"q_{target}"])), f"{name} Q at step {target}"
+        assert s.t[0] == g[f"t_{target}"]
+
+
+# ----------------------------------------------------- reference build ----
+@needs_ref
+def test_oracle_ics_match_reference():
+    for kind, res in ((0, 257), (1, 513)):
+        q_ref = np.zeros((res + 6) * 3)
+        dx_r, x0_r = np.zeros(1), np.zeros(1)
+        O.load("ref").ref_ic1(kind, res, O.dptr(q_ref), O.dptr(dx_r), O.dptr(x0_r))
+        if kind == 0:
+            q, dx, x0 = O.ic1(0, res, 0.0, 10.0, 5.0, O.SOD_L, O.SOD_R)
+        else:
+            q, dx, x0 = O.ic1(1, res, -5.0, 5.0, -4.0, O.SHU_L, O.SHU_L)
+        assert np.array_equal(bits(q), bits(q_ref)) and dx == dx_r[0] and x0 == x0_r[0]
+    for kind, res in ((0, 65), (1, 33)):
+        nx = res if kind == 0 else (res - 1) * 4 + 1
+        q_ref = np.zeros((nx + 6) * (res + 6) * 4)
+        dx_r, dy_r = np.zeros(1), np.zeros(1)
+        O.load("ref").ref_ic2(kind, res, O.dptr(q_ref), O.dptr(dx_r), O.dptr(dy_r))
+        if kind == 0:
+            q, dx, _ = O.ic2(0, nx, res, 0.0, 1.0, 0.0, 1.0, 0.8)
+        else:
+            q, dx, _ = O.ic2(1, nx, res, 0.0, 4.0, 0.0, 1.0, 1.5)  # the reference IC uses rho_pre = 1.5
+        assert np.array_equal(bits(q), bits(q_ref)) and dx == dx_r[0]
+
+
+@needs_ref
+def test_oracle_functions_match_reference_random():
+    rng = np.random.default_rng(11)
+    A, B = O.load("orc"), O.load("ref")
+    for _ in range(200):
+        r = rng.normal(size=5) * rng.choice([1e-6, 1.0, 1e3])
+        assert bits(A.orc_wenoz_left(*r)) == bits(B.ref_wenoz_left(*r))
+        assert bits(A.orc_wenoz_right(*r)) == bits(B.ref_wenoz_right(*r))
+
+
+@needs_ref
+def test_oracle_sod_full_run_matches_reference():
+    L, nxc = 7, 17
+    nx = (nxc - 1) * 2 ** (L - 1) + 1
+    q, dx, _ = O.ic1(0, nx, 0.0, 1.0, 0.5, O.SOD_L, O.SOD_R)
+    a, b = O.Solver1D("orc", L, nxc, 1.4, dx), O.Solver1D("ref", L, nxc, 1.4, dx)
+    a.set_q(q)
+    b.set_q(q)
+    pts = O.print_times(0.2, 41)
+    while a.t[0] < 0.2:
+        assert a.step(1e-4, 0.5, pts) == b.step(1e-4, 0.5, pts)
+    ma, la = a.masks()
+    mb, lb = b.masks()
+    assert np.array_equal(ma, mb) and np.array_equal(la, lb)
+    assert np.array_equal(bits(a.get_q()), bits(b.get_q()))
+    assert O.load("orc").orc1_oob(a.h) == 0
+
+
+@needs_ref
+@pytest.mark.parametrize("case", ["quad", "dmr"])
+def test_oracle_2d_matches_reference(case):
+    if case == "quad":
+        L, nxc, nyc, bc, xu, s = 5, 9, 9, 0, 1.0, 0.5
+    else:
+        L, nxc, nyc, bc, xu, s = 4, 33, 9, 1, 4.0, 1.4
+    nx = (nxc - 1) * 2 ** (L - 1) + 1
+    ny = (nyc - 1) * 2 ** (L - 1) + 1
+    q, dx, _ = O.ic2(bc, nx, ny, 0.0, xu, 0.0, 1.0, s)
+    a = O.Solver2D("orc", L, nxc, nyc, 1.4, bc, 0.0, xu, 0.0, 1.0)
+    b = O.Solver2D("ref", L, nxc, nyc, 1.4, bc, 0.0, xu, 0.0, 1.0)
+    a.set_q(q)
+    b.set_q(q)
+    pts = O.print_times(0.25, 9)
+    for _ in range(12):
+        assert a.step(1e-3, 0.4, pts) == b.step(1e-3, 0.4, pts)
+        ma, la = a.masks()
+        mb, lb = b.masks()
+        assert np.array_equal(ma, mb) and np.array_equal(la, lb)
+        assert np.array_equal(bits(a.get_q()), bits(b.get_q()))
+    assert O.load("orc").orc2_oob(a.h) == 0
+
+
+# ----------------------------------------------- closed forms, invariants ----
+def test_wavelet_weights_closed_form():
+    w = np.zeros(20)
+    O.load("orc").orc_weights(O.dptr(w))
+    w = w.reshape(4, 5)
+    # weight(i, j) at w[i, j+1]: Lagrange basis of nodes 0..3 at j + 1/2
+    assert np.array_equal(w[:, 1] * 16, [5, 15, -5, 1])    # j = 0: left-boundary predictor
+    assert np.array_equal(w[:, 2] * 16, [-1, 9, 9, -1])    # j = 1: centred predictor
+    assert np.array_equal(w[:, 3] * 16, [1, -5, 15, 5])    # j = 2: right-boundary predictor
+    for j in range(5):
+        assert abs(w[:, j].sum() - 1.0) < 1e-15          # partition of unity
+        x = j - 1 + 0.5
+        assert abs((w[:, j] * np.arange(4) ** 3).sum() - x ** 3) < 1e-12  # exact for cubics
+
+
+def test_wenoz_ideal_limit_and_mirror():
+    L = O.load("orc")
+    rng = np.random.default_rng(3)
+    for _ in range(100):
+        a, b = rng.normal(size=2)
+        v = a + b * np.arange(-2, 3)          # linear data: all smoothness indicators equal
+        left = L.orc_wenoz_left(*v)
+        assert abs(left - (a - 0.5 * b)) <= 1e-13 * (1 + abs(a) + abs(b))
+        r = rng.normal(size=5)
+        assert bits(L.orc_wenoz_right(*r)) == bits(L.orc_wenoz_left(*r[::-1]))
+    # smooth data: weights near ideal, value near the 5th-order linear scheme
+    x = 0.01 * np.arange(-2, 3)
+    v = np.sin(x) + 2
+    lin = (-3 * v[0] + 27 * v[1] + 47 * v[2] - 13 * v[3] + 2 * v[4]) / 60
+    assert abs(L.orc_wenoz_left(*v) - lin) < 1e-9
+
+
+def _phys_flux(q, dim, d=0, g=1.4):
+    if dim == 1:
+        rho, m, E = q
+        u = m / rho
+        p = (g - 1) * (E - 0.5 * m * m / rho)
+        return np.array([m, m * u + p, (E + p) * u])
+    rho, mx, my, E = q
+    u, v = mx / rho, my / rho
+    p = (g - 1) * (E - 0.5 * (mx * mx + my * my) / rho)
+    vn = u if d == 0 else v
+    return np.array([rho * vn, mx * vn + (p if d == 0 else 0), my * vn + (p if d == 1 else 0), (E + p) * vn])
+
+
+def test_face_flux_consistency():
+    L = O.load("orc")
+    rng = np.random.default_rng(5)
+    for _ in range(50):
+        rho, u, v, p = rng.uniform(0.2, 2), rng.uniform(-1, 1), rng.uniform(-1, 1), rng.uniform(0.2, 2)
+        q1 = np.array([rho, rho * u, p / 0.4 + 0.5 * rho * u * u])
+        st = np.tile(q1, 6)
+        f = np.zeros(3)
+        L.orc_flux1(1.4, O.dptr(st), rng.uniform(1, 3), O.dptr(f))
+        assert np.allclose(f, _phys_flux(q1, 1), rtol=1e-12, atol=1e-12)
+        q2 = np.array([rho, rho * u, rho * v, p / 0.4 + 0.5 * rho * (u * u + v * v)])
+        st = np.tile(q2, 6)
+        for d in range(2):
+            f = np.zeros(4)
+            L.orc_flux2(1.4, d, O.dptr(st), rng.uniform(1, 3), O.dptr(f))
+            assert np.allclose(f, _phys_flux(q2, 2, d), rtol=1e-12, atol=1e-12)
+
+
+def test_mask_invariants_2d():
+    L, nxc, nyc = 4, 33, 9
+    nx, ny = (nxc - 1) * 8 + 1, (nyc - 1) * 8 + 1
+    q, dx, _ = O.ic2(1, nx, ny, 0.0, 4.0, 0.0, 1.0, 1.4)
+    s = O.Solver2D("orc", L, nxc, nyc, 1.4, 1, 0.0, 4.0, 0.0, 1.0)
+    s.set_q(q)
+    pts = O.print_times(0.2, 21)
+    for _ in range(4):
+        s.step(1e-3, 0.4, pts)
+        m, lv = s.masks()
+        inner = m[:, 3:-3, 3:-3].astype(bool)
+        for k in range(1, 4):
+            assert not (inner[k] & ~inner[k + 1]).any(), "mask nesting"
+        for k in range(1, 5):
+            band = np.ones_like(inner[k])
+            band[2:-2, 2:-2] = False
+            assert inner[k][band].all(), "2-wide boundary band"
+            m0 = 2 ** (L - 1)
+            assert inner[k][::m0, ::m0].all(), "coarsest grid always active"
+        # level == max wherever a +-1 neighbour is active (level0 from mask0)
+        act = np.pad(inner[1], 1)
+        nb = act[:-2, 1:-1] | act[2:, 1:-1] | act[1:-1, :-2] | act[1:-1, 2:]
+        assert (lv[0][inner[1] & nb] == L - 1).all()
+        assert (lv[0][~inner[1]] == 0).all()
+
+
+def _uniform_step_1d(q, dx, cfl, g=1.4):
+    """Textbook uniform-grid WENO-Z5/Roe SSP-RK3 step with the drivers' operation order."""
+    L = O.load("orc")
+    n = q.size // 3 - 6
+    Q = q.reshape(-1, 3).copy()
+
+    def bc(A):
+        A[0:3] = A[3]
+        A[-3:] = A[-4]
+
+    def sr(A):
+        rho, m, E = A[3:-3, 0], A[3:-3, 1], A[3:-3, 2]
+        u = m / rho
+        p = (g - 1) * (E - 0.5 * (m * m) / rho)
+        return np.abs(u) + np.sqrt(g * p / rho)
+
+    def dF(A, alpha):
+        F = np.zeros((n + 1, 3))
+        for f in range(n + 1):
+            st = np.ascontiguousarray(A[f:f + 6]).ravel()  # points f-3 .. f+2
+            L.orc_flux1(g, O.dptr(st), alpha, O.dptr(F[f]))
+        return F[1:] - F[:-1]
+
+    vmax = 0.0
+    for v in sr(Q):
+        vmax = v if v > vmax else vmax
+    dt = cfl * dx / vmax
+    h = dt / (dx * 1)
+    bc(Q)
+    Qc = Q.copy()
+    Q[3:-3] = Q[3:-3] - h * dF(Q, sr(Q).max())
+    bc(Q)
+    Q[3:-3] = 0.75 * Qc[3:-3] + 0.25 * (Q[3:-3] - h * dF(Q, sr(Q).max()))
+    bc(Q)
+    Q[3:-3] = (Qc[3:-3] + 2.0 * (Q[3:-3] - h * dF(Q, sr(Q).max()))) / 3.0
+    return Q.ravel(), dt
+
+
+def test_eps0_reduces_to_uniform_wenoz():
+    """With eps = 0 nothing is thresholded: every point is active at the finest
+    level and the AWCM step is the uniform WENO-Z SSP-RK3 step (up to the
+    rounding of the forward/inverse transform round trip)."""
+    L, nxc = 5, 9
+    nx = (nxc - 1) * 16 + 1
+    q, dx, _ = O.ic1(0, nx, 0.0, 1.0, 0.5, O.SOD_L, O.SOD_R)
+    s = O.Solver1D("orc", L, nxc, 1.4, dx)
+    s.set_q(q)
+    dt = s.step(0.0, 0.5, np.array([0.0, 1.0]))
+    m, lv = s.masks()
+    assert m[1:, 3:-3].all() and (lv == L - 1).all()
+    qu, dtu = _uniform_step_1d(q, dx, 0.5)
+    assert abs(dt - dtu) <= 1e-15 * dtu
+    a, b = s.get_q().reshape(-1, 3)[3:-3], qu.reshape(-1, 3)[3:-3]
+    assert np.abs(a - b).max() <= 1e-13 * np.abs(b).max()
+
+
+def _exact_sod(x, t, left=(1.0, 0.0, 1.0), right=(0.125, 0.0, 0.1), x0=0.5, g=1.4):
+    """Exact Riemann solution (Toro, ch. 4) sampled at x, t."""
+    rl, ul, pl = left
+    rr, ur, pr = right
+    cl, cr = np.sqrt(g * pl / rl), np.sqrt(g * pr / rr)
+
+    def f(p, rk, pk, ck):
+        if p > pk:
+            A, B = 2 / ((g + 1) * rk), (g - 1) / (g + 1) * pk
+            return (p - pk) * np.sqrt(A / (p + B))
+        return 2 * ck / (g - 1) * ((p / pk) ** ((g - 1) / (2 * g)) - 1)
+
+    lo, hi = 1e-8, 10.0
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if f(mid, rl, pl, cl) + f(mid, rr, pr, cr) + ur - ul > 0:
+            hi = mid
+        else:
+            lo = mid
+    ps = 0.5 * (lo + hi)
+    us = 0.5 * (ul + ur) + 0.5 * (f(ps, rr, pr, cr) - f(ps, rl, pl, cl))
+    rho = np.empty_like(x)
+    for k, xi in enumerate(x):
+        s = (xi - x0) / t
+        if s < us:  # left of contact: rarefaction (Sod)
+            shl, stl = ul - cl, us - cl * (ps / pl) ** ((g - 1) / (2 * g))
+            if s < shl:
+                rho[k] = rl
+            elif s > stl:
+                rho[k] = rl * (ps / pl) ** (1 / g)
+            else:
+                rho[k] = rl * (2 / (g + 1) + (g - 1) / ((g + 1) * cl) * (ul - s)) ** (2 / (g - 1))
+        else:  # right: shock
+            sh = ur + cr * np.sqrt((g + 1) / (2 * g) * ps / pr + (g - 1) / (2 * g))
+            rho[k] = rr if s > sh else rr * ((ps / pr + (g - 1) / (g + 1)) / ((g - 1) / (g + 1) * ps / pr + 1))
+    return rho
+
+
+def test_sod_against_exact_riemann_solution():
+    L, nxc = 7, 17
+    nx = (nxc - 1) * 64 + 1
+    q, dx, x0 = O.ic1(0, nx, 0.0, 1.0, 0.5, O.SOD_L, O.SOD_R)
+    s = O.Solver1D("orc", L, nxc, 1.4, dx)
+    s.set_q(q)
+    pts = O.print_times(0.2, 41)
+    while s.t[0] < 0.2:
+        s.step(1e-4, 0.5, pts)
+    m, _ = s.masks()
+    act = m[1, 3:-3].astype(bool)
+    x = x0 + dx * np.arange(nx)
+    rho = s.get_q().reshape(-1, 3)[3:-3, 0]
+    ex = _exact_sod(x[act], 0.2)
+    assert np.abs(rho[act] - ex).mean() < 5e-3
