@@ -78,32 +78,35 @@ __device__ __forceinline__ void ustart(int i, int d, int iend_lim, int& is, int&
     else if (i + d + dh > iend_lim) is = iend_lim - 3 * d - dh, k = 2;
 }
 
-// ---------------------------------------------------------------- begin ----
-__global__ void k2_begin(Dims2 D, Scal2* sc, double* alpha, int force) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int na = 3 * (D.Nx + D.Ny);
-    for (int k = t; k < na; k += gridDim.x * blockDim.x) alpha[k] = 0.0;
-    if (t == 0) {
-        sc->skip = (!force && !(sc->t < sc->t_final)) ? 1 : 0;
-        sc->vmax = 0ull;
-        for (int k = 0; k < 3; k++) { sc->cntW[k] = 0; sc->cntC[k] = 0; }
-    }
+// ------------------------------------------------------------- lattices ----
+// Lattices of level lev (m = 2^(L-1-lev), h = m/2).  For every admissible size
+// they coincide with the subgrid/grid ranges of MRA_GRID_UNIFORM_2D.h:108-130
+// (checked against the reference formulas on the host at set-up).
+enum { LAT_D1 = 0, LAT_D2 = 1, LAT_D3 = 2, LAT_C = 3 };
+
+__device__ __forceinline__ Lat lattice(const Dims2& D, int lev, int kind) {
+    const int m = 1 << (D.L - 1 - lev), h = m >> 1;
+    Lat g;
+    g.dx = m;
+    g.dy = m;
+    if (kind == LAT_D1) { g.is = h; g.ie = D.Nx - 1 - h; g.js = m; g.je = D.Ny - 1 - m; }
+    else if (kind == LAT_D2) { g.is = m; g.ie = D.Nx - 1 - m; g.js = h; g.je = D.Ny - 1 - h; }
+    else if (kind == LAT_D3) { g.is = h; g.ie = D.Nx - 1 - h; g.js = h; g.je = D.Ny - 1 - h; }
+    else { g.is = m; g.ie = D.Nx - 1 - m; g.js = m; g.je = D.Ny - 1 - m; }
+    g.ni = (g.ie - g.is) / m + 1;
+    g.nj = (g.je - g.js) / m + 1;
+    return g;
 }
 
-__global__ void k2_copy(const uint8_t* __restrict__ src, uint8_t* __restrict__ dst, long long n, const Scal2* sc) {
-    if (sc->skip) return;
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) dst[t] = src[t];
-}
+// ------------------------------------------------ lifting point operations --
+// One point of one lifting pass; `act` is the point's mask byte read by the
+// caller (so a thread can issue the mask loads of all its points first).
+enum { OP_FWT_D3, OP_FWT_D12, OP_FWT_C, OP_IWT_C, OP_IWT_D12, OP_IWT_D3, OP_INT_D12, OP_INT_D3 };
 
-// ------------------------------------------------------------ forward FWT ----
-__global__ void k2_fwt_d3(Dims2 D, Lat g, int d, Wts W, double tol, double* __restrict__ rho, uint8_t* __restrict__ mask,
-                          const Scal2* sc) {
-    if (sc->skip) return;
-    int i, j;
-    if (!lat_point(g, blockIdx.x * blockDim.x + threadIdx.x, i, j)) return;
+// WAVELET_2D.h:496-523 (d3 predict + threshold)
+__device__ __forceinline__ void fwt_d3_pt(const Dims2& D, int i, int j, int d, const Wts& W, double tol,
+                                          double* rho, uint8_t* mask) {
     const long long p = gidx(D, i, j);
-    if (!mask[p]) return;
     int is, js, k1, k2;
     pstart(i, d, D.Nx - 1, is, k1);
     pstart(j, d, D.Ny - 1, js, k2);
@@ -117,21 +120,13 @@ __global__ void k2_fwt_d3(Dims2 D, Lat g, int d, Wts W, double tol, double* __re
     rho[p] = v;
     if (t_abs(v) < tol) mask[p] = 0;
 }
-
-__global__ void k2_fwt_d12(Dims2 D, Lat g1, Lat g2, int d, Wts W, double tol, double* __restrict__ rho,
-                           uint8_t* __restrict__ mask, const Scal2* sc) {
-    if (sc->skip) return;
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int n1 = g1.ni * g1.nj;
-    int i, j;
-    const bool d1 = t < n1;
-    if (d1) { if (!lat_point(g1, t, i, j)) return; }
-    else if (!lat_point(g2, t - n1, i, j)) return;
+// WAVELET_2D.h:539-583 (d1 in x / d2 in y predict + threshold)
+__device__ __forceinline__ void fwt_d12_pt(const Dims2& D, int kind, int i, int j, int d, const Wts& W, double tol,
+                                           double* rho, uint8_t* mask) {
     const long long p = gidx(D, i, j);
-    if (!mask[p]) return;
     double v = rho[p];
     int s, k;
-    if (d1) {
+    if (kind == LAT_D1) {
         pstart(i, d, D.Nx - 1, s, k);
         for (int c = 0; c < 4; c++) v -= rho[gidx(D, s + c * d, j)] * W.w[c][k + 1];
     } else {
@@ -142,63 +137,36 @@ __global__ void k2_fwt_d12(Dims2 D, Lat g1, Lat g2, int d, Wts W, double tol, do
     rho[p] = v;
     if (t_abs(v) < tol) mask[p] = 0;
 }
-
-__global__ void k2_fwt_c(Dims2 D, Lat g, int d, Wts W, double* __restrict__ rho, const uint8_t* __restrict__ mask,
-                         const Scal2* sc) {
-    if (sc->skip) return;
-    int i, j;
-    if (!lat_point(g, blockIdx.x * blockDim.x + threadIdx.x, i, j)) return;
+// WAVELET_2D.h:584-622 (update, +) and :644-683 (undo, -)
+template <int SIGN>
+__device__ __forceinline__ void lift_c_pt(const Dims2& D, int i, int j, int d, const Wts& W, double* rho) {
     const long long p = gidx(D, i, j);
-    if (!mask[p]) return;
     int is, js, k1, k2;
     ustart(i, d, D.Nx - 1, is, k1);
     ustart(j, d, D.Ny - 1, js, k2);
     double v = rho[p];
-    for (int c = 0; c < 4; c++) v += rho[gidx(D, is + c * d, j)] * W.w[c][k1 + 1];
-    for (int c = 0; c < 4; c++) v += rho[gidx(D, i, js + c * d)] * W.w[c][k2 + 1];
+    for (int c = 0; c < 4; c++) {
+        const double t = rho[gidx(D, is + c * d, j)] * W.w[c][k1 + 1];
+        if (SIGN > 0) v += t; else v -= t;
+    }
+    for (int c = 0; c < 4; c++) {
+        const double t = rho[gidx(D, i, js + c * d)] * W.w[c][k2 + 1];
+        if (SIGN > 0) v += t; else v -= t;
+    }
     for (int c2 = 0; c2 < 4; c2++)
-        for (int c1 = 0; c1 < 4; c1++)
-            v += rho[gidx(D, is + c1 * d, js + c2 * d)] * W.w[c1][k1 + 1] * W.w[c2][k2 + 1];
+        for (int c1 = 0; c1 < 4; c1++) {
+            const double t = rho[gidx(D, is + c1 * d, js + c2 * d)] * W.w[c1][k1 + 1] * W.w[c2][k2 + 1];
+            if (SIGN > 0) v += t; else v -= t;
+        }
     rho[p] = v;
 }
-
-// ------------------------------------------------------------ inverse FWT ----
-__global__ void k2_iwt_c(Dims2 D, Lat g, int d, Wts W, double* __restrict__ rho, const uint8_t* __restrict__ mask,
-                         const Scal2* sc) {
-    if (sc->skip) return;
-    int i, j;
-    if (!lat_point(g, blockIdx.x * blockDim.x + threadIdx.x, i, j)) return;
+// WAVELET_2D.h:688-734 (inverse d2 / d1)
+__device__ __forceinline__ void iwt_d12_pt(const Dims2& D, int kind, int i, int j, int d, const Wts& W, double* rho) {
     const long long p = gidx(D, i, j);
-    if (!mask[p]) { rho[p] = 0; return; }
-    int is, js, k1, k2;
-    ustart(i, d, D.Nx - 1, is, k1);
-    ustart(j, d, D.Ny - 1, js, k2);
-    double v = rho[p];
-    for (int c = 0; c < 4; c++) v -= rho[gidx(D, is + c * d, j)] * W.w[c][k1 + 1];
-    for (int c = 0; c < 4; c++) v -= rho[gidx(D, i, js + c * d)] * W.w[c][k2 + 1];
-    for (int c2 = 0; c2 < 4; c2++)
-        for (int c1 = 0; c1 < 4; c1++)
-            v -= rho[gidx(D, is + c1 * d, js + c2 * d)] * W.w[c1][k1 + 1] * W.w[c2][k2 + 1];
-    rho[p] = v;
-}
-
-// d2 then d1 in the reference; the two sub-bands read only level-lev points, so
-// one launch covers both.
-__global__ void k2_iwt_d12(Dims2 D, Lat g1, Lat g2, int d, Wts W, double* __restrict__ rho,
-                           const uint8_t* __restrict__ mask, const Scal2* sc) {
-    if (sc->skip) return;
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int n1 = g1.ni * g1.nj;
-    int i, j;
-    const bool d1 = t < n1;
-    if (d1) { if (!lat_point(g1, t, i, j)) return; }
-    else if (!lat_point(g2, t - n1, i, j)) return;
-    const long long p = gidx(D, i, j);
-    if (!mask[p]) { rho[p] = 0; return; }
     double v = rho[p];
     v *= 2.0;
     int s, k;
-    if (d1) {
+    if (kind == LAT_D1) {
         pstart(i, d, D.Nx - 1, s, k);
         for (int c = 0; c < 4; c++) v += rho[gidx(D, s + c * d, j)] * W.w[c][k + 1];
     } else {
@@ -207,14 +175,9 @@ __global__ void k2_iwt_d12(Dims2 D, Lat g1, Lat g2, int d, Wts W, double* __rest
     }
     rho[p] = v;
 }
-
-__global__ void k2_iwt_d3(Dims2 D, Lat g, int d, Wts W, double* __restrict__ rho, const uint8_t* __restrict__ mask,
-                          const Scal2* sc) {
-    if (sc->skip) return;
-    int i, j;
-    if (!lat_point(g, blockIdx.x * blockDim.x + threadIdx.x, i, j)) return;
+// WAVELET_2D.h:738-778 (inverse d3)
+__device__ __forceinline__ void iwt_d3_pt(const Dims2& D, int i, int j, int d, const Wts& W, double* rho) {
     const long long p = gidx(D, i, j);
-    if (!mask[p]) { rho[p] = 0; return; }
     int is, js, k1, k2;
     pstart(i, d, D.Nx - 1, is, k1);
     pstart(j, d, D.Ny - 1, js, k2);
@@ -227,50 +190,22 @@ __global__ void k2_iwt_d3(Dims2 D, Lat g, int d, Wts W, double* __restrict__ rho
             v -= rho[gidx(D, is + c1 * d, js + c2 * d)] * W.w[c1][k1 + 1] * W.w[c2][k2 + 1];
     rho[p] = v;
 }
-
-// ------------------------------------------------------ interpolation ------
-__global__ void k2_interp_tmp(Dims2 D, const uint8_t* __restrict__ to, const uint8_t* __restrict__ from,
-                              uint8_t* __restrict__ tmp, const Scal2* sc) {
-    if (sc->skip) return;
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long n = (long long)D.P * (D.Ny + 2 * G);
-    if (t >= n) return;
-    const int i = int(t % D.P) - G, j = int(t / D.P) - G;
-    uint8_t v = to[t];
-    if (i >= 0 && i <= D.Nx - 1 && j >= 0 && j <= D.Ny - 1 && from[t]) v = 0;
-    tmp[t] = v;
-}
-
-__global__ void k2_interp_d12(Dims2 D, Lat g1, Lat g2, int d, Wts W, double* __restrict__ Q, const uint8_t* __restrict__ tmp,
-                              const uint8_t* __restrict__ from, const Scal2* sc) {
-    if (sc->skip) return;
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int n1 = g1.ni * g1.nj;
-    int i, j;
-    const bool d1 = t < n1;
-    if (d1) { if (!lat_point(g1, t, i, j)) return; }
-    else if (!lat_point(g2, t - n1, i, j)) return;
+// WAVELET_2D.h:808-905 (interpolation of momentum and energy, d1/d2 and d3)
+__device__ __forceinline__ void int_d12_pt(const Dims2& D, int kind, int i, int j, int d, const Wts& W, double* Q) {
     const long long p = gidx(D, i, j);
-    if (tmp[p] == 0 || from[p] == 1) return;
     int s, k;
-    if (d1) pstart(i, d, D.Nx - 1, s, k);
+    if (kind == LAT_D1) pstart(i, d, D.Nx - 1, s, k);
     else pstart(j, d, D.Ny - 1, s, k);
     for (int c4 = 1; c4 < 4; c4++) {
         double* q = Q + c4 * D.plane;
         double v = 0.0;
-        if (d1) for (int c = 0; c < 4; c++) v += q[gidx(D, s + c * d, j)] * W.w[c][k + 1];
+        if (kind == LAT_D1) for (int c = 0; c < 4; c++) v += q[gidx(D, s + c * d, j)] * W.w[c][k + 1];
         else for (int c = 0; c < 4; c++) v += q[gidx(D, i, s + c * d)] * W.w[c][k + 1];
         q[p] = v;
     }
 }
-
-__global__ void k2_interp_d3(Dims2 D, Lat g, int d, Wts W, double* __restrict__ Q, const uint8_t* __restrict__ tmp,
-                             const uint8_t* __restrict__ from, const Scal2* sc) {
-    if (sc->skip) return;
-    int i, j;
-    if (!lat_point(g, blockIdx.x * blockDim.x + threadIdx.x, i, j)) return;
+__device__ __forceinline__ void int_d3_pt(const Dims2& D, int i, int j, int d, const Wts& W, double* Q) {
     const long long p = gidx(D, i, j);
-    if (tmp[p] == 0 || from[p] == 1) return;
     int is, js, k1, k2;
     pstart(i, d, D.Nx - 1, is, k1);
     pstart(j, d, D.Ny - 1, js, k2);
@@ -286,55 +221,301 @@ __global__ void k2_interp_d3(Dims2 D, Lat g, int d, Wts W, double* __restrict__ 
     }
 }
 
-// --------------------------------------------------------- mask algebra ----
-__global__ void k2_check_d3(Dims2 D, Lat g, int d, uint8_t* __restrict__ mask, Scal2* sc) {
-    if (sc->skip) return;
-    int i, j;
-    if (!lat_point(g, blockIdx.x * blockDim.x + threadIdx.x, i, j)) return;
-    if (!mask[gidx(D, i, j)]) return;
-    int is, js, k1, k2;
-    pstart(i, d, D.Nx - 1, is, k1);
-    pstart(j, d, D.Ny - 1, js, k2);
-    for (int c = 0; c < 4; c++) mset(D, mask, is + c * d, j, sc);
-    for (int c = 0; c < 4; c++) mset(D, mask, i, js + c * d, sc);
-    for (int c2 = 0; c2 < 4; c2++)
-        for (int c1 = 0; c1 < 4; c1++) mset(D, mask, is + c1 * d, js + c2 * d, sc);
+// Dispatch of one lattice point of pass OP.  For the interpolation passes
+// `mask` is the work mask (tmp) and `aux` the old mask (mask_temp).
+template <int OP>
+__device__ __forceinline__ uint8_t op_active(const uint8_t* mask, const uint8_t* aux, long long p) {
+    if (OP == OP_INT_D12 || OP == OP_INT_D3) return (mask[p] != 0 && aux[p] == 0) ? 1 : 0;
+    return mask[p];
 }
+template <int OP>
+__device__ __forceinline__ void op_point(const Dims2& D, int kind, int i, int j, int d, uint8_t act, const Wts& W,
+                                         double tol, double* Q, uint8_t* mask) {
+    if (OP == OP_IWT_C || OP == OP_IWT_D12 || OP == OP_IWT_D3) {
+        if (!act) { Q[gidx(D, i, j)] = 0; return; }
+    } else if (!act) {
+        return;
+    }
+    if (OP == OP_FWT_D3) fwt_d3_pt(D, i, j, d, W, tol, Q, mask);
+    else if (OP == OP_FWT_D12) fwt_d12_pt(D, kind, i, j, d, W, tol, Q, mask);
+    else if (OP == OP_FWT_C) lift_c_pt<1>(D, i, j, d, W, Q);
+    else if (OP == OP_IWT_C) lift_c_pt<-1>(D, i, j, d, W, Q);
+    else if (OP == OP_IWT_D12) iwt_d12_pt(D, kind, i, j, d, W, Q);
+    else if (OP == OP_IWT_D3) iwt_d3_pt(D, i, j, d, W, Q);
+    else if (OP == OP_INT_D12) int_d12_pt(D, kind, i, j, d, W, Q);
+    else int_d3_pt(D, i, j, d, W, Q);
+}
+template <int OP>
+__device__ __forceinline__ int op_kind1() {
+    return (OP == OP_FWT_D3 || OP == OP_IWT_D3 || OP == OP_INT_D3) ? LAT_D3
+           : (OP == OP_FWT_C || OP == OP_IWT_C)                     ? LAT_C
+                                                                     : LAT_D1;
+}
+template <int OP>
+__device__ __forceinline__ bool op_two() { return OP == OP_FWT_D12 || OP == OP_IWT_D12 || OP == OP_INT_D12; }
 
-__global__ void k2_check_d12(Dims2 D, Lat g1, Lat g2, int d, uint8_t* __restrict__ mask, Scal2* sc) {
+// One lifting pass of one level: each thread takes K consecutive lattice points
+// of a row, reads their K mask bytes first, then works on the active ones.
+template <int OP, int K>
+__global__ void __launch_bounds__(256) k2_lat(Dims2 D, int lev, Wts W, double tol, double* __restrict__ Q,
+                                              uint8_t* __restrict__ mask, const uint8_t* __restrict__ aux,
+                                              const Scal2* sc) {
     if (sc->skip) return;
+    const int d = 1 << (D.L - 1 - lev);
+    int kind = op_kind1<OP>();
+    Lat g = lattice(D, lev, kind);
     int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int n1 = g1.ni * g1.nj;
-    int i, j;
-    const bool d1 = t < n1;
-    if (d1) { if (!lat_point(g1, t, i, j)) return; }
-    else if (!lat_point(g2, t - n1, i, j)) return;
-    if (!mask[gidx(D, i, j)]) return;
-    int s, k;
-    if (d1) {
-        pstart(i, d, D.Nx - 1, s, k);
-        for (int c = 0; c < 4; c++) mset(D, mask, s + c * d, j, sc);
-    } else {
-        pstart(j, d, D.Ny - 1, s, k);
-        for (int c = 0; c < 4; c++) mset(D, mask, i, s + c * d, sc);
+    if (op_two<OP>()) {
+        const int n1 = ((g.ni + K - 1) / K) * g.nj;
+        if (t >= n1) { t -= n1; kind = LAT_D2; g = lattice(D, lev, LAT_D2); }
+    }
+    const int segs = (g.ni + K - 1) / K;
+    if (t >= segs * g.nj) return;
+    const int b = t / segs, a = t % segs;
+    const int j = g.js + b * g.dy;
+    uint8_t act[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int ii = a * K + k;
+        act[k] = ii < g.ni ? op_active<OP>(mask, aux, gidx(D, g.is + ii * g.dx, j)) : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int ii = a * K + k;
+        if (ii < g.ni) op_point<OP>(D, kind, g.is + ii * g.dx, j, d, act[k], W, tol, Q, mask);
     }
 }
 
-// BandingMask (PROBLEM_SHOCK_VORTEX_INTERACTION.h:205-213, 2-wide band) on mask0,
-// then the AddAdjacentZone snapshot.
+// The same pass over every point of one lattice, by one workgroup (tail levels).
+template <int OP>
+__device__ void wg_pass(const Dims2& D, int lev, const Wts& W, double tol, double* Q, uint8_t* mask, const uint8_t* aux) {
+    const int d = 1 << (D.L - 1 - lev);
+    const int nk = op_two<OP>() ? 2 : 1;
+    for (int q = 0; q < nk; q++) {
+        const int kind = q ? LAT_D2 : op_kind1<OP>();
+        const Lat g = lattice(D, lev, kind);
+        const int n = g.ni * g.nj;
+        constexpr int B = 4;  // points per thread per round, masks loaded first
+        for (int t0 = threadIdx.x * B; t0 < n; t0 += blockDim.x * B) {
+            uint8_t act[B];
+#pragma unroll
+            for (int k = 0; k < B; k++) {
+                const int t = t0 + k;
+                act[k] = t < n ? op_active<OP>(mask, aux, gidx(D, g.is + (t % g.ni) * g.dx, g.js + (t / g.ni) * g.dy)) : 0;
+            }
+#pragma unroll
+            for (int k = 0; k < B; k++) {
+                const int t = t0 + k;
+                if (t < n) op_point<OP>(D, kind, g.is + (t % g.ni) * g.dx, g.js + (t / g.ni) * g.dy, d, act[k], W, tol, Q, mask);
+            }
+        }
+    }
+}
+
+// Tail of a level cascade: levels 0..T of the forward transform (finest of
+// them first), the inverse transform or the interpolation (coarsest first),
+// in one workgroup with a barrier between passes.
+enum { TAIL_FWT = 0, TAIL_IWT = 1, TAIL_INT = 2 };
+template <int WHICH>
+__global__ void __launch_bounds__(1024) k2_tail(Dims2 D, int T, Wts W, double tol, double* __restrict__ Q,
+                                                uint8_t* __restrict__ mask, const uint8_t* __restrict__ aux,
+                                                const Scal2* sc) {
+    if (sc->skip) return;
+    if (WHICH == TAIL_FWT) {
+        for (int lev = T; lev >= 0; lev--) {
+            wg_pass<OP_FWT_D3>(D, lev, W, tol, Q, mask, aux);
+            __syncthreads();
+            wg_pass<OP_FWT_D12>(D, lev, W, tol, Q, mask, aux);
+            __syncthreads();
+            wg_pass<OP_FWT_C>(D, lev, W, tol, Q, mask, aux);
+            __syncthreads();
+        }
+    } else if (WHICH == TAIL_IWT) {
+        for (int lev = 0; lev <= T; lev++) {
+            wg_pass<OP_IWT_C>(D, lev, W, tol, Q, mask, aux);
+            __syncthreads();
+            wg_pass<OP_IWT_D12>(D, lev, W, tol, Q, mask, aux);
+            __syncthreads();
+            wg_pass<OP_IWT_D3>(D, lev, W, tol, Q, mask, aux);
+            __syncthreads();
+        }
+    } else {
+        for (int lev = 0; lev <= T; lev++) {
+            wg_pass<OP_INT_D12>(D, lev, W, tol, Q, mask, aux);
+            __syncthreads();
+            wg_pass<OP_INT_D3>(D, lev, W, tol, Q, mask, aux);
+            __syncthreads();
+        }
+    }
+}
+
+// ----------------------------------------- reconstruction check (fused) ----
+// AWCM_MANAGER_2D::CheckReconstruction (AWCM_MANAGER_2D.h:63-158), one level.
+// A cell (I, J) of level lev holds the d3 point (h+I m, h+J m), the d1 point
+// (h+I m, (J+1) m) and the d2 point ((I+1) m, h+J m).  The reference runs the
+// d3 pass before the d1/d2 passes because a d3 stencil activates d1/d2 points
+// of the same level; here each d1/d2 point instead gathers that activation
+// from the (at most six) d3 points whose clamped stencil reaches it, so all
+// three sub-bands of a level run in one pass.  Every write is a 1 and d3
+// points are neither written nor read across levels within the pass.
+__device__ __forceinline__ int clamp_start(int c, int d, int lim) {
+    int s = c - d - d / 2;
+    if (s < 0) s = 0;
+    else if (c + d + d / 2 > lim) s = lim - 3 * d;
+    return s;
+}
+
+__device__ __forceinline__ void check_cell(const Dims2& D, int lev, int I, int J, uint8_t* mask, Scal2* sc) {
+    const int m = 1 << (D.L - 1 - lev), h = m >> 1;
+    const int nI = (D.Nx - 1) / m, nJ = (D.Ny - 1) / m;
+    const int i3 = h + I * m, j3 = h + J * m;
+    const bool has1 = J + 1 <= nJ - 1, has2 = I + 1 <= nI - 1;
+    const int j1 = (J + 1) * m, i2 = (I + 1) * m;
+    // every mask byte the cell needs is loaded up front (independent loads)
+    uint8_t c1[6], c2[6];
+    const uint8_t a3 = mask[gidx(D, i3, j3)];
+    const uint8_t o1 = has1 ? mask[gidx(D, i3, j1)] : 0;
+    const uint8_t o2 = has2 ? mask[gidx(D, i2, j3)] : 0;
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+        const int o = 2 * q - 5;
+        const int jj = j1 + o * h, ii = i2 + o * h;
+        c1[q] = (has1 && jj >= h && jj <= D.Ny - 1 - h) ? mask[gidx(D, i3, jj)] : 0;
+        c2[q] = (has2 && ii >= h && ii <= D.Nx - 1 - h) ? mask[gidx(D, ii, j3)] : 0;
+    }
+    // d1 / d2 points activated by the clamped stencil of an active d3 point
+    bool a1 = o1 != 0, a2 = o2 != 0;
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+        const int o = 2 * q - 5;
+        if (c1[q]) {
+            const int s = clamp_start(j1 + o * h, m, D.Ny - 1);
+            if (j1 >= s && j1 <= s + 3 * m) a1 = true;
+        }
+        if (c2[q]) {
+            const int s = clamp_start(i2 + o * h, m, D.Nx - 1);
+            if (i2 >= s && i2 <= s + 3 * m) a2 = true;
+        }
+    }
+    if (a3) {
+        const int is = clamp_start(i3, m, D.Nx - 1), js = clamp_start(j3, m, D.Ny - 1);
+        for (int c = 0; c < 4; c++) mset(D, mask, is + c * m, j3, sc);
+        for (int c = 0; c < 4; c++) mset(D, mask, i3, js + c * m, sc);
+        for (int cc2 = 0; cc2 < 4; cc2++)
+            for (int cc1 = 0; cc1 < 4; cc1++) mset(D, mask, is + cc1 * m, js + cc2 * m, sc);
+    }
+    if (a1) {
+        const int is = clamp_start(i3, m, D.Nx - 1);
+        for (int c = 0; c < 4; c++) mset(D, mask, is + c * m, j1, sc);
+    }
+    if (a2) {
+        const int js = clamp_start(j3, m, D.Ny - 1);
+        for (int c = 0; c < 4; c++) mset(D, mask, i2, js + c * m, sc);
+    }
+}
+
+template <int K>
+__global__ void __launch_bounds__(256) k2_check(Dims2 D, int lev, uint8_t* __restrict__ mask, Scal2* sc) {
+    if (sc->skip) return;
+    const int m = 1 << (D.L - 1 - lev);
+    const int nI = (D.Nx - 1) / m, nJ = (D.Ny - 1) / m;
+    const int segs = (nI + K - 1) / K;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= segs * nJ) return;
+    const int J = t / segs, I0 = (t % segs) * K;
+#pragma unroll
+    for (int k = 0; k < K; k++)
+        if (I0 + k < nI) check_cell(D, lev, I0 + k, J, mask, sc);
+}
+
+// Levels T..0 of the check in one workgroup.
+__global__ void __launch_bounds__(1024) k2_check_tail(Dims2 D, int T, uint8_t* __restrict__ mask, Scal2* sc) {
+    if (sc->skip) return;
+    for (int lev = T; lev >= 0; lev--) {
+        const int m = 1 << (D.L - 1 - lev);
+        const int nI = (D.Nx - 1) / m, nJ = (D.Ny - 1) / m;
+        for (int t = threadIdx.x; t < nI * nJ; t += blockDim.x) check_cell(D, lev, t % nI, t / nI, mask, sc);
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------ dense mask passes --
+// Dense passes take 16 consecutive points of a row per thread: device rows are
+// 16-byte aligned (pitch P is a multiple of 16), so each mask row segment is
+// one 16-byte load.  Thread t: row jr = t / (P/16) (ghost rows included),
+// segment c: ghosted columns 16c .. 16c+15 (i = 16c - 3 + k).
+struct Seg {
+    long long p0;  // linear index of the segment's first byte
+    int i0, j;     // i of the first byte, row j
+};
+__device__ __forceinline__ bool seg_of(const Dims2& D, long long t, Seg& s) {
+    const int spr = D.P / 16;
+    const long long nseg = (long long)spr * (D.Ny + 2 * G);
+    if (t >= nseg) return false;
+    const int jr = int(t / spr), c = int(t % spr);
+    s.p0 = (long long)jr * D.P + 16 * c;
+    s.i0 = 16 * c - G;
+    s.j = jr - G;
+    return true;
+}
+union V16 {
+    uint4 v;
+    uint8_t b[16];
+};
+__device__ __forceinline__ V16 ld16(const uint8_t* p) { V16 r; r.v = *reinterpret_cast<const uint4*>(p); return r; }
+__device__ __forceinline__ void st16(uint8_t* p, const V16& x) { *reinterpret_cast<uint4*>(p) = x.v; }
+
+// mask_temp = mask0 (whole ghosted buffer) and the step's scalar resets
+__global__ void k2_begin_copy(Dims2 D, const uint8_t* __restrict__ src, uint8_t* __restrict__ dst, double* alpha,
+                              Scal2* sc, int force) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) {
+        sc->skip = (!force && !(sc->t < sc->t_final)) ? 1 : 0;
+        if (!sc->skip) { sc->residue_full = sc->dirty; sc->dirty = 0; }
+        sc->vmax = 0ull;
+        for (int k = 0; k < 3; k++) { sc->cntW[k] = 0; sc->cntC[k] = 0; }
+    }
+    const int na = 3 * (D.Nx + D.Ny);
+    if (t < na) alpha[t] = 0.0;
+    Seg s;
+    if (seg_of(D, t, s)) st16(dst + s.p0, ld16(src + s.p0));
+}
+
+// BandingMask (2-wide band) on mask0, then the AddAdjacentZone snapshot
 __global__ void k2_band_snap(Dims2 D, uint8_t* __restrict__ m0, uint8_t* __restrict__ snap, const Scal2* sc) {
     if (sc->skip) return;
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long n = (long long)D.P * (D.Ny + 2 * G);
-    if (t >= n) return;
-    const int i = int(t % D.P) - G, j = int(t / D.P) - G;
-    uint8_t v = m0[t];
-    if (i >= 0 && i <= D.Nx - 1 && j >= 0 && j <= D.Ny - 1 &&
-        !(2 <= i && i <= D.Nx - 1 - 2 && j >= 2 && j <= D.Ny - 1 - 2)) {
-        v = 1;
-        m0[t] = 1;
+    Seg s;
+    if (!seg_of(D, (long long)blockIdx.x * blockDim.x + threadIdx.x, s)) return;
+    V16 v = ld16(m0 + s.p0);
+    const bool row_band = s.j >= 0 && s.j <= D.Ny - 1 && (s.j < 2 || s.j > D.Ny - 1 - 2);
+    bool changed = false;
+    if (s.j >= 0 && s.j <= D.Ny - 1) {
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            const int i = s.i0 + k;
+            if (i >= 0 && i <= D.Nx - 1 && (row_band || i < 2 || i > D.Nx - 1 - 2) && !v.b[k]) { v.b[k] = 1; changed = true; }
+        }
     }
-    snap[t] = v;
+    if (changed) st16(m0 + s.p0, v);
+    st16(snap + s.p0, v);
+}
+
+// tmp = mask3 minus mask_temp on the grid (ghosts from mask3)
+__global__ void k2_interp_tmp(Dims2 D, const uint8_t* __restrict__ to, const uint8_t* __restrict__ from,
+                              uint8_t* __restrict__ tmp, const Scal2* sc) {
+    if (sc->skip) return;
+    Seg s;
+    if (!seg_of(D, (long long)blockIdx.x * blockDim.x + threadIdx.x, s)) return;
+    V16 v = ld16(to + s.p0);
+    if (s.j >= 0 && s.j <= D.Ny - 1) {
+        const V16 f = ld16(from + s.p0);
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            const int i = s.i0 + k;
+            if (i >= 0 && i <= D.Nx - 1 && f.b[k]) v.b[k] = 0;
+        }
+    }
+    st16(tmp + s.p0, v);
 }
 
 // Sub-band of an interior point: level lev (0..L-2) and mu (1: odd i, 2: odd j,
@@ -350,8 +531,8 @@ __device__ __forceinline__ int classify(int i, int j, const Dims2& D, int& mu) {
     return lev;
 }
 
-// AWCM_MANAGER_2D::AddAdjacentZone (AWCM_MANAGER_2D.h:14-61), all levels at once
-// from the snapshot.
+// AWCM_MANAGER_2D::AddAdjacentZone (AWCM_MANAGER_2D.h:14-61), every level at
+// once from the snapshot; one point per thread.
 __global__ void k2_adjacent(Dims2 D, const uint8_t* __restrict__ snap, uint8_t* __restrict__ mask, Scal2* sc) {
     if (sc->skip) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
@@ -372,21 +553,23 @@ __global__ void k2_adjacent(Dims2 D, const uint8_t* __restrict__ snap, uint8_t* 
     }
 }
 
-// mask_{k+1} = mask_k (ghosts included) and ComputeLevel(level_k, mask_k)
-// (AWCM_MANAGER_2D.h:215-241).  A point with no active neighbour keeps its
-// previous level, as in the reference.
-__global__ void k2_level_copy(Dims2 D, const uint8_t* __restrict__ src, uint8_t* __restrict__ dst,
-                              uint8_t* __restrict__ lvl, Scal2* sc) {
+// mask_{k+1} = mask_k (16-byte copy per thread, ghosts included)
+__global__ void k2_copy16(Dims2 D, const uint8_t* __restrict__ src, uint8_t* __restrict__ dst, const Scal2* sc) {
     if (sc->skip) return;
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long n = (long long)D.P * (D.Ny + 2 * G);
-    if (t >= n) return;
-    const uint8_t v = src[t];
-    dst[t] = v;
-    const int i = int(t % D.P) - G, j = int(t / D.P) - G;
-    if (i < 0 || i > D.Nx - 1 || j < 0 || j > D.Ny - 1) return;
+    Seg s;
+    if (seg_of(D, (long long)blockIdx.x * blockDim.x + threadIdx.x, s)) st16(dst + s.p0, ld16(src + s.p0));
+}
+
+// ComputeLevel(level_k, mask_k) (AWCM_MANAGER_2D.h:215-241), one point per
+// thread.  A point with no active neighbour keeps its previous level, as in
+// the reference.
+__global__ void k2_level(Dims2 D, const uint8_t* __restrict__ src, uint8_t* __restrict__ lvl, Scal2* sc) {
+    if (sc->skip) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= D.Nx) return;
     const long long li = (long long)i + (long long)D.Nx * j;
-    if (!v) { lvl[li] = 0; return; }
+    const long long t = gidx(D, i, j);
+    if (!src[t]) { lvl[li] = 0; return; }
     int d = 1;
     for (int lev = D.maxlev; lev >= 0; lev--) {
         if (i - d < -G || i + d > D.Nx - 1 + G || j - d < -G || j + d > D.Ny - 1 + G) { sc->flag |= 1; return; }
@@ -427,9 +610,10 @@ __global__ void k2_addpoint(Dims2 D, const uint8_t* __restrict__ snap, const uin
 
 // --------------------------------------------------------- compaction ------
 // Stage work lists: for k = 0..2 the points of mask_k at the finest level
-// (WENO-Z faces) and at coarser levels (central differences).  One atomic per
-// list per block; list order is irrelevant to the result.
-constexpr int kCompactThreads = 256, kCompactPer = 8;
+// (WENO-Z faces) and at coarser levels (central differences).  A thread takes
+// a 16-point row segment, so list entries come out in row order within a
+// thread; one atomic per list per block.  List order does not affect results.
+constexpr int kCompactThreads = 256;
 
 __global__ void __launch_bounds__(kCompactThreads)
 k2_compact(Dims2 D, const uint8_t* m0, const uint8_t* m1, const uint8_t* m2, const uint8_t* l0, const uint8_t* l1,
@@ -440,51 +624,54 @@ k2_compact(Dims2 D, const uint8_t* m0, const uint8_t* m1, const uint8_t* m2, con
     const long long N = (long long)D.Nx * D.Ny;
     const uint8_t* mk[3] = {m0, m1, m2};
     const uint8_t* lk[3] = {l0, l1, l2};
-    uint32_t flags[kCompactPer];
     int cnt[6] = {0, 0, 0, 0, 0, 0};
-    const long long b0 = (long long)blockIdx.x * kCompactThreads * kCompactPer;
-    for (int it = 0; it < kCompactPer; it++) {
-        const long long li = b0 + (long long)it * kCompactThreads + threadIdx.x;
-        uint32_t f = 0;
-        if (li < N) {
-            const int i = int(li % D.Nx), j = int(li / D.Nx);
-            const long long p = gidx(D, i, j);
-            for (int k = 0; k < 3; k++) {
-                if (!mk[k][p]) continue;
-                const int l = (lk[k][li] == D.maxlev) ? 0 : 1;
-                f |= 1u << (2 * k + l);
-                cnt[2 * k + l]++;
+    uint32_t fl[16];
+    Seg s;
+    const bool ok = seg_of(D, (long long)blockIdx.x * blockDim.x + threadIdx.x, s) && s.j >= 0 && s.j <= D.Ny - 1;
+    if (ok) {
+        V16 v[3];
+        for (int k = 0; k < 3; k++) v[k] = ld16(mk[k] + s.p0);
+        for (int e = 0; e < 16; e++) {
+            const int i = s.i0 + e;
+            uint32_t f = 0;
+            if (i >= 0 && i <= D.Nx - 1) {
+                const long long li = (long long)i + (long long)D.Nx * s.j;
+                for (int k = 0; k < 3; k++) {
+                    if (!v[k].b[e]) continue;
+                    const int l = (lk[k][li] == D.maxlev) ? 0 : 1;
+                    f |= 1u << (2 * k + l);
+                    cnt[2 * k + l]++;
+                }
             }
+            fl[e] = f;
         }
-        flags[it] = f;
     }
-    // block exclusive scan of the six counts
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     int incl[6];
     for (int q = 0; q < 6; q++) {
-        int v = cnt[q];
+        int x = cnt[q];
         for (int o = 1; o < 64; o <<= 1) {
-            const int y = __shfl_up(v, o, 64);
-            if (lane >= o) v += y;
+            const int y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
         }
-        incl[q] = v;
-        if (lane == 63) wtot[q][wid] = v;
+        incl[q] = x;
+        if (lane == 63) wtot[q][wid] = x;
     }
     __syncthreads();
     if (threadIdx.x < 6) {
-        int s = 0;
-        for (int w = 0; w < kCompactThreads / 64; w++) { const int x = wtot[threadIdx.x][w]; wtot[threadIdx.x][w] = s; s += x; }
+        int acc = 0;
+        for (int w = 0; w < kCompactThreads / 64; w++) { const int x = wtot[threadIdx.x][w]; wtot[threadIdx.x][w] = acc; acc += x; }
         int* ctr = (threadIdx.x & 1) ? &sc->cntC[threadIdx.x >> 1] : &sc->cntW[threadIdx.x >> 1];
-        base[threadIdx.x] = s ? atomicAdd(ctr, s) : 0;
+        base[threadIdx.x] = acc ? atomicAdd(ctr, acc) : 0;
     }
     __syncthreads();
+    if (!ok) return;
     int pos[6];
     for (int q = 0; q < 6; q++) pos[q] = base[q] + wtot[q][wid] + incl[q] - cnt[q];
-    for (int it = 0; it < kCompactPer; it++) {
-        const uint32_t f = flags[it];
+    for (int e = 0; e < 16; e++) {
+        const uint32_t f = fl[e];
         if (!f) continue;
-        const long long li = b0 + (long long)it * kCompactThreads + threadIdx.x;
-        const uint32_t packed = (uint32_t(li / D.Nx) << 16) | uint32_t(li % D.Nx);
+        const uint32_t packed = (uint32_t(s.j) << 16) | uint32_t(s.i0 + e);
         for (int q = 0; q < 6; q++)
             if (f & (1u << q)) lists[(long long)q * N + pos[q]++] = packed;
     }
@@ -493,30 +680,49 @@ k2_compact(Dims2 D, const uint8_t* m0, const uint8_t* m1, const uint8_t* m2, con
 // ---------------------------------------------------- CFL and LF alphas ----
 // alpha_x(j) = max_i SRx, alpha_y(i) = max_j SRy over `mask` (exact maxima), and
 // for stage 1 also max(SRx + SRy) for the CFL step (main_2d_euler(dmach).cpp:61-104).
-constexpr int kAlphaRows = 32;
+constexpr int kAlphaRows = 32;  // rows per block; each thread takes kAlphaRows/4 of them
 
 __global__ void __launch_bounds__(256) k2_alpha(Dims2 D, const uint8_t* __restrict__ mask, const double* __restrict__ Q,
                                                 double* ax, double* ay, double gamma, int with_vmax, Scal2* sc) {
     if (sc->skip) return;
+    constexpr int R = kAlphaRows / 4;
     __shared__ double colr[4][64];
     __shared__ double vr[4];
     const int i = blockIdx.x * 64 + threadIdx.x;
     const int j0 = blockIdx.y * kAlphaRows + threadIdx.y;
-    double cm = 0, vm = 0;
-    for (int r = 0; r < kAlphaRows / 4; r++) {
+    // all mask loads, then all state loads, then the arithmetic
+    uint8_t act[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
         const int j = j0 + 4 * r;
-        double sx = 0;
-        if (i < D.Nx && j < D.Ny) {
-            const long long p = gidx(D, i, j);
-            if (mask[p]) {
-                const double q[4] = {Q[p], Q[p + D.plane], Q[p + 2 * D.plane], Q[p + 3 * D.plane]};
-                sx = spec_rad2x(gamma, q);
-                const double sy = spec_rad2y(gamma, q);
-                cm = t_max(cm, sy);
-                vm = t_max(vm, sx + sy);
-            }
+        act[r] = (i < D.Nx && j < D.Ny) ? mask[gidx(D, i, j)] : 0;
+    }
+    double q[R][4];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        if (act[r]) {
+            const long long p = gidx(D, i, j0 + 4 * r);
+            q[r][0] = Q[p]; q[r][1] = Q[p + D.plane]; q[r][2] = Q[p + 2 * D.plane]; q[r][3] = Q[p + 3 * D.plane];
+        } else {
+            q[r][0] = 1; q[r][1] = 0; q[r][2] = 0; q[r][3] = 1;
         }
-        const double rm = wave_max(sx);
+    }
+    double cm = 0, vm = 0, sxr[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        double sx = 0;
+        if (act[r]) {
+            sx = spec_rad2x(gamma, q[r]);
+            const double sy = spec_rad2y(gamma, q[r]);
+            cm = t_max(cm, sy);
+            vm = t_max(vm, sx + sy);
+        }
+        sxr[r] = sx;
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int j = j0 + 4 * r;
+        const double rm = wave_max(sxr[r]);
         if (threadIdx.x == 0 && j < D.Ny && !(rm <= 0)) atomic_max_pos((unsigned long long*)&ax[j], rm);
     }
     colr[threadIdx.y][threadIdx.x] = cm;
@@ -705,9 +911,14 @@ __global__ void __launch_bounds__(256) k2_stage_coarse(Dims2 D, const uint32_t* 
     }
 }
 
-// RK combine on mask0 and RemoveResidueEuler on the interior; block 0 advances t.
-__global__ void k2_combine(Dims2 D, const uint8_t* __restrict__ m0, double* __restrict__ Q,
-                           const double* __restrict__ Q0, int do_combine, Scal2* sc) {
+// RK combine on mask0 and RemoveResidueEuler on the interior; thread 0 advances t.
+// Between steps Q is zero at every interior point outside mask0, and within a
+// step only points of mask3 (inverse transform, interpolation) or of the
+// previous mask0 (mask_temp) can become non-zero, so the residue pass only
+// writes there -- unless the host replaced Q or mask0 (residue_full).
+__global__ void k2_combine(Dims2 D, const uint8_t* __restrict__ m0, const uint8_t* __restrict__ m3,
+                           const uint8_t* __restrict__ mtmp, double* __restrict__ Q, const double* __restrict__ Q0,
+                           int do_combine, Scal2* sc) {
     if (sc->skip) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
     if (do_combine && i == 0 && j == 0) {
@@ -722,25 +933,29 @@ __global__ void k2_combine(Dims2 D, const uint8_t* __restrict__ m0, double* __re
     const long long p = gidx(D, i, j);
     if (m0[p]) {
         if (!do_combine) return;
-        for (int k = 0; k < 4; k++) {
-            const long long o = p + k * D.plane;
+        for (int q = 0; q < 4; q++) {
+            const long long o = p + q * D.plane;
             Q[o] = (Q[o] + 2.0 * Q0[o]) / 3.0;
         }
-    } else if (i >= 1 && i <= D.Nx - 2 && j >= 1 && j <= D.Ny - 2) {
-        for (int k = 0; k < 4; k++) Q[p + k * D.plane] = 0.0;
+    } else if (i >= 1 && i <= D.Nx - 2 && j >= 1 && j <= D.Ny - 2 && (sc->residue_full || m3[p] || mtmp[p])) {
+        for (int q = 0; q < 4; q++) Q[p + q * D.plane] = 0.0;
     }
 }
 
-// AoS (reference layout) <-> SoA planes
-__global__ void k2_aos_to_soa(const double* __restrict__ aos, double* __restrict__ soa, long long n, long long plane) {
+// AoS, reference pitch (Nx+6) <-> SoA planes, device pitch P
+__global__ void k2_aos_to_soa(const double* __restrict__ aos, double* __restrict__ soa, long long n, int PR, int P,
+                              long long plane) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    for (int k = 0; k < 4; k++) soa[t + k * plane] = aos[t * 4 + k];
+    const long long d = (t % PR) + (long long)P * (t / PR);
+    for (int k = 0; k < 4; k++) soa[d + k * plane] = aos[t * 4 + k];
 }
-__global__ void k2_soa_to_aos(const double* __restrict__ soa, double* __restrict__ aos, long long n, long long plane) {
+__global__ void k2_soa_to_aos(const double* __restrict__ soa, double* __restrict__ aos, long long n, int PR, int P,
+                              long long plane) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    for (int k = 0; k < 4; k++) aos[t * 4 + k] = soa[t + k * plane];
+    const long long d = (t % PR) + (long long)P * (t / PR);
+    for (int k = 0; k < 4; k++) aos[t * 4 + k] = soa[d + k * plane];
 }
 __global__ void k2_widen_levels(const uint8_t* __restrict__ src, int32_t* __restrict__ dst, long long n) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -748,6 +963,20 @@ __global__ void k2_widen_levels(const uint8_t* __restrict__ src, int32_t* __rest
 }
 
 inline unsigned nblk(long long n, int b) { return unsigned((n + b - 1) / b); }
+
+// Host mirror of lattice(): thread counts of the lattice kernels.
+Lat lat_h(const Dims2& D, int lev, int kind) {
+    const int m = 1 << (D.L - 1 - lev), h = m >> 1;
+    Lat g;
+    g.dx = g.dy = m;
+    if (kind == LAT_D1) { g.is = h; g.ie = D.Nx - 1 - h; g.js = m; g.je = D.Ny - 1 - m; }
+    else if (kind == LAT_D2) { g.is = m; g.ie = D.Nx - 1 - m; g.js = h; g.je = D.Ny - 1 - h; }
+    else if (kind == LAT_D3) { g.is = h; g.ie = D.Nx - 1 - h; g.js = h; g.je = D.Ny - 1 - h; }
+    else { g.is = m; g.ie = D.Nx - 1 - m; g.js = m; g.je = D.Ny - 1 - m; }
+    g.ni = (g.ie - g.is) / m + 1;
+    g.nj = (g.je - g.js) / m + 1;
+    return g;
+}
 
 Lat mklat(int is, int js, int ie, int je, int ires, int jres) {
     Lat g;
@@ -791,7 +1020,12 @@ int Solver2D::init(int L, int nxc_, int nyc_, double g, double dx_, double x0_, 
     D.maxlev = L - 1;
     D.Nx = (nxc - 1) * (1 << (L - 1)) + 1;
     D.Ny = (nyc - 1) * (1 << (L - 1)) + 1;
-    D.P = D.Nx + 2 * G;
+    // Row pitch padded to 16 points so every mask row is 16-byte aligned and
+    // every state row 128-byte aligned.  No stencil of a closed mask reaches past
+    // the ghost band, so the reference's wrap-around pitch (Nx+6) need not be kept
+    // on the device; the C ABI converts at upload/download.
+    D.PR = D.Nx + 2 * G;
+    D.P = (D.PR + 15) / 16 * 16;
     D.plane = (long long)D.P * (D.Ny + 2 * G);
     // MRA_GRID_INT_2D::Initialize (MRA_GRID_UNIFORM_2D.h:87-134)
     const int Nx = D.Nx, Ny = D.Ny;
@@ -806,6 +1040,17 @@ int Solver2D::init(int L, int nxc_, int nyc_, double g, double dx_, double x0_, 
         sub[lev][1] = mklat(m, int(0.5 * m), Nx - m - 1, int(Ny - 0.5 * m - 1), int(Nx / m - 1), int(Ny / m));
         sub[lev][2] = mklat(int(0.5 * m), int(0.5 * m), int(Nx - 0.5 * m - 1), int(Ny - 0.5 * m - 1), int(Nx / m), int(Ny / m));
     }
+    // the closed-form lattices used by the kernels equal the reference's ranges
+    for (int lev = 0; lev < L - 1; lev++) {
+        const Lat r[4] = {sub[lev][0], sub[lev][1], sub[lev][2], grd[lev]};
+        for (int k = 0; k < 4; k++) {
+            const Lat c = lat_h(D, lev, k);
+            if (c.is != r[k].is || c.ie != r[k].ie || c.js != r[k].js || c.je != r[k].je || c.dx != r[k].dx ||
+                c.dy != r[k].dy || c.ni != r[k].ni || c.nj != r[k].nj)
+                return AWCM_EINVAL;
+        }
+    }
+    set_tail(0);
     wavelet_weights(W.w);
     if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) return AWCM_EDEVICE;
     const size_t pb = sizeof(double) * 4 * size_t(D.plane);
@@ -827,6 +1072,7 @@ int Solver2D::init(int L, int nxc_, int nyc_, double g, double dx_, double x0_, 
     Scal2 s0;
     std::memset(&s0, 0, sizeof(s0));
     s0.cnt = 1;
+    s0.dirty = 1;
     if (hipMemcpy(sc, &s0, sizeof(s0), hipMemcpyHostToDevice) != hipSuccess) return AWCM_EDEVICE;
     int dev = 0, ncu = 256;
     hipGetDevice(&dev);
@@ -836,18 +1082,27 @@ int Solver2D::init(int L, int nxc_, int nyc_, double g, double dx_, double x0_, 
     return hipDeviceSynchronize() == hipSuccess ? AWCM_OK : AWCM_EDEVICE;
 }
 
+int Solver2D::mark_dirty() {
+    Scal2 s;
+    if (hipStreamSynchronize(stream) != hipSuccess) return AWCM_EDEVICE;
+    if (hipMemcpy(&s, sc, sizeof(s), hipMemcpyDeviceToHost) != hipSuccess) return AWCM_EDEVICE;
+    s.dirty = 1;
+    return hipMemcpy(sc, &s, sizeof(s), hipMemcpyHostToDevice) == hipSuccess ? AWCM_OK : AWCM_EDEVICE;
+}
+
 int Solver2D::upload_q(const double* q) {
-    const size_t n = size_t(D.plane);
+    const size_t n = size_t(D.PR) * (D.Ny + 2 * G);  // points in the reference layout
     if (hipMemcpyAsync(Q2, q, sizeof(double) * 4 * n, hipMemcpyHostToDevice, stream) != hipSuccess) return AWCM_EDEVICE;
-    hipLaunchKernelGGL(k2_aos_to_soa, dim3(nblk(n, 256)), dim3(256), 0, stream, Q2, Q, (long long)n, D.plane);
-    return hipStreamSynchronize(stream) == hipSuccess ? AWCM_OK : AWCM_EDEVICE;
+    hipLaunchKernelGGL(k2_aos_to_soa, dim3(nblk(n, 256)), dim3(256), 0, stream, Q2, Q, (long long)n, D.PR, D.P, D.plane);
+    if (hipStreamSynchronize(stream) != hipSuccess) return AWCM_EDEVICE;
+    return mark_dirty();
 }
 
 int Solver2D::download_q(double* q) {
-    const size_t n = size_t(D.plane);
+    const size_t n = size_t(D.PR) * (D.Ny + 2 * G);
     double* stage = nullptr;
     if (hipMalloc(&stage, sizeof(double) * 4 * n) != hipSuccess) return AWCM_ENOMEM;
-    hipLaunchKernelGGL(k2_soa_to_aos, dim3(nblk(n, 256)), dim3(256), 0, stream, Q, stage, (long long)n, D.plane);
+    hipLaunchKernelGGL(k2_soa_to_aos, dim3(nblk(n, 256)), dim3(256), 0, stream, Q, stage, (long long)n, D.PR, D.P, D.plane);
     int st = hipMemcpyAsync(q, stage, sizeof(double) * 4 * n, hipMemcpyDeviceToHost, stream) == hipSuccess ? AWCM_OK : AWCM_EDEVICE;
     if (hipStreamSynchronize(stream) != hipSuccess) st = AWCM_EDEVICE;
     hipFree(stage);
@@ -856,16 +1111,18 @@ int Solver2D::download_q(double* q) {
 
 int Solver2D::set_mask0(const uint8_t* m0) {
     if (hipStreamSynchronize(stream) != hipSuccess) return AWCM_EDEVICE;
-    return hipMemcpy(m[0], m0, size_t(D.plane), hipMemcpyHostToDevice) == hipSuccess ? AWCM_OK : AWCM_EDEVICE;
+    if (hipMemcpy2D(m[0], D.P, m0, D.PR, D.PR, D.Ny + 2 * G, hipMemcpyHostToDevice) != hipSuccess) return AWCM_EDEVICE;
+    return mark_dirty();
 }
 
 int Solver2D::get_masks(uint8_t* masks, int32_t* levels) {
     if (hipStreamSynchronize(stream) != hipSuccess) return AWCM_EDEVICE;
-    const size_t mb = size_t(D.plane), nl = size_t(D.Nx) * D.Ny;
+    const size_t mb = size_t(D.PR) * (D.Ny + 2 * G), nl = size_t(D.Nx) * D.Ny;
     if (masks) {
-        if (hipMemcpy(masks, mt, mb, hipMemcpyDeviceToHost) != hipSuccess) return AWCM_EDEVICE;
-        for (int k = 0; k < 4; k++)
-            if (hipMemcpy(masks + (k + 1) * mb, m[k], mb, hipMemcpyDeviceToHost) != hipSuccess) return AWCM_EDEVICE;
+        const uint8_t* src[5] = {mt, m[0], m[1], m[2], m[3]};
+        for (int k = 0; k < 5; k++)
+            if (hipMemcpy2D(masks + k * mb, D.PR, src[k], D.P, D.PR, D.Ny + 2 * G, hipMemcpyDeviceToHost) != hipSuccess)
+                return AWCM_EDEVICE;
     }
     if (levels) {
         int32_t* wide = nullptr;
@@ -936,83 +1193,106 @@ void Solver2D::prof_clear() {
     ev_used = 0;
 }
 
-int Solver2D::enqueue_begin(bool force) {
-    AWCM_LAUNCH(k2_begin, dim3(nblk(3 * (D.Nx + D.Ny), 256)), dim3(256), 0, stream, D, sc, alpha, force ? 1 : 0);
+constexpr int kLatK = 1;   // lattice points per thread
+constexpr int kCheckK = 1; // check cells per thread
+
+static long long lat_threads(const Dims2& D, int lev, int kind, bool two) {
+    const Lat g = lat_h(D, lev, kind);
+    long long n = (long long)((g.ni + kLatK - 1) / kLatK) * g.nj;
+    if (two) {
+        const Lat g2 = lat_h(D, lev, LAT_D2);
+        n += (long long)((g2.ni + kLatK - 1) / kLatK) * g2.nj;
+    }
+    return n;
+}
+
+int Solver2D::set_tail(int tail_cells) {
+    // coarsest levels whose cell count fits `tail_cells` run inside one workgroup
+    tail = -1;
+    for (int lev = 0; lev <= D.maxlev - 1; lev++) {
+        const int m = 1 << (D.L - 1 - lev);
+        const long long cells = (long long)((D.Nx - 1) / m) * ((D.Ny - 1) / m);
+        if (cells <= tail_cells) tail = lev;
+    }
+    if (gexec) { hipGraphExecDestroy(gexec); gexec = nullptr; }
+    if (graph) { hipGraphDestroy(graph); graph = nullptr; }
     return AWCM_OK;
 }
 
-// CheckReconstruction on `mask`: level by level, d3 then d1/d2.
+int Solver2D::enqueue_begin(bool force, bool copy_mt) {
+    const long long nseg = (long long)(D.P / 16) * (D.Ny + 2 * G);
+    const long long n = std::max<long long>(nseg, 3LL * (D.Nx + D.Ny));
+    AWCM_LAUNCH(k2_begin_copy, dim3(nblk(n, 256)), dim3(256), 0, stream, D, m[0], copy_mt ? mt : m[0], alpha, sc,
+                force ? 1 : 0);
+    return AWCM_OK;
+}
+
+// CheckReconstruction on `mask`: fused per-level passes, finest first, then the
+// tail levels in one workgroup.
 static int check_recon(Solver2D& s, uint8_t* mask) {
-    for (int lev = s.D.maxlev - 1; lev >= 0; lev--) {
-        const int d = 1 << (s.D.L - 1 - lev);
-        const Lat& g3 = s.sub[lev][2];
-        AWCM_LAUNCH(k2_check_d3, dim3(nblk((long long)g3.ni * g3.nj, 256)), dim3(256), 0, s.stream, s.D, g3, d, mask, s.sc);
-        const Lat& g1 = s.sub[lev][0];
-        const Lat& g2 = s.sub[lev][1];
-        AWCM_LAUNCH(k2_check_d12, dim3(nblk((long long)g1.ni * g1.nj + (long long)g2.ni * g2.nj, 256)), dim3(256), 0,
-                    s.stream, s.D, g1, g2, d, mask, s.sc);
+    for (int lev = s.D.maxlev - 1; lev > s.tail; lev--) {
+        const int m = 1 << (s.D.L - 1 - lev);
+        const long long nI = (s.D.Nx - 1) / m, nJ = (s.D.Ny - 1) / m;
+        AWCM_LAUNCH(k2_check<kCheckK>, dim3(nblk(((nI + kCheckK - 1) / kCheckK) * nJ, 256)), dim3(256), 0, s.stream, s.D,
+                    lev, mask, s.sc);
     }
+    if (s.tail >= 0) AWCM_LAUNCH(k2_check_tail, dim3(1), dim3(1024), 0, s.stream, s.D, s.tail, mask, s.sc);
     return AWCM_OK;
 }
 
-int Solver2D::enqueue_prepare() {
-    const long long nm = D.plane;
-    // mask_temp = mask0
-    AWCM_LAUNCH(k2_copy, dim3(nblk(nm, 256)), dim3(256), 0, stream, m[0], mt, nm, sc);
+#define AWCM_LAT(OP, lev, kind, two, Qp, mk, aux)                                                                 \
+    AWCM_LAUNCH((k2_lat<OP, kLatK>), dim3(nblk(lat_threads(D, lev, kind, two), 256)), dim3(256), 0, stream, D, lev, \
+                W, tol_host, Qp, mk, aux, sc)
+
+int Solver2D::enqueue_prepare(bool force) {
+    const long long nseg = (long long)(D.P / 16) * (D.Ny + 2 * G);
+    const unsigned bseg = nblk(nseg, 256);
+    // step scalars + mask_temp = mask0
+    if (int st = enqueue_begin(force, true)) return st;
     // forward transform, finest detail level first
-    for (int lev = D.maxlev - 1; lev >= 0; lev--) {
-        const int d = 1 << (D.L - 1 - lev);
-        const Lat& g3 = sub[lev][2];
-        const Lat& g1 = sub[lev][0];
-        const Lat& g2 = sub[lev][1];
-        const Lat& gc = grd[lev];
-        AWCM_LAUNCH(k2_fwt_d3, dim3(nblk((long long)g3.ni * g3.nj, 256)), dim3(256), 0, stream, D, g3, d, W, tol_host, Q, m[0], sc);
-        AWCM_LAUNCH(k2_fwt_d12, dim3(nblk((long long)g1.ni * g1.nj + (long long)g2.ni * g2.nj, 256)), dim3(256), 0, stream, D,
-                    g1, g2, d, W, tol_host, Q, m[0], sc);
-        AWCM_LAUNCH(k2_fwt_c, dim3(nblk((long long)gc.ni * gc.nj, 256)), dim3(256), 0, stream, D, gc, d, W, Q, m[0], sc);
+    for (int lev = D.maxlev - 1; lev > tail; lev--) {
+        AWCM_LAT(OP_FWT_D3, lev, LAT_D3, false, Q, m[0], nullptr);
+        AWCM_LAT(OP_FWT_D12, lev, LAT_D1, true, Q, m[0], nullptr);
+        AWCM_LAT(OP_FWT_C, lev, LAT_C, false, Q, m[0], nullptr);
     }
-    AWCM_LAUNCH(k2_band_snap, dim3(nblk(nm, 256)), dim3(256), 0, stream, D, m[0], snap, sc);
+    if (tail >= 0)
+        AWCM_LAUNCH(k2_tail<TAIL_FWT>, dim3(1), dim3(1024), 0, stream, D, tail, W, tol_host, Q, m[0], nullptr, sc);
+    AWCM_LAUNCH(k2_band_snap, dim3(bseg), dim3(256), 0, stream, D, m[0], snap, sc);
     AWCM_LAUNCH(k2_adjacent, dim3(nblk(D.Nx, 256), D.Ny), dim3(256), 0, stream, D, snap, m[0], sc);
     if (int st = check_recon(*this, m[0])) return st;
     for (int k = 1; k <= 3; k++) {
-        AWCM_LAUNCH(k2_level_copy, dim3(nblk(nm, 256)), dim3(256), 0, stream, D, m[k - 1], m[k], lv[k - 1], sc);
+        AWCM_LAUNCH(k2_copy16, dim3(bseg), dim3(256), 0, stream, D, m[k - 1], m[k], sc);
+        AWCM_LAUNCH(k2_level, dim3(nblk(D.Nx, 256), D.Ny), dim3(256), 0, stream, D, m[k - 1], lv[k - 1], sc);
         AWCM_LAUNCH(k2_addpoint, dim3(nblk(D.Nx - 2, 256), D.Ny - 2), dim3(256), 0, stream, D, m[k - 1], lv[k - 1], m[k], sc);
         if (int st = check_recon(*this, m[k])) return st;
     }
     // inverse transform, coarsest level first
-    for (int lev = 0; lev < D.maxlev; lev++) {
-        const int d = 1 << (D.L - 1 - lev);
-        const Lat& g3 = sub[lev][2];
-        const Lat& g1 = sub[lev][0];
-        const Lat& g2 = sub[lev][1];
-        const Lat& gc = grd[lev];
-        AWCM_LAUNCH(k2_iwt_c, dim3(nblk((long long)gc.ni * gc.nj, 256)), dim3(256), 0, stream, D, gc, d, W, Q, m[3], sc);
-        AWCM_LAUNCH(k2_iwt_d12, dim3(nblk((long long)g1.ni * g1.nj + (long long)g2.ni * g2.nj, 256)), dim3(256), 0, stream, D,
-                    g1, g2, d, W, Q, m[3], sc);
-        AWCM_LAUNCH(k2_iwt_d3, dim3(nblk((long long)g3.ni * g3.nj, 256)), dim3(256), 0, stream, D, g3, d, W, Q, m[3], sc);
+    if (tail >= 0)
+        AWCM_LAUNCH(k2_tail<TAIL_IWT>, dim3(1), dim3(1024), 0, stream, D, tail, W, tol_host, Q, m[3], nullptr, sc);
+    for (int lev = tail + 1; lev < D.maxlev; lev++) {
+        AWCM_LAT(OP_IWT_C, lev, LAT_C, false, Q, m[3], nullptr);
+        AWCM_LAT(OP_IWT_D12, lev, LAT_D1, true, Q, m[3], nullptr);
+        AWCM_LAT(OP_IWT_D3, lev, LAT_D3, false, Q, m[3], nullptr);
     }
     // interpolation of momentum and energy at new points
-    AWCM_LAUNCH(k2_interp_tmp, dim3(nblk(nm, 256)), dim3(256), 0, stream, D, m[3], mt, tmp, sc);
+    AWCM_LAUNCH(k2_interp_tmp, dim3(bseg), dim3(256), 0, stream, D, m[3], mt, tmp, sc);
     if (int st = check_recon(*this, tmp)) return st;
-    for (int lev = 0; lev < D.maxlev; lev++) {
-        const int d = 1 << (D.L - 1 - lev);
-        const Lat& g3 = sub[lev][2];
-        const Lat& g1 = sub[lev][0];
-        const Lat& g2 = sub[lev][1];
-        AWCM_LAUNCH(k2_interp_d12, dim3(nblk((long long)g1.ni * g1.nj + (long long)g2.ni * g2.nj, 256)), dim3(256), 0, stream,
-                    D, g1, g2, d, W, Q, tmp, mt, sc);
-        AWCM_LAUNCH(k2_interp_d3, dim3(nblk((long long)g3.ni * g3.nj, 256)), dim3(256), 0, stream, D, g3, d, W, Q, tmp, mt, sc);
+    if (tail >= 0)
+        AWCM_LAUNCH(k2_tail<TAIL_INT>, dim3(1), dim3(1024), 0, stream, D, tail, W, tol_host, Q, tmp, mt, sc);
+    for (int lev = tail + 1; lev < D.maxlev; lev++) {
+        AWCM_LAT(OP_INT_D12, lev, LAT_D1, true, Q, tmp, mt);
+        AWCM_LAT(OP_INT_D3, lev, LAT_D3, false, Q, tmp, mt);
     }
     return AWCM_OK;
 }
 
 int Solver2D::enqueue_step() {
-    if (int st = enqueue_begin(false)) return st;
     hipEvent_t e0 = prof ? mark() : nullptr;
-    if (int st = enqueue_prepare()) return st;
+    if (int st = enqueue_prepare(false)) return st;
     const long long nl = (long long)D.Nx * D.Ny;
-    AWCM_LAUNCH(k2_compact, dim3(nblk(nl, kCompactThreads * kCompactPer)), dim3(kCompactThreads), 0, stream, D, m[0], m[1],
-                m[2], lv[0], lv[1], lv[2], lists, sc);
+    (void)nl;
+    AWCM_LAUNCH(k2_compact, dim3(nblk((long long)(D.P / 16) * (D.Ny + 2 * G), kCompactThreads)), dim3(kCompactThreads), 0,
+                stream, D, m[0], m[1], m[2], lv[0], lv[1], lv[2], lists, sc);
     if (prof) prof_ev[2].push_back({e0, mark()});
     const dim3 ag(nblk(D.Nx, 64), nblk(D.Ny, kAlphaRows)), ab(64, 4);
     const unsigned nbc = nblk(2 * G * (D.Nx + D.Ny), 256);
@@ -1045,7 +1325,7 @@ int Solver2D::enqueue_step() {
     if (prof) pe = mark();
     AWCM_LAUNCH(k2_stage_coarse<3>, dim3(gc), dim3(256), 0, stream, D, list(0, 0), 0, lv[0], Q1, Q1, Q0, gamma, dx, sc);
     if (prof) prof_ev[1].push_back({pe, mark()});
-    AWCM_LAUNCH(k2_combine, dim3(nblk(D.Nx, 256), D.Ny), dim3(256), 0, stream, D, m[0], Q, Q0, 1, sc);
+    AWCM_LAUNCH(k2_combine, dim3(nblk(D.Nx, 256), D.Ny), dim3(256), 0, stream, D, m[0], m[3], mt, Q, Q0, 1, sc);
     return AWCM_OK;
 }
 
@@ -1068,8 +1348,8 @@ int Solver2D::launch_step() {
 }
 
 int Solver2D::enqueue_post() {
-    if (int st = enqueue_begin(true)) return st;
-    AWCM_LAUNCH(k2_combine, dim3(nblk(D.Nx, 256), D.Ny), dim3(256), 0, stream, D, m[0], Q, Q0, 0, sc);
+    if (int st = enqueue_begin(true, false)) return st;
+    AWCM_LAUNCH(k2_combine, dim3(nblk(D.Nx, 256), D.Ny), dim3(256), 0, stream, D, m[0], m[3], mt, Q, Q0, 0, sc);
     return AWCM_OK;
 }
 

@@ -131,8 +131,7 @@ int awcm_prepare_euler(awcm_solver* h, double tol) {
     }
     Solver2D& S = *h->s2;
     S.tol_host = tol;
-    int st = S.enqueue_begin(true);
-    if (!st) st = S.enqueue_prepare();
+    int st = S.enqueue_prepare(true);
     int flag = 0;
     if (!st) st = S.get_state(nullptr, nullptr, nullptr, nullptr, &flag);
     return st ? st : flag_status(flag);
@@ -276,6 +275,22 @@ int awcm_profile_read(awcm_solver* h, int kernel, double* total_ms, long long* l
     if (launches) *launches = (long long)S.prof_ev[kernel].size();
     if (points) *points = kernel == 0 ? sc.sumW : kernel == 1 ? sc.sumC : 0;
     return AWCM_OK;
+}
+
+int awcm_set_tuning(awcm_solver* h, int param, int value) {
+    if (!h) return AWCM_EINVAL;
+    if (h->dim != 2) return AWCM_ESTATE;
+    Solver2D& S = *h->s2;
+    if (hipStreamSynchronize(S.stream) != hipSuccess) return AWCM_EDEVICE;
+    if (param == AWCM_TUNE_TAIL_CELLS) {
+        if (value < 0) return AWCM_EINVAL;
+        return S.set_tail(value);
+    }
+    if (param == AWCM_TUNE_GRAPH) {
+        S.use_graph = value != 0;
+        return AWCM_OK;
+    }
+    return AWCM_EINVAL;
 }
 
 int awcm_synchronize(awcm_solver* h) {

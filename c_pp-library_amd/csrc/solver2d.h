@@ -17,6 +17,7 @@ struct Lat {
 struct Dims2 {
     int Nx, Ny, P, L, maxlev;
     long long plane;  // doubles per state plane, P * (Ny + 6)
+    int PR;           // row pitch of the reference layout, Nx + 6 (host side)
 };
 
 struct Wts {
@@ -29,6 +30,8 @@ struct Scal2 {
     double t, dt, t_final, tol, cfl;
     long long sum_active0;
     int cnt, n_print, steps, skip, do_print, flag;
+    int dirty;                // Q may be non-zero outside mask0: next residue pass is full
+    int residue_full;         // this step's residue pass zeroes every inactive point
     int cntW[3], cntC[3];     // stage work-list lengths (mask0, mask1, mask2)
     long long sumW, sumC;     // stage points processed, summed over steps
 };
@@ -76,6 +79,7 @@ struct Solver2D {
     double* ax(int s) const { return alpha + size_t(s) * (D.Nx + D.Ny); }
     double* ay(int s) const { return alpha + size_t(s) * (D.Nx + D.Ny) + D.Ny; }
 
+    int mark_dirty();
     int upload_q(const double* q);
     int download_q(double* q);
     int set_mask0(const uint8_t* m0);
@@ -85,8 +89,10 @@ struct Solver2D {
     int get_state(double* t, int* cnt, int* steps, long long* updates, int* flag);
 
     // kernel sequences (enqueued on `stream`, no host synchronisation)
-    int enqueue_begin(bool force);
-    int enqueue_prepare();
+    int tail = -1;            // levels 0..tail of each cascade run in one workgroup
+    int set_tail(int tail_cells);
+    int enqueue_begin(bool force, bool copy_mt);
+    int enqueue_prepare(bool force);
     int enqueue_step();       // begin + prepare + dt + 3 stages + combine
     int enqueue_post();
 };

@@ -69,6 +69,7 @@ def lib():
         "awcm_batch_synchronize": [V],
         "awcm_run_timed": [V, D, D, D, I, _dp, C.POINTER(I), C.POINTER(I), C.POINTER(C.c_longlong), _dp],
         "awcm_profile_enable": [V, I],
+        "awcm_set_tuning": [V, I, I],
         "awcm_profile_read": [V, I, _dp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)],
         "awcm_ic_riemann_1d": [I, D, D, D, _dp, _dp, D, _dp, _dp, _dp],
         "awcm_ic_shu_osher_1d": [I, D, D, D, _dp, D, D, _dp, _dp, _dp],
@@ -198,11 +199,15 @@ class Solver:
         _check(lib().awcm_profile_read(self.h, kernel, _d(ms), C.byref(n), C.byref(pts)), "awcm_profile_read")
         return float(ms[0]), n.value, pts.value
 
+    def set_tuning(self, param, value):
+        _check(lib().awcm_set_tuning(self.h, param, value), "awcm_set_tuning")
+
     def synchronize(self):
         _check(lib().awcm_synchronize(self.h), "awcm_synchronize")
 
 
 KERNEL_STAGE_WENO, KERNEL_STAGE_COARSE, KERNEL_PREPARE = 0, 1, 2
+TUNE_TAIL_CELLS, TUNE_GRAPH = 0, 1
 
 # 15-digit four-quadrant states (rho, u, v, p), quadrants NE, NW, SW, SE
 # (PROBLEM_QUADRANTS.h:43-73)

@@ -125,6 +125,15 @@ enum {
 int awcm_profile_enable(awcm_solver* h, int enable);
 int awcm_profile_read(awcm_solver* h, int kernel, double* total_ms, long long* launches, long long* points);
 
+/* Launch tuning of a 2-D handle (results are identical for every setting):
+ *   AWCM_TUNE_TAIL_CELLS  the coarsest levels whose lattice has at most this many
+ *                         cells run each level cascade inside one workgroup
+ *                         (default 0: every level is a launch of its own);
+ *   AWCM_TUNE_GRAPH       1: replay the step as a captured HIP graph (default),
+ *                         0: launch the kernels eagerly. */
+enum { AWCM_TUNE_TAIL_CELLS = 0, AWCM_TUNE_GRAPH = 1 };
+int awcm_set_tuning(awcm_solver* h, int param, int value);
+
 /* Block until the handle's queued device work is finished. */
 int awcm_synchronize(awcm_solver* h);
 

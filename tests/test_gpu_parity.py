@@ -126,8 +126,9 @@ def test_2d_prepare_matches_oracle(case):
     assert_same(g.download_q(), o.get_q(), mg, mo, lg, lo, f"2-D {case} prepare")
 
 
+@pytest.mark.parametrize("tail_cells", [4096, 0])
 @pytest.mark.parametrize("case", ["quad", "dmr"])
-def test_2d_steps_match_oracle(case):
+def test_2d_steps_match_oracle(case, tail_cells):
     P = _pyawcm()
     if case == "quad":
         L, nxc, nyc, bc, tf, npr = 5, 9, 9, 0, 0.25, 9
@@ -139,6 +140,7 @@ def test_2d_steps_match_oracle(case):
         xl, xu = 0.0, 4.0
     pts = O.print_times(tf, npr)
     g = P.Solver(2, L, nxc, 1.4, dx, ny_coarse=nyc, x0=xl, bc_kind=bc)
+    g.set_tuning(P.TUNE_TAIL_CELLS, tail_cells)
     g.set_schedule(pts)
     o = O.Solver2D("orc", L, nxc, nyc, 1.4, bc, xl, xu, 0.0, 1.0)
     g.upload_q(q)
@@ -152,13 +154,17 @@ def test_2d_steps_match_oracle(case):
         assert_same(g.download_q(), o.get_q(), mg, mo, lg, lo, f"2-D {case} step {n}")
 
 
-def test_2d_multistep_run_matches_oracle():
-    """Several steps queued in one awcm_run call (no host round trip per step)."""
+@pytest.mark.parametrize("tail_cells", [4096, 256, 0])
+def test_2d_multistep_run_matches_oracle(tail_cells):
+    """Several steps queued in one awcm_run call (graph replays, no host round
+    trip per step), with every level cascade split between per-level kernels
+    and the one-workgroup tail in three different ways."""
     P = _pyawcm()
-    L, nxc, nyc = 4, 33, 9
+    L, nxc, nyc = 5, 33, 9
     q, dx = dmr_setup(L, nxc, nyc)
     pts = O.print_times(0.2, 21)
     g = P.Solver(2, L, nxc, 1.4, dx, ny_coarse=nyc, x0=0.0, bc_kind=1)
+    g.set_tuning(P.TUNE_TAIL_CELLS, tail_cells)
     g.set_schedule(pts)
     g.upload_q(q)
     steps, _ = g.run(1e-3, 0.4, 0.2, 40)
