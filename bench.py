@@ -41,16 +41,14 @@ LEVELS, NXC, NYC = 7, 65, 17
 RES = (NYC - 1) * 2 ** (LEVELS - 1) + 1  # 1025
 TOL, CFL, T_FINAL, NUM_PRINT, RHO_PRE = 1e-3, 0.4, 0.2, 21, 1.4
 
-# Algorithmic cost of the dominant kernel (finest-level WENO-Z/Roe stage update).
-# FP64 operations of one face flux as the reference writes it (+ - * / sqrt,
-# WENOZ.h:236-320 with ROESOLVER.h / EULER_EQUATION.h): 1202.  A finest-level
-# point needs one new x-face and one new y-face (faces are shared by
-# neighbours), so 2 x 1202 flops, plus the 12-op difference/update.
-FLOPS_PER_WENO_POINT = 2 * 1202 + 12
-# Bytes: read the point's state (32 B), write the stage output (32 B), read the
-# RK base for stage 2 (32 B, one stage in three), the 4 B list entry; stencil
-# neighbours are the neighbours' own reads (dense limit).
-BYTES_PER_WENO_POINT = 32 + 32 + 32.0 / 3 + 4
+# Algorithmic cost of the dominant kernel, k2_stage_faces (one WENO-Z/Roe face
+# flux per thread, each face of the stage once).  FP64 operations of one face
+# flux as the reference writes it (+ - * / sqrt, WENOZ.h:236-320 with
+# ROESOLVER.h / EULER_EQUATION.h), counted mechanically: 1202.
+FLOPS_PER_FACE = 1202
+# Bytes per face: the face's own state (32 B, neighbours' states are the
+# neighbouring faces' own reads), its 4 B list entry, and the 32 B flux written.
+BYTES_PER_FACE = 32 + 4 + 32
 
 
 def ensure_built():
@@ -142,12 +140,11 @@ def main():
     assert steps_p == steps and updates_p == updates and S.t == t_end, "profiled replay diverged"
 
     value = updates / (ms / 1e3) if ms > 0 else 0.0
-    # dominant kernel: finest-level WENO-Z/Roe stage update
+    # dominant kernel: WENO-Z/Roe face fluxes of a stage
     avg_launch_s = (w_ms / w_launch) / 1e3 if w_launch else float("nan")
-    pts_per_launch = w_pts / w_launch if w_launch else 0.0
-    flops = pts_per_launch * FLOPS_PER_WENO_POINT
-    achieved_tflops = flops / avg_launch_s / 1e12 if w_launch else 0.0
-    achieved_gbs = pts_per_launch * BYTES_PER_WENO_POINT / avg_launch_s / 1e9 if w_launch else 0.0
+    faces_per_launch = w_pts / w_launch if w_launch else 0.0
+    achieved_tflops = faces_per_launch * FLOPS_PER_FACE / avg_launch_s / 1e12 if w_launch else 0.0
+    achieved_gbs = faces_per_launch * BYTES_PER_FACE / avg_launch_s / 1e9 if w_launch else 0.0
 
     out = {
         "metric": METRIC,
@@ -169,7 +166,7 @@ def main():
                    "mean_active_points": updates / max(steps, 1)},
         "roofline": {
             "bound": "fp64-valu",
-            "kernel": "k2_stage_weno (finest-level WENO-Z/Roe stage update)",
+            "kernel": "k2_stage_faces (WENO-Z/Roe face fluxes, each face of a stage once)",
             "achieved": achieved_tflops,
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
@@ -178,12 +175,13 @@ def main():
             "hbm_achieved_gbs": achieved_gbs,
             "hbm_frac": achieved_gbs / HBM_PEAK_GBS,
             "avg_launch_ms": avg_launch_s * 1e3,
-            "points_per_launch": pts_per_launch,
-            "flops_per_point": FLOPS_PER_WENO_POINT,
+            "faces_per_launch": faces_per_launch,
+            "flops_per_face": FLOPS_PER_FACE,
+            "bytes_per_face": BYTES_PER_FACE,
         },
         "breakdown_ms_per_step": {
-            "stage_weno": w_ms / max(steps, 1),
-            "stage_coarse": c_ms / max(steps, 1),
+            "stage_faces": w_ms / max(steps, 1),
+            "stage_update": c_ms / max(steps, 1),
             "prepare_euler": p_ms / max(steps, 1),
         },
     }

@@ -12,17 +12,19 @@
 //           packed (j << 16) | i.
 //
 // Kernel map (reference function -> kernels):
-//   ForwardWaveletTransformationEuler (WAVELET_2D.h:478-624)   k2_fwt_d3, k2_fwt_d12, k2_fwt_c per level
+//   ForwardWaveletTransformationEuler (WAVELET_2D.h:478-624)   k2_lat<OP_FWT_D3/D12/C> per level
 //   BandingMask + AddAdjacentZone (AWCM_MANAGER_2D.h:14-61)      k2_band_snap, k2_adjacent
-//   CheckReconstruction (AWCM_MANAGER_2D.h:63-158)               k2_check_d3, k2_check_d12 per level
-//   ComputeLevel + mask copy (AWCM_MANAGER_2D.h:215-241)         k2_level_copy
-//   AddPointForDifferentiation (AWCM_MANAGER_2D.h:160-213)       k2_addpoint
-//   InverseWaveletTransformationEuler (WAVELET_2D.h:626-782)     k2_iwt_c, k2_iwt_d12, k2_iwt_d3 per level
-//   InterpolateEuler (WAVELET_2D.h:784-907)                      k2_interp_tmp, k2_check_*, k2_interp_d12, k2_interp_d3
+//   CheckReconstruction (AWCM_MANAGER_2D.h:63-158)               k2_check per level (d3 + d1/d2 fused)
+//   mask copy + ComputeLevel + AddPointForDifferentiation         k2_level_add (one pass per mask)
+//     (AWCM_MANAGER_2D.h:160-241)
+//   InverseWaveletTransformationEuler (WAVELET_2D.h:626-782)     k2_lat<OP_IWT_C/D12/D3> per level
+//   InterpolateEuler (WAVELET_2D.h:784-907)                      k2_lat<OP_INT_D12/D3> per level
+//   stage work lists                                             k2_compact
 //   CFL / alpha reductions (main_2d_euler(dmach).cpp:61-104)     k2_alpha (+ dt in k2_bc)
 //   ghost fill (PROBLEM_QUADRANTS.h:123-203, PROBLEM_DOUBLEMACH.h:106-245)   k2_bc
-//   stage loops (main_2d_euler(dmach).cpp:106-322)               k2_stage_weno<S>, k2_stage_coarse<S>
+//   stage loops (main_2d_euler(dmach).cpp:106-322)               k2_stage_faces, k2_stage_update<S>
 //   RK combine + RemoveResidueEuler (:325-334, AWCM_MANAGER_2D.h:267-280)    k2_combine
+//   optional one-workgroup tail of the coarsest levels           k2_tail<...>, k2_check_tail
 // Every kernel returns at once when the step is skipped (t >= t_final), so
 // many steps can be queued or graph-replayed without host round trips.
 
@@ -191,17 +193,23 @@ __device__ __forceinline__ void iwt_d3_pt(const Dims2& D, int i, int j, int d, c
     rho[p] = v;
 }
 // WAVELET_2D.h:808-905 (interpolation of momentum and energy, d1/d2 and d3)
+// All stencil values of the three components are loaded before any store, so
+// the loads of one component do not wait behind the store of another.
 __device__ __forceinline__ void int_d12_pt(const Dims2& D, int kind, int i, int j, int d, const Wts& W, double* Q) {
     const long long p = gidx(D, i, j);
     int s, k;
     if (kind == LAT_D1) pstart(i, d, D.Nx - 1, s, k);
     else pstart(j, d, D.Ny - 1, s, k);
+    double x[3][4];
     for (int c4 = 1; c4 < 4; c4++) {
-        double* q = Q + c4 * D.plane;
+        const double* q = Q + c4 * D.plane;
+        for (int c = 0; c < 4; c++)
+            x[c4 - 1][c] = (kind == LAT_D1) ? q[gidx(D, s + c * d, j)] : q[gidx(D, i, s + c * d)];
+    }
+    for (int c4 = 1; c4 < 4; c4++) {
         double v = 0.0;
-        if (kind == LAT_D1) for (int c = 0; c < 4; c++) v += q[gidx(D, s + c * d, j)] * W.w[c][k + 1];
-        else for (int c = 0; c < 4; c++) v += q[gidx(D, i, s + c * d)] * W.w[c][k + 1];
-        q[p] = v;
+        for (int c = 0; c < 4; c++) v += x[c4 - 1][c] * W.w[c][k + 1];
+        Q[p + c4 * D.plane] = v;
     }
 }
 __device__ __forceinline__ void int_d3_pt(const Dims2& D, int i, int j, int d, const Wts& W, double* Q) {
@@ -209,15 +217,21 @@ __device__ __forceinline__ void int_d3_pt(const Dims2& D, int i, int j, int d, c
     int is, js, k1, k2;
     pstart(i, d, D.Nx - 1, is, k1);
     pstart(j, d, D.Ny - 1, js, k2);
+    double row[3][4], col[3][4], blk[3][16];
     for (int c4 = 1; c4 < 4; c4++) {
-        double* q = Q + c4 * D.plane;
-        double v = 0.0;
-        for (int c = 0; c < 4; c++) v += q[gidx(D, is + c * d, j)] * W.w[c][k1 + 1];
-        for (int c = 0; c < 4; c++) v += q[gidx(D, i, js + c * d)] * W.w[c][k2 + 1];
+        const double* q = Q + c4 * D.plane;
+        for (int c = 0; c < 4; c++) row[c4 - 1][c] = q[gidx(D, is + c * d, j)];
+        for (int c = 0; c < 4; c++) col[c4 - 1][c] = q[gidx(D, i, js + c * d)];
         for (int c2 = 0; c2 < 4; c2++)
-            for (int c1 = 0; c1 < 4; c1++)
-                v -= q[gidx(D, is + c1 * d, js + c2 * d)] * W.w[c1][k1 + 1] * W.w[c2][k2 + 1];
-        q[p] = v;
+            for (int c1 = 0; c1 < 4; c1++) blk[c4 - 1][4 * c2 + c1] = q[gidx(D, is + c1 * d, js + c2 * d)];
+    }
+    for (int c4 = 1; c4 < 4; c4++) {
+        double v = 0.0;
+        for (int c = 0; c < 4; c++) v += row[c4 - 1][c] * W.w[c][k1 + 1];
+        for (int c = 0; c < 4; c++) v += col[c4 - 1][c] * W.w[c][k2 + 1];
+        for (int c2 = 0; c2 < 4; c2++)
+            for (int c1 = 0; c1 < 4; c1++) v -= blk[c4 - 1][4 * c2 + c1] * W.w[c1][k1 + 1] * W.w[c2][k2 + 1];
+        Q[p + c4 * D.plane] = v;
     }
 }
 
@@ -467,18 +481,29 @@ __device__ __forceinline__ void st16(uint8_t* p, const V16& x) { *reinterpret_ca
 
 // mask_temp = mask0 (whole ghosted buffer) and the step's scalar resets
 __global__ void k2_begin_copy(Dims2 D, const uint8_t* __restrict__ src, uint8_t* __restrict__ dst, double* alpha,
-                              Scal2* sc, int force) {
+                              uint8_t* z1, uint8_t* z2, uint8_t* z3, Scal2* sc, int force) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t == 0) {
         sc->skip = (!force && !(sc->t < sc->t_final)) ? 1 : 0;
         if (!sc->skip) { sc->residue_full = sc->dirty; sc->dirty = 0; }
         sc->vmax = 0ull;
-        for (int k = 0; k < 3; k++) { sc->cntW[k] = 0; sc->cntC[k] = 0; }
+        for (int k = 0; k < 3; k++) { sc->cntW[k] = 0; sc->cntC[k] = 0; sc->cntXS[k] = 0; sc->cntYS[k] = 0; }
     }
     const int na = 3 * (D.Nx + D.Ny);
     if (t < na) alpha[t] = 0.0;
     Seg s;
-    if (seg_of(D, t, s)) st16(dst + s.p0, ld16(src + s.p0));
+    if (seg_of(D, t, s)) {
+        st16(dst + s.p0, ld16(src + s.p0));
+        // masks 1..3 are rebuilt from mask0 this step; clearing them here lets
+        // the fused copy/level/add-point pass write nothing but ones
+        if (z1) {
+            V16 zero;
+            zero.v = make_uint4(0, 0, 0, 0);
+            st16(z1 + s.p0, zero);
+            st16(z2 + s.p0, zero);
+            st16(z3 + s.p0, zero);
+        }
+    }
 }
 
 // BandingMask (2-wide band) on mask0, then the AddAdjacentZone snapshot
@@ -553,78 +578,76 @@ __global__ void k2_adjacent(Dims2 D, const uint8_t* __restrict__ snap, uint8_t* 
     }
 }
 
-// mask_{k+1} = mask_k (16-byte copy per thread, ghosts included)
-__global__ void k2_copy16(Dims2 D, const uint8_t* __restrict__ src, uint8_t* __restrict__ dst, const Scal2* sc) {
+// One pass per mask: mask_{k+1} = mask_k (ghosts included), ComputeLevel(level_k,
+// mask_k) (AWCM_MANAGER_2D.h:215-241) and AddPointForDifferentiation(mask_{k+1},
+// level_k) (AWCM_MANAGER_2D.h:160-213).  mask_{k+1} was cleared at the start of
+// the step, so the copy and the scatter both write only ones and cannot undo
+// each other.  The reference snapshots mask_{k+1} right after copying it from
+// mask_k, so mask_k is the snapshot; a point's own level is all the scatter
+// needs.  A point with no active neighbour keeps its previous level, as in the
+// reference.  Grid: x over the ghosted row (Nx+6 points), y over ghosted rows.
+__global__ void k2_level_add(Dims2 D, const uint8_t* __restrict__ src, uint8_t* __restrict__ dst,
+                             uint8_t* __restrict__ lvl, Scal2* sc) {
     if (sc->skip) return;
-    Seg s;
-    if (seg_of(D, (long long)blockIdx.x * blockDim.x + threadIdx.x, s)) st16(dst + s.p0, ld16(src + s.p0));
-}
-
-// ComputeLevel(level_k, mask_k) (AWCM_MANAGER_2D.h:215-241), one point per
-// thread.  A point with no active neighbour keeps its previous level, as in
-// the reference.
-__global__ void k2_level(Dims2 D, const uint8_t* __restrict__ src, uint8_t* __restrict__ lvl, Scal2* sc) {
-    if (sc->skip) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
-    if (i >= D.Nx) return;
-    const long long li = (long long)i + (long long)D.Nx * j;
+    const int i = int(blockIdx.x * blockDim.x + threadIdx.x) - G, j = int(blockIdx.y) - G;
+    if (i > D.Nx - 1 + G) return;
     const long long t = gidx(D, i, j);
-    if (!src[t]) { lvl[li] = 0; return; }
+    const uint8_t v = src[t];
+    if (v) dst[t] = 1;
+    if (i < 0 || i > D.Nx - 1 || j < 0 || j > D.Ny - 1) return;
+    const long long li = (long long)i + (long long)D.Nx * j;
+    if (!v) { lvl[li] = 0; return; }
+    int lev = lvl[li];  // kept when no neighbour is active
     int d = 1;
-    for (int lev = D.maxlev; lev >= 0; lev--) {
-        if (i - d < -G || i + d > D.Nx - 1 + G || j - d < -G || j + d > D.Ny - 1 + G) { sc->flag |= 1; return; }
-        if (src[t - d] || src[t + d] || src[t - (long long)d * D.P] || src[t + (long long)d * D.P]) {
-            lvl[li] = (uint8_t)lev;
-            return;
-        }
+    for (int l = D.maxlev; l >= 0; l--) {
+        if (i - d < -G || i + d > D.Nx - 1 + G || j - d < -G || j + d > D.Ny - 1 + G) { sc->flag |= 1; break; }
+        if (src[t - d] || src[t + d] || src[t - (long long)d * D.P] || src[t + (long long)d * D.P]) { lev = l; break; }
         d *= 2;
     }
-}
-
-// AWCM_MANAGER_2D::AddPointForDifferentiation (AWCM_MANAGER_2D.h:160-213).  The
-// reference snapshots mask_{k+1} right after copying it from mask_k, so mask_k
-// is the snapshot.
-__global__ void k2_addpoint(Dims2 D, const uint8_t* __restrict__ snap, const uint8_t* __restrict__ lvl,
-                            uint8_t* __restrict__ mask, Scal2* sc) {
-    if (sc->skip) return;
-    const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x, j = 1 + blockIdx.y;
-    if (i > D.Nx - 2 || j > D.Ny - 2) return;
-    if (!snap[gidx(D, i, j)]) return;
-    const int lev = lvl[(long long)i + (long long)D.Nx * j];
-    const int d = 1 << (D.L - 1 - lev);  // grid[lev].dx
+    lvl[li] = (uint8_t)lev;
+    if (i < 1 || i > D.Nx - 2 || j < 1 || j > D.Ny - 2) return;
+    const int dd = 1 << (D.L - 1 - lev);  // grid[lev].dx
     const int NX = D.Nx - 1, NY = D.Ny - 1;
     if (lev == D.maxlev) {
-        for (int c = -3; c <= 3; c++) { mset(D, mask, i + c, j, sc); mset(D, mask, i, j + c, sc); }
+        for (int c = -3; c <= 3; c++) { mset(D, dst, i + c, j, sc); mset(D, dst, i, j + c, sc); }
         return;
     }
     int lo, hi;
-    if (i - 2 * d < 0) lo = -1, hi = 2;
-    else if (i + 2 * d > NX) lo = -2, hi = 1;
+    if (i - 2 * dd < 0) lo = -1, hi = 2;
+    else if (i + 2 * dd > NX) lo = -2, hi = 1;
     else lo = -2, hi = 2;
-    for (int c = lo; c <= hi; c++) mset(D, mask, i + d * c, j, sc);
-    if (j - 2 * d < 0) lo = -1, hi = 2;
-    else if (j + 2 * d > NY) lo = -2, hi = 1;
+    for (int c = lo; c <= hi; c++) mset(D, dst, i + dd * c, j, sc);
+    if (j - 2 * dd < 0) lo = -1, hi = 2;
+    else if (j + 2 * dd > NY) lo = -2, hi = 1;
     else lo = -2, hi = 2;
-    for (int c = lo; c <= hi; c++) mset(D, mask, i, j + d * c, sc);
+    for (int c = lo; c <= hi; c++) mset(D, dst, i, j + dd * c, sc);
 }
 
 // --------------------------------------------------------- compaction ------
-// Stage work lists: for k = 0..2 the points of mask_k at the finest level
-// (WENO-Z faces) and at coarser levels (central differences).  A thread takes
-// a 16-point row segment, so list entries come out in row order within a
-// thread; one atomic per list per block.  List order does not affect results.
+// Stage work lists, for each stage mask k = 0..2:
+//   W  points of mask_k at the finest level (WENO-Z faces),
+//   C  points of mask_k at coarser levels (central differences),
+//   XS W points whose left neighbour is not a W point (their left x-face is
+//      not the right face of another W point),
+//   YS W points whose bottom neighbour is not a W point.
+// A thread takes a 16-point row segment, so entries come out in row order
+// within a thread; one atomic per list per block.  List order does not affect
+// any result.
 constexpr int kCompactThreads = 256;
+constexpr int kNL = 12;
 
 __global__ void __launch_bounds__(kCompactThreads)
 k2_compact(Dims2 D, const uint8_t* m0, const uint8_t* m1, const uint8_t* m2, const uint8_t* l0, const uint8_t* l1,
            const uint8_t* l2, uint32_t* lists, Scal2* sc) {
     if (sc->skip) return;
-    __shared__ int wtot[6][kCompactThreads / 64];
-    __shared__ int base[6];
+    __shared__ int wtot[kNL][kCompactThreads / 64];
+    __shared__ int base[kNL];
     const long long N = (long long)D.Nx * D.Ny;
     const uint8_t* mk[3] = {m0, m1, m2};
     const uint8_t* lk[3] = {l0, l1, l2};
-    int cnt[6] = {0, 0, 0, 0, 0, 0};
+    int cnt[kNL];
+#pragma unroll
+    for (int q = 0; q < kNL; q++) cnt[q] = 0;
     uint32_t fl[16];
     Seg s;
     const bool ok = seg_of(D, (long long)blockIdx.x * blockDim.x + threadIdx.x, s) && s.j >= 0 && s.j <= D.Ny - 1;
@@ -638,17 +661,23 @@ k2_compact(Dims2 D, const uint8_t* m0, const uint8_t* m1, const uint8_t* m2, con
                 const long long li = (long long)i + (long long)D.Nx * s.j;
                 for (int k = 0; k < 3; k++) {
                     if (!v[k].b[e]) continue;
-                    const int l = (lk[k][li] == D.maxlev) ? 0 : 1;
-                    f |= 1u << (2 * k + l);
-                    cnt[2 * k + l]++;
+                    if (lk[k][li] != D.maxlev) { f |= 1u << (4 * k + 1); cnt[4 * k + 1]++; continue; }
+                    f |= 1u << (4 * k);
+                    cnt[4 * k]++;
+                    const long long p = s.p0 + e;
+                    const bool lw = i >= 1 && mk[k][p - 1] && lk[k][li - 1] == D.maxlev;
+                    const bool bw = s.j >= 1 && mk[k][p - D.P] && lk[k][li - D.Nx] == D.maxlev;
+                    if (!lw) { f |= 1u << (4 * k + 2); cnt[4 * k + 2]++; }
+                    if (!bw) { f |= 1u << (4 * k + 3); cnt[4 * k + 3]++; }
                 }
             }
             fl[e] = f;
         }
     }
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    int incl[6];
-    for (int q = 0; q < 6; q++) {
+    int incl[kNL];
+#pragma unroll
+    for (int q = 0; q < kNL; q++) {
         int x = cnt[q];
         for (int o = 1; o < 64; o <<= 1) {
             const int y = __shfl_up(x, o, 64);
@@ -658,21 +687,24 @@ k2_compact(Dims2 D, const uint8_t* m0, const uint8_t* m1, const uint8_t* m2, con
         if (lane == 63) wtot[q][wid] = x;
     }
     __syncthreads();
-    if (threadIdx.x < 6) {
+    if (threadIdx.x < kNL) {
         int acc = 0;
         for (int w = 0; w < kCompactThreads / 64; w++) { const int x = wtot[threadIdx.x][w]; wtot[threadIdx.x][w] = acc; acc += x; }
-        int* ctr = (threadIdx.x & 1) ? &sc->cntC[threadIdx.x >> 1] : &sc->cntW[threadIdx.x >> 1];
+        const int k = threadIdx.x >> 2, ty = threadIdx.x & 3;
+        int* ctr = ty == 0 ? &sc->cntW[k] : ty == 1 ? &sc->cntC[k] : ty == 2 ? &sc->cntXS[k] : &sc->cntYS[k];
         base[threadIdx.x] = acc ? atomicAdd(ctr, acc) : 0;
     }
     __syncthreads();
     if (!ok) return;
-    int pos[6];
-    for (int q = 0; q < 6; q++) pos[q] = base[q] + wtot[q][wid] + incl[q] - cnt[q];
+    int pos[kNL];
+#pragma unroll
+    for (int q = 0; q < kNL; q++) pos[q] = base[q] + wtot[q][wid] + incl[q] - cnt[q];
     for (int e = 0; e < 16; e++) {
         const uint32_t f = fl[e];
         if (!f) continue;
         const uint32_t packed = (uint32_t(s.j) << 16) | uint32_t(s.i0 + e);
-        for (int q = 0; q < 6; q++)
+#pragma unroll
+        for (int q = 0; q < kNL; q++)
             if (f & (1u << q)) lists[(long long)q * N + pos[q]++] = packed;
     }
 }
@@ -820,45 +852,73 @@ __device__ __forceinline__ void stage_out(const Dims2& D, long long p, const dou
     }
 }
 
-// Finest-level points: WENO-Z/Roe flux differences in x and y.
-template <int S>
-__global__ void __launch_bounds__(128) k2_stage_weno(Dims2 D, const uint32_t* __restrict__ list, int which,
-                                                     const double* __restrict__ Qa, const double* __restrict__ Qs,
-                                                     double* __restrict__ Qo, const double* __restrict__ ax,
-                                                     const double* __restrict__ ay, double gdx, Scal2* sc) {
+// WENO-Z/Roe face fluxes of one stage, each face once.  Face slot p = gidx(i,j)
+// holds the face to the right of (FX) / above (FY) point (i, j).  Faces needed:
+// the right and top faces of every finest-level point, plus the left face of
+// each x-run start and the bottom face of each y-run start.  One face per
+// thread; the flux is WENOZ_EULER_SOLVER_2D::ComputeFluxX/Y (WENOZ.h:236-320) of
+// the six states around the face -- the same call the reference makes, once
+// per face instead of once per adjacent point.
+__global__ void __launch_bounds__(128) k2_stage_faces(Dims2 D, const uint32_t* __restrict__ lw,
+                                                      const uint32_t* __restrict__ lxs, const uint32_t* __restrict__ lys,
+                                                      int which, const double* __restrict__ Qs, double* __restrict__ FX,
+                                                      double* __restrict__ FY, const double* __restrict__ ax,
+                                                      const double* __restrict__ ay, Scal2* sc) {
     if (sc->skip) return;
-    const int n = sc->cntW[which];
-    const double dt = sc->dt;
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
-        const uint32_t e = list[t];
+    const int nW = sc->cntW[which], nX = sc->cntXS[which], nY = sc->cntYS[which];
+    const int nx = nW + nX, ntot = nx + nW + nY;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < ntot; t += gridDim.x * blockDim.x) {
+        const bool xdir = t < nx;
+        const int u = xdir ? t : t - nx;
+        uint32_t e;
+        int shift;  // 0: the face after the point; -1: the face before it
+        if (xdir) { if (u < nW) { e = lw[u]; shift = 0; } else { e = lxs[u - nW]; shift = -1; } }
+        else { if (u < nW) { e = lw[u]; shift = 0; } else { e = lys[u - nW]; shift = -1; } }
         const int i = int(e & 0xffffu), j = int(e >> 16);
-        const long long p = gidx(D, i, j);
-        double s[7][4];
-        double fa[4], fb[4], r[4];
-        for (int q = 0; q < 7; q++) get4(Qs, D.plane, p + (q - 3), s[q]);
-        face_flux2<0>(kFluxGamma, s, ax[j], fa);
-        face_flux2<0>(kFluxGamma, s + 1, ax[j], fb);
-        for (int k = 0; k < 4; k++) r[k] = fb[k] - fa[k];
-        for (int q = 0; q < 7; q++) get4(Qs, D.plane, p + (long long)(q - 3) * D.P, s[q]);
-        face_flux2<1>(kFluxGamma, s, ay[i], fa);
-        face_flux2<1>(kFluxGamma, s + 1, ay[i], fb);
-        for (int k = 0; k < 4; k++) r[k] = r[k] + (fb[k] - fa[k]);
-        stage_out<S>(D, p, r, gdx * 1, dt, Qa, Qs, Qo);
+        double s[6][4], f[4];
+        if (xdir) {
+            const long long p = gidx(D, i + shift, j);  // face between p and p + 1
+            for (int q = 0; q < 6; q++) get4(Qs, D.plane, p + (q - 2), s[q]);
+            face_flux2<0>(kFluxGamma, s, ax[j], f);
+            put4(FX, D.plane, p, f);
+        } else {
+            const long long p = gidx(D, i, j + shift);  // face between p and p + P
+            for (int q = 0; q < 6; q++) get4(Qs, D.plane, p + (long long)(q - 2) * D.P, s[q]);
+            face_flux2<1>(kFluxGamma, s, ay[i], f);
+            put4(FY, D.plane, p, f);
+        }
     }
 }
 
-// Coarser-level points: central differences of the physical flux at spacing d
-// with the one-sided variants of AWCM_2D.h:166-187.
+// Stage update of every point of the stage mask: finest-level points difference
+// their face fluxes (dF = right - left, dG = top - bottom: main_2d_euler(dmach).cpp
+// :123-128); coarser points take central differences of the physical flux at
+// spacing d with the one-sided variants of AWCM_2D.h:166-187.
 template <int S>
-__global__ void __launch_bounds__(256) k2_stage_coarse(Dims2 D, const uint32_t* __restrict__ list, int which,
+__global__ void __launch_bounds__(256) k2_stage_update(Dims2 D, const uint32_t* __restrict__ lw,
+                                                       const uint32_t* __restrict__ lc, int which,
                                                        const uint8_t* __restrict__ lvl, const double* __restrict__ Qa,
                                                        const double* __restrict__ Qs, double* __restrict__ Qo,
+                                                       const double* __restrict__ FX, const double* __restrict__ FY,
                                                        double gamma, double gdx, Scal2* sc) {
     if (sc->skip) return;
-    const int n = sc->cntC[which];
+    const int nW = sc->cntW[which], nC = sc->cntC[which];
     const double dt = sc->dt;
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
-        const uint32_t e = list[t];
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nW + nC; t += gridDim.x * blockDim.x) {
+        if (t < nW) {
+            const uint32_t e = lw[t];
+            const int i = int(e & 0xffffu), j = int(e >> 16);
+            const long long p = gidx(D, i, j);
+            double a[4], b[4], c[4], d[4], r[4];
+            get4(FX, D.plane, p - 1, a);
+            get4(FX, D.plane, p, b);
+            get4(FY, D.plane, p - D.P, c);
+            get4(FY, D.plane, p, d);
+            for (int k = 0; k < 4; k++) r[k] = (b[k] - a[k]) + (d[k] - c[k]);
+            stage_out<S>(D, p, r, gdx * 1, dt, Qa, Qs, Qo);
+            continue;
+        }
+        const uint32_t e = lc[t - nW];
         const int i = int(e & 0xffffu), j = int(e >> 16);
         const long long p = gidx(D, i, j);
         const int lev = lvl[(long long)i + (long long)D.Nx * j];
@@ -928,6 +988,7 @@ __global__ void k2_combine(Dims2 D, const uint8_t* __restrict__ m0, const uint8_
         sc->sum_active0 += (long long)sc->cntW[0] + sc->cntC[0];
         sc->sumW += (long long)sc->cntW[0] + sc->cntW[1] + sc->cntW[2];
         sc->sumC += (long long)sc->cntC[0] + sc->cntC[1] + sc->cntC[2];
+        for (int k = 0; k < 3; k++) sc->sumF += 2LL * sc->cntW[k] + sc->cntXS[k] + sc->cntYS[k];
     }
     if (i >= D.Nx) return;
     const long long p = gidx(D, i, j);
@@ -1061,7 +1122,8 @@ int Solver2D::init(int L, int nxc_, int nyc_, double g, double dx_, double x0_, 
     if ((st = dalloc(&mt, mb)) || (st = dalloc(&snap, mb)) || (st = dalloc(&tmp, mb))) return st;
     for (int k = 0; k < 4; k++) if ((st = dalloc(&m[k], mb)) || (st = dalloc(&lv[k], nl))) return st;
     if ((st = dalloc(&alpha, sizeof(double) * 3 * (Nx + Ny)))) return st;
-    if ((st = dalloc(&lists, sizeof(uint32_t) * 6 * nl))) return st;
+    if ((st = dalloc(&lists, sizeof(uint32_t) * 12 * nl))) return st;
+    if ((st = dalloc(&FX, pb)) || (st = dalloc(&FY, pb))) return st;
     if ((st = dalloc(&pt, sizeof(double) * 4096))) return st;
     if ((st = dalloc(&sc, sizeof(Scal2)))) return st;
     hipMemset(Q, 0, pb); hipMemset(Q0, 0, pb); hipMemset(Q1, 0, pb); hipMemset(Q2, 0, pb);
@@ -1222,8 +1284,8 @@ int Solver2D::set_tail(int tail_cells) {
 int Solver2D::enqueue_begin(bool force, bool copy_mt) {
     const long long nseg = (long long)(D.P / 16) * (D.Ny + 2 * G);
     const long long n = std::max<long long>(nseg, 3LL * (D.Nx + D.Ny));
-    AWCM_LAUNCH(k2_begin_copy, dim3(nblk(n, 256)), dim3(256), 0, stream, D, m[0], copy_mt ? mt : m[0], alpha, sc,
-                force ? 1 : 0);
+    AWCM_LAUNCH(k2_begin_copy, dim3(nblk(n, 256)), dim3(256), 0, stream, D, m[0], copy_mt ? mt : m[0], alpha,
+                copy_mt ? m[1] : nullptr, m[2], m[3], sc, force ? 1 : 0);
     return AWCM_OK;
 }
 
@@ -1261,9 +1323,8 @@ int Solver2D::enqueue_prepare(bool force) {
     AWCM_LAUNCH(k2_adjacent, dim3(nblk(D.Nx, 256), D.Ny), dim3(256), 0, stream, D, snap, m[0], sc);
     if (int st = check_recon(*this, m[0])) return st;
     for (int k = 1; k <= 3; k++) {
-        AWCM_LAUNCH(k2_copy16, dim3(bseg), dim3(256), 0, stream, D, m[k - 1], m[k], sc);
-        AWCM_LAUNCH(k2_level, dim3(nblk(D.Nx, 256), D.Ny), dim3(256), 0, stream, D, m[k - 1], lv[k - 1], sc);
-        AWCM_LAUNCH(k2_addpoint, dim3(nblk(D.Nx - 2, 256), D.Ny - 2), dim3(256), 0, stream, D, m[k - 1], lv[k - 1], m[k], sc);
+        AWCM_LAUNCH(k2_level_add, dim3(nblk(D.Nx + 2 * G, 256), D.Ny + 2 * G), dim3(256), 0, stream, D, m[k - 1], m[k],
+                    lv[k - 1], sc);
         if (int st = check_recon(*this, m[k])) return st;
     }
     // inverse transform, coarsest level first
@@ -1274,14 +1335,17 @@ int Solver2D::enqueue_prepare(bool force) {
         AWCM_LAT(OP_IWT_D12, lev, LAT_D1, true, Q, m[3], nullptr);
         AWCM_LAT(OP_IWT_D3, lev, LAT_D3, false, Q, m[3], nullptr);
     }
-    // interpolation of momentum and energy at new points
-    AWCM_LAUNCH(k2_interp_tmp, dim3(bseg), dim3(256), 0, stream, D, m[3], mt, tmp, sc);
-    if (int st = check_recon(*this, tmp)) return st;
+    // interpolation of momentum and energy at new points.  The reference builds
+    // tmp = CheckReconstruction(mask3 minus mask_temp) and interpolates where tmp
+    // is set and mask_temp is not (WAVELET_2D.h:789-794,813).  mask3 is closed
+    // under CheckReconstruction, so the closure only ever adds points of mask3
+    // and tmp minus mask_temp is exactly mask3 minus mask_temp: the work mask is
+    // mask3 itself, tested against mask_temp, and the closure pass is not needed.
     if (tail >= 0)
-        AWCM_LAUNCH(k2_tail<TAIL_INT>, dim3(1), dim3(1024), 0, stream, D, tail, W, tol_host, Q, tmp, mt, sc);
+        AWCM_LAUNCH(k2_tail<TAIL_INT>, dim3(1), dim3(1024), 0, stream, D, tail, W, tol_host, Q, m[3], mt, sc);
     for (int lev = tail + 1; lev < D.maxlev; lev++) {
-        AWCM_LAT(OP_INT_D12, lev, LAT_D1, true, Q, tmp, mt);
-        AWCM_LAT(OP_INT_D3, lev, LAT_D3, false, Q, tmp, mt);
+        AWCM_LAT(OP_INT_D12, lev, LAT_D1, true, Q, m[3], mt);
+        AWCM_LAT(OP_INT_D3, lev, LAT_D3, false, Q, m[3], mt);
     }
     return AWCM_OK;
 }
@@ -1298,33 +1362,36 @@ int Solver2D::enqueue_step() {
     const unsigned nbc = nblk(2 * G * (D.Nx + D.Ny), 256);
     const unsigned gw = grid_blocks * 2, gc = grid_blocks;
     hipEvent_t pe = nullptr;
-    // stage 1: alpha over mask3 of Q (+ CFL), ghosts of Q, mask2/level2 -> Q2
-    AWCM_LAUNCH(k2_alpha, ag, ab, 0, stream, D, m[3], Q, ax(0), ay(0), gamma, 1, sc);
-    AWCM_LAUNCH(k2_bc, dim3(nbc), dim3(256), 0, stream, D, Q, bc_kind, x0, dx, 1, pt, sc);
-    if (prof) pe = mark();
-    AWCM_LAUNCH(k2_stage_weno<1>, dim3(gw), dim3(128), 0, stream, D, list(2, 1), 2, Q, Q, Q2, ax(0), ay(0), dx, sc);
-    if (prof) prof_ev[0].push_back({pe, mark()});
-    if (prof) pe = mark();
-    AWCM_LAUNCH(k2_stage_coarse<1>, dim3(gc), dim3(256), 0, stream, D, list(2, 0), 2, lv[2], Q, Q, Q2, gamma, dx, sc);
-    if (prof) prof_ev[1].push_back({pe, mark()});
-    // stage 2: ghosts of Q2, alpha over mask2 of Q2, mask1/level1 -> Q1
-    AWCM_LAUNCH(k2_bc, dim3(nbc), dim3(256), 0, stream, D, Q2, bc_kind, x0, dx, 0, pt, sc);
-    AWCM_LAUNCH(k2_alpha, ag, ab, 0, stream, D, m[2], Q2, ax(1), ay(1), gamma, 0, sc);
-    if (prof) pe = mark();
-    AWCM_LAUNCH(k2_stage_weno<2>, dim3(gw), dim3(128), 0, stream, D, list(1, 1), 1, Q, Q2, Q1, ax(1), ay(1), dx, sc);
-    if (prof) prof_ev[0].push_back({pe, mark()});
-    if (prof) pe = mark();
-    AWCM_LAUNCH(k2_stage_coarse<2>, dim3(gc), dim3(256), 0, stream, D, list(1, 0), 1, lv[1], Q, Q2, Q1, gamma, dx, sc);
-    if (prof) prof_ev[1].push_back({pe, mark()});
-    // stage 3: ghosts of Q1, alpha over mask1 of Q1, mask0/level0 -> Q0
-    AWCM_LAUNCH(k2_bc, dim3(nbc), dim3(256), 0, stream, D, Q1, bc_kind, x0, dx, 0, pt, sc);
-    AWCM_LAUNCH(k2_alpha, ag, ab, 0, stream, D, m[1], Q1, ax(2), ay(2), gamma, 0, sc);
-    if (prof) pe = mark();
-    AWCM_LAUNCH(k2_stage_weno<3>, dim3(gw), dim3(128), 0, stream, D, list(0, 1), 0, Q1, Q1, Q0, ax(2), ay(2), dx, sc);
-    if (prof) prof_ev[0].push_back({pe, mark()});
-    if (prof) pe = mark();
-    AWCM_LAUNCH(k2_stage_coarse<3>, dim3(gc), dim3(256), 0, stream, D, list(0, 0), 0, lv[0], Q1, Q1, Q0, gamma, dx, sc);
-    if (prof) prof_ev[1].push_back({pe, mark()});
+    // stage s reads Qs (ghosts filled), uses alpha set s and the lists of mask k
+    struct StageArgs { int k; const double* Qa; const double* Qs; double* Qo; };
+    const StageArgs sa[3] = {{2, Q, Q, Q2}, {1, Q, Q2, Q1}, {0, Q1, Q1, Q0}};
+    for (int s = 0; s < 3; s++) {
+        const StageArgs& g = sa[s];
+        if (s == 0) {
+            // alpha over mask3 of Q (+ CFL maximum), then the ghosts of Q and dt
+            AWCM_LAUNCH(k2_alpha, ag, ab, 0, stream, D, m[3], Q, ax(0), ay(0), gamma, 1, sc);
+            AWCM_LAUNCH(k2_bc, dim3(nbc), dim3(256), 0, stream, D, Q, bc_kind, x0, dx, 1, pt, sc);
+        } else {
+            // ghosts of the previous stage's output, then alpha over its mask
+            AWCM_LAUNCH(k2_bc, dim3(nbc), dim3(256), 0, stream, D, const_cast<double*>(g.Qs), bc_kind, x0, dx, 0, pt, sc);
+            AWCM_LAUNCH(k2_alpha, ag, ab, 0, stream, D, m[g.k + 1], g.Qs, ax(s), ay(s), gamma, 0, sc);
+        }
+        if (prof) pe = mark();
+        AWCM_LAUNCH(k2_stage_faces, dim3(gw), dim3(128), 0, stream, D, list(g.k, 0), list(g.k, 2), list(g.k, 3), g.k,
+                    g.Qs, FX, FY, ax(s), ay(s), sc);
+        if (prof) prof_ev[0].push_back({pe, mark()});
+        if (prof) pe = mark();
+        if (s == 0)
+            AWCM_LAUNCH(k2_stage_update<1>, dim3(gc), dim3(256), 0, stream, D, list(g.k, 0), list(g.k, 1), g.k, lv[g.k],
+                        g.Qa, g.Qs, g.Qo, FX, FY, gamma, dx, sc);
+        else if (s == 1)
+            AWCM_LAUNCH(k2_stage_update<2>, dim3(gc), dim3(256), 0, stream, D, list(g.k, 0), list(g.k, 1), g.k, lv[g.k],
+                        g.Qa, g.Qs, g.Qo, FX, FY, gamma, dx, sc);
+        else
+            AWCM_LAUNCH(k2_stage_update<3>, dim3(gc), dim3(256), 0, stream, D, list(g.k, 0), list(g.k, 1), g.k, lv[g.k],
+                        g.Qa, g.Qs, g.Qo, FX, FY, gamma, dx, sc);
+        if (prof) prof_ev[1].push_back({pe, mark()});
+    }
     AWCM_LAUNCH(k2_combine, dim3(nblk(D.Nx, 256), D.Ny), dim3(256), 0, stream, D, m[0], m[3], mt, Q, Q0, 1, sc);
     return AWCM_OK;
 }

@@ -255,6 +255,7 @@ int awcm_profile_enable(awcm_solver* h, int enable) {
     if (hipMemcpy(&sc, S.sc, sizeof(sc), hipMemcpyDeviceToHost) != hipSuccess) return AWCM_EDEVICE;
     sc.sumW = 0;
     sc.sumC = 0;
+    sc.sumF = 0;
     return hipMemcpy(S.sc, &sc, sizeof(sc), hipMemcpyHostToDevice) == hipSuccess ? AWCM_OK : AWCM_EDEVICE;
 }
 
@@ -273,7 +274,7 @@ int awcm_profile_read(awcm_solver* h, int kernel, double* total_ms, long long* l
     if (hipMemcpy(&sc, S.sc, sizeof(sc), hipMemcpyDeviceToHost) != hipSuccess) return AWCM_EDEVICE;
     if (total_ms) *total_ms = tot;
     if (launches) *launches = (long long)S.prof_ev[kernel].size();
-    if (points) *points = kernel == 0 ? sc.sumW : kernel == 1 ? sc.sumC : 0;
+    if (points) *points = kernel == 0 ? sc.sumF : kernel == 1 ? sc.sumW + sc.sumC : 0;
     return AWCM_OK;
 }
 

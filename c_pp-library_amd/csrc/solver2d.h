@@ -32,8 +32,9 @@ struct Scal2 {
     int cnt, n_print, steps, skip, do_print, flag;
     int dirty;                // Q may be non-zero outside mask0: next residue pass is full
     int residue_full;         // this step's residue pass zeroes every inactive point
-    int cntW[3], cntC[3];     // stage work-list lengths (mask0, mask1, mask2)
-    long long sumW, sumC;     // stage points processed, summed over steps
+    int cntW[3], cntC[3];     // stage work-list lengths (mask0, mask1, mask2): finest / coarser points
+    int cntXS[3], cntYS[3];   // finest points whose left / bottom neighbour is not a finest point
+    long long sumW, sumC, sumF;  // stage points and faces processed, summed over steps
 };
 
 struct Solver2D {
@@ -51,7 +52,8 @@ struct Solver2D {
     uint8_t *mt = nullptr, *m[4] = {}, *snap = nullptr, *tmp = nullptr;
     uint8_t* lv[4] = {};
     double* alpha = nullptr;     // 3 sets of (Ny alpha_x, Nx alpha_y)
-    uint32_t* lists = nullptr;   // 6 lists of Nx*Ny packed (j << 16 | i)
+    uint32_t* lists = nullptr;   // 12 lists of Nx*Ny packed (j << 16 | i): W, C, XS, YS per stage mask
+    double *FX = nullptr, *FY = nullptr;  // face fluxes: right / top face of each point, 4 planes
     double* pt = nullptr;        // print times
     Scal2* sc = nullptr;
     int grid_blocks = 0;         // persistent grid for list kernels
@@ -75,7 +77,8 @@ struct Solver2D {
     ~Solver2D();
     int init(int L, int nxc, int nyc, double gamma, double dx, double x0, int bc_kind);
     int dalloc(void* pp, size_t bytes);
-    uint32_t* list(int k, int weno) const { return lists + size_t(2 * k + (weno ? 0 : 1)) * size_t(D.Nx) * D.Ny; }
+    // type 0: finest-level points, 1: coarser points, 2: x-run starts, 3: y-run starts
+    uint32_t* list(int k, int type) const { return lists + size_t(4 * k + type) * size_t(D.Nx) * D.Ny; }
     double* ax(int s) const { return alpha + size_t(s) * (D.Nx + D.Ny); }
     double* ay(int s) const { return alpha + size_t(s) * (D.Nx + D.Ny) + D.Ny; }
 
