@@ -25,6 +25,39 @@ struct Inst1D {
     int steps;
     int status;                // 0 ok, <0 error
     int pad;
+    // Trajectory tape for the discrete adjoint (null when not recording).  Step
+    // n of the recording stores, at tape_m + n * tape_stride, the masks
+    // mask_temp, mask0..mask3 (5 x (Nx+6) bytes) and level0..level2 (3 x Nx
+    // bytes) that the step used, at tape_s + 4n its dt and the three stage
+    // alphas, and every ckpt_every steps the state Q at the step's start
+    // (3 x (Nx+6) doubles) at ckpt + (n / ckpt_every) * 3 * (Nx+6).
+    uint8_t* tape_m;
+    double* tape_s;
+    double* ckpt;
+    long long tape_stride;
+    int tape_n;                // steps recorded so far
+    int tape_cap;              // steps the tape can hold
+    int ckpt_every;
+    int pad2;
+};
+
+// Arguments of the 1-D adjoint kernel: the tape of a recorded trajectory
+// (layout as in Inst1D) and the adjoint work buffers, all device memory.
+struct Adj1DArgs {
+    const uint8_t* tape_m;
+    const double* tape_s;
+    const double* ckpt;
+    long long tape_stride;
+    int n_steps;      // recorded steps to replay / reverse
+    int ckpt_every;   // spacing of the recorded checkpoints
+    int seg_steps;    // steps per recompute segment (a multiple of ckpt_every)
+    int pad;
+    double* seg;      // seg_steps x 3 stage inputs x 3 planes x (Nx+6)
+    double* Qb;       // 3 planes x (Nx+6): adjoint state
+    double* Xb;       // 3 planes x (Nx+6)
+    double* Cb;       // 3 planes x (Nx+6): adjoint of the RK copy
+    double* Gb;       // 3 planes x (Nx+6): adjoint of dF per point
+    double* FV;       // (Nx+1) x 18: face-flux products
 };
 
 }  // namespace awcm

@@ -44,6 +44,21 @@ int get_masks1(Batch1D& B, int idx, uint8_t* masks, int32_t* levels);
 int set_mask0_1(Batch1D& B, int idx, const uint8_t* m0);
 int set_schedule1(Batch1D& B, const double* pt, int npt);
 
+// Tape and work buffers of the 1-D discrete adjoint (one per handle).
+struct Adj1DHost {
+    std::vector<void*> allocs;
+    uint8_t* tape_m = nullptr;
+    double* tape_s = nullptr;
+    double* ckpt = nullptr;
+    long long stride = 0;
+    int cap = 0, steps = 0, K = 0;
+    double* seg = nullptr;
+    int seg_steps = 0;
+    double *Qb = nullptr, *Xb = nullptr, *Cb = nullptr, *Gb = nullptr, *FV = nullptr, *Qsave = nullptr;
+    ~Adj1DHost();
+    int dalloc(void* pp, size_t bytes);
+};
+
 }  // namespace awcm
 
 struct awcm_batch {
@@ -54,5 +69,6 @@ struct awcm_solver {
     int dim = 1;
     awcm::Batch1D b1;       // dim 1
     awcm::Solver2D* s2 = nullptr;  // dim 2
-    ~awcm_solver() { delete s2; }
+    awcm::Adj1DHost* adj = nullptr;  // dim 1, after awcm_adjoint_record
+    ~awcm_solver() { delete s2; delete adj; }
 };

@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libawcm_hip.so")
 
 AWCM_OK = 0
-ERRORS = {-1: "EINVAL", -2: "EDEVICE", -3: "ENOMEM", -4: "ENONFINITE", -5: "ERANGE", -6: "ESTATE"}
+ERRORS = {-1: "EINVAL", -2: "EDEVICE", -3: "ENOMEM", -4: "ENONFINITE", -5: "ERANGE", -6: "ESTATE", -7: "EDISABLED"}
 BC_TRANSMISSIVE = 0
 BC_DOUBLE_MACH = 1
 
@@ -71,6 +71,9 @@ def lib():
         "awcm_profile_enable": [V, I],
         "awcm_set_tuning": [V, I, I],
         "awcm_profile_read": [V, I, _dp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)],
+        "awcm_adjoint_record": [V, D, D, D, I, I, _dp, C.POINTER(I), C.POINTER(I)],
+        "awcm_adjoint_replay": [V],
+        "awcm_adjoint": [V, _dp, I, _dp],
         "awcm_ic_riemann_1d": [I, D, D, D, _dp, _dp, D, _dp, _dp, _dp],
         "awcm_ic_shu_osher_1d": [I, D, D, D, _dp, D, D, _dp, _dp, _dp],
         "awcm_ic_quadrants_2d": [I, D, _dp, D, _dp, _dp],
@@ -204,6 +207,30 @@ class Solver:
 
     def synchronize(self):
         _check(lib().awcm_synchronize(self.h), "awcm_synchronize")
+
+    # ---- discrete adjoint (1-D) ----
+    def adjoint_record(self, tol, cfl, t_final, max_steps, checkpoint_every=32):
+        """awcm_run that also records the frozen trajectory; returns steps."""
+        t = np.array([self.t])
+        cnt = C.c_int(self.cnt)
+        steps = C.c_int(0)
+        _check(lib().awcm_adjoint_record(self.h, tol, cfl, t_final, max_steps, checkpoint_every, _d(t),
+                                         C.byref(cnt), C.byref(steps)), "awcm_adjoint_record")
+        self.t = float(t[0])
+        self.cnt = cnt.value
+        return steps.value
+
+    def adjoint_replay(self):
+        """Frozen forward over the recorded steps, from the current state."""
+        _check(lib().awcm_adjoint_replay(self.h), "awcm_adjoint_replay")
+
+    def adjoint(self, dJ_dQfinal, checkpoint_every=0):
+        """dJ/dQ0 through the recorded steps (layout of upload_q)."""
+        g = np.ascontiguousarray(dJ_dQfinal, dtype=np.float64)
+        assert g.size == self.npts * self.ncomp
+        out = np.empty_like(g)
+        _check(lib().awcm_adjoint(self.h, _d(g), checkpoint_every, _d(out)), "awcm_adjoint")
+        return out
 
 
 KERNEL_STAGE_WENO, KERNEL_STAGE_COARSE, KERNEL_PREPARE = 0, 1, 2

@@ -38,7 +38,8 @@ enum {
     AWCM_ENOMEM = -3,     /* device allocation failed                           */
     AWCM_ENONFINITE = -4, /* state not finite / non-positive (sqrt of negative) */
     AWCM_ERANGE = -5,     /* a mask scatter left the ghosted domain             */
-    AWCM_ESTATE = -6      /* call not valid in the handle's current state       */
+    AWCM_ESTATE = -6,     /* call not valid in the handle's current state       */
+    AWCM_EDISABLED = -7   /* entry point switched off in this build             */
 };
 
 /* Ghost-fill kinds (the per-problem BOUNDARY_MANAGER classes). */
@@ -105,6 +106,40 @@ int awcm_set_schedule(awcm_solver* h, const double* print_times, int n_print);
  * and updates (sum over steps of |mask0|) may be NULL. */
 int awcm_run(awcm_solver* h, double tol, double cfl, double t_final, int max_steps, double* t, int* cnt,
              int* steps_done, long long* updates);
+
+/* ---- discrete adjoint (1-D) ------------------------------------------------
+ * The reference has no adjoint; these entry points differentiate its 1-D time
+ * loop (the awcm_run path) with the adaptive data frozen per step.
+ *
+ * awcm_adjoint_record: as awcm_run (same arguments, bit-identical results),
+ * and records for every step its masks mask_temp/mask0..3, levels level0..2,
+ * dt and the three stage alphas, plus the state at the start of every
+ * checkpoint_every-th step.  Replaces any earlier recording.  max_steps also
+ * sizes the tape (about 8 (Nx+6) bytes per step plus 24 (Nx+6) bytes per
+ * checkpoint). */
+int awcm_adjoint_record(awcm_solver* h, double tol, double cfl, double t_final, int max_steps, int checkpoint_every,
+                        double* t, int* cnt, int* steps_done);
+
+/* Frozen forward: replays the recorded steps from the current state (upload it
+ * first) with the recorded masks, levels, dt and alphas.  From the recording's
+ * initial state it reproduces the recorded run bit for bit; from a perturbed
+ * state it is the map whose derivative awcm_adjoint computes. */
+int awcm_adjoint_replay(awcm_solver* h);
+
+/* dJ/dQ0 = (dQ_final/dQ0)^T dJ/dQ_final through the recorded steps (frozen
+ * masks, levels, dt and alphas), by checkpoint-and-recompute: the steps are
+ * reversed in segments of checkpoint_every steps (a positive multiple of the
+ * recording's spacing; 0 = the recording's), each recomputed from its
+ * checkpoint.  Both arrays use the layout of awcm_upload_q; ghost entries of
+ * dJ_dQfinal are ignored and those of dJ_dQ0 are returned as 0.  The device
+ * state Q is left unchanged.  Deterministic: the result does not depend on
+ * the segment length and is identical between calls.
+ * Returns AWCM_EDISABLED in this build: the backward kernel's first device
+ * run ended in a GPU memory fault whose cause is not yet identified (its
+ * device code runs clean, and matches finite differences, in the host
+ * emulation of tests/native/emu1d.cpp, including under AddressSanitizer with
+ * 1024 emulated threads), so it is not launched on a GPU until it is. */
+int awcm_adjoint(awcm_solver* h, const double* dJ_dQfinal, int checkpoint_every, double* dJ_dQ0);
 
 /* As awcm_run, but all n_steps are queued with no host synchronisation in
  * between (steps past t_final are no-ops on the device) and elapsed_ms is the

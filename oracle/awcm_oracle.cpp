@@ -516,9 +516,52 @@ struct Solver1D {
                 out[p] = D02 * f_rr[p] + D12 * f_r[p] + D22 * f_c[p] + D32 * f_l[p] + D2 * f_ll[p];
         }
     }
+    // The three SSP-RK3 stages and RemoveResidue of one driver-loop iteration
+    // (main_1d_euler.cpp:97-257), with dt and the stage alphas given.
+    void stages(double dt, const double* al) {
+        bc();
+        Qc = Q; Qt = Q;
+        for (int i = 0; i <= Nx - 1; i++) {
+            if (mk(m[2], i) == 0) continue;
+            const int lev = lv[2][i];
+            double f[3]; dF(i, lev, al[0], f);
+            const double cc = dt / (dx * dxl[lev]);
+            for (int k = 0; k < 3; k++) q(Q, i)[k] = q(Q, i)[k] - cc * f[k];
+        }
+        bc();
+        Qt = Q;
+        for (int i = 0; i <= Nx - 1; i++) {
+            if (mk(m[1], i) == 0) continue;
+            const int lev = lv[1][i];
+            double f[3]; dF(i, lev, al[1], f);
+            const double cc = dt / (dx * dxl[lev]);
+            for (int k = 0; k < 3; k++) q(Q, i)[k] = 0.75 * q(Qc, i)[k] + 0.25 * (q(Q, i)[k] - cc * f[k]);
+        }
+        bc();
+        Qt = Q;
+        for (int i = 0; i <= Nx - 1; i++) {
+            if (mk(m[0], i) == 0) continue;
+            const int lev = lv[0][i];
+            double f[3]; dF(i, lev, al[2], f);
+            const double cc = dt / (dx * dxl[lev]);
+            for (int k = 0; k < 3; k++) q(Q, i)[k] = (q(Qc, i)[k] + 2.0 * (q(Q, i)[k] - cc * f[k])) / 3.0;
+        }
+        post();
+    }
+    // Frozen-trajectory tape for finite-difference checks of the discrete
+    // adjoint: per step the masks, levels, dt and stage alphas it used.
+    struct TapeStep {
+        std::vector<uint8_t> mt, m[4];
+        std::vector<int> lv[3];
+        double dt, al[3];
+    };
+    std::vector<TapeStep> tape;
+    bool recording = false;
+
     // One iteration of the driver loop (main_1d_euler.cpp:61-260).  Returns dt;
     // *cnt is the print counter (starts at 1), *t the time.
     double step(double tol, double cfl, const double* pt, int npt, int* cnt, double* t) {
+        std::vector<uint8_t> m_start = m[0];
         prepare(tol);
         double vmax = 0;
         for (int i = 0; i <= Nx - 1; i++) {
@@ -530,40 +573,68 @@ struct Solver1D {
         double dt = cfl * dx / vmax;
         bool do_print = false;
         if (*cnt < npt && *t + dt > pt[*cnt]) { dt = pt[*cnt] - *t; do_print = true; }
+        // alphas: stage s uses mask_{3-s} of the stage input (main_1d_euler.cpp:86-95)
         bc();
-        double a = alpha(3);
-        Qc = Q; Qt = Q;
-        for (int i = 0; i <= Nx - 1; i++) {
-            if (mk(m[2], i) == 0) continue;
-            const int lev = lv[2][i];
-            double f[3]; dF(i, lev, a, f);
-            const double cc = dt / (dx * dxl[lev]);
-            for (int k = 0; k < 3; k++) q(Q, i)[k] = q(Q, i)[k] - cc * f[k];
+        double al[3];
+        al[0] = alpha(3);
+        {
+            // stage inputs for alpha(2) and alpha(1) are only known stage by stage
+            // (they depend on the stage results), so run the stages inline
+            Qc = Q; Qt = Q;
+            for (int i = 0; i <= Nx - 1; i++) {
+                if (mk(m[2], i) == 0) continue;
+                const int lev = lv[2][i];
+                double f[3]; dF(i, lev, al[0], f);
+                const double cc = dt / (dx * dxl[lev]);
+                for (int k = 0; k < 3; k++) q(Q, i)[k] = q(Q, i)[k] - cc * f[k];
+            }
+            bc();
+            al[1] = alpha(2);
+            Qt = Q;
+            for (int i = 0; i <= Nx - 1; i++) {
+                if (mk(m[1], i) == 0) continue;
+                const int lev = lv[1][i];
+                double f[3]; dF(i, lev, al[1], f);
+                const double cc = dt / (dx * dxl[lev]);
+                for (int k = 0; k < 3; k++) q(Q, i)[k] = 0.75 * q(Qc, i)[k] + 0.25 * (q(Q, i)[k] - cc * f[k]);
+            }
+            bc();
+            al[2] = alpha(1);
+            Qt = Q;
+            for (int i = 0; i <= Nx - 1; i++) {
+                if (mk(m[0], i) == 0) continue;
+                const int lev = lv[0][i];
+                double f[3]; dF(i, lev, al[2], f);
+                const double cc = dt / (dx * dxl[lev]);
+                for (int k = 0; k < 3; k++) q(Q, i)[k] = (q(Qc, i)[k] + 2.0 * (q(Q, i)[k] - cc * f[k])) / 3.0;
+            }
+            post();
         }
-        bc();
-        a = alpha(2);
-        Qt = Q;
-        for (int i = 0; i <= Nx - 1; i++) {
-            if (mk(m[1], i) == 0) continue;
-            const int lev = lv[1][i];
-            double f[3]; dF(i, lev, a, f);
-            const double cc = dt / (dx * dxl[lev]);
-            for (int k = 0; k < 3; k++) q(Q, i)[k] = 0.75 * q(Qc, i)[k] + 0.25 * (q(Q, i)[k] - cc * f[k]);
+        if (recording) {
+            TapeStep s;
+            s.mt = m_start;
+            for (int k = 0; k < 4; k++) s.m[k] = m[k];
+            for (int k = 0; k < 3; k++) s.lv[k] = lv[k];
+            s.dt = dt;
+            for (int k = 0; k < 3; k++) s.al[k] = al[k];
+            tape.push_back(std::move(s));
         }
-        bc();
-        a = alpha(1);
-        Qt = Q;
-        for (int i = 0; i <= Nx - 1; i++) {
-            if (mk(m[0], i) == 0) continue;
-            const int lev = lv[0][i];
-            double f[3]; dF(i, lev, a, f);
-            const double cc = dt / (dx * dxl[lev]);
-            for (int k = 0; k < 3; k++) q(Q, i)[k] = (q(Qc, i)[k] + 2.0 * (q(Q, i)[k] - cc * f[k])) / 3.0;
-        }
-        post();
         *t += dt;
         if (do_print) (*cnt)++;
         return dt;
+    }
+    // Step n of the tape with masks, levels, dt and alphas frozen: the same
+    // arithmetic as step(), since the FWT consults mask_temp at every point
+    // before thresholding it (WAVELET_1D.h:282-335).
+    void replay(int n) {
+        const TapeStep& s = tape[n];
+        std::vector<uint8_t> mcopy = s.mt;
+        fwt(mcopy, -1.0);
+        for (int k = 0; k < 4; k++) m[k] = s.m[k];
+        for (int k = 0; k < 3; k++) lv[k] = s.lv[k];
+        iwt(m[3]);
+        interp(mcopy, m[3]);
+        stages(s.dt, s.al);
     }
 };
 
@@ -1249,6 +1320,25 @@ double orc1_step(void* h, double tol, double cfl, const double* pt, int npt, int
     return static_cast<Solver1D*>(h)->step(tol, cfl, pt, npt, cnt, t);
 }
 long orc1_oob(void* h) { return static_cast<Solver1D*>(h)->oob; }
+// Frozen-trajectory tape: record the following orc1_step calls, then replay
+// them from any state (finite-difference checks of the discrete adjoint).
+void orc1_record(void* h, int on) {
+    auto* s = static_cast<Solver1D*>(h);
+    s->recording = on != 0;
+    if (on) s->tape.clear();
+}
+int orc1_tape_steps(void* h) { return int(static_cast<Solver1D*>(h)->tape.size()); }
+void orc1_tape_scalars(void* h, double* out) {
+    auto* s = static_cast<Solver1D*>(h);
+    for (size_t n = 0; n < s->tape.size(); n++) {
+        out[4 * n] = s->tape[n].dt;
+        for (int k = 0; k < 3; k++) out[4 * n + 1 + k] = s->tape[n].al[k];
+    }
+}
+void orc1_replay(void* h, int first, int count) {
+    auto* s = static_cast<Solver1D*>(h);
+    for (int n = first; n < first + count && n < int(s->tape.size()); n++) s->replay(n);
+}
 // WENOZ_EULER_SOLVER_1D::ComputeFlux on explicit states (6 x 3 doubles)
 void orc_flux1(double gamma, const double* states, double alpha, double* flux) {
     const double* s[6] = {states, states + 3, states + 6, states + 9, states + 12, states + 15};

@@ -1,0 +1,145 @@
+// CPU emulation of the 1-D HIP kernels (k1_solve with tape recording, and
+// k1_adjoint), built with g++ against tests/native/emu/hip/hip_runtime.h.  The
+// device code is compiled unchanged; only the launch is replaced by a call.
+// Test infrastructure only: it lets the CPU suite check the adjoint's device
+// code against finite differences without a GPU.
+#include <vector>
+
+#include "awcm1d.hip"
+#include "adjoint1d_dev.h"
+
+namespace awcm {
+uint8_t sm[1 << 21];  // dynamic LDS of the emulated workgroup
+}
+
+using namespace awcm;
+
+namespace {
+
+struct Emu {
+    Inst1D in{};
+    std::vector<double> Q, Qc, Qt, F, pt;
+    std::vector<uint8_t> masks, levels, tape_m;
+    std::vector<double> tape_s, ckpt, seg, Qb, Xb, Cb, Gb, FV;
+    int steps = 0, K = 1;
+    int NP() const { return in.Nx + 6; }
+    int nxp() const { return (NP() + 15) / 16 * 16; }
+};
+
+void launch_solve(Emu& e, int mode, int max_steps) {
+    int order = 0, qctr = 0;
+    k1_solve(&e.in, 1, &order, &qctr, mode, max_steps, e.nxp());
+}
+
+}  // namespace
+
+extern "C" {
+
+void* emu_create(int L, int nxc, double gamma, double dx, const double* pt, int npt) {
+    auto* e = new Emu();
+    Inst1D& in = e->in;
+    in.L = L;
+    in.nxc = nxc;
+    in.Nx = (nxc - 1) * (1 << (L - 1)) + 1;
+    in.gamma = gamma;
+    in.dx = dx;
+    in.cnt = 1;
+    const int NP = e->NP();
+    e->Q.assign(3 * NP, 0.0);
+    e->Qc = e->Q;
+    e->Qt = e->Q;
+    e->F.assign(3 * (in.Nx + 1), 0.0);
+    e->masks.assign(5 * NP, 0);
+    std::fill(e->masks.begin() + NP, e->masks.begin() + 2 * NP, 1);
+    e->levels.assign(4 * in.Nx, 0);
+    e->pt.assign(pt, pt + npt);
+    in.Q = e->Q.data(); in.Qc = e->Qc.data(); in.Qt = e->Qt.data(); in.F = e->F.data();
+    in.masks = e->masks.data(); in.levels = e->levels.data();
+    in.print_times = e->pt.data(); in.n_print = npt;
+    return e;
+}
+
+void emu_destroy(void* h) { delete static_cast<Emu*>(h); }
+
+void emu_set_q(void* h, const double* q) {
+    auto* e = static_cast<Emu*>(h);
+    const int NP = e->NP();
+    for (int i = 0; i < NP; i++)
+        for (int c = 0; c < 3; c++) e->Q[c * NP + i] = q[i * 3 + c];
+}
+
+void emu_get_q(void* h, double* q) {
+    auto* e = static_cast<Emu*>(h);
+    const int NP = e->NP();
+    for (int i = 0; i < NP; i++)
+        for (int c = 0; c < 3; c++) q[i * 3 + c] = e->Q[c * NP + i];
+}
+
+int emu_record(void* h, double tol, double cfl, double t_final, int max_steps, int K) {
+    auto* e = static_cast<Emu*>(h);
+    Inst1D& in = e->in;
+    // each recording starts a fresh run: t = 0, print counter 1, the masks and
+    // levels of a new solver (mask0 all ones, AWCM_1D.h:23-34)
+    const int NPr = e->NP();
+    in.t = 0.0; in.cnt = 1; in.steps = 0; in.sum_active0 = 0; in.status = 0;
+    std::fill(e->masks.begin(), e->masks.end(), 0);
+    std::fill(e->masks.begin() + NPr, e->masks.begin() + 2 * NPr, 1);
+    std::fill(e->levels.begin(), e->levels.end(), 0);
+    const long long NP = e->NP();
+    const long long stride = (5 * NP + 3LL * in.Nx + 255) / 256 * 256;
+    e->tape_m.assign(size_t(stride) * max_steps, 0);
+    e->tape_s.assign(4 * size_t(max_steps), 0.0);
+    e->ckpt.assign(3 * NP * size_t(max_steps / K + 1), 0.0);
+    e->K = K;
+    in.tol = tol; in.cfl = cfl; in.t_final = t_final;
+    in.tape_m = e->tape_m.data(); in.tape_s = e->tape_s.data(); in.ckpt = e->ckpt.data();
+    in.tape_stride = stride; in.tape_n = 0; in.tape_cap = max_steps; in.ckpt_every = K;
+    launch_solve(*e, 1, max_steps);
+    e->steps = in.tape_n;
+    in.tape_m = nullptr; in.tape_s = nullptr; in.ckpt = nullptr;
+    return e->steps;
+}
+
+static Adj1DArgs args(Emu& e, int seg_steps) {
+    Adj1DArgs A{};
+    const long long NP = e.NP();
+    A.tape_m = e.tape_m.data();
+    A.tape_s = e.tape_s.data();
+    A.ckpt = e.ckpt.data();
+    A.tape_stride = (5 * NP + 3LL * e.in.Nx + 255) / 256 * 256;
+    A.n_steps = e.steps;
+    A.ckpt_every = e.K;
+    A.seg_steps = seg_steps;
+    e.seg.assign(9 * NP * size_t(seg_steps), 0.0);
+    e.Qb.assign(3 * NP, 0.0);
+    e.Xb.assign(3 * NP, 0.0);
+    e.Cb.assign(3 * NP, 0.0);
+    e.Gb.assign(3 * NP, 0.0);
+    e.FV.assign(18 * size_t(e.in.Nx + 1), 0.0);
+    A.seg = e.seg.data(); A.Qb = e.Qb.data(); A.Xb = e.Xb.data(); A.Cb = e.Cb.data();
+    A.Gb = e.Gb.data(); A.FV = e.FV.data();
+    return A;
+}
+
+void emu_replay(void* h) {
+    auto* e = static_cast<Emu*>(h);
+    k1_adjoint(&e->in, args(*e, e->K), 0, e->nxp());
+}
+
+void emu_adjoint(void* h, const double* gf, int seg_steps, double* g0) {
+    auto* e = static_cast<Emu*>(h);
+    const int NP = e->NP();
+    std::vector<double> Qsave = e->Q;
+    Adj1DArgs A = args(*e, seg_steps ? seg_steps : e->K);
+    for (int i = 3; i < NP - 3; i++)
+        for (int c = 0; c < 3; c++) e->Qb[c * NP + i] = gf[i * 3 + c];
+    k1_adjoint(&e->in, A, 1, e->nxp());
+    e->Q = Qsave;
+    e->in.Q = e->Q.data();
+    for (int i = 0; i < NP; i++)
+        for (int c = 0; c < 3; c++) g0[i * 3 + c] = (i < 3 || i >= NP - 3) ? 0.0 : e->Qb[c * NP + i];
+}
+
+int emu_status(void* h) { return static_cast<Emu*>(h)->in.status; }
+
+}  // extern "C"

@@ -76,6 +76,10 @@ def load(prefix="orc"):
         sig["orc_ic2"] = (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                     C.c_double, C.c_double, _dp, C.c_double, _dp, _dp, _dp])
         sig["orc1_oob"] = (C.c_long, [V])
+        sig["orc1_record"] = (None, [V, C.c_int])
+        sig["orc1_tape_steps"] = (C.c_int, [V])
+        sig["orc1_tape_scalars"] = (None, [V, _dp])
+        sig["orc1_replay"] = (None, [V, C.c_int, C.c_int])
         sig["orc2_oob"] = (C.c_long, [V])
     else:
         sig["ref2_create"] = (V, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
@@ -178,6 +182,25 @@ class Solver1D:
         pts = np.ascontiguousarray(pts, dtype=np.float64)
         return getattr(self.lib, f"{self.p}1_step")(self.h, tol, cfl, dptr(pts), len(pts),
                                                        C.byref(self.cnt), dptr(self.t))
+
+    # frozen-trajectory tape (oracle only)
+    def record(self, on=True):
+        self.lib.orc1_record(self.h, 1 if on else 0)
+
+    def tape_steps(self):
+        return self.lib.orc1_tape_steps(self.h)
+
+    def tape_scalars(self):
+        """(steps, 4) array: dt and the three stage alphas of each recorded step."""
+        n = self.tape_steps()
+        out = np.zeros(4 * max(n, 1))
+        self.lib.orc1_tape_scalars(self.h, dptr(out))
+        return out[:4 * n].reshape(n, 4)
+
+    def replay(self, first=0, count=None):
+        """Replay recorded steps from the current state with everything frozen."""
+        count = self.tape_steps() - first if count is None else count
+        self.lib.orc1_replay(self.h, first, count)
 
 
 class Solver2D:

@@ -1,0 +1,357 @@
+"""CPU checks of the discrete adjoint's building blocks.
+
+* The reverse-mode products of awcm/adjoint.h (host build of the product
+  header) against central finite differences of the forward functions in
+  awcm/physics.h, on random non-degenerate states.
+* The oracle's frozen replay (masks, levels, dt and alphas from a tape)
+  reproduces its own forward run bit for bit, which is what makes finite
+  differences of the replay a reference gradient for the adjoint.
+* The device code of the 1-D solver and adjoint kernels, compiled unchanged
+  for the host by tests/native/emu1d.cpp (one emulated thread, barriers as
+  no-ops -- valid because every pass between barriers is race-free): its
+  forward and frozen replay are bit-identical to the oracle's, and its
+  adjoint matches central finite differences of the oracle's frozen replay.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+_dp = C.POINTER(C.c_double)
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+@pytest.fixture(scope="module")
+def shim(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("shim") / "vjp_shim.so")
+    src = os.path.join(ROOT, "tests", "native", "vjp_shim.cpp")
+    inc = os.path.join(ROOT, "c_pp-library_amd", "include")
+    subprocess.run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-I", inc, src, "-o", out],
+                   check=True)
+    L = C.CDLL(out)
+    D = C.c_double
+    L.shim_face_flux1.argtypes = [D, _dp, D, _dp]
+    L.shim_face_flux1_vjp.argtypes = [D, _dp, D, _dp, _dp]
+    L.shim_flux_f1.argtypes = [D, _dp, _dp]
+    L.shim_flux_f1_vjp.argtypes = [D, _dp, _dp, _dp]
+    L.shim_wenoz_left.argtypes = [_dp]
+    L.shim_wenoz_left.restype = D
+    L.shim_wenoz_left_vjp.argtypes = [_dp, D, _dp]
+    for f in (L.shim_face_flux1, L.shim_face_flux1_vjp, L.shim_flux_f1, L.shim_flux_f1_vjp, L.shim_wenoz_left_vjp):
+        f.restype = None
+    return L
+
+
+def random_states(rng, n):
+    rho = rng.uniform(0.5, 2.0, n)
+    u = rng.uniform(-1.0, 1.0, n)
+    p = rng.uniform(0.5, 2.0, n)
+    return np.stack([rho, rho * u, p / 0.4 + 0.5 * rho * u * u], axis=1)
+
+
+def fd_vjp(f, x, ybar, h):
+    """(df/dx)^T ybar by central differences, component by component."""
+    g = np.zeros_like(x)
+    for k in range(x.size):
+        xp, xm = x.copy(), x.copy()
+        xp.flat[k] += h * max(1.0, abs(x.flat[k]))
+        xm.flat[k] -= h * max(1.0, abs(x.flat[k]))
+        g.flat[k] = np.dot(f(xp) - f(xm), ybar) / (xp.flat[k] - xm.flat[k])
+    return g
+
+
+def test_flux_vjp_matches_fd(shim):
+    rng = np.random.default_rng(1)
+    for _ in range(20):
+        q = random_states(rng, 1)[0].copy()
+        fb = rng.standard_normal(3)
+
+        def f(x):
+            out = np.zeros(3)
+            shim.shim_flux_f1(1.4, _d(x), _d(out))
+            return out
+
+        g = np.zeros(3)
+        shim.shim_flux_f1_vjp(1.4, _d(q), _d(fb), _d(g))
+        ref = fd_vjp(f, q, fb, 1e-6)
+        assert np.allclose(g, ref, rtol=1e-7, atol=1e-8 * np.abs(ref).max())
+
+
+def test_wenoz_vjp_matches_fd(shim):
+    rng = np.random.default_rng(2)
+    for _ in range(50):
+        x = rng.standard_normal(5)
+        ob = rng.standard_normal()
+
+        def f(y):
+            return np.array([shim.shim_wenoz_left(_d(np.ascontiguousarray(y)))])
+
+        g = np.zeros(5)
+        shim.shim_wenoz_left_vjp(_d(x), ob, _d(g))
+        ref = fd_vjp(f, x, np.array([ob]), 1e-7)
+        assert np.allclose(g, ref, rtol=1e-6, atol=1e-7 * max(1.0, np.abs(ref).max()))
+
+
+def test_wenoz_vjp_uniform_stencil_gives_linear_weights(shim):
+    """At an exactly uniform stencil the product is the linear-weight one:
+    0.1 q0 + 0.6 q1 + 0.3 q2 = (-3, 27, 47, -13, 2)/60 of ll..rr, also for a
+    value whose multiples round (3 * 0.7 is inexact)."""
+    for v in (0.7, 3.857143, 1.0):
+        x = np.full(5, v)
+        g = np.zeros(5)
+        shim.shim_wenoz_left_vjp(_d(x), 1.0, _d(g))
+        assert np.allclose(g, np.array([-3, 27, 47, -13, 2]) / 60.0, rtol=0, atol=1e-15), (v, g)
+
+
+def test_face_flux_vjp_matches_fd(shim):
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        s = random_states(rng, 6)
+        alpha = 2.5
+        fb = rng.standard_normal(3)
+
+        def f(x):
+            out = np.zeros(3)
+            shim.shim_face_flux1(1.4, _d(np.ascontiguousarray(x)), alpha, _d(out))
+            return out
+
+        g = np.zeros(18)
+        shim.shim_face_flux1_vjp(1.4, _d(s), alpha, _d(fb), _d(g))
+        ref = fd_vjp(f, s.reshape(-1).copy(), fb, 1e-6)
+        err = np.abs(g - ref).max() / np.abs(ref).max()
+        assert err < 1e-6, err
+
+
+def test_oracle_frozen_replay_reproduces_forward():
+    q, dx, _ = O.ic1(1, 1025, -5.0, 5.0, -4.0, O.SHU_L, O.SHU_L)
+    o = O.Solver1D("orc", 7, 17, 1.4, dx)
+    o.set_q(q)
+    pts = O.print_times(1.8, 2)
+    o.record()
+    for _ in range(60):
+        o.step(1e-4, 0.5, pts)
+    qf = o.get_q()
+    sc = o.tape_scalars()
+    assert sc.shape == (60, 4) and np.all(sc > 0)
+    o.set_q(q)
+    o.replay()
+    assert np.array_equal(o.get_q().view(np.uint64), qf.view(np.uint64))
+
+
+@pytest.fixture(scope="module")
+def emu(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("emu") / "emu1d.so")
+    pkg = os.path.join(ROOT, "c_pp-library_amd")
+    subprocess.run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-fPIC", "-shared",
+                    "-I", os.path.join(ROOT, "tests", "native", "emu"), "-I", os.path.join(pkg, "include"),
+                    "-I", os.path.join(pkg, "csrc"), "-I", os.path.join(ROOT, "include"), "-x", "c++",
+                    os.path.join(ROOT, "tests", "native", "emu1d.cpp"), "-o", out], check=True)
+    E = C.CDLL(out)
+    V = C.c_void_p
+    E.emu_create.restype = V
+    E.emu_create.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, _dp, C.c_int]
+    E.emu_destroy.argtypes = [V]
+    E.emu_set_q.argtypes = [V, _dp]
+    E.emu_get_q.argtypes = [V, _dp]
+    E.emu_record.argtypes = [V, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int]
+    E.emu_record.restype = C.c_int
+    E.emu_replay.argtypes = [V]
+    E.emu_adjoint.argtypes = [V, _dp, C.c_int, _dp]
+    E.emu_status.argtypes = [V]
+    return E
+
+
+def test_emulated_kernels_forward_replay_and_adjoint(emu):
+    """Shu-Osher at 513 points to t = 0.06 (about 30 steps)."""
+    L, T = 6, 0.06
+    nx = 16 * 2 ** (L - 1) + 1
+    q, dx, x0 = O.ic1(1, nx, -5.0, 5.0, -4.0, O.SHU_L, O.SHU_L)
+    pts = O.print_times(T, 2)
+    h = emu.emu_create(L, 17, 1.4, dx, _d(pts), len(pts))
+    try:
+        emu.emu_set_q(h, _d(q))
+        n = emu.emu_record(h, 1e-4, 0.5, T, 1000, 8)
+        qe = np.zeros_like(q)
+        emu.emu_get_q(h, _d(qe))
+        o = O.Solver1D("orc", L, 17, 1.4, dx)
+        o.set_q(q)
+        o.record()
+        for _ in range(n):
+            o.step(1e-4, 0.5, pts)
+        assert n > 20 and emu.emu_status(h) == 0
+        assert np.array_equal(qe.view(np.uint64), o.get_q().view(np.uint64))
+        rng = np.random.default_rng(5)
+        x = x0 + dx * np.arange(nx)
+        s = np.clip((x + 3.9) / 8.9, 0, 1)
+        basis = np.stack([np.sin((k + 1) * np.pi * s) for k in range(16)])
+        v = np.zeros((nx + 6, 3))
+        v[3:-3, 0] = rng.standard_normal(16) @ basis
+        v = v.reshape(-1)
+        # frozen replay: emulated kernel == oracle, from a perturbed state
+        qp = q + 1e-6 * v
+        emu.emu_set_q(h, _d(qp))
+        emu.emu_replay(h)
+        o.set_q(qp)
+        o.replay()
+        got = np.zeros_like(q)
+        emu.emu_get_q(h, _d(got))
+        assert np.array_equal(got.view(np.uint64), o.get_q().view(np.uint64))
+        # adjoint vs central FD of the oracle's frozen replay
+        rho_t = qe.reshape(-1, 3)[3:-3, 0] + 0.01 * np.sin(0.05 * np.arange(nx))
+
+        def J(q0):
+            o.set_q(q0)
+            o.replay()
+            r = o.get_q().reshape(-1, 3)[3:-3, 0] - rho_t
+            return float(r @ r)
+
+        gf = np.zeros((nx + 6, 3))
+        gf[3:-3, 0] = 2 * (qe.reshape(-1, 3)[3:-3, 0] - rho_t)
+        g0 = np.zeros_like(q)
+        emu.emu_set_q(h, _d(q))
+        emu.emu_adjoint(h, _d(gf.reshape(-1)), 0, _d(g0))
+        fd = (J(q + 1e-6 * v) - J(q - 1e-6 * v)) / 2e-6
+        assert abs(g0 @ v - fd) <= 1e-6 * abs(fd), (g0 @ v, fd)
+        # segment length does not change the result
+        g1 = np.zeros_like(q)
+        emu.emu_adjoint(h, _d(gf.reshape(-1)), 24, _d(g1))
+        assert np.array_equal(g0.view(np.uint64), g1.view(np.uint64))
+    finally:
+        emu.emu_destroy(h)
+
+
+class EmuSolver:
+    """The emulated 1-D kernels (record, frozen replay, adjoint) for one solve."""
+
+    def __init__(self, E, L, dx, t_final):
+        self.E = E
+        self.pts = O.print_times(t_final, 2)
+        self.h = E.emu_create(L, 17, 1.4, dx, _d(self.pts), len(self.pts))
+        self.n = 16 * 2 ** (L - 1) + 7
+
+    def close(self):
+        self.E.emu_destroy(self.h)
+
+    def set_q(self, q):
+        self.E.emu_set_q(self.h, _d(np.ascontiguousarray(q)))
+
+    def get_q(self):
+        q = np.zeros(3 * self.n)
+        self.E.emu_get_q(self.h, _d(q))
+        return q
+
+    def record(self, q0, tol, t_final, K):
+        self.set_q(q0)
+        return self.E.emu_record(self.h, tol, 0.5, t_final, 20000, K)
+
+    def replay(self, q0):
+        self.set_q(q0)
+        self.E.emu_replay(self.h)
+        return self.get_q()
+
+    def adjoint(self, gf, K=0):
+        g = np.zeros(3 * self.n)
+        self.E.emu_adjoint(self.h, _d(np.ascontiguousarray(gf)), K, _d(g))
+        return g
+
+
+def density_direction(nx, x0, dx, rng, lo=-3.9):
+    x = x0 + dx * np.arange(nx)
+    s = np.clip((x - lo) / (5.0 - lo), 0, 1)
+    basis = np.stack([np.sin((k + 1) * np.pi * s) for k in range(16)])
+    v = np.zeros((nx + 6, 3))
+    v[3:-3, 0] = rng.standard_normal(16) @ basis
+    return v.reshape(-1)
+
+
+def test_torch_autograd_wrapper_on_emulated_kernels(emu):
+    """A torch.autograd.Function around the frozen replay (forward) and the
+    adjoint (backward) of the emulated kernels: J.backward() gives the
+    gradient that central finite differences of the oracle's replay give."""
+    torch = pytest.importorskip("torch")
+    L, T = 6, 0.06
+    nx = 16 * 2 ** (L - 1) + 1
+    q, dx, x0 = O.ic1(1, nx, -5.0, 5.0, -4.0, O.SHU_L, O.SHU_L)
+    S = EmuSolver(emu, L, dx, T)
+    try:
+        S.record(q, 1e-4, T, 8)
+        qf = S.get_q()
+        rho_t = torch.from_numpy(qf.reshape(-1, 3)[3:-3, 0] * 1.01)
+
+        class FrozenAWCM(torch.autograd.Function):
+            @staticmethod
+            def forward(ctx, q0):
+                return torch.from_numpy(S.replay(q0.detach().numpy()))
+
+            @staticmethod
+            def backward(ctx, g):
+                return torch.from_numpy(S.adjoint(g.detach().numpy()))
+
+        q0 = torch.from_numpy(q.copy()).requires_grad_(True)
+        J = ((FrozenAWCM.apply(q0).view(-1, 3)[3:-3, 0] - rho_t) ** 2).sum()
+        J.backward()
+        v = density_direction(nx, x0, dx, np.random.default_rng(21))
+
+        def Jn(qq):
+            r = S.replay(qq).reshape(-1, 3)[3:-3, 0] - rho_t.numpy()
+            return float(r @ r)
+
+        fd = (Jn(q + 1e-6 * v) - Jn(q - 1e-6 * v)) / 2e-6
+        ad = float(q0.grad.numpy() @ v)
+        assert abs(ad - fd) <= 1e-6 * abs(fd), (ad, fd)
+    finally:
+        S.close()
+
+
+def test_emulated_adjoint_full_shu_osher(emu):
+    """The Shu-Osher adjoint case at its full size: 9 levels (4097 points),
+    eps 1e-4, CFL 0.5, t = 1.8 (about 7100 steps), checkpoints every 32 steps.
+    Over this horizon the frozen map is only piecewise smooth (WENO-Z's
+    |b0 - b2| kinks), so central differences of the oracle's replay resolve
+    the derivative to a few 1e-5 at best (h = 1e-7: steps of 1e-6 are off by
+    up to 4e-2, steps below 5e-8 are dominated by rounding); the adjoint is
+    held to 2e-4 of the h = 1e-7 quotient."""
+    L, T = 9, 1.8
+    nx = 16 * 2 ** (L - 1) + 1
+    q, dx, x0 = O.ic1(1, nx, -5.0, 5.0, -4.0, O.SHU_L, O.SHU_L)
+    rng = np.random.default_rng(2026)
+    amp = 0.2 * (1 + 0.05 * rng.standard_normal())
+    qt, _, _ = O.ic1(1, nx, -5.0, 5.0, -4.0, O.SHU_L, O.SHU_L, amp=amp)
+    S = EmuSolver(emu, L, dx, T)
+    try:
+        # target: the plain forward run from the perturbed right-state amplitude
+        steps_t = S.record(qt, 1e-4, T, 32)
+        rho_t = S.get_q().reshape(-1, 3)[3:-3, 0].copy()
+        n = S.record(q, 1e-4, T, 32)
+        assert n > 6000 and steps_t > 6000
+        qf = S.get_q()
+        o = O.Solver1D("orc", L, 17, 1.4, dx)
+        o.set_q(q)
+        o.record()
+        while o.t[0] < T:
+            o.step(1e-4, 0.5, S.pts)
+        assert o.tape_steps() == n
+        assert np.array_equal(o.get_q().view(np.uint64), qf.view(np.uint64))
+        gf = np.zeros((nx + 6, 3))
+        gf[3:-3, 0] = 2 * (qf.reshape(-1, 3)[3:-3, 0] - rho_t)
+        g = S.adjoint(gf.reshape(-1))
+
+        def J(q0):
+            o.set_q(q0)
+            o.replay()
+            r = o.get_q().reshape(-1, 3)[3:-3, 0] - rho_t
+            return float(r @ r)
+
+        v = density_direction(nx, x0, dx, np.random.default_rng(7))
+        fd = (J(q + 1e-7 * v) - J(q - 1e-7 * v)) / 2e-7
+        assert abs(g @ v - fd) <= 2e-4 * abs(fd), (g @ v, fd)
+    finally:
+        S.close()
