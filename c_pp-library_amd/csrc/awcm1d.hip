@@ -59,7 +59,7 @@ __global__ void __launch_bounds__(1024) k1_solve(Inst1D* insts, int n_inst, cons
         __syncthreads();
         if (threadIdx.x == 0) {
             st.t = in.t; st.tol = in.tol; st.cfl = in.cfl; st.t_final = in.t_final;
-            st.sum_active0 = 0; st.cnt = in.cnt; st.steps = 0; st.n_print = in.n_print;
+            st.sum_active0 = 0; st.cnt = in.cnt; st.steps = 0; st.n_print = in.n_print; st.bad = 0;
         }
         __syncthreads();
         if (mode == 0) {
@@ -69,7 +69,7 @@ __global__ void __launch_bounds__(1024) k1_solve(Inst1D* insts, int n_inst, cons
         } else {
             const int budget = c.rec ? min(max_steps, in.tape_cap - in.tape_n) : max_steps;
             for (int s = 0; s < budget; s++) {
-                if (!(st.t < st.t_final)) break;
+                if (!(st.t < st.t_final) || st.bad) break;
                 step(c, st, in.print_times);
             }
         }
@@ -80,6 +80,7 @@ __global__ void __launch_bounds__(1024) k1_solve(Inst1D* insts, int n_inst, cons
             for (int i = threadIdx.x; i < in.Nx; i += blockDim.x) in.levels[k * in.Nx + i] = c.lv[k][i];
         if (threadIdx.x == 0) {
             if (s_flag) in.status = -5;
+            if (st.bad) in.status = -4;
             in.t = st.t; in.cnt = st.cnt; in.steps += st.steps; in.sum_active0 += st.sum_active0;
             if (c.rec) in.tape_n += st.steps;
         }

@@ -27,6 +27,15 @@ struct Batch1D {
     double* d_pt = nullptr;
     hipStream_t stream = nullptr;
     std::vector<void*> allocs;
+    // graph of the whole-batch launch (counter reset + solve kernel), rebuilt
+    // when its launch shape or step budget changes
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int g_threads = 0, g_wgs = 0, g_steps = -1;
+    // device snapshot of every instance's state (autotuner probes)
+    std::vector<double*> snap_q;
+    std::vector<uint8_t*> snap_m;
+    std::vector<Inst1D> snap_h;
 
     ~Batch1D();
     int init(int n, const awcm_instance_1d* params, const double* pt, int npt);
@@ -34,8 +43,11 @@ struct Batch1D {
     int push();
     int pull();
     int launch(int mode, int max_steps, int first, int count);
+    int launch_graph(int max_steps);
     const int* d_order_for(int first, int count);
-    int check_status();
+    int check_status(bool nonfinite_ok = false);
+    int snapshot();
+    int restore();
 };
 
 int upload_q1(Batch1D& B, int idx, const double* q);

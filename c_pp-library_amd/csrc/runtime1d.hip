@@ -17,6 +17,12 @@ namespace awcm {
 
 __global__ void k1_solve(Inst1D* insts, int n_inst, const int* order, int* qctr, int mode, int max_steps, int nxp);
 
+// Work-queue counter reset as a kernel, so the batch launch can be captured
+// into a graph without a memset node.
+__global__ void k1_reset_counter(int* qctr) {
+    if (threadIdx.x == 0) *qctr = 0;
+}
+
 namespace {
 
 constexpr int G = 3;
@@ -26,6 +32,8 @@ size_t lds_bytes(int nxp) { return size_t(11) * nxp; }
 }  // namespace
 
 Batch1D::~Batch1D() {
+    if (gexec) hipGraphExecDestroy(gexec);
+    if (graph) hipGraphDestroy(graph);
     for (void* p : allocs) hipFree(p);
     if (stream) hipStreamDestroy(stream);
 }
@@ -124,6 +132,68 @@ int Batch1D::launch(int mode, int max_steps, int first, int count) {
     return hipGetLastError() == hipSuccess ? AWCM_OK : AWCM_EDEVICE;
 }
 
+// The whole batch as one graph launch: reset the work counter, then one
+// persistent k1_solve over all instances (largest first).  The graph is
+// re-captured only when the launch shape or the step budget changes.
+int Batch1D::launch_graph(int max_steps) {
+    const int nxp = ((max_nx + 2 * G) + 15) / 16 * 16;
+    const int wgs = workgroups > 0 ? std::min(workgroups, n) : n;
+    if (!gexec || g_threads != threads || g_wgs != wgs || g_steps != max_steps) {
+        if (gexec) { hipGraphExecDestroy(gexec); gexec = nullptr; }
+        if (graph) { hipGraphDestroy(graph); graph = nullptr; }
+        if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return AWCM_EDEVICE;
+        hipLaunchKernelGGL(k1_reset_counter, dim3(1), dim3(64), 0, stream, d_qctr);
+        hipLaunchKernelGGL(k1_solve, dim3(wgs), dim3(threads), lds_bytes(nxp), stream, d_inst, n, d_order, d_qctr, 1,
+                           max_steps, nxp);
+        if (hipStreamEndCapture(stream, &graph) != hipSuccess) return AWCM_EDEVICE;
+        if (hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0) != hipSuccess) return AWCM_EDEVICE;
+        g_threads = threads;
+        g_wgs = wgs;
+        g_steps = max_steps;
+    }
+    return hipGraphLaunch(gexec, stream) == hipSuccess ? AWCM_OK : AWCM_EDEVICE;
+}
+
+// Device copy of every instance's state (Q, masks, levels) and of the host
+// descriptors; restore() puts it back.  Used to run autotuner probes without
+// changing the solve.
+int Batch1D::snapshot() {
+    if (snap_q.empty()) {
+        snap_q.resize(n);
+        snap_m.resize(n);
+        for (int k = 0; k < n; k++) {
+            const size_t NP = size_t(h[k].Nx) + 2 * G;
+            if (dalloc(&snap_q[k], sizeof(double) * 3 * NP)) return AWCM_ENOMEM;
+            if (dalloc(&snap_m[k], 5 * NP + 4 * size_t(h[k].Nx))) return AWCM_ENOMEM;
+        }
+    }
+    for (int k = 0; k < n; k++) {
+        const size_t NP = size_t(h[k].Nx) + 2 * G;
+        if (hipMemcpyAsync(snap_q[k], h[k].Q, sizeof(double) * 3 * NP, hipMemcpyDeviceToDevice, stream) != hipSuccess ||
+            hipMemcpyAsync(snap_m[k], h[k].masks, 5 * NP, hipMemcpyDeviceToDevice, stream) != hipSuccess ||
+            hipMemcpyAsync(snap_m[k] + 5 * NP, h[k].levels, 4 * size_t(h[k].Nx), hipMemcpyDeviceToDevice, stream) !=
+                hipSuccess)
+            return AWCM_EDEVICE;
+    }
+    snap_h = h;
+    return hipStreamSynchronize(stream) == hipSuccess ? AWCM_OK : AWCM_EDEVICE;
+}
+
+int Batch1D::restore() {
+    if (snap_q.empty()) return AWCM_ESTATE;
+    for (int k = 0; k < n; k++) {
+        const size_t NP = size_t(h[k].Nx) + 2 * G;
+        if (hipMemcpyAsync(h[k].Q, snap_q[k], sizeof(double) * 3 * NP, hipMemcpyDeviceToDevice, stream) != hipSuccess ||
+            hipMemcpyAsync(h[k].masks, snap_m[k], 5 * NP, hipMemcpyDeviceToDevice, stream) != hipSuccess ||
+            hipMemcpyAsync(h[k].levels, snap_m[k] + 5 * NP, 4 * size_t(h[k].Nx), hipMemcpyDeviceToDevice, stream) !=
+                hipSuccess)
+            return AWCM_EDEVICE;
+    }
+    h = snap_h;
+    if (int st = push()) return st;
+    return hipStreamSynchronize(stream) == hipSuccess ? AWCM_OK : AWCM_EDEVICE;
+}
+
 // For a launch over a sub-range (sequential mode) the order is the identity.
 const int* Batch1D::d_order_for(int first, int count) {
     if (first == 0 && count == n) return d_order;
@@ -135,9 +205,15 @@ const int* Batch1D::d_order_for(int first, int count) {
     return d_identity;
 }
 
-int Batch1D::check_status() {
-    for (int k = 0; k < n; k++)
-        if (h[k].status) return h[k].status == -5 ? AWCM_ERANGE : AWCM_EDEVICE;
+// First error status over the instances.  With nonfinite_ok, an instance
+// whose state broke down (status AWCM_ENONFINITE, stopped at its last finite
+// time) is not an error of the call: the batch reports it per instance.
+int Batch1D::check_status(bool nonfinite_ok) {
+    for (int k = 0; k < n; k++) {
+        const int s = h[k].status;
+        if (s == 0 || (s == AWCM_ENONFINITE && nonfinite_ok)) continue;
+        return s == -5 ? AWCM_ERANGE : s == AWCM_ENONFINITE ? AWCM_ENONFINITE : AWCM_EDEVICE;
+    }
     return AWCM_OK;
 }
 
@@ -187,9 +263,9 @@ int awcm_batch_run(awcm_batch* b, int max_steps) {
     if (!b || max_steps < 0) return AWCM_EINVAL;
     Batch1D& B = b->b1;
     int st = B.push();
-    if (!st) st = B.launch(1, max_steps, 0, B.n);
+    if (!st) st = B.launch_graph(max_steps);
     if (!st) st = B.pull();
-    if (!st) st = B.check_status();
+    if (!st) st = B.check_status(true);
     return st;
 }
 
@@ -199,7 +275,55 @@ int awcm_batch_run_sequential(awcm_batch* b, int max_steps) {
     int st = B.push();
     for (int k = 0; k < B.n && !st; k++) st = B.launch(1, max_steps, k, 1);
     if (!st) st = B.pull();
-    if (!st) st = B.check_status();
+    if (!st) st = B.check_status(true);
+    return st;
+}
+
+int awcm_batch_autotune(awcm_batch* b, int probe_steps, int* threads, int* workgroups, double* probe_ms) {
+    if (!b || probe_steps <= 0) return AWCM_EINVAL;
+    Batch1D& B = b->b1;
+    // candidate launch shapes: workgroup size x number of persistent workers
+    // (0 = one workgroup per instance); each is timed on the same probe
+    // (probe_steps steps of every instance from the current state), which is
+    // then undone, so tuning does not change the solve
+    static const int kThreads[3] = {256, 512, 1024};
+    static const int kWgFactor[3] = {0, 1, 2};  // 0: n workgroups; f: f x (number of CUs)
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return AWCM_EDEVICE;
+    const int keep_t = B.threads, keep_w = B.workgroups;
+    int st = B.push();
+    if (!st) st = B.snapshot();
+    if (st) return st;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return AWCM_EDEVICE;
+    double best = 1e300;
+    int bt = keep_t, bw = keep_w, idx = 0;
+    for (int ti = 0; ti < 3 && !st; ti++) {
+        for (int wi = 0; wi < 3 && !st; wi++, idx++) {
+            B.threads = kThreads[ti];
+            B.workgroups = kWgFactor[wi] * cus;
+            if (B.workgroups >= B.n) B.workgroups = 0;
+            st = B.launch_graph(probe_steps);  // capture outside the timed region
+            if (!st) st = B.restore();
+            if (!st && hipEventRecord(e0, B.stream) != hipSuccess) st = AWCM_EDEVICE;
+            if (!st) st = B.launch_graph(probe_steps);
+            if (!st && hipEventRecord(e1, B.stream) != hipSuccess) st = AWCM_EDEVICE;
+            float ms = 0;
+            if (!st && hipEventSynchronize(e1) != hipSuccess) st = AWCM_EDEVICE;
+            if (!st && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) st = AWCM_EDEVICE;
+            if (!st) st = B.restore();
+            if (probe_ms) probe_ms[idx] = st ? -1.0 : ms;
+            if (!st && ms < best) { best = ms; bt = B.threads; bw = B.workgroups; }
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    B.threads = st ? keep_t : bt;
+    B.workgroups = st ? keep_w : bw;
+    if (threads) *threads = B.threads;
+    if (workgroups) *workgroups = B.workgroups;
     return st;
 }
 

@@ -436,6 +436,7 @@ struct LoopState {
     double t, tol, cfl, t_final;
     long long sum_active0;
     int cnt, steps, n_print;
+    int bad;  // a wave speed was not finite: the state has broken down
 };
 
 // Tape writers of the adjoint recording (see Inst1D).
@@ -464,15 +465,26 @@ __device__ void step(const Ctx1& c, LoopState& st, const double* print_times) {
     prepare(c, st.tol);
     if (c.rec) tape_masks(c, rn);
     double v = 0;
+    bool finite = true;
     for (int i = threadIdx.x; i < c.g.Nx; i += blockDim.x) {
         if (c.m[3][i + G] == 0) continue;
         double q[3];
         load3(c, c.Q, i, q);
         const double cs = eos_c1(c.gamma, q);
         const double u = q[1] / q[0];
-        v = t_max(v, t_abs(u) + cs);
+        const double s = t_abs(u) + cs;
+        // the reference's MAX drops a NaN depending on the scan order, so a
+        // broken-down state (negative pressure, vacuum) is detected explicitly
+        finite = finite && s <= 1.7976931348623157e308;
+        v = t_max(v, s);
     }
-    const double vmax = block_max(v, c.red);
+    if (!finite) atomicOr(&st.bad, 1);
+    const double vmax = block_max(v, c.red);  // its barriers publish st.bad
+    if (st.bad || !(vmax > 0)) {
+        if (threadIdx.x == 0) st.bad = 1;
+        __syncthreads();
+        return;
+    }
     double dt = st.cfl * c.dx / vmax;
     const double t = st.t;
     const int cnt = st.cnt;

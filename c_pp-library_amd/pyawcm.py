@@ -65,6 +65,7 @@ def lib():
         "awcm_batch_set_launch": [V, I, I],
         "awcm_batch_run": [V, I],
         "awcm_batch_run_sequential": [V, I],
+        "awcm_batch_autotune": [V, I, C.POINTER(I), C.POINTER(I), _dp],
         "awcm_batch_state": [V, I, _dp, C.POINTER(I), C.POINTER(I), C.POINTER(C.c_longlong), C.POINTER(I)],
         "awcm_batch_synchronize": [V],
         "awcm_run_timed": [V, D, D, D, I, _dp, C.POINTER(I), C.POINTER(I), C.POINTER(C.c_longlong), _dp],
@@ -337,6 +338,13 @@ class Batch1D:
 
     def run_sequential(self, max_steps=1 << 30):
         _check(lib().awcm_batch_run_sequential(self.h, max_steps), "awcm_batch_run_sequential")
+
+    def autotune(self, probe_steps=20):
+        """Pick the batched launch shape by timing probes; returns (threads, workgroups, probe_ms[9])."""
+        t, w = C.c_int(), C.c_int()
+        ms = np.zeros(9)
+        _check(lib().awcm_batch_autotune(self.h, probe_steps, C.byref(t), C.byref(w), _d(ms)), "awcm_batch_autotune")
+        return t.value, w.value, ms
 
     def state(self, idx):
         t = np.zeros(1)

@@ -215,8 +215,21 @@ int awcm_batch_get_masks(awcm_batch* b, int idx, uint8_t* masks, int32_t* levels
  * workgroups per launch (0 = one per instance). */
 int awcm_batch_set_launch(awcm_batch* b, int threads, int workgroups);
 /* Run every instance to its t_final (or max_steps steps) in one persistent
- * launch; instances are dispatched largest-first from a device work queue. */
+ * launch; instances are dispatched largest-first from a device work queue.
+ * An instance whose state breaks down (a wave speed not finite) stops at its
+ * last finite time with status AWCM_ENONFINITE in awcm_batch_state; that is
+ * not an error of the call.
+ * The launch (work-counter reset + solve kernel) is captured once into a HIP
+ * graph per launch shape and step budget and replayed. */
 int awcm_batch_run(awcm_batch* b, int max_steps);
+/* Project autotuner for the batched launch: times every candidate shape
+ * (workgroup size 256 / 512 / 1024 x one workgroup per instance or 1x / 2x
+ * the CU count of persistent workers) on probe_steps steps of every instance
+ * from the current state, restores that state after each probe, and keeps the
+ * fastest (as awcm_batch_set_launch would).  probe_ms, if not NULL, receives
+ * the 9 probe times in that order (thread-major).  No counterpart in the
+ * reference, which has no batched form. */
+int awcm_batch_autotune(awcm_batch* b, int probe_steps, int* threads, int* workgroups, double* probe_ms);
 /* The same solves, one launch per instance, in index order (the baseline the
  * batched form is measured against). */
 int awcm_batch_run_sequential(awcm_batch* b, int max_steps);

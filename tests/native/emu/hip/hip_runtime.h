@@ -44,5 +44,11 @@ inline T atomicMax(T* p, U v) {
     if (T(v) > old) *p = T(v);
     return old;
 }
+template <class T, class U>
+inline T atomicOr(T* p, U v) {
+    const T old = *p;
+    *p = old | T(v);
+    return old;
+}
 using std::max;
 using std::min;

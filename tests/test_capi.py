@@ -87,3 +87,28 @@ def test_invalid_arguments_rejected_before_device_work():
     assert lib.awcm_create_2d(14, 65, 17, 1.4, 0.001, 0.0, 1, C.byref(h)) == -1  # grid too large
     assert lib.awcm_run(None, 1e-3, 0.4, 0.2, 1, None, None, None, None) == -1
     assert lib.awcm_ic_double_mach_2d(1, 1.4, 1.4, None, None) == -1
+
+
+def test_splitmix64_known_answers():
+    """The generator of the batched workload is splitmix64: its published
+    outputs for seed 1234567, and doubles in [0, 1) from the top 53 bits."""
+    import workloads
+    g = workloads.SplitMix64(1234567)
+    assert [g.next() for _ in range(5)] == [6457827717110365317, 3203168211198807973, 9817491932198370423,
+                                            4593380528125082431, 16408922859458223821]
+    g = workloads.SplitMix64(0)
+    u = [g.uniform() for _ in range(1000)]
+    assert 0.0 <= min(u) and max(u) < 1.0 and abs(np.mean(u) - 0.5) < 0.05
+
+
+def test_batched_workload_distribution():
+    P = _lib()
+    import workloads
+    inst = workloads.shock_tube_instances(512)
+    assert inst == workloads.shock_tube_instances(512)
+    L = np.array([i[0] for i in inst])
+    assert set(L.tolist()) == {6, 7, 8, 9}
+    assert {i[7] for i in inst} == {1e-3, 1e-4}
+    a = np.array([i[1:7] for i in inst])
+    assert a[:, :2].min() >= 0.1 and a[:, :2].max() <= 1.0 and a[:, 2:4].min() >= 0.05 and a[:, 2:4].max() <= 1.0
+    assert a[:, 4].min() >= -0.5 and a[:, 4].max() <= 0.5 and a[:, 5].min() >= 0.3 and a[:, 5].max() <= 0.7

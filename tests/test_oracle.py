@@ -363,3 +363,40 @@ def test_sod_against_exact_riemann_solution():
     rho = s.get_q().reshape(-1, 3)[3:-3, 0]
     ex = _exact_sod(x[act], 0.2)
     assert np.abs(rho[act] - ex).mean() < 5e-3
+
+
+def test_batched_workload_instance_that_breaks_down():
+    """Instance 448 of the seeded batched workload breaks down in the oracle
+    (bit-exact with the reference): after 1558 steps, the next PrepareEuler
+    leaves a mask3 point with a non-finite wave speed |u| + c.  The
+    reference's MAX (a ternary) drops that NaN from the CFL maximum, so its
+    time only turns NaN at step 1657; the GPU stops the instance at the first
+    non-finite wave speed instead."""
+    import workloads
+    L, rl, rr, pl, pr, u, x0, eps = workloads.shock_tube_instances(512)[448]
+    nx = 16 * 2 ** (L - 1) + 1
+    q, dx, _ = O.ic1(0, nx, 0.0, 1.0, x0, (rl, u, pl), (rr, u, pr))
+    o = O.Solver1D("orc", L, 17, 1.4, dx)
+    o.set_q(q)
+    pts = O.print_times(0.15, 41)
+
+    def nonfinite_speed_after_prepare():
+        saved, (m, lv) = o.get_q(), o.masks()
+        o.prepare(eps)
+        Q = o.get_q().reshape(-1, 3)[3:-3]
+        m3 = o.masks()[0][4][3:-3] > 0
+        with np.errstate(all="ignore"):
+            p = 0.4 * (Q[:, 2] - 0.5 * Q[:, 1] ** 2 / Q[:, 0])
+            s = np.abs(Q[:, 1] / Q[:, 0]) + np.sqrt(1.4 * p / Q[:, 0])
+        o.set_q(saved)
+        o.set_mask0(m[1])
+        return not np.all(np.isfinite(s[m3]))
+
+    for n in range(1558):
+        o.step(eps, 0.5, pts)
+    assert np.isfinite(o.t[0])
+    assert nonfinite_speed_after_prepare()
+    for n in range(1558, 1657):
+        assert np.isfinite(o.t[0]), n
+        o.step(eps, 0.5, pts)
+    assert not np.isfinite(o.t[0])
