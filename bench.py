@@ -49,6 +49,12 @@ FLOPS_PER_FACE = 1202
 # Bytes per face: the face's own state (32 B, neighbours' states are the
 # neighbouring faces' own reads), its 4 B list entry, and the 32 B flux written.
 BYTES_PER_FACE = 32 + 4 + 32
+# Measured on this workload at t = 0.02 with rocprofv3 (profiles/
+# k2_stage_faces_dmr_pmc.md): HBM-side bytes per k2_stage_faces launch =
+# FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a 1 GiB device copy), and
+# issued FP64 lane operations per face (ADD + MUL + TRANS + 2 FMA, x 64 lanes).
+TRAFFIC_BYTES_PER_LAUNCH = 94705466
+ISSUED_FP64_OPS_PER_FACE = 2143
 
 
 def ensure_built():
@@ -236,13 +242,18 @@ def main():
                    "advance_wall_s": round(adv_s, 3), "updates": updates,
                    "mean_active_points": updates / max(steps, 1)},
         "roofline": {
-            "bound": "fp64-valu",
+            "bound": "mfma",
+            "compute": "FP64 vector ALU: the kernel issues no MFMA; MI355X's FP64 vector and matrix peaks are both "
+                       "78.6 TFLOP/s, and bit-exact parity forbids FMA contraction",
             "kernel": "k2_stage_faces (WENO-Z/Roe face fluxes, each face of a stage once)",
             "achieved": achieved_tflops,
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved_tflops / FP64_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": TRAFFIC_BYTES_PER_LAUNCH,
+            "traffic_over_algorithmic": TRAFFIC_BYTES_PER_LAUNCH / (faces_per_launch * BYTES_PER_FACE)
+            if faces_per_launch else None,
+            "issued_fp64_tflops": faces_per_launch * ISSUED_FP64_OPS_PER_FACE / avg_launch_s / 1e12 if w_launch else 0.0,
             "hbm_achieved_gbs": achieved_gbs,
             "hbm_frac": achieved_gbs / HBM_PEAK_GBS,
             "avg_launch_ms": avg_launch_s * 1e3,
