@@ -1,0 +1,52 @@
+"""The reference-named C++ class API on the GPU: AWCM_1D/2D::PrepareEuler runs
+through awcm_prepare_euler, and the drivers' time loop runs through the class
+API (host stages, device PrepareEuler every step).
+
+tests/native/_build/awcm_api_probe is the same source that, built against the
+reference's headers and run on the CPU, printed tests/golden/api_probe_*.txt.
+Bar: identical text, i.e. every field (ghosts included), mask and level field
+bit-identical, and t and dt identical at every step:
+  prepare  two consecutive PrepareEuler calls on Sod and Shu-Osher (1-D, 6
+           levels, 481 points) and on the quadrants and double-Mach problems
+           (2-D, 4 levels, 65x65 and 129x33)
+  loop     40 steps of main_1d_euler.cpp's loop (WENO-Z face-flux pair on Sod
+           and Shu-Osher; WENO5-JS ComputedF on Sod, as the shipped driver) and
+           of main_2d_euler(quad|dmach).cpp's loop
+"""
+import os
+import subprocess
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PROBE = os.path.join(ROOT, "tests", "native", "_build", "awcm_api_probe")
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def run(mode, *args):
+    import pyawcm
+    pyawcm.lib()  # raises if the HIP library is not built
+    if not os.path.exists(PROBE):
+        pytest.fail("tests/native/_build/awcm_api_probe is not built (run __graft_entry__.build())")
+    print(f"api probe: {mode} {' '.join(args)}", flush=True)
+    out = subprocess.run([PROBE, mode, *args], capture_output=True, text=True, timeout=110)
+    assert out.returncode == 0, f"probe {mode} failed ({out.returncode}): {out.stderr[-2000:]}"
+    return out.stdout.splitlines()
+
+
+def compare(got, name):
+    with open(os.path.join(GOLDEN, f"api_probe_{name}.txt")) as f:
+        want = f.read().splitlines()
+    bad = [(w, g) for w, g in zip(want, got) if w != g]
+    assert len(got) == len(want), f"{len(got)} lines, expected {len(want)}; first differences {bad[:4]}"
+    assert not bad, f"differs from the reference (expected, got): {bad[:6]}"
+
+
+def test_prepare_euler_matches_reference():
+    compare(run("prepare"), "prepare")
+
+
+def test_driver_loop_matches_reference():
+    compare(run("loop", "40"), "loop")
