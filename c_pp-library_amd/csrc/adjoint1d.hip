@@ -49,8 +49,21 @@ Adj1DArgs args_of(const Adj1DHost& H) {
     return A;
 }
 
+// k1_adjoint keeps the masks in LDS only: an instance whose 11 * nxp bytes do
+// not fit one workgroup's LDS next to the kernel's static LDS is refused.
+int adjoint_fits(int nxp) {
+    int dev = 0, cap = 0;
+    hipFuncAttributes fa;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cap, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
+        hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k1_adjoint)) != hipSuccess)
+        return AWCM_EDEVICE;
+    return size_t(11) * nxp + fa.sharedSizeBytes <= size_t(cap) ? AWCM_OK : AWCM_EINVAL;
+}
+
 int launch_adjoint(Batch1D& B, const Adj1DHost& H, int mode) {
     const int nxp = nxp_of(B.h[0]);
+    if (int st = adjoint_fits(nxp)) return st;
     hipLaunchKernelGGL(k1_adjoint, dim3(1), dim3(1024), size_t(11) * nxp, B.stream, B.d_inst, args_of(H), mode, nxp);
     if (hipGetLastError() != hipSuccess) return AWCM_EDEVICE;
     int st = B.pull();

@@ -1,7 +1,8 @@
 // 1-D AWCM compressible-Euler solver on gfx950.
 //
 // One workgroup owns one independent solve (a "tube").  Its five masks and four
-// level fields live in LDS for the whole launch; the conserved state Q, the RK
+// level fields live in LDS for the whole launch (in HBM, workgroup-private, for
+// instances too long for one workgroup's LDS); the conserved state Q, the RK
 // copies and the face-flux buffer live in HBM (planes of Nx+6 doubles, SoA) and
 // stay L2-resident at these sizes.  A launch runs the reference's time loop
 // (Wavelet/main_1d_euler.cpp:61-260, WENO-Z flux pair in place of the shipped
@@ -21,11 +22,16 @@ namespace awcm {
 
 // mode 0: PrepareEuler only; mode 2: PostProcessEuler only; mode 1: time steps until t >= t_final or max_steps
 // steps of this launch.  Instances are taken from `order` through the atomic
-// work counter *qctr (the host zeroes it).  Dynamic LDS: 11 * nxp bytes where
-// nxp >= max(Nx) + 6 over the instances.
+// work counter *qctr (the host zeroes it).  The masks and levels of the
+// instance in hand occupy 11 * nxp bytes (nxp >= max(Nx) + 6 over the
+// instances): dynamic LDS, or with kHbm -- for instances whose masks do not
+// fit one workgroup's LDS -- this workgroup's slice of `hbm` (gridDim.x slices
+// of 11 * nxp bytes; workgroup-local, ordered by the same barriers).
+template <bool kHbm>
 __global__ void __launch_bounds__(1024) k1_solve(Inst1D* insts, int n_inst, const int* order, int* qctr, int mode,
-                                                 int max_steps, int nxp) {
-    extern __shared__ uint8_t sm[];
+                                                 int max_steps, int nxp, uint8_t* hbm) {
+    extern __shared__ uint8_t lds_masks[];
+    uint8_t* const sm = kHbm ? hbm + size_t(blockIdx.x) * 11 * size_t(nxp) : lds_masks;
     __shared__ double W[4][5];
     __shared__ double red[16];
     __shared__ int s_idx, s_flag;
@@ -86,5 +92,8 @@ __global__ void __launch_bounds__(1024) k1_solve(Inst1D* insts, int n_inst, cons
         }
     }
 }
+
+template __global__ void k1_solve<false>(Inst1D*, int, const int*, int*, int, int, int, uint8_t*);
+template __global__ void k1_solve<true>(Inst1D*, int, const int*, int*, int, int, int, uint8_t*);
 
 }  // namespace awcm

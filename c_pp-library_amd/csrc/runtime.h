@@ -32,6 +32,12 @@ struct Batch1D {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int g_threads = 0, g_wgs = 0, g_steps = -1;
+    const uint8_t* g_hbm = nullptr;
+    // masks and levels of instances too long for one workgroup's LDS: an
+    // 11 * nxp-byte slice per workgroup in HBM (grown on demand, never shrunk)
+    size_t lds_cap = 0;           // dynamic LDS one k1_solve workgroup may use
+    uint8_t* d_hbm_masks = nullptr;
+    size_t hbm_masks_bytes = 0;
     // device snapshot of every instance's state (autotuner probes)
     std::vector<double*> snap_q;
     std::vector<uint8_t*> snap_m;
@@ -45,6 +51,7 @@ struct Batch1D {
     int launch(int mode, int max_steps, int first, int count);
     int launch_graph(int max_steps);
     const int* d_order_for(int first, int count);
+    int mask_space(int wgs, int nxp, uint8_t** hbm);
     int check_status(bool nonfinite_ok = false);
     int snapshot();
     int restore();

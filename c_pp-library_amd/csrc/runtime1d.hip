@@ -15,7 +15,9 @@
 
 namespace awcm {
 
-__global__ void k1_solve(Inst1D* insts, int n_inst, const int* order, int* qctr, int mode, int max_steps, int nxp);
+template <bool kHbm>
+__global__ void k1_solve(Inst1D* insts, int n_inst, const int* order, int* qctr, int mode, int max_steps, int nxp,
+                         uint8_t* hbm);
 
 // Work-queue counter reset as a kernel, so the batch launch can be captured
 // into a graph without a memset node.
@@ -49,6 +51,15 @@ int Batch1D::init(int n_, const awcm_instance_1d* params, const double* pt, int 
         if ((long long)(p.nx_coarse - 1) * (1 << (p.num_levels - 1)) + 1 > 65535) return AWCM_EINVAL;
     }
     if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) return AWCM_EDEVICE;
+    {
+        int dev = 0, cap = 0;
+        hipFuncAttributes fa;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cap, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
+            hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k1_solve<false>)) != hipSuccess)
+            return AWCM_EDEVICE;
+        lds_cap = size_t(cap) > fa.sharedSizeBytes ? size_t(cap) - fa.sharedSizeBytes : 0;
+    }
     h.resize(n);
     max_nx = 0;
     for (int k = 0; k < n; k++) {
@@ -123,12 +134,40 @@ int Batch1D::pull() {
     return hipStreamSynchronize(stream) == hipSuccess ? AWCM_OK : AWCM_EDEVICE;
 }
 
+// Where a launch of wgs workgroups keeps its masks: LDS (*hbm = nullptr) when
+// 11 * nxp bytes fit, else a slice per workgroup of an HBM buffer.  Never
+// called during graph capture (it may allocate).
+int Batch1D::mask_space(int wgs, int nxp, uint8_t** hbm) {
+    *hbm = nullptr;
+    if (lds_bytes(nxp) <= lds_cap) return AWCM_OK;
+    const size_t need = lds_bytes(nxp) * size_t(wgs);
+    if (need > hbm_masks_bytes) {
+        if (hipStreamSynchronize(stream) != hipSuccess) return AWCM_EDEVICE;
+        if (d_hbm_masks) {
+            allocs.erase(std::find(allocs.begin(), allocs.end(), static_cast<void*>(d_hbm_masks)));
+            hipFree(d_hbm_masks);
+            d_hbm_masks = nullptr;
+            hbm_masks_bytes = 0;
+        }
+        if (int st = dalloc(&d_hbm_masks, need)) return st;
+        hbm_masks_bytes = need;
+    }
+    *hbm = d_hbm_masks;
+    return AWCM_OK;
+}
+
 int Batch1D::launch(int mode, int max_steps, int first, int count) {
     const int nxp = ((max_nx + 2 * G) + 15) / 16 * 16;
-    if (hipMemsetAsync(d_qctr, 0, sizeof(int), stream) != hipSuccess) return AWCM_EDEVICE;
     const int wgs = workgroups > 0 ? std::min(workgroups, count) : count;
-    hipLaunchKernelGGL(k1_solve, dim3(wgs), dim3(threads), lds_bytes(nxp), stream, d_inst + first, count,
-                       d_order_for(first, count), d_qctr, mode, max_steps, nxp);
+    uint8_t* hbm = nullptr;
+    if (int st = mask_space(wgs, nxp, &hbm)) return st;
+    if (hipMemsetAsync(d_qctr, 0, sizeof(int), stream) != hipSuccess) return AWCM_EDEVICE;
+    if (hbm)
+        hipLaunchKernelGGL(k1_solve<true>, dim3(wgs), dim3(threads), 0, stream, d_inst + first, count,
+                           d_order_for(first, count), d_qctr, mode, max_steps, nxp, hbm);
+    else
+        hipLaunchKernelGGL(k1_solve<false>, dim3(wgs), dim3(threads), lds_bytes(nxp), stream, d_inst + first, count,
+                           d_order_for(first, count), d_qctr, mode, max_steps, nxp, hbm);
     return hipGetLastError() == hipSuccess ? AWCM_OK : AWCM_EDEVICE;
 }
 
@@ -138,18 +177,25 @@ int Batch1D::launch(int mode, int max_steps, int first, int count) {
 int Batch1D::launch_graph(int max_steps) {
     const int nxp = ((max_nx + 2 * G) + 15) / 16 * 16;
     const int wgs = workgroups > 0 ? std::min(workgroups, n) : n;
-    if (!gexec || g_threads != threads || g_wgs != wgs || g_steps != max_steps) {
+    uint8_t* hbm = nullptr;
+    if (int st = mask_space(wgs, nxp, &hbm)) return st;
+    if (!gexec || g_threads != threads || g_wgs != wgs || g_steps != max_steps || g_hbm != hbm) {
         if (gexec) { hipGraphExecDestroy(gexec); gexec = nullptr; }
         if (graph) { hipGraphDestroy(graph); graph = nullptr; }
         if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return AWCM_EDEVICE;
         hipLaunchKernelGGL(k1_reset_counter, dim3(1), dim3(64), 0, stream, d_qctr);
-        hipLaunchKernelGGL(k1_solve, dim3(wgs), dim3(threads), lds_bytes(nxp), stream, d_inst, n, d_order, d_qctr, 1,
-                           max_steps, nxp);
+        if (hbm)
+            hipLaunchKernelGGL(k1_solve<true>, dim3(wgs), dim3(threads), 0, stream, d_inst, n, d_order, d_qctr, 1,
+                               max_steps, nxp, hbm);
+        else
+            hipLaunchKernelGGL(k1_solve<false>, dim3(wgs), dim3(threads), lds_bytes(nxp), stream, d_inst, n, d_order,
+                               d_qctr, 1, max_steps, nxp, hbm);
         if (hipStreamEndCapture(stream, &graph) != hipSuccess) return AWCM_EDEVICE;
         if (hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0) != hipSuccess) return AWCM_EDEVICE;
         g_threads = threads;
         g_wgs = wgs;
         g_steps = max_steps;
+        g_hbm = hbm;
     }
     return hipGraphLaunch(gexec, stream) == hipSuccess ? AWCM_OK : AWCM_EDEVICE;
 }

@@ -116,7 +116,9 @@ int awcm_run(awcm_solver* h, double tol, double cfl, double t_final, int max_ste
  * dt and the three stage alphas, plus the state at the start of every
  * checkpoint_every-th step.  Replaces any earlier recording.  max_steps also
  * sizes the tape (about 8 (Nx+6) bytes per step plus 24 (Nx+6) bytes per
- * checkpoint). */
+ * checkpoint).  Replay and reverse keep the masks in LDS, so they return
+ * AWCM_EINVAL for instances longer than one workgroup's LDS holds (about
+ * 14 800 finest points on gfx950); recording has no such limit. */
 int awcm_adjoint_record(awcm_solver* h, double tol, double cfl, double t_final, int max_steps, int checkpoint_every,
                         double* t, int* cnt, int* steps_done);
 

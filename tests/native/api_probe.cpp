@@ -9,7 +9,7 @@
 // reference build printed.
 //
 //   awcm_api_probe host <dir>     host-side classes; writes Print() files in <dir>
-//   awcm_api_probe prepare        AWCM_1D/2D::PrepareEuler on four small problems
+//   awcm_api_probe prepare        AWCM_1D/2D::PrepareEuler on five problems
 //   awcm_api_probe loop <steps>   the drivers' time loop through the class API
 #include <cinttypes>
 #include <cstdint>
@@ -576,7 +576,8 @@ struct Cfg2 {
     double tol;
     bool dmr;
 };
-const Cfg1 kCfg1[] = {{"sod", 6, 16, 1e-4, false}, {"shu_osher", 6, 16, 1e-4, true}};
+// the last is main_1d_euler.cpp's own configuration (:14-17): 15361 points
+const Cfg1 kCfg1[] = {{"sod", 6, 16, 1e-4, false}, {"shu_osher", 6, 16, 1e-4, true}, {"sod_driver", 11, 16, 1e-6, false}};
 const Cfg2 kCfg2[] = {{"quad", 4, 9, 9, 1e-3, false}, {"dmr", 4, 17, 5, 5e-3, true}};
 
 void init1(const Cfg1& c, MRA_GRID_INT_1D& mra, GRID_UNIFORM_1D& g, GRID_UNIFORM_1D& gx, FIELD_UNIFORM_1D<V3>& Q,
@@ -869,8 +870,10 @@ int main(int argc, char** argv) {
     }
     if (mode == "loop" && argc > 2) {
         const int steps = std::atoi(argv[2]);
-        for (const Cfg1& c : kCfg1) loop1(c, steps, false);
+        loop1(kCfg1[0], steps, false);
+        loop1(kCfg1[1], steps, false);
         loop1(kCfg1[0], steps, true);
+        loop1(kCfg1[2], steps, true);
         loop2<BOUNDARY_MANAGER_QUAD>(kCfg2[0], steps);
         loop2<BOUNDARY_MANAGER_DMACH>(kCfg2[1], steps);
         return 0;

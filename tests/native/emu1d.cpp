@@ -9,7 +9,8 @@
 #include "adjoint1d_dev.h"
 
 namespace awcm {
-uint8_t sm[1 << 21];  // dynamic LDS of the emulated workgroup
+uint8_t sm[1 << 21];         // dynamic LDS of the emulated workgroup (k1_adjoint)
+uint8_t lds_masks[1 << 21];  // dynamic LDS of the emulated workgroup (k1_solve<false>)
 }
 
 using namespace awcm;
@@ -26,14 +27,25 @@ struct Emu {
     int nxp() const { return (NP() + 15) / 16 * 16; }
 };
 
+// with masks_in_hbm, k1_solve<true> keeps the masks in a heap slice instead
+// of the emulated LDS (the path of instances too long for one workgroup's LDS)
+bool masks_in_hbm = false;
+
 void launch_solve(Emu& e, int mode, int max_steps) {
     int order = 0, qctr = 0;
-    k1_solve(&e.in, 1, &order, &qctr, mode, max_steps, e.nxp());
+    if (masks_in_hbm) {
+        std::vector<uint8_t> hbm(size_t(11) * e.nxp(), 0xA5);
+        k1_solve<true>(&e.in, 1, &order, &qctr, mode, max_steps, e.nxp(), hbm.data());
+    } else {
+        k1_solve<false>(&e.in, 1, &order, &qctr, mode, max_steps, e.nxp(), nullptr);
+    }
 }
 
 }  // namespace
 
 extern "C" {
+
+void emu_set_masks_in_hbm(int on) { masks_in_hbm = on != 0; }
 
 void* emu_create(int L, int nxc, double gamma, double dx, const double* pt, int npt) {
     auto* e = new Emu();

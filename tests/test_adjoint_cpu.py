@@ -166,7 +166,40 @@ def emu(tmp_path_factory):
     E.emu_replay.argtypes = [V]
     E.emu_adjoint.argtypes = [V, _dp, C.c_int, _dp]
     E.emu_status.argtypes = [V]
+    E.emu_set_masks_in_hbm.argtypes = [C.c_int]
     return E
+
+
+def test_emulated_solve_with_masks_in_hbm_matches_lds(emu):
+    """k1_solve<true> (masks and levels in a per-workgroup HBM slice, the path
+    of instances too long for one workgroup's LDS) reproduces k1_solve<false>
+    and the oracle bit for bit: Sod at 513 points, 25 steps."""
+    L, T = 6, 0.05
+    nx = 16 * 2 ** (L - 1) + 1
+    q, dx, _ = O.ic1(0, nx, 0.0, 1.0, 0.5, O.SOD_L, O.SOD_R)
+    pts = O.print_times(T, 3)
+    out = []
+    for hbm in (0, 1):
+        emu.emu_set_masks_in_hbm(hbm)
+        h = emu.emu_create(L, 17, 1.4, dx, _d(pts), len(pts))
+        try:
+            emu.emu_set_q(h, _d(q))
+            n = emu.emu_record(h, 1e-4, 0.5, T, 1000, 8)
+            qe = np.zeros_like(q)
+            emu.emu_get_q(h, _d(qe))
+            assert emu.emu_status(h) == 0
+            out.append((n, qe))
+        finally:
+            emu.emu_destroy(h)
+    emu.emu_set_masks_in_hbm(0)
+    (n0, q0), (n1, q1) = out
+    o = O.Solver1D("orc", L, 17, 1.4, dx)
+    o.set_q(q)
+    for _ in range(n0):
+        o.step(1e-4, 0.5, pts)
+    assert n0 == n1 and n0 > 20
+    assert np.array_equal(q0.view(np.uint64), q1.view(np.uint64))
+    assert np.array_equal(q1.view(np.uint64), o.get_q().view(np.uint64))
 
 
 def test_emulated_kernels_forward_replay_and_adjoint(emu):

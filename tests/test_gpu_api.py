@@ -7,11 +7,14 @@ reference's headers and run on the CPU, printed tests/golden/api_probe_*.txt.
 Bar: identical text, i.e. every field (ghosts included), mask and level field
 bit-identical, and t and dt identical at every step:
   prepare  two consecutive PrepareEuler calls on Sod and Shu-Osher (1-D, 6
-           levels, 481 points) and on the quadrants and double-Mach problems
-           (2-D, 4 levels, 65x65 and 129x33)
+           levels, 481 points), on Sod as main_1d_euler.cpp configures it (11
+           levels, 15361 points: masks beyond one workgroup's LDS, so the
+           device keeps them in HBM) and on the quadrants and double-Mach
+           problems (2-D, 4 levels, 65x65 and 129x33)
   loop     40 steps of main_1d_euler.cpp's loop (WENO-Z face-flux pair on Sod
-           and Shu-Osher; WENO5-JS ComputedF on Sod, as the shipped driver) and
-           of main_2d_euler(quad|dmach).cpp's loop
+           and Shu-Osher; WENO5-JS ComputedF, as the shipped driver, on Sod at
+           481 points and in the driver's own 15361-point configuration) and of
+           main_2d_euler(quad|dmach).cpp's loop
 """
 import os
 import subprocess
