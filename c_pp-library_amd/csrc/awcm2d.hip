@@ -7,7 +7,8 @@
 //           (ARRAY_2D.h:112), so every stencil address is the reference's.
 //   masks   uint8, same ghosted index; mask_temp, mask0..3, a snapshot and the
 //           interpolation work mask.
-//   levels  uint8, index i + Nx*j (no ghosts).
+//   levels  uint8, the masks' ghosted index (ghost entries hold the max level
+//           and are never read), so a row segment is one aligned 16-byte load.
 //   lists   per stage: compacted finest-level points and coarse points,
 //           packed (j << 16) | i.
 //
@@ -595,16 +596,15 @@ __global__ void k2_level_add(Dims2 D, const uint8_t* __restrict__ src, uint8_t* 
     const uint8_t v = src[t];
     if (v) dst[t] = 1;
     if (i < 0 || i > D.Nx - 1 || j < 0 || j > D.Ny - 1) return;
-    const long long li = (long long)i + (long long)D.Nx * j;
-    if (!v) { lvl[li] = 0; return; }
-    int lev = lvl[li];  // kept when no neighbour is active
+    if (!v) { lvl[t] = 0; return; }
+    int lev = lvl[t];  // kept when no neighbour is active
     int d = 1;
     for (int l = D.maxlev; l >= 0; l--) {
         if (i - d < -G || i + d > D.Nx - 1 + G || j - d < -G || j + d > D.Ny - 1 + G) { sc->flag |= 1; break; }
         if (src[t - d] || src[t + d] || src[t - (long long)d * D.P] || src[t + (long long)d * D.P]) { lev = l; break; }
         d *= 2;
     }
-    lvl[li] = (uint8_t)lev;
+    lvl[t] = (uint8_t)lev;
     if (i < 1 || i > D.Nx - 2 || j < 1 || j > D.Ny - 2) return;
     const int dd = 1 << (D.L - 1 - lev);  // grid[lev].dx
     const int NX = D.Nx - 1, NY = D.Ny - 1;
@@ -652,21 +652,36 @@ k2_compact(Dims2 D, const uint8_t* m0, const uint8_t* m1, const uint8_t* m2, con
     Seg s;
     const bool ok = seg_of(D, (long long)blockIdx.x * blockDim.x + threadIdx.x, s) && s.j >= 0 && s.j <= D.Ny - 1;
     if (ok) {
-        V16 v[3];
-        for (int k = 0; k < 3; k++) v[k] = ld16(mk[k] + s.p0);
+        // every byte the segment needs, as independent loads: masks and levels
+        // of the row and of the row below (16-byte aligned), and of the point
+        // left of the segment (row j >= 0 has ghost rows and a ghost column
+        // before it, so p0 - 1 and p0 - P lie inside the buffer)
+        V16 mr[3], lr[3], mb[3], lb[3];
+        uint8_t ml[3], ll[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            mr[k] = ld16(mk[k] + s.p0);
+            lr[k] = ld16(lk[k] + s.p0);
+            mb[k] = ld16(mk[k] + s.p0 - D.P);
+            lb[k] = ld16(lk[k] + s.p0 - D.P);
+            ml[k] = mk[k][s.p0 - 1];
+            ll[k] = lk[k][s.p0 - 1];
+        }
+        const uint8_t top = uint8_t(D.maxlev);
+#pragma unroll
         for (int e = 0; e < 16; e++) {
             const int i = s.i0 + e;
             uint32_t f = 0;
             if (i >= 0 && i <= D.Nx - 1) {
-                const long long li = (long long)i + (long long)D.Nx * s.j;
+#pragma unroll
                 for (int k = 0; k < 3; k++) {
-                    if (!v[k].b[e]) continue;
-                    if (lk[k][li] != D.maxlev) { f |= 1u << (4 * k + 1); cnt[4 * k + 1]++; continue; }
+                    if (!mr[k].b[e]) continue;
+                    if (lr[k].b[e] != top) { f |= 1u << (4 * k + 1); cnt[4 * k + 1]++; continue; }
                     f |= 1u << (4 * k);
                     cnt[4 * k]++;
-                    const long long p = s.p0 + e;
-                    const bool lw = i >= 1 && mk[k][p - 1] && lk[k][li - 1] == D.maxlev;
-                    const bool bw = s.j >= 1 && mk[k][p - D.P] && lk[k][li - D.Nx] == D.maxlev;
+                    const bool lm = e ? mr[k].b[e - 1] && lr[k].b[e - 1] == top : ml[k] && ll[k] == top;
+                    const bool lw = i >= 1 && lm;
+                    const bool bw = s.j >= 1 && mb[k].b[e] && lb[k].b[e] == top;
                     if (!lw) { f |= 1u << (4 * k + 2); cnt[4 * k + 2]++; }
                     if (!bw) { f |= 1u << (4 * k + 3); cnt[4 * k + 3]++; }
                 }
@@ -712,16 +727,52 @@ k2_compact(Dims2 D, const uint8_t* m0, const uint8_t* m1, const uint8_t* m2, con
 // ---------------------------------------------------- CFL and LF alphas ----
 // alpha_x(j) = max_i SRx, alpha_y(i) = max_j SRy over `mask` (exact maxima), and
 // for stage 1 also max(SRx + SRy) for the CFL step (main_2d_euler(dmach).cpp:61-104).
+// k2_alpha: a block takes a tile of 64 columns x kAlphaRows rows and writes its
+// row, column and CFL maxima as partials.  The ghost-fill kernel that follows
+// on the stream (k2_bc) reduces them into alpha_x, alpha_y and the CFL maximum.
+// Maxima combine as unsigned bit patterns of non-negative doubles starting
+// from 0, with partials that are not > 0 (NaN kept) replaced by 0: exact, and
+// in every case the value that atomicMax of each positive partial into a
+// zeroed array gives.
 constexpr int kAlphaRows = 32;  // rows per block; each thread takes kAlphaRows/4 of them
 
+__device__ __forceinline__ double bits_max(double a, double b) {
+    return (unsigned long long)__double_as_longlong(b) > (unsigned long long)__double_as_longlong(a) ? b : a;
+}
+__device__ __forceinline__ double pos_or_zero(double v) { return (v <= 0) ? 0.0 : v; }
+
+// bit-pattern maximum of p[0], p[stride], ..., p[(n-1) stride] and 0, with the
+// loads issued eight at a time
+__device__ __forceinline__ double strided_bits_max(const double* p, long long stride, int n) {
+    double m = 0.0;
+    int b = 0;
+    for (; b + 8 <= n; b += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = p[(b + u) * stride];
+#pragma unroll
+        for (int u = 0; u < 8; u++) m = bits_max(m, v[u]);
+    }
+    for (; b < n; b++) m = bits_max(m, p[b * stride]);
+    return m;
+}
+
+struct AlphaPart {
+    double* px;  // [ntx][Ny]  row maxima of each tile column
+    double* py;  // [nty][Nx]  column maxima of each tile row
+    double* pv;  // [nty][ntx] CFL maxima per tile
+    int ntx, nty;
+};
+
 __global__ void __launch_bounds__(256) k2_alpha(Dims2 D, const uint8_t* __restrict__ mask, const double* __restrict__ Q,
-                                                double* ax, double* ay, double gamma, int with_vmax, Scal2* sc) {
+                                                AlphaPart P, double gamma, const Scal2* sc) {
     if (sc->skip) return;
     constexpr int R = kAlphaRows / 4;
     __shared__ double colr[4][64];
     __shared__ double vr[4];
-    const int i = blockIdx.x * 64 + threadIdx.x;
-    const int j0 = blockIdx.y * kAlphaRows + threadIdx.y;
+    const int tx = blockIdx.x, ty = blockIdx.y;
+    const int i = tx * 64 + threadIdx.x;
+    const int j0 = ty * kAlphaRows + threadIdx.y;
     // all mask loads, then all state loads, then the arithmetic
     uint8_t act[R];
 #pragma unroll
@@ -755,7 +806,7 @@ __global__ void __launch_bounds__(256) k2_alpha(Dims2 D, const uint8_t* __restri
     for (int r = 0; r < R; r++) {
         const int j = j0 + 4 * r;
         const double rm = wave_max(sxr[r]);
-        if (threadIdx.x == 0 && j < D.Ny && !(rm <= 0)) atomic_max_pos((unsigned long long*)&ax[j], rm);
+        if (threadIdx.x == 0 && j < D.Ny) P.px[(long long)tx * D.Ny + j] = pos_or_zero(rm);
     }
     colr[threadIdx.y][threadIdx.x] = cm;
     const double wv = wave_max(vm);
@@ -763,11 +814,8 @@ __global__ void __launch_bounds__(256) k2_alpha(Dims2 D, const uint8_t* __restri
     __syncthreads();
     if (threadIdx.y == 0) {
         const double c = t_max(t_max(colr[0][threadIdx.x], colr[1][threadIdx.x]), t_max(colr[2][threadIdx.x], colr[3][threadIdx.x]));
-        if (i < D.Nx && !(c <= 0)) atomic_max_pos((unsigned long long*)&ay[i], c);
-        if (with_vmax && threadIdx.x == 0) {
-            const double v = t_max(t_max(vr[0], vr[1]), t_max(vr[2], vr[3]));
-            if (!(v <= 0)) atomic_max_pos(&sc->vmax, v);
-        }
+        if (i < D.Nx) P.py[(long long)ty * D.Nx + i] = pos_or_zero(c);
+        if (threadIdx.x == 0) P.pv[ty * P.ntx + tx] = pos_or_zero(t_max(t_max(vr[0], vr[1]), t_max(vr[2], vr[3])));
     }
 }
 
@@ -781,16 +829,36 @@ __device__ __forceinline__ void get4(const double* Q, long long plane, long long
 
 // Ghost fill of one field at time t; with compute_dt, block 0 first turns the
 // CFL maximum into dt with the output-time clip (main_2d_euler(dmach).cpp:76-83).
-__global__ void k2_bc(Dims2 D, double* Q, int kind, double x0, double dx, int compute_dt, const double* pt, Scal2* sc) {
+// Blocks nbc.. of the grid reduce the alpha partials k2_alpha left (one row or
+// column per thread); with compute_dt, block 0 first reduces the CFL partials
+// and turns the maximum into dt with the output-time clip
+// (main_2d_euler(dmach).cpp:76-83).  The other blocks fill the ghosts.
+__global__ void __launch_bounds__(256) k2_bc(Dims2 D, double* Q, int kind, double x0, double dx, int compute_dt,
+                                             const double* pt, AlphaPart P, double* ax, double* ay, int nbc, Scal2* sc) {
     if (sc->skip) return;
-    if (compute_dt && blockIdx.x == 0 && threadIdx.x == 0) {
-        const double vmax = __longlong_as_double((long long)sc->vmax);
-        double dt = sc->cfl * dx / vmax;
-        int dp = 0;
-        if (sc->cnt < sc->n_print && sc->t + dt > pt[sc->cnt]) { dt = pt[sc->cnt] - sc->t; dp = 1; }
-        sc->dt = dt;
-        sc->do_print = dp;
-        if (!(vmax > 0) || !(vmax < INFINITY)) sc->flag |= 2;
+    if ((int)blockIdx.x >= nbc) {
+        const int r = (blockIdx.x - nbc) * blockDim.x + threadIdx.x;
+        if (r < D.Ny) ax[r] = strided_bits_max(P.px + r, D.Ny, P.ntx);
+        else if (r < D.Ny + D.Nx) ay[r - D.Ny] = strided_bits_max(P.py + (r - D.Ny), D.Nx, P.nty);
+        return;
+    }
+    if (compute_dt && blockIdx.x == 0) {
+        __shared__ double red[256];
+        double m = 0.0;
+        for (int k = threadIdx.x; k < P.ntx * P.nty; k += blockDim.x) m = bits_max(m, P.pv[k]);
+        red[threadIdx.x] = m;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double vmax = 0.0;
+            for (int k = 0; k < (int)blockDim.x; k++) vmax = bits_max(vmax, red[k]);
+            sc->vmax = (unsigned long long)__double_as_longlong(vmax);
+            double dt = sc->cfl * dx / vmax;
+            int dp = 0;
+            if (sc->cnt < sc->n_print && sc->t + dt > pt[sc->cnt]) { dt = pt[sc->cnt] - sc->t; dp = 1; }
+            sc->dt = dt;
+            sc->do_print = dp;
+            if (!(vmax > 0) || !(vmax < INFINITY)) sc->flag |= 2;
+        }
     }
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int nl = G * D.Ny, nb = G * D.Nx;
@@ -924,7 +992,7 @@ __global__ void __launch_bounds__(256) k2_stage_update(Dims2 D, const uint32_t* 
         const uint32_t e = lc[t - nW];
         const int i = int(e & 0xffffu), j = int(e >> 16);
         const long long p = gidx(D, i, j);
-        const int lev = lvl[(long long)i + (long long)D.Nx * j];
+        const int lev = lvl[p];
         const int d = 1 << (D.L - 1 - lev);
         // a closed mask keeps every used stencil point inside the ghost band
         const bool okx = (i <= d) ? (i - d >= -G && i + 2 * d <= D.Nx - 1 + G)
@@ -1021,9 +1089,12 @@ __global__ void k2_soa_to_aos(const double* __restrict__ soa, double* __restrict
     const long long d = (t % PR) + (long long)P * (t / PR);
     for (int k = 0; k < 4; k++) aos[t * 4 + k] = soa[d + k * plane];
 }
-__global__ void k2_widen_levels(const uint8_t* __restrict__ src, int32_t* __restrict__ dst, long long n) {
+// level field (ghosted device layout) -> int32, index i + Nx*j
+__global__ void k2_widen_levels(Dims2 D, const uint8_t* __restrict__ src, int32_t* __restrict__ dst) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) dst[t] = src[t];
+    if (t >= (long long)D.Nx * D.Ny) return;
+    const int i = int(t % D.Nx), j = int(t / D.Nx);
+    dst[t] = src[gidx(D, i, j)];
 }
 
 inline unsigned nblk(long long n, int b) { return unsigned((n + b - 1) / b); }
@@ -1063,6 +1134,9 @@ Solver2D::~Solver2D() {
     for (hipEvent_t e : ev_pool) hipEventDestroy(e);
     for (void* p : allocs) hipFree(p);
     if (stream) hipStreamDestroy(stream);
+    if (s_int) hipStreamDestroy(s_int);
+    for (hipEvent_t e : {ev_m3, ev_int})
+        if (e) hipEventDestroy(e);
 }
 
 int Solver2D::dalloc(void* pp, size_t bytes) {
@@ -1116,15 +1190,23 @@ int Solver2D::init(int L, int nxc_, int nyc_, double g, double dx_, double x0_, 
     }
     set_tail(0);
     wavelet_weights(W.w);
-    if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) return AWCM_EDEVICE;
+    if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&s_int, hipStreamNonBlocking) != hipSuccess)
+        return AWCM_EDEVICE;
+    for (hipEvent_t* e : {&ev_m3, &ev_int})
+        if (hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess) return AWCM_EDEVICE;
     const size_t pb = sizeof(double) * 4 * size_t(D.plane);
     const size_t mb = size_t(D.plane);
     const size_t nl = size_t(Nx) * Ny;
     int st;
     if ((st = dalloc(&Q, pb)) || (st = dalloc(&Q0, pb)) || (st = dalloc(&Q1, pb)) || (st = dalloc(&Q2, pb))) return st;
     if ((st = dalloc(&mt, mb)) || (st = dalloc(&snap, mb)) || (st = dalloc(&tmp, mb))) return st;
-    for (int k = 0; k < 4; k++) if ((st = dalloc(&m[k], mb)) || (st = dalloc(&lv[k], nl))) return st;
+    for (int k = 0; k < 4; k++) if ((st = dalloc(&m[k], mb)) || (st = dalloc(&lv[k], mb))) return st;
     if ((st = dalloc(&alpha, sizeof(double) * 3 * (Nx + Ny)))) return st;
+    {
+        const size_t ntx = (size_t(Nx) + 63) / 64, nty = (size_t(Ny) + kAlphaRows - 1) / kAlphaRows;
+        if ((st = dalloc(&apart, sizeof(double) * (ntx * Ny + nty * Nx + ntx * nty)))) return st;
+    }
     if ((st = dalloc(&lists, sizeof(uint32_t) * 12 * nl))) return st;
     if ((st = dalloc(&FX, pb)) || (st = dalloc(&FY, pb))) return st;
     if ((st = dalloc(&pt, sizeof(double) * 4096))) return st;
@@ -1132,7 +1214,7 @@ int Solver2D::init(int L, int nxc_, int nyc_, double g, double dx_, double x0_, 
     hipMemset(Q, 0, pb); hipMemset(Q0, 0, pb); hipMemset(Q1, 0, pb); hipMemset(Q2, 0, pb);
     // AWCM_2D ctor (AWCM_2D.h:26-41): masks false except mask0; levels = max level
     hipMemset(mt, 0, mb); hipMemset(snap, 0, mb); hipMemset(tmp, 0, mb);
-    for (int k = 0; k < 4; k++) { hipMemset(m[k], k == 0 ? 1 : 0, mb); hipMemset(lv[k], D.maxlev, nl); }
+    for (int k = 0; k < 4; k++) { hipMemset(m[k], k == 0 ? 1 : 0, mb); hipMemset(lv[k], D.maxlev, mb); }
     hipMemset(alpha, 0, sizeof(double) * 3 * (Nx + Ny));
     Scal2 s0;
     std::memset(&s0, 0, sizeof(s0));
@@ -1194,7 +1276,7 @@ int Solver2D::get_masks(uint8_t* masks, int32_t* levels) {
         if (hipMalloc(&wide, sizeof(int32_t) * nl) != hipSuccess) return AWCM_ENOMEM;
         int st = AWCM_OK;
         for (int k = 0; k < 4 && !st; k++) {
-            hipLaunchKernelGGL(k2_widen_levels, dim3(nblk(nl, 256)), dim3(256), 0, stream, lv[k], wide, (long long)nl);
+            hipLaunchKernelGGL(k2_widen_levels, dim3(nblk(nl, 256)), dim3(256), 0, stream, D, lv[k], wide);
             if (hipMemcpyAsync(levels + k * nl, wide, sizeof(int32_t) * nl, hipMemcpyDeviceToHost, stream) != hipSuccess)
                 st = AWCM_EDEVICE;
             if (hipStreamSynchronize(stream) != hipSuccess) st = AWCM_EDEVICE;
@@ -1305,9 +1387,10 @@ static int check_recon(Solver2D& s, uint8_t* mask) {
     return AWCM_OK;
 }
 
-#define AWCM_LAT(OP, lev, kind, two, Qp, mk, aux)                                                                 \
-    AWCM_LAUNCH((k2_lat<OP, kLatK>), dim3(nblk(lat_threads(D, lev, kind, two), 256)), dim3(256), 0, stream, D, lev, \
+#define AWCM_LAT_ON(strm, OP, lev, kind, two, Qp, mk, aux)                                                       \
+    AWCM_LAUNCH((k2_lat<OP, kLatK>), dim3(nblk(lat_threads(D, lev, kind, two), 256)), dim3(256), 0, strm, D, lev, \
                 W, tol_host, Qp, mk, aux, sc)
+#define AWCM_LAT(OP, lev, kind, two, Qp, mk, aux) AWCM_LAT_ON(stream, OP, lev, kind, two, Qp, mk, aux)
 
 int Solver2D::enqueue_prepare(bool force) {
     const long long nseg = (long long)(D.P / 16) * (D.Ny + 2 * G);
@@ -1330,6 +1413,23 @@ int Solver2D::enqueue_prepare(bool force) {
                     lv[k - 1], sc);
         if (int st = check_recon(*this, m[k])) return st;
     }
+    // mask3 is final.  Interpolation of momentum and energy at new points
+    // (planes 1..3) on the side stream, the inverse transform of density
+    // (plane 0) here; both only read mask3 and mask_temp.  The reference builds
+    // tmp = CheckReconstruction(mask3 minus mask_temp) and interpolates where tmp
+    // is set and mask_temp is not (WAVELET_2D.h:789-794,813).  mask3 is closed
+    // under CheckReconstruction, so the closure only ever adds points of mask3
+    // and tmp minus mask_temp is exactly mask3 minus mask_temp: the work mask is
+    // mask3 itself, tested against mask_temp, and the closure pass is not needed.
+    if (hipEventRecord(ev_m3, stream) != hipSuccess || hipStreamWaitEvent(s_int, ev_m3, 0) != hipSuccess)
+        return AWCM_EDEVICE;
+    if (tail >= 0)
+        AWCM_LAUNCH(k2_tail<TAIL_INT>, dim3(1), dim3(1024), 0, s_int, D, tail, W, tol_host, Q, m[3], mt, sc);
+    for (int lev = tail + 1; lev < D.maxlev; lev++) {
+        AWCM_LAT_ON(s_int, OP_INT_D12, lev, LAT_D1, true, Q, m[3], mt);
+        AWCM_LAT_ON(s_int, OP_INT_D3, lev, LAT_D3, false, Q, m[3], mt);
+    }
+    if (hipEventRecord(ev_int, s_int) != hipSuccess) return AWCM_EDEVICE;
     // inverse transform, coarsest level first
     if (tail >= 0)
         AWCM_LAUNCH(k2_tail<TAIL_IWT>, dim3(1), dim3(1024), 0, stream, D, tail, W, tol_host, Q, m[3], nullptr, sc);
@@ -1338,31 +1438,25 @@ int Solver2D::enqueue_prepare(bool force) {
         AWCM_LAT(OP_IWT_D12, lev, LAT_D1, true, Q, m[3], nullptr);
         AWCM_LAT(OP_IWT_D3, lev, LAT_D3, false, Q, m[3], nullptr);
     }
-    // interpolation of momentum and energy at new points.  The reference builds
-    // tmp = CheckReconstruction(mask3 minus mask_temp) and interpolates where tmp
-    // is set and mask_temp is not (WAVELET_2D.h:789-794,813).  mask3 is closed
-    // under CheckReconstruction, so the closure only ever adds points of mask3
-    // and tmp minus mask_temp is exactly mask3 minus mask_temp: the work mask is
-    // mask3 itself, tested against mask_temp, and the closure pass is not needed.
-    if (tail >= 0)
-        AWCM_LAUNCH(k2_tail<TAIL_INT>, dim3(1), dim3(1024), 0, stream, D, tail, W, tol_host, Q, m[3], mt, sc);
-    for (int lev = tail + 1; lev < D.maxlev; lev++) {
-        AWCM_LAT(OP_INT_D12, lev, LAT_D1, true, Q, m[3], mt);
-        AWCM_LAT(OP_INT_D3, lev, LAT_D3, false, Q, m[3], mt);
-    }
+    // join the side stream
+    if (hipStreamWaitEvent(stream, ev_int, 0) != hipSuccess) return AWCM_EDEVICE;
     return AWCM_OK;
 }
 
 int Solver2D::enqueue_step() {
     hipEvent_t e0 = prof ? mark() : nullptr;
     if (int st = enqueue_prepare(false)) return st;
-    const long long nl = (long long)D.Nx * D.Ny;
-    (void)nl;
     AWCM_LAUNCH(k2_compact, dim3(nblk((long long)(D.P / 16) * (D.Ny + 2 * G), kCompactThreads)), dim3(kCompactThreads), 0,
                 stream, D, m[0], m[1], m[2], lv[0], lv[1], lv[2], lists, sc);
     if (prof) prof_ev[2].push_back({e0, mark()});
     const dim3 ag(nblk(D.Nx, 64), nblk(D.Ny, kAlphaRows)), ab(64, 4);
-    const unsigned nbc = nblk(2 * G * (D.Nx + D.Ny), 256);
+    AlphaPart ap;
+    ap.px = apart;
+    ap.py = apart + size_t(ag.x) * D.Ny;
+    ap.pv = ap.py + size_t(ag.y) * D.Nx;
+    ap.ntx = int(ag.x);
+    ap.nty = int(ag.y);
+    const unsigned nbc = nblk(2 * G * (D.Nx + D.Ny), 256), nred = nblk(D.Nx + D.Ny, 256);
     const unsigned gw = grid_blocks * 2, gc = grid_blocks;
     hipEvent_t pe = nullptr;
     // stage s reads Qs (ghosts filled), uses alpha set s and the lists of mask k
@@ -1370,15 +1464,13 @@ int Solver2D::enqueue_step() {
     const StageArgs sa[3] = {{2, Q, Q, Q2}, {1, Q, Q2, Q1}, {0, Q1, Q1, Q0}};
     for (int s = 0; s < 3; s++) {
         const StageArgs& g = sa[s];
-        if (s == 0) {
-            // alpha over mask3 of Q (+ CFL maximum), then the ghosts of Q and dt
-            AWCM_LAUNCH(k2_alpha, ag, ab, 0, stream, D, m[3], Q, ax(0), ay(0), gamma, 1, sc);
-            AWCM_LAUNCH(k2_bc, dim3(nbc), dim3(256), 0, stream, D, Q, bc_kind, x0, dx, 1, pt, sc);
-        } else {
-            // ghosts of the previous stage's output, then alpha over its mask
-            AWCM_LAUNCH(k2_bc, dim3(nbc), dim3(256), 0, stream, D, const_cast<double*>(g.Qs), bc_kind, x0, dx, 0, pt, sc);
-            AWCM_LAUNCH(k2_alpha, ag, ab, 0, stream, D, m[g.k + 1], g.Qs, ax(s), ay(s), gamma, 0, sc);
-        }
+        // alpha partials over the stage's outer mask (interior points only),
+        // then the ghost fill of Qs, which also reduces the partials (and, in
+        // stage 1, turns the CFL maximum into dt)
+        const int mk = s == 0 ? 3 : g.k + 1;
+        AWCM_LAUNCH(k2_alpha, ag, ab, 0, stream, D, m[mk], g.Qs, ap, gamma, sc);
+        AWCM_LAUNCH(k2_bc, dim3(nbc + nred), dim3(256), 0, stream, D, const_cast<double*>(g.Qs), bc_kind, x0, dx,
+                    s == 0 ? 1 : 0, pt, ap, ax(s), ay(s), int(nbc), sc);
         if (prof) pe = mark();
         AWCM_LAUNCH(k2_stage_faces, dim3(gw), dim3(128), 0, stream, D, list(g.k, 0), list(g.k, 2), list(g.k, 3), g.k,
                     g.Qs, FX, FY, ax(s), ay(s), sc);

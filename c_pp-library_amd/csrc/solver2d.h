@@ -46,12 +46,19 @@ struct Solver2D {
     Lat grd[16]{};
     Wts W{};
     hipStream_t stream = nullptr;
+    // Side stream of PrepareEuler, forked from and joined back into `stream` by
+    // events (graph capture records it as a parallel branch): the interpolation
+    // of momentum and energy (planes 1..3) runs beside the inverse transform of
+    // density (plane 0); both only read mask3 and mask_temp.
+    hipStream_t s_int = nullptr;
+    hipEvent_t ev_m3 = nullptr, ev_int = nullptr;
     std::vector<void*> allocs;
 
     double *Q = nullptr, *Q0 = nullptr, *Q1 = nullptr, *Q2 = nullptr;  // 4 planes each
     uint8_t *mt = nullptr, *m[4] = {}, *snap = nullptr, *tmp = nullptr;
     uint8_t* lv[4] = {};
     double* alpha = nullptr;     // 3 sets of (Ny alpha_x, Nx alpha_y)
+    double* apart = nullptr;     // alpha partials per tile (row, column and CFL maxima)
     uint32_t* lists = nullptr;   // 12 lists of Nx*Ny packed (j << 16 | i): W, C, XS, YS per stage mask
     double *FX = nullptr, *FY = nullptr;  // face fluxes: right / top face of each point, 4 planes
     double* pt = nullptr;        // print times
