@@ -275,7 +275,7 @@ template <int OP, int K>
 __global__ void __launch_bounds__(256) k2_lat(Dims2 D, int lev, Wts W, double tol, double* __restrict__ Q,
                                               uint8_t* __restrict__ mask, const uint8_t* __restrict__ aux,
                                               const Scal2* sc) {
-    if (sc->skip) return;
+    const int skip = sc->skip;  // tested after the mask loads
     const int d = 1 << (D.L - 1 - lev);
     int kind = op_kind1<OP>();
     Lat g = lattice(D, lev, kind);
@@ -294,6 +294,7 @@ __global__ void __launch_bounds__(256) k2_lat(Dims2 D, int lev, Wts W, double to
         const int ii = a * K + k;
         act[k] = ii < g.ni ? op_active<OP>(mask, aux, gidx(D, g.is + ii * g.dx, j)) : 0;
     }
+    if (skip) return;
 #pragma unroll
     for (int k = 0; k < K; k++) {
         const int ii = a * K + k;
@@ -380,24 +381,38 @@ __device__ __forceinline__ int clamp_start(int c, int d, int lim) {
     return s;
 }
 
-__device__ __forceinline__ void check_cell(const Dims2& D, int lev, int I, int J, uint8_t* mask, Scal2* sc) {
+// every mask byte a cell needs, loaded up front (independent loads)
+struct CellMasks {
+    uint8_t a3, o1, o2, c1[6], c2[6];
+};
+__device__ __forceinline__ CellMasks check_cell_load(const Dims2& D, int lev, int I, int J, const uint8_t* mask) {
     const int m = 1 << (D.L - 1 - lev), h = m >> 1;
     const int nI = (D.Nx - 1) / m, nJ = (D.Ny - 1) / m;
     const int i3 = h + I * m, j3 = h + J * m;
     const bool has1 = J + 1 <= nJ - 1, has2 = I + 1 <= nI - 1;
     const int j1 = (J + 1) * m, i2 = (I + 1) * m;
-    // every mask byte the cell needs is loaded up front (independent loads)
-    uint8_t c1[6], c2[6];
-    const uint8_t a3 = mask[gidx(D, i3, j3)];
-    const uint8_t o1 = has1 ? mask[gidx(D, i3, j1)] : 0;
-    const uint8_t o2 = has2 ? mask[gidx(D, i2, j3)] : 0;
+    CellMasks c;
+    c.a3 = mask[gidx(D, i3, j3)];
+    c.o1 = has1 ? mask[gidx(D, i3, j1)] : 0;
+    c.o2 = has2 ? mask[gidx(D, i2, j3)] : 0;
 #pragma unroll
     for (int q = 0; q < 6; q++) {
         const int o = 2 * q - 5;
         const int jj = j1 + o * h, ii = i2 + o * h;
-        c1[q] = (has1 && jj >= h && jj <= D.Ny - 1 - h) ? mask[gidx(D, i3, jj)] : 0;
-        c2[q] = (has2 && ii >= h && ii <= D.Nx - 1 - h) ? mask[gidx(D, ii, j3)] : 0;
+        c.c1[q] = (has1 && jj >= h && jj <= D.Ny - 1 - h) ? mask[gidx(D, i3, jj)] : 0;
+        c.c2[q] = (has2 && ii >= h && ii <= D.Nx - 1 - h) ? mask[gidx(D, ii, j3)] : 0;
     }
+    return c;
+}
+
+__device__ __forceinline__ void check_cell_apply(const Dims2& D, int lev, int I, int J, const CellMasks& cm,
+                                                 uint8_t* mask, Scal2* sc) {
+    const int m = 1 << (D.L - 1 - lev), h = m >> 1;
+    const int i3 = h + I * m, j3 = h + J * m;
+    const int j1 = (J + 1) * m, i2 = (I + 1) * m;
+    const uint8_t a3 = cm.a3, o1 = cm.o1, o2 = cm.o2;
+    const uint8_t* c1 = cm.c1;
+    const uint8_t* c2 = cm.c2;
     // d1 / d2 points activated by the clamped stencil of an active d3 point
     bool a1 = o1 != 0, a2 = o2 != 0;
 #pragma unroll
@@ -429,18 +444,29 @@ __device__ __forceinline__ void check_cell(const Dims2& D, int lev, int I, int J
     }
 }
 
+__device__ __forceinline__ void check_cell(const Dims2& D, int lev, int I, int J, uint8_t* mask, Scal2* sc) {
+    check_cell_apply(D, lev, I, J, check_cell_load(D, lev, I, J, mask), mask, sc);
+}
+
+// The skip flag is read beside the mask loads, not before them: a skipped
+// step costs the loads, a live one saves a round trip.
 template <int K>
 __global__ void __launch_bounds__(256) k2_check(Dims2 D, int lev, uint8_t* __restrict__ mask, Scal2* sc) {
-    if (sc->skip) return;
+    const int skip = sc->skip;
     const int m = 1 << (D.L - 1 - lev);
     const int nI = (D.Nx - 1) / m, nJ = (D.Ny - 1) / m;
     const int segs = (nI + K - 1) / K;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= segs * nJ) return;
     const int J = t / segs, I0 = (t % segs) * K;
+    CellMasks cm[K];
 #pragma unroll
     for (int k = 0; k < K; k++)
-        if (I0 + k < nI) check_cell(D, lev, I0 + k, J, mask, sc);
+        if (I0 + k < nI) cm[k] = check_cell_load(D, lev, I0 + k, J, mask);
+    if (skip) return;
+#pragma unroll
+    for (int k = 0; k < K; k++)
+        if (I0 + k < nI) check_cell_apply(D, lev, I0 + k, J, cm[k], mask, sc);
 }
 
 // Levels T..0 of the check in one workgroup.
@@ -509,10 +535,11 @@ __global__ void k2_begin_copy(Dims2 D, const uint8_t* __restrict__ src, uint8_t*
 
 // BandingMask (2-wide band) on mask0, then the AddAdjacentZone snapshot
 __global__ void k2_band_snap(Dims2 D, uint8_t* __restrict__ m0, uint8_t* __restrict__ snap, const Scal2* sc) {
-    if (sc->skip) return;
+    const int skip = sc->skip;  // tested after the mask load
     Seg s;
     if (!seg_of(D, (long long)blockIdx.x * blockDim.x + threadIdx.x, s)) return;
     V16 v = ld16(m0 + s.p0);
+    if (skip) return;
     const bool row_band = s.j >= 0 && s.j <= D.Ny - 1 && (s.j < 2 || s.j > D.Ny - 1 - 2);
     bool changed = false;
     if (s.j >= 0 && s.j <= D.Ny - 1) {
@@ -560,12 +587,14 @@ __device__ __forceinline__ int classify(int i, int j, const Dims2& D, int& mu) {
 // AWCM_MANAGER_2D::AddAdjacentZone (AWCM_MANAGER_2D.h:14-61), every level at
 // once from the snapshot; one point per thread.
 __global__ void k2_adjacent(Dims2 D, const uint8_t* __restrict__ snap, uint8_t* __restrict__ mask, Scal2* sc) {
-    if (sc->skip) return;
+    const int skip = sc->skip;  // tested after the snapshot load
     const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
     if (i >= D.Nx) return;
     int mu;
     const int lev = classify(i, j, D, mu);
-    if (lev < 0 || !snap[gidx(D, i, j)]) return;
+    if (lev < 0) return;
+    const uint8_t sv = snap[gidx(D, i, j)];
+    if (skip || !sv) return;
     if (lev == D.maxlev - 1) {
         mset(D, mask, i - 1, j, sc); mset(D, mask, i + 1, j, sc);
         mset(D, mask, i, j - 1, sc); mset(D, mask, i, j + 1, sc);
@@ -589,11 +618,12 @@ __global__ void k2_adjacent(Dims2 D, const uint8_t* __restrict__ snap, uint8_t* 
 // reference.  Grid: x over the ghosted row (Nx+6 points), y over ghosted rows.
 __global__ void k2_level_add(Dims2 D, const uint8_t* __restrict__ src, uint8_t* __restrict__ dst,
                              uint8_t* __restrict__ lvl, Scal2* sc) {
-    if (sc->skip) return;
+    const int skip = sc->skip;  // tested after the mask load
     const int i = int(blockIdx.x * blockDim.x + threadIdx.x) - G, j = int(blockIdx.y) - G;
     if (i > D.Nx - 1 + G) return;
     const long long t = gidx(D, i, j);
     const uint8_t v = src[t];
+    if (skip) return;
     if (v) dst[t] = 1;
     if (i < 0 || i > D.Nx - 1 || j < 0 || j > D.Ny - 1) return;
     if (!v) { lvl[t] = 0; return; }
@@ -639,7 +669,7 @@ constexpr int kNL = 12;
 __global__ void __launch_bounds__(kCompactThreads)
 k2_compact(Dims2 D, const uint8_t* m0, const uint8_t* m1, const uint8_t* m2, const uint8_t* l0, const uint8_t* l1,
            const uint8_t* l2, uint32_t* lists, Scal2* sc) {
-    if (sc->skip) return;
+    const int skip = sc->skip;  // tested after the mask and level loads
     __shared__ int wtot[kNL][kCompactThreads / 64];
     __shared__ int base[kNL];
     const long long N = (long long)D.Nx * D.Ny;
@@ -689,6 +719,7 @@ k2_compact(Dims2 D, const uint8_t* m0, const uint8_t* m1, const uint8_t* m2, con
             fl[e] = f;
         }
     }
+    if (skip) return;  // uniform over the block, before its barriers
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     int incl[kNL];
 #pragma unroll
@@ -766,7 +797,7 @@ struct AlphaPart {
 
 __global__ void __launch_bounds__(256) k2_alpha(Dims2 D, const uint8_t* __restrict__ mask, const double* __restrict__ Q,
                                                 AlphaPart P, double gamma, const Scal2* sc) {
-    if (sc->skip) return;
+    const int skip = sc->skip;  // tested after the mask loads
     constexpr int R = kAlphaRows / 4;
     __shared__ double colr[4][64];
     __shared__ double vr[4];
@@ -780,6 +811,7 @@ __global__ void __launch_bounds__(256) k2_alpha(Dims2 D, const uint8_t* __restri
         const int j = j0 + 4 * r;
         act[r] = (i < D.Nx && j < D.Ny) ? mask[gidx(D, i, j)] : 0;
     }
+    if (skip) return;  // uniform over the block, before its barrier
     double q[R][4];
 #pragma unroll
     for (int r = 0; r < R; r++) {
@@ -835,11 +867,16 @@ __device__ __forceinline__ void get4(const double* Q, long long plane, long long
 // (main_2d_euler(dmach).cpp:76-83).  The other blocks fill the ghosts.
 __global__ void __launch_bounds__(256) k2_bc(Dims2 D, double* Q, int kind, double x0, double dx, int compute_dt,
                                              const double* pt, AlphaPart P, double* ax, double* ay, int nbc, Scal2* sc) {
-    if (sc->skip) return;
+    const int skip = sc->skip;  // tested after the loads, before any store
     if ((int)blockIdx.x >= nbc) {
         const int r = (blockIdx.x - nbc) * blockDim.x + threadIdx.x;
-        if (r < D.Ny) ax[r] = strided_bits_max(P.px + r, D.Ny, P.ntx);
-        else if (r < D.Ny + D.Nx) ay[r - D.Ny] = strided_bits_max(P.py + (r - D.Ny), D.Nx, P.nty);
+        if (r < D.Ny) {
+            const double m = strided_bits_max(P.px + r, D.Ny, P.ntx);
+            if (!skip) ax[r] = m;
+        } else if (r < D.Ny + D.Nx) {
+            const double m = strided_bits_max(P.py + (r - D.Ny), D.Nx, P.nty);
+            if (!skip) ay[r - D.Ny] = m;
+        }
         return;
     }
     if (compute_dt && blockIdx.x == 0) {
@@ -848,7 +885,7 @@ __global__ void __launch_bounds__(256) k2_bc(Dims2 D, double* Q, int kind, doubl
         for (int k = threadIdx.x; k < P.ntx * P.nty; k += blockDim.x) m = bits_max(m, P.pv[k]);
         red[threadIdx.x] = m;
         __syncthreads();
-        if (threadIdx.x == 0) {
+        if (threadIdx.x == 0 && !skip) {
             double vmax = 0.0;
             for (int k = 0; k < (int)blockDim.x; k++) vmax = bits_max(vmax, red[k]);
             sc->vmax = (unsigned long long)__double_as_longlong(vmax);
@@ -875,7 +912,7 @@ __global__ void __launch_bounds__(256) k2_bc(Dims2 D, double* Q, int kind, doubl
         const int si = side == 0 ? 0 : side == 1 ? D.Nx - 1 : i;
         const int sj = side == 2 ? 0 : side == 3 ? D.Ny - 1 : j;
         get4(Q, D.plane, gidx(D, si, sj), v);
-        put4(Q, D.plane, p, v);
+        if (!skip) put4(Q, D.plane, p, v);
         return;
     }
     // double Mach reflection (EOS at gamma = 1.4, as BOUNDARY_MANAGER_DMACH)
@@ -903,7 +940,7 @@ __global__ void __launch_bounds__(256) k2_bc(Dims2 D, double* Q, int kind, doubl
             v[0] = 1.4; v[1] = 1.4 * 0; v[2] = 1.4 * 0; v[3] = eos_E2(1.4, 1.4, 0, 0, 1);
         }
     }
-    put4(Q, D.plane, p, v);
+    if (!skip) put4(Q, D.plane, p, v);
 }
 
 // ------------------------------------------------------------ RK stages ----
@@ -1050,8 +1087,11 @@ __global__ void __launch_bounds__(256) k2_stage_update(Dims2 D, const uint32_t* 
 __global__ void k2_combine(Dims2 D, const uint8_t* __restrict__ m0, const uint8_t* __restrict__ m3,
                            const uint8_t* __restrict__ mtmp, double* __restrict__ Q, const double* __restrict__ Q0,
                            int do_combine, Scal2* sc) {
-    if (sc->skip) return;
+    const int skip = sc->skip;  // tested after the mask load
     const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    const long long p = gidx(D, i < D.Nx ? i : D.Nx - 1, j);
+    const uint8_t a0 = m0[p];
+    if (skip) return;
     if (do_combine && i == 0 && j == 0) {
         sc->t = sc->t + sc->dt;
         if (sc->do_print) sc->cnt = sc->cnt + 1;
@@ -1062,8 +1102,7 @@ __global__ void k2_combine(Dims2 D, const uint8_t* __restrict__ m0, const uint8_
         for (int k = 0; k < 3; k++) sc->sumF += 2LL * sc->cntW[k] + sc->cntXS[k] + sc->cntYS[k];
     }
     if (i >= D.Nx) return;
-    const long long p = gidx(D, i, j);
-    if (m0[p]) {
+    if (a0) {
         if (!do_combine) return;
         for (int q = 0; q < 4; q++) {
             const long long o = p + q * D.plane;
