@@ -1382,6 +1382,19 @@ void Solver2D::prof_clear() {
 constexpr int kLatK = 1;   // lattice points per thread
 constexpr int kCheckK = 1; // check cells per thread
 
+// Threads per workgroup of a lattice or check pass over n points.  Every lane
+// of these passes loads from its own cache lines (stencil points are m apart
+// in x and m rows apart in y), so a CU spends a cycle per lane and load on
+// address processing; a coarse lattice in few full workgroups serialises on
+// a few CUs.  Smaller workgroups spread it over the whole chip.
+constexpr int kLatMinBlock = 16;    // smallest workgroup (one wave, 16 lanes live)
+constexpr int kLatMinBlocks = 512;  // workgroup count to reach before shrinking stops
+static int lat_block(long long n) {
+    int b = 256;
+    while (b > kLatMinBlock && nblk(n, b) < kLatMinBlocks) b >>= 1;
+    return b;
+}
+
 static long long lat_threads(const Dims2& D, int lev, int kind, bool two) {
     const Lat g = lat_h(D, lev, kind);
     long long n = (long long)((g.ni + kLatK - 1) / kLatK) * g.nj;
@@ -1419,16 +1432,21 @@ static int check_recon(Solver2D& s, uint8_t* mask) {
     for (int lev = s.D.maxlev - 1; lev > s.tail; lev--) {
         const int m = 1 << (s.D.L - 1 - lev);
         const long long nI = (s.D.Nx - 1) / m, nJ = (s.D.Ny - 1) / m;
-        AWCM_LAUNCH(k2_check<kCheckK>, dim3(nblk(((nI + kCheckK - 1) / kCheckK) * nJ, 256)), dim3(256), 0, s.stream, s.D,
-                    lev, mask, s.sc);
+        const long long n = ((nI + kCheckK - 1) / kCheckK) * nJ;
+        const int b = lat_block(n);
+        AWCM_LAUNCH(k2_check<kCheckK>, dim3(nblk(n, b)), dim3(b), 0, s.stream, s.D, lev, mask, s.sc);
     }
     if (s.tail >= 0) AWCM_LAUNCH(k2_check_tail, dim3(1), dim3(1024), 0, s.stream, s.D, s.tail, mask, s.sc);
     return AWCM_OK;
 }
 
-#define AWCM_LAT_ON(strm, OP, lev, kind, two, Qp, mk, aux)                                                       \
-    AWCM_LAUNCH((k2_lat<OP, kLatK>), dim3(nblk(lat_threads(D, lev, kind, two), 256)), dim3(256), 0, strm, D, lev, \
-                W, tol_host, Qp, mk, aux, sc)
+#define AWCM_LAT_ON(strm, OP, lev, kind, two, Qp, mk, aux)                                                  \
+    do {                                                                                                    \
+        const long long n_ = lat_threads(D, lev, kind, two);                                                \
+        const int b_ = lat_block(n_);                                                                       \
+        AWCM_LAUNCH((k2_lat<OP, kLatK>), dim3(nblk(n_, b_)), dim3(b_), 0, strm, D, lev, W, tol_host, Qp, mk, \
+                    aux, sc);                                                                               \
+    } while (0)
 #define AWCM_LAT(OP, lev, kind, two, Qp, mk, aux) AWCM_LAT_ON(stream, OP, lev, kind, two, Qp, mk, aux)
 
 int Solver2D::enqueue_prepare(bool force) {
