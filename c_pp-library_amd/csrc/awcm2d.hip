@@ -584,27 +584,81 @@ __device__ __forceinline__ int classify(int i, int j, const Dims2& D, int& mu) {
     return lev;
 }
 
+// Row segments of B bytes (B = 2, 4, 8, 16) of the ghosted layout; P is a
+// multiple of 16, so segments never straddle rows.
+template <int B> struct VecB;
+template <> struct VecB<2> { using T = uint16_t; };
+template <> struct VecB<4> { using T = uint32_t; };
+template <> struct VecB<8> { using T = uint2; };
+template <> struct VecB<16> { using T = uint4; };
+template <int B>
+union UB {
+    typename VecB<B>::T v;
+    uint8_t b[B];
+};
+template <int B>
+__device__ __forceinline__ UB<B> ldb(const uint8_t* p) {
+    UB<B> r;
+    r.v = *reinterpret_cast<const typename VecB<B>::T*>(p);
+    return r;
+}
+template <int B>
+__device__ __forceinline__ void stb(uint8_t* p, const UB<B>& x) {
+    *reinterpret_cast<typename VecB<B>::T*>(p) = x.v;
+}
+template <int B>
+__device__ __forceinline__ bool segb_of(const Dims2& D, long long t, Seg& s) {
+    const int spr = D.P / B;
+    const long long nseg = (long long)spr * (D.Ny + 2 * G);
+    if (t >= nseg) return false;
+    const int jr = int(t / spr), c = int(t % spr);
+    s.p0 = (long long)jr * D.P + B * c;
+    s.i0 = B * c - G;
+    s.j = jr - G;
+    return true;
+}
+// bit k set where byte k of the segment is nonzero
+template <int B>
+__device__ __forceinline__ uint32_t nonzero_bits(const UB<B>& v) {
+    uint32_t m = 0;
+#pragma unroll
+    for (int k = 0; k < B; k++) m |= v.b[k] ? 1u << k : 0u;
+    return m;
+}
+
+// Bytes per thread of the adjacent-zone and level/add-point passes: a thread
+// works through its segment's active points one after another, so wide
+// segments serialise dense regions, narrow ones pay per-thread overhead on
+// the (mostly empty) rest of the grid.
+constexpr int kMaskB = 4;
+
 // AWCM_MANAGER_2D::AddAdjacentZone (AWCM_MANAGER_2D.h:14-61), every level at
-// once from the snapshot; one point per thread.
+// once from the snapshot, a B-point row segment per thread.
+template <int B>
 __global__ void k2_adjacent(Dims2 D, const uint8_t* __restrict__ snap, uint8_t* __restrict__ mask, Scal2* sc) {
     const int skip = sc->skip;  // tested after the snapshot load
-    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
-    if (i >= D.Nx) return;
-    int mu;
-    const int lev = classify(i, j, D, mu);
-    if (lev < 0) return;
-    const uint8_t sv = snap[gidx(D, i, j)];
-    if (skip || !sv) return;
-    if (lev == D.maxlev - 1) {
-        mset(D, mask, i - 1, j, sc); mset(D, mask, i + 1, j, sc);
-        mset(D, mask, i, j - 1, sc); mset(D, mask, i, j + 1, sc);
-    } else {
-        const int m = 1 << (D.L - 1 - lev);
-        const int dq = m / 4, dh = m / 2;
-        mset(D, mask, i - dq, j, sc); mset(D, mask, i + dq, j, sc);
-        mset(D, mask, i, j - dq, sc); mset(D, mask, i, j + dq, sc);
-        mset(D, mask, i - dh, j, sc); mset(D, mask, i + dh, j, sc);
-        mset(D, mask, i, j - dh, sc); mset(D, mask, i, j + dh, sc);
+    Seg s;
+    if (!segb_of<B>(D, (long long)blockIdx.x * blockDim.x + threadIdx.x, s)) return;
+    if (s.j < 1 || s.j > D.Ny - 2) return;  // no interior point (classify is -1)
+    uint32_t vm = nonzero_bits<B>(ldb<B>(snap + s.p0));
+    if (skip) return;
+    const int j = s.j;
+    for (; vm; vm &= vm - 1) {
+        const int i = s.i0 + __builtin_ctz(vm);
+        int mu;
+        const int lev = classify(i, j, D, mu);  // -1 for ghost, padding and boundary bytes
+        if (lev < 0) continue;
+        if (lev == D.maxlev - 1) {
+            mset(D, mask, i - 1, j, sc); mset(D, mask, i + 1, j, sc);
+            mset(D, mask, i, j - 1, sc); mset(D, mask, i, j + 1, sc);
+        } else {
+            const int m = 1 << (D.L - 1 - lev);
+            const int dq = m / 4, dh = m / 2;
+            mset(D, mask, i - dq, j, sc); mset(D, mask, i + dq, j, sc);
+            mset(D, mask, i, j - dq, sc); mset(D, mask, i, j + dq, sc);
+            mset(D, mask, i - dh, j, sc); mset(D, mask, i + dh, j, sc);
+            mset(D, mask, i, j - dh, sc); mset(D, mask, i, j + dh, sc);
+        }
     }
 }
 
@@ -615,42 +669,93 @@ __global__ void k2_adjacent(Dims2 D, const uint8_t* __restrict__ snap, uint8_t* 
 // each other.  The reference snapshots mask_{k+1} right after copying it from
 // mask_k, so mask_k is the snapshot; a point's own level is all the scatter
 // needs.  A point with no active neighbour keeps its previous level, as in the
-// reference.  Grid: x over the ghosted row (Nx+6 points), y over ghosted rows.
+// reference.
+// A thread takes a B-point row segment of the ghosted rows and is the only
+// writer of its segment of level_k (mask_{k+1} is written with single-byte
+// stores of ones, which other segments' scatters may also write).  The
+// nearest-neighbour test of the segment's points comes from the segment, the
+// rows above and below and the two edge bytes; only isolated points search
+// wider.
+template <int B>
 __global__ void k2_level_add(Dims2 D, const uint8_t* __restrict__ src, uint8_t* __restrict__ dst,
                              uint8_t* __restrict__ lvl, Scal2* sc) {
+    constexpr uint32_t kAll = (1u << B) - 1u;
     const int skip = sc->skip;  // tested after the mask load
-    const int i = int(blockIdx.x * blockDim.x + threadIdx.x) - G, j = int(blockIdx.y) - G;
-    if (i > D.Nx - 1 + G) return;
-    const long long t = gidx(D, i, j);
-    const uint8_t v = src[t];
+    Seg s;
+    if (!segb_of<B>(D, (long long)blockIdx.x * blockDim.x + threadIdx.x, s)) return;
+    const UB<B> v = ldb<B>(src + s.p0);
     if (skip) return;
-    if (v) dst[t] = 1;
-    if (i < 0 || i > D.Nx - 1 || j < 0 || j > D.Ny - 1) return;
-    if (!v) { lvl[t] = 0; return; }
-    int lev = lvl[t];  // kept when no neighbour is active
-    int d = 1;
-    for (int l = D.maxlev; l >= 0; l--) {
-        if (i - d < -G || i + d > D.Nx - 1 + G || j - d < -G || j + d > D.Ny - 1 + G) { sc->flag |= 1; break; }
-        if (src[t - d] || src[t + d] || src[t - (long long)d * D.P] || src[t + (long long)d * D.P]) { lev = l; break; }
-        d *= 2;
-    }
-    lvl[t] = (uint8_t)lev;
-    if (i < 1 || i > D.Nx - 2 || j < 1 || j > D.Ny - 2) return;
-    const int dd = 1 << (D.L - 1 - lev);  // grid[lev].dx
-    const int NX = D.Nx - 1, NY = D.Ny - 1;
-    if (lev == D.maxlev) {
-        for (int c = -3; c <= 3; c++) { mset(D, dst, i + c, j, sc); mset(D, dst, i, j + c, sc); }
+    // bytes past the last ghost column are row padding, not mask points
+    const int last = D.Nx - 1 + G - s.i0;
+    const uint32_t vm = nonzero_bits<B>(v) & (last < 0 ? 0u : last >= B - 1 ? kAll : (2u << last) - 1u);
+    const bool row_in = s.j >= 0 && s.j <= D.Ny - 1;
+    // interior columns of the segment: k in [kb, ke]
+    const int kb = s.i0 < 0 ? -s.i0 : 0;
+    const int ke = D.Nx - 1 - s.i0 < B - 1 ? D.Nx - 1 - s.i0 : B - 1;
+    const uint32_t im = (row_in && ke >= kb) ? (((2u << ke) - 1u) & ~((1u << kb) - 1u)) : 0u;
+    for (uint32_t w = vm & ~im; w; w &= w - 1) dst[s.p0 + __builtin_ctz(w)] = 1;  // ghost points: copy only
+    if (!im) return;
+    const uint32_t act = vm & im;
+    if (!act) {
+        if (im == kAll) {
+            UB<B> z;
+#pragma unroll
+            for (int k = 0; k < B; k++) z.b[k] = 0;
+            stb<B>(lvl + s.p0, z);
+        } else {
+            for (uint32_t w = im; w; w &= w - 1) lvl[s.p0 + __builtin_ctz(w)] = 0;
+        }
         return;
     }
-    int lo, hi;
-    if (i - 2 * dd < 0) lo = -1, hi = 2;
-    else if (i + 2 * dd > NX) lo = -2, hi = 1;
-    else lo = -2, hi = 2;
-    for (int c = lo; c <= hi; c++) mset(D, dst, i + dd * c, j, sc);
-    if (j - 2 * dd < 0) lo = -1, hi = 2;
-    else if (j + 2 * dd > NY) lo = -2, hi = 1;
-    else lo = -2, hi = 2;
-    for (int c = lo; c <= hi; c++) mset(D, dst, i, j + dd * c, sc);
+    UB<B> L = ldb<B>(lvl + s.p0);
+    const UB<B> up = ldb<B>(src + s.p0 - D.P), dn = ldb<B>(src + s.p0 + D.P);
+    // edge bytes: live whenever the segment's first or last point is interior
+    const uint8_t lft = (act & 1u) ? src[s.p0 - 1] : 0;
+    const uint8_t rgt = (act >> (B - 1)) & 1u ? src[s.p0 + B] : 0;
+    uint32_t nbm = 0;
+#pragma unroll
+    for (int k = 0; k < B; k++) {
+        const uint8_t l = k > 0 ? v.b[k - 1] : lft, r = k < B - 1 ? v.b[k + 1] : rgt;
+        nbm |= (l | r | up.b[k] | dn.b[k]) ? 1u << k : 0u;
+        if (((im & ~vm) >> k) & 1u) L.b[k] = 0;  // inactive interior point
+    }
+    stb<B>(lvl + s.p0, L);  // active points are rewritten below, in program order
+    const int j = s.j;
+    const int NX = D.Nx - 1, NY = D.Ny - 1;
+    for (uint32_t w = act; w; w &= w - 1) {
+        const int k = __builtin_ctz(w);
+        const long long t = s.p0 + k;
+        const int i = s.i0 + k;
+        dst[t] = 1;
+        int lev;
+        if ((nbm >> k) & 1u) {
+            lev = D.maxlev;  // an active neighbour at distance 1 (always inside the ghosted grid)
+        } else {
+            lev = lvl[t];  // kept when no neighbour is active
+            int d = 1;
+            for (int l = D.maxlev; l >= 0; l--) {
+                if (i - d < -G || i + d > D.Nx - 1 + G || j - d < -G || j + d > D.Ny - 1 + G) { sc->flag |= 1; break; }
+                if (src[t - d] || src[t + d] || src[t - (long long)d * D.P] || src[t + (long long)d * D.P]) { lev = l; break; }
+                d *= 2;
+            }
+        }
+        lvl[t] = (uint8_t)lev;
+        if (i < 1 || i > D.Nx - 2 || j < 1 || j > D.Ny - 2) continue;
+        const int dd = 1 << (D.L - 1 - lev);  // grid[lev].dx
+        if (lev == D.maxlev) {
+            for (int c = -3; c <= 3; c++) { mset(D, dst, i + c, j, sc); mset(D, dst, i, j + c, sc); }
+            continue;
+        }
+        int lo, hi;
+        if (i - 2 * dd < 0) lo = -1, hi = 2;
+        else if (i + 2 * dd > NX) lo = -2, hi = 1;
+        else lo = -2, hi = 2;
+        for (int c = lo; c <= hi; c++) mset(D, dst, i + dd * c, j, sc);
+        if (j - 2 * dd < 0) lo = -1, hi = 2;
+        else if (j + 2 * dd > NY) lo = -2, hi = 1;
+        else lo = -2, hi = 2;
+        for (int c = lo; c <= hi; c++) mset(D, dst, i, j + dd * c, sc);
+    }
 }
 
 // --------------------------------------------------------- compaction ------
@@ -1463,10 +1568,11 @@ int Solver2D::enqueue_prepare(bool force) {
     if (tail >= 0)
         AWCM_LAUNCH(k2_tail<TAIL_FWT>, dim3(1), dim3(1024), 0, stream, D, tail, W, tol_host, Q, m[0], nullptr, sc);
     AWCM_LAUNCH(k2_band_snap, dim3(bseg), dim3(256), 0, stream, D, m[0], snap, sc);
-    AWCM_LAUNCH(k2_adjacent, dim3(nblk(D.Nx, 256), D.Ny), dim3(256), 0, stream, D, snap, m[0], sc);
+    const unsigned bmask = nblk((long long)(D.P / kMaskB) * (D.Ny + 2 * G), 256);
+    AWCM_LAUNCH(k2_adjacent<kMaskB>, dim3(bmask), dim3(256), 0, stream, D, snap, m[0], sc);
     if (int st = check_recon(*this, m[0])) return st;
     for (int k = 1; k <= 3; k++) {
-        AWCM_LAUNCH(k2_level_add, dim3(nblk(D.Nx + 2 * G, 256), D.Ny + 2 * G), dim3(256), 0, stream, D, m[k - 1], m[k],
+        AWCM_LAUNCH(k2_level_add<kMaskB>, dim3(bmask), dim3(256), 0, stream, D, m[k - 1], m[k],
                     lv[k - 1], sc);
         if (int st = check_recon(*this, m[k])) return st;
     }
