@@ -332,7 +332,7 @@ __device__ void step_T(const Ctx1& c, const Adj1DArgs& A, int n, const double* X
 //         dJ/dQ before the first on return; Q is used as work space.
 // One workgroup; dynamic LDS 11 * nxp bytes, nxp >= Nx + 6.
 __global__ void __launch_bounds__(1024) k1_adjoint(Inst1D* inst, Adj1DArgs A, int mode, int nxp) {
-    extern __shared__ uint8_t sm[];
+    AWCM_DYNAMIC_LDS uint8_t sm[];
     __shared__ double W[4][5];
     __shared__ double red[16];
     __shared__ double sc[4];

@@ -30,7 +30,7 @@ namespace awcm {
 template <bool kHbm>
 __global__ void __launch_bounds__(1024) k1_solve(Inst1D* insts, int n_inst, const int* order, int* qctr, int mode,
                                                  int max_steps, int nxp, uint8_t* hbm) {
-    extern __shared__ uint8_t lds_masks[];
+    AWCM_DYNAMIC_LDS uint8_t lds_masks[];
     uint8_t* const sm = kHbm ? hbm + size_t(blockIdx.x) * 11 * size_t(nxp) : lds_masks;
     __shared__ double W[4][5];
     __shared__ double red[16];

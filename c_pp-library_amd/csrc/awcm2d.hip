@@ -553,24 +553,6 @@ __global__ void k2_band_snap(Dims2 D, uint8_t* __restrict__ m0, uint8_t* __restr
     st16(snap + s.p0, v);
 }
 
-// tmp = mask3 minus mask_temp on the grid (ghosts from mask3)
-__global__ void k2_interp_tmp(Dims2 D, const uint8_t* __restrict__ to, const uint8_t* __restrict__ from,
-                              uint8_t* __restrict__ tmp, const Scal2* sc) {
-    if (sc->skip) return;
-    Seg s;
-    if (!seg_of(D, (long long)blockIdx.x * blockDim.x + threadIdx.x, s)) return;
-    V16 v = ld16(to + s.p0);
-    if (s.j >= 0 && s.j <= D.Ny - 1) {
-        const V16 f = ld16(from + s.p0);
-#pragma unroll
-        for (int k = 0; k < 16; k++) {
-            const int i = s.i0 + k;
-            if (i >= 0 && i <= D.Nx - 1 && f.b[k]) v.b[k] = 0;
-        }
-    }
-    st16(tmp + s.p0, v);
-}
-
 // Sub-band of an interior point: level lev (0..L-2) and mu (1: odd i, 2: odd j,
 // 3: both), or -1 for coarsest-grid and boundary points.  Equivalent to
 // membership of the subgrid(lev, mu) lattices of MRA_GRID_UNIFORM_2D.h:114-130.

@@ -9,6 +9,12 @@
 namespace awcm {
 
 constexpr int kGhost = 3;   // ghost width of every state and mask field
+
+// Declaration keyword of a kernel's dynamic LDS array.  The host emulations
+// of the kernels (tests/native/emu*) define it before including this file.
+#ifndef AWCM_DYNAMIC_LDS
+#define AWCM_DYNAMIC_LDS extern __shared__
+#endif
 constexpr int kMaxLevels = 16;
 
 // Max of a non-negative double over a wavefront, exact (max is order-free).

@@ -52,3 +52,11 @@ inline T atomicOr(T* p, U v) {
 }
 using std::max;
 using std::min;
+
+// Runs f() as one workgroup: here a single thread.
+template <class F>
+void emu_launch(unsigned, F f) {
+    blockDim.x = 1;
+    threadIdx.x = 0;
+    f();
+}

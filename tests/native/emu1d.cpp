@@ -1,9 +1,16 @@
 // CPU emulation of the 1-D HIP kernels (k1_solve with tape recording, and
-// k1_adjoint), built with g++ against tests/native/emu/hip/hip_runtime.h.  The
-// device code is compiled unchanged; only the launch is replaced by a call.
-// Test infrastructure only: it lets the CPU suite check the adjoint's device
-// code against finite differences without a GPU.
+// k1_adjoint), built with g++ against tests/native/emu/hip/hip_runtime.h (one
+// emulated thread per workgroup) or against tests/native/emu_mt/hip/
+// hip_runtime.h with EMU_THREADS threads (a workgroup of real threads and
+// barriers, for ThreadSanitizer).  The device code is compiled unchanged;
+// only the launch is replaced by emu_launch.  Test infrastructure only: it
+// lets the CPU suite check the adjoint's device code against finite
+// differences, and its barrier discipline, without a GPU.
 #include <vector>
+
+#ifndef EMU_THREADS
+#define EMU_THREADS 1
+#endif
 
 #include "awcm1d.hip"
 #include "adjoint1d_dev.h"
@@ -35,9 +42,9 @@ void launch_solve(Emu& e, int mode, int max_steps) {
     int order = 0, qctr = 0;
     if (masks_in_hbm) {
         std::vector<uint8_t> hbm(size_t(11) * e.nxp(), 0xA5);
-        k1_solve<true>(&e.in, 1, &order, &qctr, mode, max_steps, e.nxp(), hbm.data());
+        emu_launch(EMU_THREADS, [&] { k1_solve<true>(&e.in, 1, &order, &qctr, mode, max_steps, e.nxp(), hbm.data()); });
     } else {
-        k1_solve<false>(&e.in, 1, &order, &qctr, mode, max_steps, e.nxp(), nullptr);
+        emu_launch(EMU_THREADS, [&] { k1_solve<false>(&e.in, 1, &order, &qctr, mode, max_steps, e.nxp(), nullptr); });
     }
 }
 
@@ -135,7 +142,8 @@ static Adj1DArgs args(Emu& e, int seg_steps) {
 
 void emu_replay(void* h) {
     auto* e = static_cast<Emu*>(h);
-    k1_adjoint(&e->in, args(*e, e->K), 0, e->nxp());
+    const Adj1DArgs A = args(*e, e->K);
+    emu_launch(EMU_THREADS, [&] { k1_adjoint(&e->in, A, 0, e->nxp()); });
 }
 
 void emu_adjoint(void* h, const double* gf, int seg_steps, double* g0) {
@@ -145,7 +153,7 @@ void emu_adjoint(void* h, const double* gf, int seg_steps, double* g0) {
     Adj1DArgs A = args(*e, seg_steps ? seg_steps : e->K);
     for (int i = 3; i < NP - 3; i++)
         for (int c = 0; c < 3; c++) e->Qb[c * NP + i] = gf[i * 3 + c];
-    k1_adjoint(&e->in, A, 1, e->nxp());
+    emu_launch(EMU_THREADS, [&] { k1_adjoint(&e->in, A, 1, e->nxp()); });
     e->Q = Qsave;
     e->in.Q = e->Q.data();
     for (int i = 0; i < NP; i++)
