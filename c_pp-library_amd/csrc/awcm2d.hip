@@ -506,13 +506,17 @@ union V16 {
 __device__ __forceinline__ V16 ld16(const uint8_t* p) { V16 r; r.v = *reinterpret_cast<const uint4*>(p); return r; }
 __device__ __forceinline__ void st16(uint8_t* p, const V16& x) { *reinterpret_cast<uint4*>(p) = x.v; }
 
-// mask_temp = mask0 (whole ghosted buffer) and the step's scalar resets
+// mask_temp = mask0 (whole ghosted buffer) and the step's scalar resets.
+// The residue pass of a step (or of PostProcessEuler) zeroes only points that
+// the step's masks touch, unless `dirty` says Q may be non-zero elsewhere; a
+// PrepareEuler on its own (prepare_only) runs no residue pass but drops points
+// from mask0, so it leaves `dirty` set for the next pass.
 __global__ void k2_begin_copy(Dims2 D, const uint8_t* __restrict__ src, uint8_t* __restrict__ dst, double* alpha,
-                              uint8_t* z1, uint8_t* z2, uint8_t* z3, Scal2* sc, int force) {
+                              uint8_t* z1, uint8_t* z2, uint8_t* z3, Scal2* sc, int force, int prepare_only) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t == 0) {
         sc->skip = (!force && !(sc->t < sc->t_final)) ? 1 : 0;
-        if (!sc->skip) { sc->residue_full = sc->dirty; sc->dirty = 0; }
+        if (!sc->skip) { sc->residue_full = sc->dirty; sc->dirty = prepare_only ? 1 : 0; }
         sc->vmax = 0ull;
         for (int k = 0; k < 3; k++) { sc->cntW[k] = 0; sc->cntC[k] = 0; sc->cntXS[k] = 0; sc->cntYS[k] = 0; }
     }
@@ -1505,11 +1509,11 @@ int Solver2D::set_tail(int tail_cells) {
     return AWCM_OK;
 }
 
-int Solver2D::enqueue_begin(bool force, bool copy_mt) {
+int Solver2D::enqueue_begin(bool force, bool copy_mt, bool prepare_only) {
     const long long nseg = (long long)(D.P / 16) * (D.Ny + 2 * G);
     const long long n = std::max<long long>(nseg, 3LL * (D.Nx + D.Ny));
     AWCM_LAUNCH(k2_begin_copy, dim3(nblk(n, 256)), dim3(256), 0, stream, D, m[0], copy_mt ? mt : m[0], alpha,
-                copy_mt ? m[1] : nullptr, m[2], m[3], sc, force ? 1 : 0);
+                copy_mt ? m[1] : nullptr, m[2], m[3], sc, force ? 1 : 0, prepare_only ? 1 : 0);
     return AWCM_OK;
 }
 
@@ -1540,7 +1544,8 @@ int Solver2D::enqueue_prepare(bool force) {
     const long long nseg = (long long)(D.P / 16) * (D.Ny + 2 * G);
     const unsigned bseg = nblk(nseg, 256);
     // step scalars + mask_temp = mask0
-    if (int st = enqueue_begin(force, true)) return st;
+    // forced: a PrepareEuler of its own, not the first part of a step
+    if (int st = enqueue_begin(force, true, force)) return st;
     // forward transform, finest detail level first
     for (int lev = D.maxlev - 1; lev > tail; lev--) {
         AWCM_LAT(OP_FWT_D3, lev, LAT_D3, false, Q, m[0], nullptr);
@@ -1655,7 +1660,7 @@ int Solver2D::launch_step() {
 }
 
 int Solver2D::enqueue_post() {
-    if (int st = enqueue_begin(true, false)) return st;
+    if (int st = enqueue_begin(true, false, false)) return st;
     AWCM_LAUNCH(k2_combine, dim3(nblk(D.Nx, 256), D.Ny), dim3(256), 0, stream, D, m[0], m[3], mt, Q, Q0, 0, sc);
     return AWCM_OK;
 }

@@ -101,7 +101,7 @@ struct Solver2D {
     // kernel sequences (enqueued on `stream`, no host synchronisation)
     int tail = -1;            // levels 0..tail of each cascade run in one workgroup
     int set_tail(int tail_cells);
-    int enqueue_begin(bool force, bool copy_mt);
+    int enqueue_begin(bool force, bool copy_mt, bool prepare_only);
     int enqueue_prepare(bool force);
     int enqueue_step();       // begin + prepare + dt + 3 stages + combine
     int enqueue_post();

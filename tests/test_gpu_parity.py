@@ -178,6 +178,81 @@ def test_2d_multistep_run_matches_oracle(tail_cells):
     assert_same(g.download_q(), o.get_q(), mg, mo, lg, lo, "2-D DMR 40 steps")
 
 
+def test_2d_prepare_then_run_matches_oracle():
+    """awcm_prepare_euler on its own, then steps: the standalone PrepareEuler
+    drops points from mask0 without a residue pass, so the next step's
+    residue pass must zero every point outside mask0 (as RemoveResidueEuler
+    always does), not only the points the step's masks touch."""
+    P = _pyawcm()
+    L, nxc, nyc = 5, 33, 9
+    q, dx = dmr_setup(L, nxc, nyc)
+    pts = O.print_times(0.2, 21)
+    g = P.Solver(2, L, nxc, 1.4, dx, ny_coarse=nyc, x0=0.0, bc_kind=1)
+    g.set_schedule(pts)
+    o = O.Solver2D("orc", L, nxc, nyc, 1.4, 1, 0.0, 4.0, 0.0, 1.0)
+    g.upload_q(q)
+    o.set_q(q)
+    for rep in range(2):
+        g.prepare(1e-3)
+        o.prepare(1e-3)
+        steps, _ = g.run(1e-3, 0.4, 0.2, 5)
+        for _ in range(5):
+            o.step(1e-3, 0.4, pts)
+        assert steps == 5 and g.t == o.t[0]
+        mg, lg = g.masks()
+        mo, lo = o.masks()
+        assert_same(g.download_q(), o.get_q(), mg, mo, lg, lo, f"2-D DMR prepare then 5 steps, round {rep}")
+
+
+@pytest.mark.parametrize("case", ["dmr", "quad"])
+def test_2d_full_size_matches_oracle(case):
+    """The 2-D headline configurations at their full sizes: double Mach
+    reflection on 4097 x 1025 (7 levels, 65 x 17 coarse, rho_pre 1.4) and the
+    four-quadrant Riemann problem on 2049 x 2049 (7 levels, 33 x 33 coarse),
+    eps 1e-3, CFL 0.4.  PrepareEuler of the initial condition on its own; then,
+    from the initial condition again as the drivers run it, 30 steps queued 10
+    at a time (graph replays).  Masks, levels, t, the print counter and all four
+    fields are bit-identical to the oracle after the PrepareEuler and after
+    every 10 steps.  (A PrepareEuler on its own followed by steps applies the
+    transform twice before the first stage; on the four-quadrant state the
+    reference scheme breaks down at once that way, and so do oracle and GPU.)"""
+    P = _pyawcm()
+    if case == "dmr":
+        L, nxc, nyc, bc, xl, xu, tf, npr = 7, 65, 17, 1, 0.0, 4.0, 0.2, 21
+        q, dx = dmr_setup(L, nxc, nyc)
+    else:
+        L, nxc, nyc, bc, xl, xu, tf, npr = 7, 33, 33, 0, 0.0, 1.0, 0.25, 9
+        q, dx = quad_setup(L, nxc)
+    pts = O.print_times(tf, npr)
+
+    def pair():
+        g = P.Solver(2, L, nxc, 1.4, dx, ny_coarse=nyc, x0=xl, bc_kind=bc)
+        g.set_schedule(pts)
+        o = O.Solver2D("orc", L, nxc, nyc, 1.4, bc, xl, xu, 0.0, 1.0)
+        g.upload_q(q)
+        o.set_q(q)
+        return g, o
+
+    g, o = pair()
+    g.prepare(1e-3)
+    o.prepare(1e-3)
+    mg, lg = g.masks()
+    mo, lo = o.masks()
+    assert_same(g.download_q(), o.get_q(), mg, mo, lg, lo, f"2-D {case} full size prepare")
+    del g, o
+    g, o = pair()
+    for chunk in range(3):
+        steps, _ = g.run(1e-3, 0.4, tf, 10)
+        for _ in range(10):
+            o.step(1e-3, 0.4, pts)
+        assert steps == 10 and g.t == o.t[0] and g.cnt == o.cnt.value, f"chunk {chunk}: t {g.t} vs {o.t[0]}"
+        mg, lg = g.masks()
+        mo, lo = o.masks()
+        qg = g.download_q()
+        assert np.isfinite(qg).all()
+        assert_same(qg, o.get_q(), mg, mo, lg, lo, f"2-D {case} full size, {10 * (chunk + 1)} steps")
+
+
 def test_batch_matches_oracle():
     P = _pyawcm()
     rng = np.random.default_rng(7)
