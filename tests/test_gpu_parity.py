@@ -253,6 +253,130 @@ def test_2d_full_size_matches_oracle(case):
         assert_same(qg, o.get_q(), mg, mo, lg, lo, f"2-D {case} full size, {10 * (chunk + 1)} steps")
 
 
+@pytest.mark.parametrize("seed", [0, 1, 2, 3, 4, 5])
+def test_2d_api_sequences_match_oracle(seed):
+    """Seeded random sequences of C-ABI calls on one solver, mirrored on the
+    oracle and compared bit for bit after every call: PrepareEuler on its own,
+    queued runs of 1..5 steps, uploads of a perturbed state, mask0 pushed back,
+    and (GPU only, no effect on results) the tail and graph-replay settings.
+    State carried between calls (the residue flag, mask_temp, captured graphs)
+    must never make the result depend on the call sequence.  Where the
+    reference scheme breaks down (repeated PrepareEuler before a step can),
+    the GPU must report it in the same run of steps as the oracle."""
+    P = _pyawcm()
+    rng = np.random.default_rng(seed)
+    L, nxc, nyc, tf = 4, 33, 9, 0.2
+    q, dx = dmr_setup(L, nxc, nyc)
+    pts = O.print_times(tf, 21)
+    g = P.Solver(2, L, nxc, 1.4, dx, ny_coarse=nyc, x0=0.0, bc_kind=1)
+    g.set_schedule(pts)
+    o = O.Solver2D("orc", L, nxc, nyc, 1.4, 1, 0.0, 4.0, 0.0, 1.0)
+    g.upload_q(q)
+    o.set_q(q)
+    ops = []
+    for _ in range(16):
+        k = int(rng.integers(0, 6))
+        if k == 0:
+            ops.append("prepare")
+            g.prepare(1e-3)
+            o.prepare(1e-3)
+        elif k in (1, 2):
+            n = int(rng.integers(1, 6))
+            ops.append(f"run {n}")
+            try:
+                steps, _ = g.run(1e-3, 0.4, tf, n)
+            except P.AwcmError as e:
+                # the reference scheme itself breaks down on some sequences
+                # (repeated PrepareEuler before a step): so must the oracle
+                assert "ENONFINITE" in str(e), ops
+                broke = False
+                for _ in range(n):
+                    dt = o.step(1e-3, 0.4, pts)
+                    if not (np.isfinite(dt) and np.isfinite(o.get_q()).all()):
+                        broke = True
+                        break
+                assert broke, f"seed {seed}: GPU broke down after {ops}, the oracle did not"
+                return
+            for _ in range(steps):
+                o.step(1e-3, 0.4, pts)
+            assert g.t == o.t[0] and g.cnt == o.cnt.value, ops
+        elif k == 3:
+            ops.append("upload perturbed")
+            qn = g.download_q() * (1.0 + 1e-3 * rng.standard_normal(g.npts * g.ncomp))
+            g.upload_q(qn)
+            o.set_q(qn)
+        elif k == 4:
+            ops.append("mask0 pushed back")
+            m0 = o.masks()[0][1].copy()
+            g.set_mask0(m0)
+            o.set_mask0(m0)
+        else:
+            tail, graph = int(rng.choice([0, 256, 4096])), int(rng.integers(0, 2))
+            ops.append(f"tail {tail} graph {graph}")
+            g.set_tuning(P.TUNE_TAIL_CELLS, tail)
+            g.set_tuning(P.TUNE_GRAPH, graph)
+        mg, lg = g.masks()
+        mo, lo = o.masks()
+        assert_same(g.download_q(), o.get_q(), mg, mo, lg, lo, f"seed {seed} after {ops}")
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2, 3])
+def test_1d_api_sequences_match_oracle(seed):
+    """The 1-D solver under seeded random sequences of C-ABI calls, mirrored
+    on the oracle and compared bit for bit after every call: PrepareEuler on
+    its own, queued runs of 1..8 steps, uploads of a perturbed state and mask0
+    pushed back.  A breakdown of the reference scheme must be reported by the
+    GPU in the same run of steps as by the oracle."""
+    P = _pyawcm()
+    rng = np.random.default_rng(100 + seed)
+    L, tf = 7, 0.2
+    q, dx = sod_setup(L)
+    pts = O.print_times(tf, 41)
+    g = P.Solver(1, L, 17, 1.4, dx)
+    g.set_schedule(pts)
+    o = O.Solver1D("orc", L, 17, 1.4, dx)
+    g.upload_q(q)
+    o.set_q(q)
+    ops = []
+    for _ in range(16):
+        k = int(rng.integers(0, 5))
+        if k == 0:
+            ops.append("prepare")
+            g.prepare(1e-4)
+            o.prepare(1e-4)
+        elif k in (1, 2):
+            n = int(rng.integers(1, 9))
+            ops.append(f"run {n}")
+            try:
+                steps, _ = g.run(1e-4, 0.5, tf, n)
+            except P.AwcmError as e:
+                assert "ENONFINITE" in str(e), ops
+                broke = False
+                for _ in range(n):
+                    dt = o.step(1e-4, 0.5, pts)
+                    if not (np.isfinite(dt) and np.isfinite(o.get_q()).all()):
+                        broke = True
+                        break
+                assert broke, f"seed {seed}: GPU broke down after {ops}, the oracle did not"
+                return
+            for _ in range(steps):
+                o.step(1e-4, 0.5, pts)
+            assert g.t == o.t[0] and g.cnt == o.cnt.value, ops
+        elif k == 3:
+            ops.append("upload perturbed")
+            qn = g.download_q() * (1.0 + 1e-3 * rng.standard_normal(g.npts * g.ncomp))
+            g.upload_q(qn)
+            o.set_q(qn)
+        else:
+            ops.append("mask0 pushed back")
+            m0 = o.masks()[0][1].copy()
+            g.set_mask0(m0)
+            o.set_mask0(m0)
+        mg, lg = g.masks()
+        mo, lo = o.masks()
+        assert_same(g.download_q(), o.get_q(), mg, mo, lg, lo, f"seed {seed} after {ops}")
+
+
 def test_batch_matches_oracle():
     P = _pyawcm()
     rng = np.random.default_rng(7)
