@@ -104,7 +104,7 @@ __device__ __forceinline__ Lat lattice(const Dims2& D, int lev, int kind) {
 // ------------------------------------------------ lifting point operations --
 // One point of one lifting pass; `act` is the point's mask byte read by the
 // caller (so a thread can issue the mask loads of all its points first).
-enum { OP_FWT_D3, OP_FWT_D12, OP_FWT_C, OP_IWT_C, OP_IWT_D12, OP_IWT_D3, OP_INT_D12, OP_INT_D3 };
+enum { OP_FWT_D3, OP_FWT_D12, OP_FWT_C, OP_IWT_C, OP_IWT_D12, OP_IWT_D3, OP_INT_D12, OP_INT_D3, OP_IAG_D12, OP_IAG_D3 };
 
 // WAVELET_2D.h:496-523 (d3 predict + threshold)
 __device__ __forceinline__ void fwt_d3_pt(const Dims2& D, int i, int j, int d, const Wts& W, double tol,
@@ -193,54 +193,64 @@ __device__ __forceinline__ void iwt_d3_pt(const Dims2& D, int i, int j, int d, c
             v -= rho[gidx(D, is + c1 * d, js + c2 * d)] * W.w[c1][k1 + 1] * W.w[c2][k2 + 1];
     rho[p] = v;
 }
-// WAVELET_2D.h:808-905 (interpolation of momentum and energy, d1/d2 and d3)
-// All stencil values of the three components are loaded before any store, so
-// the loads of one component do not wait behind the store of another.
+// WAVELET_2D.h:808-905 (interpolation of momentum and energy, d1/d2 and d3:
+// components C0 = 1..3) and WAVELET_2D.h:937-1039 (InterpolateAllGridEuler:
+// all four, C0 = 0).  All stencil values of the components are loaded before
+// any store, so the loads of one component do not wait behind the store of
+// another.
+template <int C0>
 __device__ __forceinline__ void int_d12_pt(const Dims2& D, int kind, int i, int j, int d, const Wts& W, double* Q) {
     const long long p = gidx(D, i, j);
     int s, k;
     if (kind == LAT_D1) pstart(i, d, D.Nx - 1, s, k);
     else pstart(j, d, D.Ny - 1, s, k);
-    double x[3][4];
-    for (int c4 = 1; c4 < 4; c4++) {
+    double x[4 - C0][4];
+    for (int c4 = C0; c4 < 4; c4++) {
         const double* q = Q + c4 * D.plane;
         for (int c = 0; c < 4; c++)
-            x[c4 - 1][c] = (kind == LAT_D1) ? q[gidx(D, s + c * d, j)] : q[gidx(D, i, s + c * d)];
+            x[c4 - C0][c] = (kind == LAT_D1) ? q[gidx(D, s + c * d, j)] : q[gidx(D, i, s + c * d)];
     }
-    for (int c4 = 1; c4 < 4; c4++) {
+    for (int c4 = C0; c4 < 4; c4++) {
         double v = 0.0;
-        for (int c = 0; c < 4; c++) v += x[c4 - 1][c] * W.w[c][k + 1];
+        for (int c = 0; c < 4; c++) v += x[c4 - C0][c] * W.w[c][k + 1];
         Q[p + c4 * D.plane] = v;
     }
 }
+template <int C0>
 __device__ __forceinline__ void int_d3_pt(const Dims2& D, int i, int j, int d, const Wts& W, double* Q) {
     const long long p = gidx(D, i, j);
     int is, js, k1, k2;
     pstart(i, d, D.Nx - 1, is, k1);
     pstart(j, d, D.Ny - 1, js, k2);
-    double row[3][4], col[3][4], blk[3][16];
-    for (int c4 = 1; c4 < 4; c4++) {
+    double row[4 - C0][4], col[4 - C0][4], blk[4 - C0][16];
+    for (int c4 = C0; c4 < 4; c4++) {
         const double* q = Q + c4 * D.plane;
-        for (int c = 0; c < 4; c++) row[c4 - 1][c] = q[gidx(D, is + c * d, j)];
-        for (int c = 0; c < 4; c++) col[c4 - 1][c] = q[gidx(D, i, js + c * d)];
+        for (int c = 0; c < 4; c++) row[c4 - C0][c] = q[gidx(D, is + c * d, j)];
+        for (int c = 0; c < 4; c++) col[c4 - C0][c] = q[gidx(D, i, js + c * d)];
         for (int c2 = 0; c2 < 4; c2++)
-            for (int c1 = 0; c1 < 4; c1++) blk[c4 - 1][4 * c2 + c1] = q[gidx(D, is + c1 * d, js + c2 * d)];
+            for (int c1 = 0; c1 < 4; c1++) blk[c4 - C0][4 * c2 + c1] = q[gidx(D, is + c1 * d, js + c2 * d)];
     }
-    for (int c4 = 1; c4 < 4; c4++) {
+    for (int c4 = C0; c4 < 4; c4++) {
         double v = 0.0;
-        for (int c = 0; c < 4; c++) v += row[c4 - 1][c] * W.w[c][k1 + 1];
-        for (int c = 0; c < 4; c++) v += col[c4 - 1][c] * W.w[c][k2 + 1];
+        for (int c = 0; c < 4; c++) v += row[c4 - C0][c] * W.w[c][k1 + 1];
+        for (int c = 0; c < 4; c++) v += col[c4 - C0][c] * W.w[c][k2 + 1];
         for (int c2 = 0; c2 < 4; c2++)
-            for (int c1 = 0; c1 < 4; c1++) v -= blk[c4 - 1][4 * c2 + c1] * W.w[c1][k1 + 1] * W.w[c2][k2 + 1];
+            for (int c1 = 0; c1 < 4; c1++) v -= blk[c4 - C0][4 * c2 + c1] * W.w[c1][k1 + 1] * W.w[c2][k2 + 1];
         Q[p + c4 * D.plane] = v;
     }
 }
 
-// Dispatch of one lattice point of pass OP.  For the interpolation passes
-// `mask` is the work mask (tmp) and `aux` the old mask (mask_temp).
+// Dispatch of one lattice point of pass OP.  For the interpolation passes of
+// PrepareEuler `mask` is the work mask (mask3) and `aux` the old mask
+// (mask_temp).  For those of InterpolateAllGridEuler `aux` is mask_from: the
+// reference's work mask is CheckReconstruction of (every point but mask_from's
+// finest-grid points), tested as `mask_temp && !mask_from`; the closure only
+// adds points, and the points it can add are all in mask_from, so the test is
+// `!mask_from` alone (WAVELET_2D.h:917-922,943).
 template <int OP>
 __device__ __forceinline__ uint8_t op_active(const uint8_t* mask, const uint8_t* aux, long long p) {
     if (OP == OP_INT_D12 || OP == OP_INT_D3) return (mask[p] != 0 && aux[p] == 0) ? 1 : 0;
+    if (OP == OP_IAG_D12 || OP == OP_IAG_D3) return aux[p] == 0 ? 1 : 0;
     return mask[p];
 }
 template <int OP>
@@ -257,17 +267,21 @@ __device__ __forceinline__ void op_point(const Dims2& D, int kind, int i, int j,
     else if (OP == OP_IWT_C) lift_c_pt<-1>(D, i, j, d, W, Q);
     else if (OP == OP_IWT_D12) iwt_d12_pt(D, kind, i, j, d, W, Q);
     else if (OP == OP_IWT_D3) iwt_d3_pt(D, i, j, d, W, Q);
-    else if (OP == OP_INT_D12) int_d12_pt(D, kind, i, j, d, W, Q);
-    else int_d3_pt(D, i, j, d, W, Q);
+    else if (OP == OP_INT_D12) int_d12_pt<1>(D, kind, i, j, d, W, Q);
+    else if (OP == OP_INT_D3) int_d3_pt<1>(D, i, j, d, W, Q);
+    else if (OP == OP_IAG_D12) int_d12_pt<0>(D, kind, i, j, d, W, Q);
+    else int_d3_pt<0>(D, i, j, d, W, Q);
 }
 template <int OP>
 __device__ __forceinline__ int op_kind1() {
-    return (OP == OP_FWT_D3 || OP == OP_IWT_D3 || OP == OP_INT_D3) ? LAT_D3
+    return (OP == OP_FWT_D3 || OP == OP_IWT_D3 || OP == OP_INT_D3 || OP == OP_IAG_D3) ? LAT_D3
            : (OP == OP_FWT_C || OP == OP_IWT_C)                     ? LAT_C
                                                                      : LAT_D1;
 }
 template <int OP>
-__device__ __forceinline__ bool op_two() { return OP == OP_FWT_D12 || OP == OP_IWT_D12 || OP == OP_INT_D12; }
+__device__ __forceinline__ bool op_two() {
+    return OP == OP_FWT_D12 || OP == OP_IWT_D12 || OP == OP_INT_D12 || OP == OP_IAG_D12;
+}
 
 // One lifting pass of one level: each thread takes K consecutive lattice points
 // of a row, reads their K mask bytes first, then works on the active ones.
@@ -1591,6 +1605,36 @@ int Solver2D::enqueue_prepare(bool force) {
     // join the side stream
     if (hipStreamWaitEvent(stream, ev_int, 0) != hipSuccess) return AWCM_EDEVICE;
     return AWCM_OK;
+}
+
+// WAVELET_2D::InterpolateAllGridEuler (WAVELET_2D.h:912-1044) on the device
+// state: every finest-grid point not in mask_from gets its coarse-to-fine
+// prediction, all four components, level by level (d1 and d2 of a level read
+// only the level's coarse points, so they share a launch; d3 reads them).
+// mask_from is in the reference layout; it is held in `tmp`.  The step
+// scalars are kept, except that Q is now non-zero outside mask0.
+int Solver2D::interpolate_all(const uint8_t* mask_from) {
+    if (hipStreamSynchronize(stream) != hipSuccess) return AWCM_EDEVICE;
+    if (hipMemcpy2D(tmp, D.P, mask_from, D.PR, D.PR, D.Ny + 2 * G, hipMemcpyHostToDevice) != hipSuccess)
+        return AWCM_EDEVICE;
+    Scal2 s0;
+    if (hipMemcpy(&s0, sc, sizeof(s0), hipMemcpyDeviceToHost) != hipSuccess) return AWCM_EDEVICE;
+    Scal2 s1 = s0;
+    s1.skip = 0;  // the passes test `skip`, which a finished run leaves set
+    if (hipMemcpy(sc, &s1, sizeof(s1), hipMemcpyHostToDevice) != hipSuccess) return AWCM_EDEVICE;
+    int st = AWCM_OK;
+    auto passes = [&]() -> int {
+        for (int lev = 0; lev < D.maxlev; lev++) {
+            AWCM_LAT(OP_IAG_D12, lev, LAT_D1, true, Q, tmp, tmp);
+            AWCM_LAT(OP_IAG_D3, lev, LAT_D3, false, Q, tmp, tmp);
+        }
+        return AWCM_OK;
+    };
+    st = passes();
+    if (hipStreamSynchronize(stream) != hipSuccess && !st) st = AWCM_EDEVICE;
+    s0.dirty = 1;
+    if (hipMemcpy(sc, &s0, sizeof(s0), hipMemcpyHostToDevice) != hipSuccess && !st) st = AWCM_EDEVICE;
+    return st;
 }
 
 int Solver2D::enqueue_step() {

@@ -137,6 +137,12 @@ int awcm_prepare_euler(awcm_solver* h, double tol) {
     return st ? st : flag_status(flag);
 }
 
+int awcm_interpolate_all_grid(awcm_solver* h, const uint8_t* mask_from) {
+    if (!h || !mask_from) return AWCM_EINVAL;
+    if (h->dim != 2) return AWCM_ESTATE;
+    return h->s2->interpolate_all(mask_from);
+}
+
 int awcm_post_process_euler(awcm_solver* h) {
     if (!h) return AWCM_EINVAL;
     if (h->dim == 1) {

@@ -55,7 +55,8 @@ struct Solver2D {
     std::vector<void*> allocs;
 
     double *Q = nullptr, *Q0 = nullptr, *Q1 = nullptr, *Q2 = nullptr;  // 4 planes each
-    uint8_t *mt = nullptr, *m[4] = {}, *snap = nullptr, *tmp = nullptr;
+    uint8_t *mt = nullptr, *m[4] = {}, *snap = nullptr;
+    uint8_t* tmp = nullptr;      // mask_from of interpolate_all
     uint8_t* lv[4] = {};
     double* alpha = nullptr;     // 3 sets of (Ny alpha_x, Nx alpha_y)
     double* apart = nullptr;     // alpha partials per tile (row, column and CFL maxima)
@@ -97,6 +98,7 @@ struct Solver2D {
     int set_schedule(const double* p, int n);
     int set_state(double t, int cnt, double t_final, double tol, double cfl);
     int get_state(double* t, int* cnt, int* steps, long long* updates, int* flag);
+    int interpolate_all(const uint8_t* mask_from);  // synchronous
 
     // kernel sequences (enqueued on `stream`, no host synchronisation)
     int tail = -1;            // levels 0..tail of each cascade run in one workgroup

@@ -54,6 +54,7 @@ def lib():
         "awcm_set_mask0": [V, _u8p],
         "awcm_get_masks": [V, _u8p, _i32p],
         "awcm_prepare_euler": [V, D],
+        "awcm_interpolate_all_grid": [V, _u8p],
         "awcm_post_process_euler": [V],
         "awcm_set_schedule": [V, _dp, I],
         "awcm_run": [V, D, D, D, I, _dp, C.POINTER(I), C.POINTER(I), C.POINTER(C.c_longlong)],
@@ -160,6 +161,13 @@ class Solver:
 
     def prepare(self, tol):
         _check(lib().awcm_prepare_euler(self.h, tol), "awcm_prepare_euler")
+
+    def interpolate_all_grid(self, mask_from):
+        """WAVELET_2D::InterpolateAllGridEuler (Wavelet/WAVELET_2D.h:912-1044) on
+        the device field; mask_from is ghosted uint8 in the reference layout."""
+        m = np.ascontiguousarray(mask_from, dtype=np.uint8)
+        assert m.size == self.npts
+        _check(lib().awcm_interpolate_all_grid(self.h, m.ctypes.data_as(_u8p)), "awcm_interpolate_all_grid")
 
     def post_process(self):
         _check(lib().awcm_post_process_euler(self.h), "awcm_post_process_euler")

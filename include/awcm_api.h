@@ -1225,8 +1225,79 @@ public:
     }
 };
 
+namespace awcm_api_detail {
+// The device solver of a 2-D field: validates the field's extent and creates
+// the solver on first use (dx and the x of column 0 come from the field's grid).
+inline awcm_solver* open_2d(Handle& dev, const MRA_GRID_INT_2D& mra_grid,
+                            const FIELD_UNIFORM_2D<VECTOR_4D<double>>& q, const char* who) {
+    const int Nx = mra_grid.Nx, Ny = mra_grid.Ny;
+    if (q.ghost_width != 3 || q.i_start != 0 || q.j_start != 0 || q.i_res != Nx || q.j_res != Ny)
+        throw std::invalid_argument(std::string(who) +
+                                    ": the field must span (0..Nx-1) x (0..Ny-1) with ghost width 3");
+    if (!dev.h)
+        check(awcm_create_2d(mra_grid.num_levels, mra_grid.Nx_coarse, mra_grid.Ny_coarse, 1.4, q.grid.dx,
+                             q.grid.x_min, AWCM_BC_TRANSMISSIVE, &dev.h),
+              "awcm_create_2d");
+    return dev.h;
+}
+}  // namespace awcm_api_detail
+
+// WAVELET_2D (Wavelet/WAVELET_2D.h:10-35): the interpolating wavelet of order
+// 2N.  N and weight(i, j) = l_i(j + 1/2), the Lagrange basis on the nodes
+// 0..2N-1 evaluated at the midpoints, are the reference's public members; the
+// device kernels hold their own copy of the N = 2 table, the only order the
+// AWCM classes use.  Of its transforms, InterpolateAllGridEuler is exposed: the
+// one a driver calls directly (main_2d_euler.cpp:334); the others run inside
+// AWCM_2D::PrepareEuler on the device.  A WAVELET_2D that is AWCM_2D's member
+// shares that object's device solver; a free-standing one has its own.
+class WAVELET_2D {
+public:
+    const int N;
+    ARRAY_2D<double> weight;
+
+    WAVELET_2D(const int& N_ = 2) : N(N_) {
+        if (N != 2) throw std::invalid_argument("WAVELET_2D: only N = 2 (the 4-point wavelet) is supported");
+        weight.Initialize(0, -1, 2 * N, 2 * N + 1);
+        for (int j = -1; j < 2 * N; j++)
+            for (int i = 0; i < 2 * N; i++) {
+                double l = 1.0;
+                for (int k = 0; k < 2 * N; k++)
+                    if (k != i) l *= double(j + 0.5 - k) / double(i - k);
+                weight(i, j) = l;
+            }
+    }
+    WAVELET_2D(const WAVELET_2D&) = delete;
+    WAVELET_2D& operator=(const WAVELET_2D&) = delete;
+
+    // WAVELET_2D::InterpolateAllGridEuler (WAVELET_2D.h:912-1044) on the GPU:
+    // every finest-grid point not in mask_from gets its prediction from the
+    // coarser levels, all four components.  The field goes down and comes back.
+    void InterpolateAllGridEuler(FIELD_UNIFORM_2D<VECTOR_4D<double>>& q, const MRA_GRID_INT_2D& mra_grid,
+                                 const FIELD_UNIFORM_2D<bool>& mask_from) const {
+        using awcm_api_detail::check;
+        const int Nx = mra_grid.Nx, Ny = mra_grid.Ny;
+        if (mask_from.ghost_width != 3 || mask_from.i_start != 0 || mask_from.j_start != 0 ||
+            mask_from.i_res != Nx || mask_from.j_res != Ny)
+            throw std::invalid_argument(
+                "WAVELET_2D::InterpolateAllGridEuler: mask_from must span (0..Nx-1) x (0..Ny-1) with ghost width 3");
+        awcm_solver* h = awcm_api_detail::open_2d(*dev, mra_grid, q, "WAVELET_2D::InterpolateAllGridEuler");
+        const size_t NP = size_t(Nx + 6) * size_t(Ny + 6);
+        std::unique_ptr<uint8_t[]> m(new uint8_t[NP]);
+        for (size_t n = 0; n < NP; n++) m[n] = mask_from.arr.val[n] ? 1 : 0;
+        check(awcm_upload_q(h, q.arr.val[0].values), "awcm_upload_q");
+        check(awcm_interpolate_all_grid(h, m.get()), "awcm_interpolate_all_grid");
+        check(awcm_download_q(h, q.arr.val[0].values), "awcm_download_q");
+    }
+
+private:
+    friend class AWCM_2D;
+    mutable awcm_api_detail::Handle own;
+    awcm_api_detail::Handle* dev = &own;
+};
+
 class AWCM_2D {
 public:
+    WAVELET_2D wavelet;
     MRA_GRID_INT_2D& mra_grid;
     FIELD_UNIFORM_2D<bool> mask_temp, mask0, mask1, mask2, mask3;
     FIELD_UNIFORM_2D<int> level0, level1, level2, level3;
@@ -1241,6 +1312,7 @@ public:
     AWCM_2D(MRA_GRID_INT_2D& mra_grid_, const double& gamma = 1.4)
         : mra_grid(mra_grid_), d1(2.0 / 3.0), d2(-1.0 / 12.0) {
         (void)gamma;
+        wavelet.dev = &dev;
         const int Nx = mra_grid.Nx, Ny = mra_grid.Ny;
         FIELD_UNIFORM_2D<bool>* ms[5] = {&mask_temp, &mask0, &mask1, &mask2, &mask3};
         FIELD_UNIFORM_2D<int>* ls[4] = {&level0, &level1, &level2, &level3};
@@ -1263,12 +1335,7 @@ public:
     void PrepareEuler(FIELD_UNIFORM_2D<VECTOR_4D<double>>& q, const double& tol = 1e-6) {
         using awcm_api_detail::check;
         const int Nx = mra_grid.Nx, Ny = mra_grid.Ny;
-        if (q.ghost_width != 3 || q.i_start != 0 || q.j_start != 0 || q.i_res != Nx || q.j_res != Ny)
-            throw std::invalid_argument("AWCM_2D::PrepareEuler: the field must span (0..Nx-1) x (0..Ny-1) with ghost width 3");
-        if (!dev.h)
-            check(awcm_create_2d(mra_grid.num_levels, mra_grid.Nx_coarse, mra_grid.Ny_coarse, 1.4, q.grid.dx,
-                                 q.grid.x_min, AWCM_BC_TRANSMISSIVE, &dev.h),
-                  "awcm_create_2d");
+        awcm_api_detail::open_2d(dev, mra_grid, q, "AWCM_2D::PrepareEuler");
         const size_t NP = size_t(Nx + 6) * size_t(Ny + 6);
         awcm_api_detail::push_mask0(dev.h, mask0, NP);
         check(awcm_upload_q(dev.h, q.arr.val[0].values), "awcm_upload_q");

@@ -91,6 +91,14 @@ int awcm_get_masks(awcm_solver* h, uint8_t* masks, int32_t* levels);
  * transform and interpolation of the new points, on the device. */
 int awcm_prepare_euler(awcm_solver* h, double tol);
 
+/* WAVELET_2D::InterpolateAllGridEuler(Q, mra_grid, mask_from)
+ * (Wavelet/WAVELET_2D.h:912-1044), called by Wavelet/main_2d_euler.cpp:334
+ * after the time loop: every finest-grid point not in mask_from (ghosted
+ * uint8, reference layout) gets its interpolating-wavelet prediction from the
+ * coarser levels, all four components, on the device state.  2-D only
+ * (AWCM_ESTATE on a 1-D handle).  Synchronous. */
+int awcm_interpolate_all_grid(awcm_solver* h, const uint8_t* mask_from);
+
 /* AWCM_*::PostProcessEuler (AWCM_1D.h:122-125, AWCM_2D.h:149-152). */
 int awcm_post_process_euler(awcm_solver* h);
 

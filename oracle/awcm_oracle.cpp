@@ -968,6 +968,25 @@ struct Solver2D {
             for (int i = 0; i <= ie; i++)
                 if (mk(from, i, j)) mk(tmp, i, j) = 0;
         check_recon(tmp);
+        interp_passes(from, tmp, 1);
+    }
+    // WAVELET_2D::InterpolateAllGridEuler(q, mra, mask_from) (WAVELET_2D.h:912-1044):
+    // mask_temp is every point (ghosts included) except the finest grid's points
+    // of mask_from, closed by CheckReconstruction; then the passes of
+    // InterpolateEuler over all four components.
+    void interpolate_all(std::vector<uint8_t>& from) {
+        const int ie = Nx - 1, je = Ny - 1;
+        std::vector<uint8_t> tmp(from.size(), 1);
+        for (int j = 0; j <= je; j++)
+            for (int i = 0; i <= ie; i++)
+                if (mk(from, i, j)) mk(tmp, i, j) = 0;
+        check_recon(tmp);
+        interp_passes(from, tmp, 0);
+    }
+    // coarse-to-fine d1, d2, d3 predictions of components p0..3 at the points of
+    // `tmp` not in `from` (WAVELET_2D.h:796-905 and :922-1041)
+    void interp_passes(std::vector<uint8_t>& from, std::vector<uint8_t>& tmp, int p0) {
+        const int ie = Nx - 1, je = Ny - 1;
         for (int lev = 0; lev < maxlev; lev++) {
             const int d = mra[lev], dh = d / 2;
             {   const GridI2& g = sg(lev, 1);
@@ -977,7 +996,7 @@ struct Solver2D {
                         if (mk(tmp, i, j) == 0 || mk(from, i, j) == 1) continue;
                         int is = i - d - dh, iend = i + d + dh, k = 1;
                         if (is < 0) is = 0, k = 0; else if (iend > ie) is = ie - 3 * d, k = 2;
-                        for (int p = 1; p < 4; p++) {
+                        for (int p = p0; p < 4; p++) {
                             double& v = q(Q, i, j)[p];
                             v = 0.0;
                             int mm = is;
@@ -992,7 +1011,7 @@ struct Solver2D {
                         if (mk(tmp, i, j) == 0 || mk(from, i, j) == 1) continue;
                         int js = j - d - dh, jend = j + d + dh, k = 1;
                         if (js < 0) js = 0, k = 0; else if (jend > je) js = je - 3 * d, k = 2;
-                        for (int p = 1; p < 4; p++) {
+                        for (int p = p0; p < 4; p++) {
                             double& v = q(Q, i, j)[p];
                             v = 0.0;
                             int n = js;
@@ -1008,7 +1027,7 @@ struct Solver2D {
                         int is = i - d - dh, js = j - d - dh, iend = i + d + dh, jend = j + d + dh, k1 = 1, k2 = 1;
                         if (is < 0) is = 0, k1 = 0; else if (iend > ie) is = ie - 3 * d, k1 = 2;
                         if (js < 0) js = 0, k2 = 0; else if (jend > je) js = je - 3 * d, k2 = 2;
-                        for (int p = 1; p < 4; p++) {
+                        for (int p = p0; p < 4; p++) {
                             double& v = q(Q, i, j)[p];
                             v = 0.0;
                             int mm = is, n = js;
@@ -1373,6 +1392,11 @@ void orc2_get_masks(void* h, uint8_t* masks, int32_t* levels) {
 }
 void orc2_set_mask0(void* h, const uint8_t* m0) { auto* s = static_cast<Solver2D*>(h); std::memcpy(s->m[0].data(), m0, s->m[0].size()); }
 void orc2_prepare(void* h, double tol) { static_cast<Solver2D*>(h)->prepare(tol); }
+void orc2_interpolate_all(void* h, const uint8_t* mask_from) {
+    auto* s = static_cast<Solver2D*>(h);
+    std::vector<uint8_t> from(mask_from, mask_from + s->m[0].size());
+    s->interpolate_all(from);
+}
 double orc2_step(void* h, double tol, double cfl, const double* pt, int npt, int* cnt, double* t) {
     return static_cast<Solver2D*>(h)->step(tol, cfl, pt, npt, cnt, t);
 }

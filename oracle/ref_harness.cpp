@@ -397,6 +397,12 @@ void ref2_set_mask0(void* h, const uint8_t* m0) {
     for (int k = 0; k < s->awcm->mask0.arr.ij_res; k++) s->awcm->mask0.arr.val[k] = m0[k] != 0;
 }
 void ref2_prepare(void* h, double tol) { auto* s = static_cast<Ref2D*>(h); s->awcm->PrepareEuler(s->Q, tol); }
+void ref2_interpolate_all(void* h, const uint8_t* mask_from) {
+    auto* s = static_cast<Ref2D*>(h);
+    FIELD_UNIFORM_2D<bool> from(s->awcm->mask0);
+    for (int k = 0; k < from.arr.ij_res; k++) from.arr.val[k] = mask_from[k] != 0;
+    s->awcm->wavelet.InterpolateAllGridEuler(s->Q, s->mra_grid, from);
+}
 double ref2_step(void* h, double tol, double cfl, const double* pt, int npt, int* cnt, double* t) {
     return static_cast<Ref2D*>(h)->step(tol, cfl, pt, npt, cnt, t);
 }

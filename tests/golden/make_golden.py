@@ -5,7 +5,7 @@ from the reference's own headers).  Every vector below is produced by the
 reference's classes driven by oracle/ref_harness.cpp; the committed .npz files
 hold data only (inputs and expected outputs).
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [file.npz ...]   (default: all files)
 """
 import os
 import sys
@@ -116,16 +116,51 @@ def traj2(kind, L, nxc, nyc, tol, cfl, tf, npr, snaps):
     return out
 
 
+def all_grid2(kind, L, nxc, nyc, steps, seed):
+    """WAVELET_2D::InterpolateAllGridEuler on the state after `steps` steps,
+    from mask0, from mask2, from a random mask and from no points at all (each
+    from that same state), then two more steps after the one from mask0."""
+    tr = traj2(kind, L, nxc, nyc, 1e-3, 0.4, 0.25 if kind == "quad" else 0.2, 9 if kind == "quad" else 21, [steps])
+    bc, xu = (0, 1.0) if kind == "quad" else (1, 4.0)
+    s = O.Solver2D("ref", L, nxc, nyc, 1.4, bc, 0.0, xu, 0.0, 1.0)
+    q_in, m = tr[f"q_{steps}"], tr[f"masks_{steps}"]
+    rng = np.random.default_rng(seed)
+    froms = {"mask0": m[1], "mask2": m[3], "random": (rng.random(m[1].shape) < 0.3).astype(np.uint8),
+             "none": np.zeros_like(m[1])}
+    out = dict(levels=L, nx_coarse=nxc, ny_coarse=nyc, bc=bc, xu=xu, q_in=q_in, print_times=tr["print_times"],
+               q0=tr["q0"], steps=steps, t_in=tr[f"t_{steps}"], cnt_in=tr[f"cnt_{steps}"])
+    for name, f in froms.items():
+        s.set_q(q_in)
+        s.interpolate_all(f)
+        out[f"from_{name}"] = f.astype(np.uint8)
+        out[f"q_{name}"] = s.get_q()
+    # the driver's order: its run, then the whole grid from mask0; here the run
+    # then goes on for two steps from that field and mask0
+    s.set_q(tr["q0"])
+    for _ in range(steps):
+        s.step(1e-3, 0.4, tr["print_times"])
+    s.interpolate_all(s.masks()[0][1].copy())
+    for _ in range(2):
+        s.step(1e-3, 0.4, tr["print_times"])
+    mm, lv = s.masks()
+    out.update(q_after=s.get_q(), masks_after=mm.astype(np.uint8), levels_after=lv.astype(np.int8), t_after=s.t[0])
+    return out
+
+
 def main():
     if not O.have_ref():
         sys.exit("oracle/_ref/libawcm_ref.so is not built (make -C oracle)")
-    rng = np.random.default_rng(20261203)
-    np.savez_compressed(os.path.join(HERE, "kat_functions.npz"), **kats(rng))
-    np.savez_compressed(os.path.join(HERE, "sod_1d_L6.npz"), **traj1("sod", 6, 9, 1e-4, 0.5, 0.2, 41, [1, 5, 40]))
-    np.savez_compressed(os.path.join(HERE, "shu_osher_1d_L6.npz"),
-                        **traj1("shu", 6, 9, 1e-4, 0.5, 1.8, 41, [1, 10, 30]))
-    np.savez_compressed(os.path.join(HERE, "quad_2d_L4.npz"), **traj2("quad", 4, 9, 9, 1e-3, 0.4, 0.25, 9, [1, 6]))
-    np.savez_compressed(os.path.join(HERE, "dmr_2d_L4.npz"), **traj2("dmr", 4, 17, 6, 1e-3, 0.4, 0.2, 21, [1, 6]))
+    files = {
+        "kat_functions.npz": lambda: kats(np.random.default_rng(20261203)),
+        "sod_1d_L6.npz": lambda: traj1("sod", 6, 9, 1e-4, 0.5, 0.2, 41, [1, 5, 40]),
+        "shu_osher_1d_L6.npz": lambda: traj1("shu", 6, 9, 1e-4, 0.5, 1.8, 41, [1, 10, 30]),
+        "quad_2d_L4.npz": lambda: traj2("quad", 4, 9, 9, 1e-3, 0.4, 0.25, 9, [1, 6]),
+        "dmr_2d_L4.npz": lambda: traj2("dmr", 4, 17, 6, 1e-3, 0.4, 0.2, 21, [1, 6]),
+        "all_grid_quad_2d_L4.npz": lambda: all_grid2("quad", 4, 5, 5, 6, 31),
+        "all_grid_dmr_2d_L3.npz": lambda: all_grid2("dmr", 3, 17, 5, 6, 32),
+    }
+    for name in sys.argv[1:] or list(files):
+        np.savez_compressed(os.path.join(HERE, name), **files[name]())
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)), "bytes")

@@ -623,6 +623,14 @@ void run_prepare() {
             std::snprintf(tag, sizeof tag, "prepare.%s.%d", c.name, k);
             hash_awcm2(awcm, Q, tag);
         }
+        // the whole grid from mask1's points, then from mask0's
+        // (WAVELET_2D::InterpolateAllGridEuler, as main_2d_euler.cpp:334)
+        awcm.wavelet.InterpolateAllGridEuler(Q, mra, awcm.mask1);
+        std::snprintf(tag, sizeof tag, "prepare.%s.all_grid1", c.name);
+        hash_awcm2(awcm, Q, tag);
+        awcm.wavelet.InterpolateAllGridEuler(Q, mra, awcm.mask0);
+        std::snprintf(tag, sizeof tag, "prepare.%s.all_grid0", c.name);
+        hash_awcm2(awcm, Q, tag);
         std::fflush(stdout);
     }
 }
@@ -842,6 +850,10 @@ void loop2(const Cfg2& c, int steps) {
     }
     char tag[64];
     std::snprintf(tag, sizeof tag, "loop.%s.final", c.name);
+    hash_awcm2(awcm, Q, tag);
+    // the end of main_2d_euler.cpp (:334): the whole grid from mask0's points
+    awcm.wavelet.InterpolateAllGridEuler(Q, mra, awcm.mask0);
+    std::snprintf(tag, sizeof tag, "loop.%s.all_grid", c.name);
     hash_awcm2(awcm, Q, tag);
     std::fflush(stdout);
 }

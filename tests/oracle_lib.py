@@ -62,6 +62,7 @@ def load(prefix="orc"):
         f"{p}2_get_masks": (None, [V, _u8p, _i32p]),
         f"{p}2_set_mask0": (None, [V, _u8p]),
         f"{p}2_prepare": (None, [V, C.c_double]),
+        f"{p}2_interpolate_all": (None, [V, _u8p]),
         f"{p}2_step": (C.c_double, [V, C.c_double, C.c_double, _dp, C.c_int, _ip, _dp]),
         f"{p}_flux1": (None, [C.c_double, _dp, C.c_double, _dp]),
         f"{p}_flux2": (None, [C.c_double, C.c_int, _dp, C.c_double, _dp]),
@@ -254,6 +255,12 @@ class Solver2D:
 
     def prepare(self, tol):
         getattr(self.lib, f"{self.p}2_prepare")(self.h, tol)
+
+    def interpolate_all(self, mask_from):
+        """WAVELET_2D::InterpolateAllGridEuler with mask_from ((ny+6, nx+6) bytes)."""
+        m = np.ascontiguousarray(mask_from, dtype=np.uint8).reshape(-1)
+        assert m.size == self.npts
+        getattr(self.lib, f"{self.p}2_interpolate_all")(self.h, u8ptr(m))
 
     def step(self, tol, cfl, pts):
         pts = np.ascontiguousarray(pts, dtype=np.float64)

@@ -15,7 +15,7 @@ against this library it must print the same text.  That covers bit-for-bit:
 The GPU half (PrepareEuler and the drivers' time loop) is tests/test_gpu_api.py.
 
 Where the reference tree is present, the probe is also rebuilt from it and run
-live, and the shipped drivers (Wavelet/main_1d_euler.cpp,
+live, and the shipped drivers (Wavelet/main_1d_euler.cpp, main_2d_euler.cpp,
 main_2d_euler(dmach).cpp, main_2d_euler(quad).cpp) are compiled and linked
 against this library with only `void main` -> `int main` and sprintf_s ->
 snprintf changed.
@@ -33,7 +33,7 @@ PROBE = os.path.join(NATIVE, "_build", "awcm_api_probe")
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 REF = "/root/reference"
 REF_PROBE = os.path.join(ROOT, "oracle", "_ref", "awcm_api_probe_ref")
-DRIVERS = ["main_1d_euler", "main_2d_euler(dmach)", "main_2d_euler(quad)"]
+DRIVERS = ["main_1d_euler", "main_2d_euler", "main_2d_euler(dmach)", "main_2d_euler(quad)"]
 
 
 def have_ref():

@@ -86,6 +86,7 @@ def test_invalid_arguments_rejected_before_device_work():
     assert lib.awcm_create_2d(7, 65, 17, 1.4, 0.001, 0.0, 9, C.byref(h)) == -1  # unknown BC kind
     assert lib.awcm_create_2d(14, 65, 17, 1.4, 0.001, 0.0, 1, C.byref(h)) == -1  # grid too large
     assert lib.awcm_run(None, 1e-3, 0.4, 0.2, 1, None, None, None, None) == -1
+    assert lib.awcm_interpolate_all_grid(None, None) == -1
     assert lib.awcm_ic_double_mach_2d(1, 1.4, 1.4, None, None) == -1
 
 

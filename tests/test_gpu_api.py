@@ -10,11 +10,13 @@ bit-identical, and t and dt identical at every step:
            levels, 481 points), on Sod as main_1d_euler.cpp configures it (11
            levels, 15361 points: masks beyond one workgroup's LDS, so the
            device keeps them in HBM) and on the quadrants and double-Mach
-           problems (2-D, 4 levels, 65x65 and 129x33)
+           problems (2-D, 4 levels, 65x65 and 129x33), each 2-D one followed
+           by WAVELET_2D::InterpolateAllGridEuler from mask1, then from mask0
   loop     40 steps of main_1d_euler.cpp's loop (WENO-Z face-flux pair on Sod
            and Shu-Osher; WENO5-JS ComputedF, as the shipped driver, on Sod at
            481 points and in the driver's own 15361-point configuration) and of
-           main_2d_euler(quad|dmach).cpp's loop
+           main_2d_euler(quad|dmach).cpp's loop, each 2-D loop ending with
+           the whole grid from mask0 as main_2d_euler.cpp:334 does
 """
 import os
 import subprocess

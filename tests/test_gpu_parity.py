@@ -253,12 +253,99 @@ def test_2d_full_size_matches_oracle(case):
         assert_same(qg, o.get_q(), mg, mo, lg, lo, f"2-D {case} full size, {10 * (chunk + 1)} steps")
 
 
+@pytest.mark.parametrize("case", ["quad", "dmr", "dmr_full"])
+def test_2d_interpolate_all_grid_matches_oracle(case):
+    """WAVELET_2D::InterpolateAllGridEuler through awcm_interpolate_all_grid.
+    After a run (GPU and oracle in step; the last case is double Mach
+    reflection at 4097 x 1025), both interpolate the same field from mask0
+    (the driver's call, main_2d_euler.cpp:334), mask1, mask3, a sparse and a
+    dense random mask and no points at all: fields bit-identical.  Then, from
+    the driver's call, the run goes on on both (a field that is non-zero
+    outside mask0 entering PrepareEuler): masks, levels, fields, t
+    bit-identical, for as long as the reference scheme keeps a physical
+    state (see below)."""
+    P = _pyawcm()
+    if case == "quad":
+        L, nxc, nyc, bc, xl, xu, tf, npr, n0 = 5, 9, 9, 0, 0.0, 1.0, 0.25, 9, 8
+        q, dx = quad_setup(L, nxc)
+    elif case == "dmr":
+        L, nxc, nyc, bc, xl, xu, tf, npr, n0 = 4, 33, 9, 1, 0.0, 4.0, 0.2, 21, 8
+        q, dx = dmr_setup(L, nxc, nyc)
+    else:
+        L, nxc, nyc, bc, xl, xu, tf, npr, n0 = 7, 65, 17, 1, 0.0, 4.0, 0.2, 21, 10
+        q, dx = dmr_setup(L, nxc, nyc)
+    pts = O.print_times(tf, npr)
+    rng = np.random.default_rng(17)
+    g = P.Solver(2, L, nxc, 1.4, dx, ny_coarse=nyc, x0=xl, bc_kind=bc)
+    g.set_schedule(pts)
+    g.upload_q(q)
+    o = O.Solver2D("orc", L, nxc, nyc, 1.4, bc, xl, xu, 0.0, 1.0)
+    steps, _ = g.run(1e-3, 0.4, tf, n0)
+    assert steps == n0
+    o.set_q(q)
+    for _ in range(n0):
+        o.step(1e-3, 0.4, pts)
+    assert g.t == o.t[0]
+    mg, lg = g.masks()
+    mo, lo = o.masks()
+    qs = g.download_q()
+    assert_same(qs, o.get_q(), mg, mo, lg, lo, f"{case} after {n0} steps")
+    froms = {"mask1": mg[2], "mask3": mg[4], "sparse": (rng.random(mg[1].shape) < 0.1).astype(np.uint8),
+             "dense": (rng.random(mg[1].shape) < 0.7).astype(np.uint8), "none": np.zeros_like(mg[1]),
+             "mask0": mg[1]}
+    for name, f in froms.items():
+        print(f"all grid {case} from {name}", flush=True)
+        g.upload_q(qs)
+        o.set_q(qs)
+        g.interpolate_all_grid(f)
+        o.interpolate_all(f)
+        qg, qo = g.download_q(), o.get_q()
+        assert np.array_equal(qg.view(np.uint64), qo.view(np.uint64)), \
+            f"{case} from {name}: {int((qg != qo).sum())} values differ"
+    # the field of the driver's call (from mask0) goes on.  The interpolated
+    # points feed the next transforms, and on two of the cases the reference
+    # scheme then leaves the physical states: on the four-quadrant state the
+    # second step gives NaN (the reference build too, at the same points as
+    # the oracle), and on the full-size double Mach state the second step's
+    # PrepareEuler activates points of negative pressure, whose NaN sound
+    # speed makes the reference's maximum of the wave speeds (so dt) depend on
+    # the order its loop visits them.  Up to there the fields are bit-identical;
+    # on the four-quadrant case the GPU field must hold NaN on the same step.
+    for n in range(1 if case == "dmr_full" else 3):
+        o.step(1e-3, 0.4, pts)
+        g.run(1e-3, 0.4, tf, 1)
+        what = f"{case} step {n} after the interpolation"
+        assert g.t == o.t[0] and g.cnt == o.cnt.value, what
+        mg2, lg2 = g.masks()
+        mo2, lo2 = o.masks()
+        qg, qo = g.download_q(), o.get_q()
+        nan = np.isnan(qo)
+        if nan.any():
+            assert np.isnan(qg).any(), f"{what}: the oracle broke down, the GPU did not"
+            break
+        assert_same(qg, qo, mg2, mo2, lg2, lo2, what)
+
+
+def test_interpolate_all_grid_rejects_1d_handle():
+    """InterpolateAllGridEuler is a 2-D routine: a 1-D handle is refused
+    (AWCM_ESTATE) and its field is left as it was."""
+    P = _pyawcm()
+    q, dx = sod_setup(5, 17)
+    g = P.Solver(1, 5, 17, 1.4, dx)
+    g.upload_q(q)
+    with pytest.raises(P.AwcmError) as e:
+        g.interpolate_all_grid(np.ones(g.npts, np.uint8))
+    assert e.value.code == -6  # AWCM_ESTATE
+    assert np.array_equal(g.download_q(), q)
+
+
 @pytest.mark.parametrize("seed", [0, 1, 2, 3, 4, 5])
 def test_2d_api_sequences_match_oracle(seed):
     """Seeded random sequences of C-ABI calls on one solver, mirrored on the
     oracle and compared bit for bit after every call: PrepareEuler on its own,
     queued runs of 1..5 steps, uploads of a perturbed state, mask0 pushed back,
-    and (GPU only, no effect on results) the tail and graph-replay settings.
+    InterpolateAllGridEuler from mask0, and (GPU only, no effect on results)
+    the tail and graph-replay settings.
     State carried between calls (the residue flag, mask_temp, captured graphs)
     must never make the result depend on the call sequence.  Where the
     reference scheme breaks down (repeated PrepareEuler before a step can),
@@ -275,11 +362,16 @@ def test_2d_api_sequences_match_oracle(seed):
     o.set_q(q)
     ops = []
     for _ in range(16):
-        k = int(rng.integers(0, 6))
+        k = int(rng.integers(0, 7))
         if k == 0:
             ops.append("prepare")
             g.prepare(1e-3)
             o.prepare(1e-3)
+        elif k == 6:
+            ops.append("all grid from mask0")
+            m0 = o.masks()[0][1].copy()
+            g.interpolate_all_grid(m0)
+            o.interpolate_all(m0)
         elif k in (1, 2):
             n = int(rng.integers(1, 6))
             ops.append(f"run {n}")

@@ -8,8 +8,9 @@
   transposed index would break: dyadic wavelet weights, WENO-Z ideal-weight
   limit and mirror symmetry, flux consistency F(Q,...,Q) = F(Q) (which needs
   R L = I), nested masks, the always-active band, eps = 0 reducing the AWCM
-  step to a uniform-grid WENO-Z SSP-RK3 step, and Sod against the exact
-  Riemann solution.
+  step to a uniform-grid WENO-Z SSP-RK3 step, Sod against the exact
+  Riemann solution, and InterpolateAllGridEuler reproducing bicubic fields
+  and leaving mask_from's points alone.
 """
 import os
 
@@ -89,6 +90,31 @@ def test_oracle_matches_golden_2d(name):
         assert s.t[0] == g[f"t_{target}"]
 
 
+@pytest.mark.parametrize("name", ["all_grid_quad_2d_L4.npz", "all_grid_dmr_2d_L3.npz"])
+def test_oracle_all_grid_matches_golden(name):
+    """WAVELET_2D::InterpolateAllGridEuler from four masks on a stepped state,
+    and two steps after the driver's call (a field that is non-zero outside
+    mask0 going into the next PrepareEuler)."""
+    g = load(name)
+    args = (int(g["levels"]), int(g["nx_coarse"]), int(g["ny_coarse"]), 1.4, int(g["bc"]), 0.0, float(g["xu"]), 0.0, 1.0)
+    s = O.Solver2D("orc", *args)
+    for f in ("mask0", "mask2", "random", "none"):
+        s.set_q(g["q_in"])
+        s.interpolate_all(g[f"from_{f}"])
+        assert np.array_equal(bits(s.get_q()), bits(g[f"q_{f}"])), f"{name} from {f}"
+    s = O.Solver2D("orc", *args)
+    s.set_q(g["q0"])
+    for _ in range(int(g["steps"])):
+        s.step(1e-3, 0.4, g["print_times"])
+    assert np.array_equal(bits(s.get_q()), bits(g["q_in"])) and s.t[0] == g["t_in"]
+    s.interpolate_all(s.masks()[0][1].copy())
+    for _ in range(2):
+        s.step(1e-3, 0.4, g["print_times"])
+    m, lv = s.masks()
+    assert np.array_equal(m, g["masks_after"].reshape(m.shape)) and np.array_equal(lv, g["levels_after"].reshape(lv.shape))
+    assert np.array_equal(bits(s.get_q()), bits(g["q_after"])) and s.t[0] == g["t_after"]
+
+
 # ----------------------------------------------------- reference build ----
 @needs_ref
 def test_oracle_ics_match_reference():
@@ -157,6 +183,47 @@ def test_oracle_2d_matches_reference(case):
     b.set_q(q)
     pts = O.print_times(0.25, 9)
     for _ in range(12):
+        assert a.step(1e-3, 0.4, pts) == b.step(1e-3, 0.4, pts)
+        ma, la = a.masks()
+        mb, lb = b.masks()
+        assert np.array_equal(ma, mb) and np.array_equal(la, lb)
+        assert np.array_equal(bits(a.get_q()), bits(b.get_q()))
+    assert O.load("orc").orc2_oob(a.h) == 0
+
+
+@needs_ref
+@pytest.mark.parametrize("case", ["quad", "dmr"])
+def test_oracle_all_grid_matches_reference(case):
+    if case == "quad":
+        L, nxc, nyc, bc, xu, s = 5, 9, 9, 0, 1.0, 0.5
+    else:
+        L, nxc, nyc, bc, xu, s = 4, 33, 9, 1, 4.0, 1.4
+    nx = (nxc - 1) * 2 ** (L - 1) + 1
+    ny = (nyc - 1) * 2 ** (L - 1) + 1
+    q, dx, _ = O.ic2(bc, nx, ny, 0.0, xu, 0.0, 1.0, s)
+    a = O.Solver2D("orc", L, nxc, nyc, 1.4, bc, 0.0, xu, 0.0, 1.0)
+    b = O.Solver2D("ref", L, nxc, nyc, 1.4, bc, 0.0, xu, 0.0, 1.0)
+    a.set_q(q)
+    b.set_q(q)
+    pts = O.print_times(0.25, 9)
+    rng = np.random.default_rng(7)
+    for _ in range(5):
+        a.step(1e-3, 0.4, pts)
+        b.step(1e-3, 0.4, pts)
+    m = a.masks()[0]
+    q5 = a.get_q()
+    # mask1, mask3 and the random masks hold points whose field the step's
+    # residue removal zeroed: their results are not physical states, but
+    # they are still the same arithmetic
+    for f in (m[2], m[4], (rng.random(m[1].shape) < 0.1).astype(np.uint8),
+              (rng.random(m[1].shape) < 0.7).astype(np.uint8), m[1]):
+        a.set_q(q5)
+        b.set_q(q5)
+        a.interpolate_all(f)
+        b.interpolate_all(f)
+        assert np.array_equal(bits(a.get_q()), bits(b.get_q()))
+    # the driver's call (from mask0), then the run goes on from that field
+    for _ in range(3):
         assert a.step(1e-3, 0.4, pts) == b.step(1e-3, 0.4, pts)
         ma, la = a.masks()
         mb, lb = b.masks()
@@ -252,6 +319,73 @@ def test_mask_invariants_2d():
         nb = act[:-2, 1:-1] | act[2:, 1:-1] | act[1:-1, :-2] | act[1:-1, 2:]
         assert (lv[0][inner[1] & nb] == L - 1).all()
         assert (lv[0][~inner[1]] == 0).all()
+
+
+def _bicubic_fields(nx, ny, rng):
+    """Four fields, each a random polynomial of degree <= 3 in x and in y."""
+    x = np.arange(-3, nx + 3) / (nx - 1)
+    y = np.arange(-3, ny + 3) / (ny - 1)
+    X, Y = np.meshgrid(x, y)
+    out = np.zeros((ny + 6, nx + 6, 4))
+    for p in range(4):
+        c = rng.uniform(-1.0, 1.0, (4, 4))
+        out[:, :, p] = sum(c[a, b] * X ** a * Y ** b for a in range(4) for b in range(4))
+    return out
+
+
+@pytest.mark.parametrize("bc,L,nxc,nyc", [(0, 5, 5, 5), (1, 4, 9, 5)])
+def test_all_grid_reproduces_bicubic_fields(bc, L, nxc, nyc):
+    """The 4-point interpolating wavelet (and its one-sided forms at the
+    walls) is exact for cubics, and its d3 prediction P_x + P_y - P_xy is exact
+    for products of cubics: a bicubic field known only on the coarsest grid
+    and on the outer ring of the finest grid (the boundary layer no sub-band
+    lattice reaches, MRA_GRID_UNIFORM_2D.h:102-124) comes back everywhere to
+    rounding, in all four components.  The other points start from garbage,
+    so a component, sub-band or level that is skipped, a transposed operand
+    or a wrong stencil index fails."""
+    rng = np.random.default_rng(5 + bc)
+    nx, ny = (nxc - 1) * 2 ** (L - 1) + 1, (nyc - 1) * 2 ** (L - 1) + 1
+    s = O.Solver2D("orc", L, nxc, nyc, 1.4, bc, 0.0, 1.0, 0.0, 1.0)
+    exact = _bicubic_fields(nx, ny, rng)
+    m = 2 ** (L - 1)
+    coarse = np.zeros((ny + 6, nx + 6), dtype=np.uint8)
+    coarse[3:-3:m, 3:-3:m] = 1
+    work = np.zeros(coarse.shape, bool)  # points a sub-band lattice reaches
+    work[4:-4, 4:-4] = True
+    for f in (coarse, coarse | (rng.random(coarse.shape) < 0.2)):
+        q = np.where((work & (f == 0))[:, :, None], rng.uniform(-50, 50, exact.shape), exact)
+        s.set_q(q.ravel())
+        s.interpolate_all(f)
+        got = s.get_q().reshape(exact.shape)
+        inner = (slice(3, -3), slice(3, -3))
+        err = np.abs(got[inner] - exact[inner]).max()
+        assert err < 1e-12, f"max error {err}"
+        # the outer ring and the ghost cells are never written
+        assert np.array_equal(bits(got[~work]), bits(q[~work]))
+
+
+def test_all_grid_leaves_mask_from_points():
+    """Points in mask_from, the coarsest grid (never a detail point) and the
+    outer ring of the finest grid keep their bits, every other point gets a
+    new value; mask_from covering everything changes nothing."""
+    L, nxc, nyc = 4, 17, 6
+    nx, ny = (nxc - 1) * 8 + 1, (nyc - 1) * 8 + 1
+    rng = np.random.default_rng(9)
+    s = O.Solver2D("orc", L, nxc, nyc, 1.4, 1, 0.0, 4.0, 0.0, 1.0)
+    q = rng.uniform(0.5, 2.0, (ny + 6) * (nx + 6) * 4)
+    for f in (np.ones((ny + 6, nx + 6), np.uint8), (rng.random((ny + 6, nx + 6)) < 0.5).astype(np.uint8),
+              np.zeros((ny + 6, nx + 6), np.uint8)):
+        s.set_q(q)
+        s.interpolate_all(f)
+        got = s.get_q().reshape(ny + 6, nx + 6, 4)
+        keep = f.astype(bool).copy()
+        keep[3:-3:8, 3:-3:8] = True
+        assert np.array_equal(bits(got[keep]), bits(q.reshape(got.shape)[keep]))
+        changed = (bits(got) != bits(q.reshape(got.shape))).any(axis=2)
+        work = np.zeros_like(keep)  # all but the outer ring of the finest grid
+        work[4:-4, 4:-4] = True
+        assert not changed[~work].any()
+        assert changed[work & ~keep].all()
 
 
 def _uniform_step_1d(q, dx, cfl, g=1.4):
