@@ -104,7 +104,10 @@ __device__ __forceinline__ Lat lattice(const Dims2& D, int lev, int kind) {
 // ------------------------------------------------ lifting point operations --
 // One point of one lifting pass; `act` is the point's mask byte read by the
 // caller (so a thread can issue the mask loads of all its points first).
-enum { OP_FWT_D3, OP_FWT_D12, OP_FWT_C, OP_IWT_C, OP_IWT_D12, OP_IWT_D3, OP_INT_D12, OP_INT_D3, OP_IAG_D12, OP_IAG_D3 };
+enum {
+    OP_FWT_D3, OP_FWT_D12, OP_FWT_C, OP_IWT_C, OP_IWT_D12, OP_IWT_D3,
+    OP_INT_D12, OP_INT_D3, OP_IAG_D12, OP_IAG_D3, OP_SINT_D12, OP_SINT_D3
+};
 
 // WAVELET_2D.h:496-523 (d3 predict + threshold)
 __device__ __forceinline__ void fwt_d3_pt(const Dims2& D, int i, int j, int d, const Wts& W, double tol,
@@ -194,43 +197,43 @@ __device__ __forceinline__ void iwt_d3_pt(const Dims2& D, int i, int j, int d, c
     rho[p] = v;
 }
 // WAVELET_2D.h:808-905 (interpolation of momentum and energy, d1/d2 and d3:
-// components C0 = 1..3) and WAVELET_2D.h:937-1039 (InterpolateAllGridEuler:
-// all four, C0 = 0).  All stencil values of the components are loaded before
-// any store, so the loads of one component do not wait behind the store of
-// another.
-template <int C0>
+// components C0..C1-1 = 1..3), :937-1039 (InterpolateAllGridEuler: all four)
+// and :372-470 (Interpolate of the scalar field: component 0).  All stencil
+// values of the components are loaded before any store, so the loads of one
+// component do not wait behind the store of another.
+template <int C0, int C1>
 __device__ __forceinline__ void int_d12_pt(const Dims2& D, int kind, int i, int j, int d, const Wts& W, double* Q) {
     const long long p = gidx(D, i, j);
     int s, k;
     if (kind == LAT_D1) pstart(i, d, D.Nx - 1, s, k);
     else pstart(j, d, D.Ny - 1, s, k);
-    double x[4 - C0][4];
-    for (int c4 = C0; c4 < 4; c4++) {
+    double x[C1 - C0][4];
+    for (int c4 = C0; c4 < C1; c4++) {
         const double* q = Q + c4 * D.plane;
         for (int c = 0; c < 4; c++)
             x[c4 - C0][c] = (kind == LAT_D1) ? q[gidx(D, s + c * d, j)] : q[gidx(D, i, s + c * d)];
     }
-    for (int c4 = C0; c4 < 4; c4++) {
+    for (int c4 = C0; c4 < C1; c4++) {
         double v = 0.0;
         for (int c = 0; c < 4; c++) v += x[c4 - C0][c] * W.w[c][k + 1];
         Q[p + c4 * D.plane] = v;
     }
 }
-template <int C0>
+template <int C0, int C1>
 __device__ __forceinline__ void int_d3_pt(const Dims2& D, int i, int j, int d, const Wts& W, double* Q) {
     const long long p = gidx(D, i, j);
     int is, js, k1, k2;
     pstart(i, d, D.Nx - 1, is, k1);
     pstart(j, d, D.Ny - 1, js, k2);
-    double row[4 - C0][4], col[4 - C0][4], blk[4 - C0][16];
-    for (int c4 = C0; c4 < 4; c4++) {
+    double row[C1 - C0][4], col[C1 - C0][4], blk[C1 - C0][16];
+    for (int c4 = C0; c4 < C1; c4++) {
         const double* q = Q + c4 * D.plane;
         for (int c = 0; c < 4; c++) row[c4 - C0][c] = q[gidx(D, is + c * d, j)];
         for (int c = 0; c < 4; c++) col[c4 - C0][c] = q[gidx(D, i, js + c * d)];
         for (int c2 = 0; c2 < 4; c2++)
             for (int c1 = 0; c1 < 4; c1++) blk[c4 - C0][4 * c2 + c1] = q[gidx(D, is + c1 * d, js + c2 * d)];
     }
-    for (int c4 = C0; c4 < 4; c4++) {
+    for (int c4 = C0; c4 < C1; c4++) {
         double v = 0.0;
         for (int c = 0; c < 4; c++) v += row[c4 - C0][c] * W.w[c][k1 + 1];
         for (int c = 0; c < 4; c++) v += col[c4 - C0][c] * W.w[c][k2 + 1];
@@ -242,14 +245,16 @@ __device__ __forceinline__ void int_d3_pt(const Dims2& D, int i, int j, int d, c
 
 // Dispatch of one lattice point of pass OP.  For the interpolation passes of
 // PrepareEuler `mask` is the work mask (mask3) and `aux` the old mask
-// (mask_temp).  For those of InterpolateAllGridEuler `aux` is mask_from: the
+// (mask_temp); for those of Interpolate / InterpolateEuler(from, to) `mask`
+// is the work mask (CheckReconstruction of to minus from) and `aux` is from.  For those of InterpolateAllGridEuler `aux` is mask_from: the
 // reference's work mask is CheckReconstruction of (every point but mask_from's
 // finest-grid points), tested as `mask_temp && !mask_from`; the closure only
 // adds points, and the points it can add are all in mask_from, so the test is
 // `!mask_from` alone (WAVELET_2D.h:917-922,943).
 template <int OP>
 __device__ __forceinline__ uint8_t op_active(const uint8_t* mask, const uint8_t* aux, long long p) {
-    if (OP == OP_INT_D12 || OP == OP_INT_D3) return (mask[p] != 0 && aux[p] == 0) ? 1 : 0;
+    if (OP == OP_INT_D12 || OP == OP_INT_D3 || OP == OP_SINT_D12 || OP == OP_SINT_D3)
+        return (mask[p] != 0 && aux[p] == 0) ? 1 : 0;
     if (OP == OP_IAG_D12 || OP == OP_IAG_D3) return aux[p] == 0 ? 1 : 0;
     return mask[p];
 }
@@ -267,20 +272,22 @@ __device__ __forceinline__ void op_point(const Dims2& D, int kind, int i, int j,
     else if (OP == OP_IWT_C) lift_c_pt<-1>(D, i, j, d, W, Q);
     else if (OP == OP_IWT_D12) iwt_d12_pt(D, kind, i, j, d, W, Q);
     else if (OP == OP_IWT_D3) iwt_d3_pt(D, i, j, d, W, Q);
-    else if (OP == OP_INT_D12) int_d12_pt<1>(D, kind, i, j, d, W, Q);
-    else if (OP == OP_INT_D3) int_d3_pt<1>(D, i, j, d, W, Q);
-    else if (OP == OP_IAG_D12) int_d12_pt<0>(D, kind, i, j, d, W, Q);
-    else int_d3_pt<0>(D, i, j, d, W, Q);
+    else if (OP == OP_INT_D12) int_d12_pt<1, 4>(D, kind, i, j, d, W, Q);
+    else if (OP == OP_INT_D3) int_d3_pt<1, 4>(D, i, j, d, W, Q);
+    else if (OP == OP_IAG_D12) int_d12_pt<0, 4>(D, kind, i, j, d, W, Q);
+    else if (OP == OP_IAG_D3) int_d3_pt<0, 4>(D, i, j, d, W, Q);
+    else if (OP == OP_SINT_D12) int_d12_pt<0, 1>(D, kind, i, j, d, W, Q);
+    else int_d3_pt<0, 1>(D, i, j, d, W, Q);
 }
 template <int OP>
 __device__ __forceinline__ int op_kind1() {
-    return (OP == OP_FWT_D3 || OP == OP_IWT_D3 || OP == OP_INT_D3 || OP == OP_IAG_D3) ? LAT_D3
+    return (OP == OP_FWT_D3 || OP == OP_IWT_D3 || OP == OP_INT_D3 || OP == OP_IAG_D3 || OP == OP_SINT_D3) ? LAT_D3
            : (OP == OP_FWT_C || OP == OP_IWT_C)                     ? LAT_C
                                                                      : LAT_D1;
 }
 template <int OP>
 __device__ __forceinline__ bool op_two() {
-    return OP == OP_FWT_D12 || OP == OP_IWT_D12 || OP == OP_INT_D12 || OP == OP_IAG_D12;
+    return OP == OP_FWT_D12 || OP == OP_IWT_D12 || OP == OP_INT_D12 || OP == OP_IAG_D12 || OP == OP_SINT_D12;
 }
 
 // One lifting pass of one level: each thread takes K consecutive lattice points
@@ -520,7 +527,8 @@ union V16 {
 __device__ __forceinline__ V16 ld16(const uint8_t* p) { V16 r; r.v = *reinterpret_cast<const uint4*>(p); return r; }
 __device__ __forceinline__ void st16(uint8_t* p, const V16& x) { *reinterpret_cast<uint4*>(p) = x.v; }
 
-// mask_temp = mask0 (whole ghosted buffer) and the step's scalar resets.
+// mask_temp = mask0 (whole ghosted buffer; dst == src when mask_temp is to be
+// left alone), masks 1..3 cleared (z1 non-null) and the step's scalar resets.
 // The residue pass of a step (or of PostProcessEuler) zeroes only points that
 // the step's masks touch, unless `dirty` says Q may be non-zero elsewhere; a
 // PrepareEuler on its own (prepare_only) runs no residue pass but drops points
@@ -569,6 +577,17 @@ __global__ void k2_band_snap(Dims2 D, uint8_t* __restrict__ m0, uint8_t* __restr
     }
     if (changed) st16(m0 + s.p0, v);
     st16(snap + s.p0, v);
+}
+
+// work = to, cleared at the finest-grid points of from (WAVELET_2D.h:364-368):
+// the start of the work mask of Interpolate / InterpolateEuler(from, to)
+__global__ void k2_to_minus_from(Dims2 D, const uint8_t* __restrict__ from, const uint8_t* __restrict__ to,
+                                 uint8_t* __restrict__ work) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= D.plane) return;
+    const int i = int(t % D.P) - G, j = int(t / D.P) - G;
+    const bool fine = i >= 0 && i <= D.Nx - 1 && j >= 0 && j <= D.Ny - 1;
+    work[t] = (fine && from[t]) ? 0 : to[t];
 }
 
 // Sub-band of an interior point: level lev (0..L-2) and mu (1: odd i, 2: odd j,
@@ -1344,7 +1363,8 @@ int Solver2D::init(int L, int nxc_, int nyc_, double g, double dx_, double x0_, 
     const size_t nl = size_t(Nx) * Ny;
     int st;
     if ((st = dalloc(&Q, pb)) || (st = dalloc(&Q0, pb)) || (st = dalloc(&Q1, pb)) || (st = dalloc(&Q2, pb))) return st;
-    if ((st = dalloc(&mt, mb)) || (st = dalloc(&snap, mb)) || (st = dalloc(&tmp, mb))) return st;
+    if ((st = dalloc(&mt, mb)) || (st = dalloc(&snap, mb)) || (st = dalloc(&tmp, mb)) || (st = dalloc(&work, mb)))
+        return st;
     for (int k = 0; k < 4; k++) if ((st = dalloc(&m[k], mb)) || (st = dalloc(&lv[k], mb))) return st;
     if ((st = dalloc(&alpha, sizeof(double) * 3 * (Nx + Ny)))) return st;
     {
@@ -1357,7 +1377,7 @@ int Solver2D::init(int L, int nxc_, int nyc_, double g, double dx_, double x0_, 
     if ((st = dalloc(&sc, sizeof(Scal2)))) return st;
     hipMemset(Q, 0, pb); hipMemset(Q0, 0, pb); hipMemset(Q1, 0, pb); hipMemset(Q2, 0, pb);
     // AWCM_2D ctor (AWCM_2D.h:26-41): masks false except mask0; levels = max level
-    hipMemset(mt, 0, mb); hipMemset(snap, 0, mb); hipMemset(tmp, 0, mb);
+    hipMemset(mt, 0, mb); hipMemset(snap, 0, mb); hipMemset(tmp, 0, mb); hipMemset(work, 0, mb);
     for (int k = 0; k < 4; k++) { hipMemset(m[k], k == 0 ? 1 : 0, mb); hipMemset(lv[k], D.maxlev, mb); }
     hipMemset(alpha, 0, sizeof(double) * 3 * (Nx + Ny));
     Scal2 s0;
@@ -1523,11 +1543,11 @@ int Solver2D::set_tail(int tail_cells) {
     return AWCM_OK;
 }
 
-int Solver2D::enqueue_begin(bool force, bool copy_mt, bool prepare_only) {
+int Solver2D::enqueue_begin(bool force, bool copy_mt, bool prepare_only, bool clear_masks) {
     const long long nseg = (long long)(D.P / 16) * (D.Ny + 2 * G);
     const long long n = std::max<long long>(nseg, 3LL * (D.Nx + D.Ny));
     AWCM_LAUNCH(k2_begin_copy, dim3(nblk(n, 256)), dim3(256), 0, stream, D, m[0], copy_mt ? mt : m[0], alpha,
-                copy_mt ? m[1] : nullptr, m[2], m[3], sc, force ? 1 : 0, prepare_only ? 1 : 0);
+                clear_masks ? m[1] : nullptr, m[2], m[3], sc, force ? 1 : 0, prepare_only ? 1 : 0);
     return AWCM_OK;
 }
 
@@ -1554,12 +1574,13 @@ static int check_recon(Solver2D& s, uint8_t* mask) {
     } while (0)
 #define AWCM_LAT(OP, lev, kind, two, Qp, mk, aux) AWCM_LAT_ON(stream, OP, lev, kind, two, Qp, mk, aux)
 
-int Solver2D::enqueue_prepare(bool force) {
+int Solver2D::enqueue_prepare(bool force, bool euler) {
     const long long nseg = (long long)(D.P / 16) * (D.Ny + 2 * G);
     const unsigned bseg = nblk(nseg, 256);
-    // step scalars + mask_temp = mask0
+    // step scalars + mask_temp = mask0 (PrepareEuler only: AWCM_2D::Prepare
+    // leaves mask_temp alone, AWCM_2D.h:52-79)
     // forced: a PrepareEuler of its own, not the first part of a step
-    if (int st = enqueue_begin(force, true, force)) return st;
+    if (int st = enqueue_begin(force, euler, force, true)) return st;
     // forward transform, finest detail level first
     for (int lev = D.maxlev - 1; lev > tail; lev--) {
         AWCM_LAT(OP_FWT_D3, lev, LAT_D3, false, Q, m[0], nullptr);
@@ -1576,6 +1597,18 @@ int Solver2D::enqueue_prepare(bool force) {
         AWCM_LAUNCH(k2_level_add<kMaskB>, dim3(bmask), dim3(256), 0, stream, D, m[k - 1], m[k],
                     lv[k - 1], sc);
         if (int st = check_recon(*this, m[k])) return st;
+    }
+    if (!euler) {
+        // AWCM_2D::Prepare: the inverse transform of the scalar field (plane 0)
+        // and no interpolation
+        if (tail >= 0)
+            AWCM_LAUNCH(k2_tail<TAIL_IWT>, dim3(1), dim3(1024), 0, stream, D, tail, W, tol_host, Q, m[3], nullptr, sc);
+        for (int lev = tail + 1; lev < D.maxlev; lev++) {
+            AWCM_LAT(OP_IWT_C, lev, LAT_C, false, Q, m[3], nullptr);
+            AWCM_LAT(OP_IWT_D12, lev, LAT_D1, true, Q, m[3], nullptr);
+            AWCM_LAT(OP_IWT_D3, lev, LAT_D3, false, Q, m[3], nullptr);
+        }
+        return AWCM_OK;
     }
     // mask3 is final.  Interpolation of momentum and energy at new points
     // (planes 1..3) on the side stream, the inverse transform of density
@@ -1635,6 +1668,58 @@ int Solver2D::interpolate_all(const uint8_t* mask_from) {
     s0.dirty = 1;
     if (hipMemcpy(sc, &s0, sizeof(s0), hipMemcpyHostToDevice) != hipSuccess && !st) st = AWCM_EDEVICE;
     return st;
+}
+
+// WAVELET_2D::Interpolate (WAVELET_2D.h:360-473, the scalar field in plane 0)
+// or InterpolateEuler(from, to) (WAVELET_2D.h:784-907, planes 1..3) on the
+// device state.  The work mask is CheckReconstruction of (to minus from's
+// finest-grid points), built in `work`; from is held in `tmp`.  Both masks are
+// in the reference layout.  As interpolate_all, the step scalars are kept and
+// the next residue pass is a full one.
+int Solver2D::interpolate(const uint8_t* mask_from, const uint8_t* mask_to, bool euler) {
+    if (hipStreamSynchronize(stream) != hipSuccess) return AWCM_EDEVICE;
+    if (hipMemcpy2D(tmp, D.P, mask_from, D.PR, D.PR, D.Ny + 2 * G, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy2D(work, D.P, mask_to, D.PR, D.PR, D.Ny + 2 * G, hipMemcpyHostToDevice) != hipSuccess)
+        return AWCM_EDEVICE;
+    Scal2 s0;
+    if (hipMemcpy(&s0, sc, sizeof(s0), hipMemcpyDeviceToHost) != hipSuccess) return AWCM_EDEVICE;
+    Scal2 s1 = s0;
+    s1.skip = 0;  // the check and lattice passes test `skip`
+    if (hipMemcpy(sc, &s1, sizeof(s1), hipMemcpyHostToDevice) != hipSuccess) return AWCM_EDEVICE;
+    auto passes = [&]() -> int {
+        AWCM_LAUNCH(k2_to_minus_from, dim3(nblk(D.plane, 256)), dim3(256), 0, stream, D, tmp, work, work);
+        if (int st = check_recon(*this, work)) return st;
+        for (int lev = 0; lev < D.maxlev; lev++) {
+            if (euler) {
+                AWCM_LAT(OP_INT_D12, lev, LAT_D1, true, Q, work, tmp);
+                AWCM_LAT(OP_INT_D3, lev, LAT_D3, false, Q, work, tmp);
+            } else {
+                AWCM_LAT(OP_SINT_D12, lev, LAT_D1, true, Q, work, tmp);
+                AWCM_LAT(OP_SINT_D3, lev, LAT_D3, false, Q, work, tmp);
+            }
+        }
+        return AWCM_OK;
+    };
+    int st = passes();
+    if (hipStreamSynchronize(stream) != hipSuccess && !st) st = AWCM_EDEVICE;
+    s0.dirty = 1;
+    if (hipMemcpy(sc, &s0, sizeof(s0), hipMemcpyHostToDevice) != hipSuccess && !st) st = AWCM_EDEVICE;
+    return st;
+}
+
+// The scalar field of AWCM_2D::Prepare / WAVELET_2D::Interpolate: plane 0 of Q,
+// (Nx+6) x (Ny+6) doubles in the reference layout on the host.
+int Solver2D::upload_scalar(const double* q) {
+    if (hipStreamSynchronize(stream) != hipSuccess) return AWCM_EDEVICE;
+    if (hipMemcpy2D(Q, sizeof(double) * D.P, q, sizeof(double) * D.PR, sizeof(double) * D.PR, D.Ny + 2 * G,
+                    hipMemcpyHostToDevice) != hipSuccess)
+        return AWCM_EDEVICE;
+    return mark_dirty();
+}
+int Solver2D::download_scalar(double* q) {
+    if (hipStreamSynchronize(stream) != hipSuccess) return AWCM_EDEVICE;
+    return hipMemcpy2D(q, sizeof(double) * D.PR, Q, sizeof(double) * D.P, sizeof(double) * D.PR, D.Ny + 2 * G,
+                       hipMemcpyDeviceToHost) == hipSuccess ? AWCM_OK : AWCM_EDEVICE;
 }
 
 int Solver2D::enqueue_step() {
@@ -1704,7 +1789,7 @@ int Solver2D::launch_step() {
 }
 
 int Solver2D::enqueue_post() {
-    if (int st = enqueue_begin(true, false, false)) return st;
+    if (int st = enqueue_begin(true, false, false, false)) return st;
     AWCM_LAUNCH(k2_combine, dim3(nblk(D.Nx, 256), D.Ny), dim3(256), 0, stream, D, m[0], m[3], mt, Q, Q0, 0, sc);
     return AWCM_OK;
 }

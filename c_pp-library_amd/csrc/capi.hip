@@ -137,6 +137,47 @@ int awcm_prepare_euler(awcm_solver* h, double tol) {
     return st ? st : flag_status(flag);
 }
 
+int awcm_upload_scalar(awcm_solver* h, const double* q) {
+    if (!h || !q) return AWCM_EINVAL;
+    if (h->dim != 2) return AWCM_ESTATE;
+    return h->s2->upload_scalar(q);
+}
+
+int awcm_download_scalar(awcm_solver* h, double* q) {
+    if (!h || !q) return AWCM_EINVAL;
+    if (h->dim != 2) return AWCM_ESTATE;
+    return h->s2->download_scalar(q);
+}
+
+int awcm_prepare(awcm_solver* h, double tol) {
+    if (!h) return AWCM_EINVAL;
+    if (h->dim != 2) return AWCM_ESTATE;
+    Solver2D& S = *h->s2;
+    S.tol_host = tol;
+    int st = S.enqueue_prepare(true, false);
+    int flag = 0;
+    if (!st) st = S.get_state(nullptr, nullptr, nullptr, nullptr, &flag);
+    return st ? st : flag_status(flag);
+}
+
+int awcm_post_process(awcm_solver* h) {
+    if (!h) return AWCM_EINVAL;
+    if (h->dim != 2) return AWCM_ESTATE;
+    return awcm_post_process_euler(h);
+}
+
+int awcm_interpolate(awcm_solver* h, const uint8_t* mask_from, const uint8_t* mask_to) {
+    if (!h || !mask_from || !mask_to) return AWCM_EINVAL;
+    if (h->dim != 2) return AWCM_ESTATE;
+    return h->s2->interpolate(mask_from, mask_to, false);
+}
+
+int awcm_interpolate_euler(awcm_solver* h, const uint8_t* mask_from, const uint8_t* mask_to) {
+    if (!h || !mask_from || !mask_to) return AWCM_EINVAL;
+    if (h->dim != 2) return AWCM_ESTATE;
+    return h->s2->interpolate(mask_from, mask_to, true);
+}
+
 int awcm_interpolate_all_grid(awcm_solver* h, const uint8_t* mask_from) {
     if (!h || !mask_from) return AWCM_EINVAL;
     if (h->dim != 2) return AWCM_ESTATE;

@@ -56,7 +56,8 @@ struct Solver2D {
 
     double *Q = nullptr, *Q0 = nullptr, *Q1 = nullptr, *Q2 = nullptr;  // 4 planes each
     uint8_t *mt = nullptr, *m[4] = {}, *snap = nullptr;
-    uint8_t* tmp = nullptr;      // mask_from of interpolate_all
+    uint8_t* tmp = nullptr;      // mask_from of interpolate_all / interpolate
+    uint8_t* work = nullptr;     // work mask of interpolate
     uint8_t* lv[4] = {};
     double* alpha = nullptr;     // 3 sets of (Ny alpha_x, Nx alpha_y)
     double* apart = nullptr;     // alpha partials per tile (row, column and CFL maxima)
@@ -99,12 +100,15 @@ struct Solver2D {
     int set_state(double t, int cnt, double t_final, double tol, double cfl);
     int get_state(double* t, int* cnt, int* steps, long long* updates, int* flag);
     int interpolate_all(const uint8_t* mask_from);  // synchronous
+    int interpolate(const uint8_t* mask_from, const uint8_t* mask_to, bool euler);  // synchronous
+    int upload_scalar(const double* q);
+    int download_scalar(double* q);
 
     // kernel sequences (enqueued on `stream`, no host synchronisation)
     int tail = -1;            // levels 0..tail of each cascade run in one workgroup
     int set_tail(int tail_cells);
-    int enqueue_begin(bool force, bool copy_mt, bool prepare_only);
-    int enqueue_prepare(bool force);
+    int enqueue_begin(bool force, bool copy_mt, bool prepare_only, bool clear_masks);
+    int enqueue_prepare(bool force, bool euler = true);
     int enqueue_step();       // begin + prepare + dt + 3 stages + combine
     int enqueue_post();
 };

@@ -55,6 +55,12 @@ def lib():
         "awcm_get_masks": [V, _u8p, _i32p],
         "awcm_prepare_euler": [V, D],
         "awcm_interpolate_all_grid": [V, _u8p],
+        "awcm_upload_scalar": [V, _dp],
+        "awcm_download_scalar": [V, _dp],
+        "awcm_prepare": [V, D],
+        "awcm_post_process": [V],
+        "awcm_interpolate": [V, _u8p, _u8p],
+        "awcm_interpolate_euler": [V, _u8p, _u8p],
         "awcm_post_process_euler": [V],
         "awcm_set_schedule": [V, _dp, I],
         "awcm_run": [V, D, D, D, I, _dp, C.POINTER(I), C.POINTER(I), C.POINTER(C.c_longlong)],
@@ -161,6 +167,38 @@ class Solver:
 
     def prepare(self, tol):
         _check(lib().awcm_prepare_euler(self.h, tol), "awcm_prepare_euler")
+
+    # ---- the scalar path (2-D): AWCM_2D::Prepare / PostProcess, WAVELET_2D::Interpolate
+    def upload_scalar(self, q):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        assert q.size == self.npts
+        _check(lib().awcm_upload_scalar(self.h, _d(q)), "awcm_upload_scalar")
+
+    def download_scalar(self):
+        q = np.empty(self.npts)
+        _check(lib().awcm_download_scalar(self.h, _d(q)), "awcm_download_scalar")
+        return q
+
+    def prepare_scalar(self, tol):
+        _check(lib().awcm_prepare(self.h, tol), "awcm_prepare")
+
+    def post_process_scalar(self):
+        _check(lib().awcm_post_process(self.h), "awcm_post_process")
+
+    def interpolate(self, mask_from, mask_to):
+        """WAVELET_2D::Interpolate(q, mra, mask_from, mask_to) on the scalar field."""
+        f = np.ascontiguousarray(mask_from, dtype=np.uint8)
+        t = np.ascontiguousarray(mask_to, dtype=np.uint8)
+        assert f.size == self.npts and t.size == self.npts
+        _check(lib().awcm_interpolate(self.h, f.ctypes.data_as(_u8p), t.ctypes.data_as(_u8p)), "awcm_interpolate")
+
+    def interpolate_euler(self, mask_from, mask_to):
+        """WAVELET_2D::InterpolateEuler(q, mra, mask_from, mask_to)."""
+        f = np.ascontiguousarray(mask_from, dtype=np.uint8)
+        t = np.ascontiguousarray(mask_to, dtype=np.uint8)
+        assert f.size == self.npts and t.size == self.npts
+        _check(lib().awcm_interpolate_euler(self.h, f.ctypes.data_as(_u8p), t.ctypes.data_as(_u8p)),
+               "awcm_interpolate_euler")
 
     def interpolate_all_grid(self, mask_from):
         """WAVELET_2D::InterpolateAllGridEuler (Wavelet/WAVELET_2D.h:912-1044) on

@@ -1228,8 +1228,8 @@ public:
 namespace awcm_api_detail {
 // The device solver of a 2-D field: validates the field's extent and creates
 // the solver on first use (dx and the x of column 0 come from the field's grid).
-inline awcm_solver* open_2d(Handle& dev, const MRA_GRID_INT_2D& mra_grid,
-                            const FIELD_UNIFORM_2D<VECTOR_4D<double>>& q, const char* who) {
+template <class T>
+awcm_solver* open_2d(Handle& dev, const MRA_GRID_INT_2D& mra_grid, const FIELD_UNIFORM_2D<T>& q, const char* who) {
     const int Nx = mra_grid.Nx, Ny = mra_grid.Ny;
     if (q.ghost_width != 3 || q.i_start != 0 || q.j_start != 0 || q.i_res != Nx || q.j_res != Ny)
         throw std::invalid_argument(std::string(who) +
@@ -1240,15 +1240,33 @@ inline awcm_solver* open_2d(Handle& dev, const MRA_GRID_INT_2D& mra_grid,
               "awcm_create_2d");
     return dev.h;
 }
+// A mask argument spans the finest grid; its ghost width may be anything (the
+// drivers pass ghost-3 masks and, main_2d_advect.cpp:217, a FIELD_UNIFORM_2D
+// built on the grid alone, ghost width 0).  Only its finest-grid points are
+// read, as in the reference.
+inline void check_mask_2d(const MRA_GRID_INT_2D& mra_grid, const FIELD_UNIFORM_2D<bool>& m, const char* who) {
+    if (m.i_start != 0 || m.j_start != 0 || m.i_res != mra_grid.Nx || m.j_res != mra_grid.Ny)
+        throw std::invalid_argument(std::string(who) + ": masks must span (0..Nx-1) x (0..Ny-1)");
+}
+// the mask in the C ABI's layout: (Nx+6) x (Ny+6) bytes, ghosts it lacks zero
+inline std::unique_ptr<uint8_t[]> mask_bytes(const FIELD_UNIFORM_2D<bool>& m) {
+    const int Nx = m.i_res, Ny = m.j_res, g = m.ghost_width < 3 ? m.ghost_width : 3;
+    const size_t P = size_t(Nx) + 6;
+    std::unique_ptr<uint8_t[]> b(new uint8_t[P * (size_t(Ny) + 6)]());
+    for (int j = -g; j <= Ny - 1 + g; j++)
+        for (int i = -g; i <= Nx - 1 + g; i++) b[size_t(i + 3) + P * size_t(j + 3)] = m(i, j) ? 1 : 0;
+    return b;
+}
 }  // namespace awcm_api_detail
 
 // WAVELET_2D (Wavelet/WAVELET_2D.h:10-35): the interpolating wavelet of order
 // 2N.  N and weight(i, j) = l_i(j + 1/2), the Lagrange basis on the nodes
 // 0..2N-1 evaluated at the midpoints, are the reference's public members; the
 // device kernels hold their own copy of the N = 2 table, the only order the
-// AWCM classes use.  Of its transforms, InterpolateAllGridEuler is exposed: the
-// one a driver calls directly (main_2d_euler.cpp:334); the others run inside
-// AWCM_2D::PrepareEuler on the device.  A WAVELET_2D that is AWCM_2D's member
+// AWCM classes use.  Of its transforms, the interpolations are exposed (the
+// drivers call them directly: main_2d_euler.cpp:334, main_2d_advect.cpp:219);
+// the forward and inverse transforms run inside AWCM_2D::Prepare and
+// PrepareEuler on the device.  A WAVELET_2D that is AWCM_2D's member
 // shares that object's device solver; a free-standing one has its own.
 class WAVELET_2D {
 public:
@@ -1269,23 +1287,48 @@ public:
     WAVELET_2D(const WAVELET_2D&) = delete;
     WAVELET_2D& operator=(const WAVELET_2D&) = delete;
 
+    // WAVELET_2D::Interpolate (WAVELET_2D.h:360-473) on the GPU: the scalar
+    // field at the points of CheckReconstruction(mask_to minus mask_from) not
+    // in mask_from.  The field goes down and comes back.
+    void Interpolate(FIELD_UNIFORM_2D<double>& q, const MRA_GRID_INT_2D& mra_grid,
+                     const FIELD_UNIFORM_2D<bool>& mask_from, const FIELD_UNIFORM_2D<bool>& mask_to) const {
+        using awcm_api_detail::check;
+        const char* who = "WAVELET_2D::Interpolate";
+        awcm_api_detail::check_mask_2d(mra_grid, mask_from, who);
+        awcm_api_detail::check_mask_2d(mra_grid, mask_to, who);
+        awcm_solver* h = awcm_api_detail::open_2d(*dev, mra_grid, q, who);
+        check(awcm_upload_scalar(h, q.arr.val), "awcm_upload_scalar");
+        check(awcm_interpolate(h, awcm_api_detail::mask_bytes(mask_from).get(),
+                               awcm_api_detail::mask_bytes(mask_to).get()),
+              "awcm_interpolate");
+        check(awcm_download_scalar(h, q.arr.val), "awcm_download_scalar");
+    }
+    // WAVELET_2D::InterpolateEuler (WAVELET_2D.h:784-907) on the GPU: the same
+    // for momentum and energy.
+    void InterpolateEuler(FIELD_UNIFORM_2D<VECTOR_4D<double>>& q, const MRA_GRID_INT_2D& mra_grid,
+                          const FIELD_UNIFORM_2D<bool>& mask_from, const FIELD_UNIFORM_2D<bool>& mask_to) const {
+        using awcm_api_detail::check;
+        const char* who = "WAVELET_2D::InterpolateEuler";
+        awcm_api_detail::check_mask_2d(mra_grid, mask_from, who);
+        awcm_api_detail::check_mask_2d(mra_grid, mask_to, who);
+        awcm_solver* h = awcm_api_detail::open_2d(*dev, mra_grid, q, who);
+        check(awcm_upload_q(h, q.arr.val[0].values), "awcm_upload_q");
+        check(awcm_interpolate_euler(h, awcm_api_detail::mask_bytes(mask_from).get(),
+                                     awcm_api_detail::mask_bytes(mask_to).get()),
+              "awcm_interpolate_euler");
+        check(awcm_download_q(h, q.arr.val[0].values), "awcm_download_q");
+    }
     // WAVELET_2D::InterpolateAllGridEuler (WAVELET_2D.h:912-1044) on the GPU:
     // every finest-grid point not in mask_from gets its prediction from the
     // coarser levels, all four components.  The field goes down and comes back.
     void InterpolateAllGridEuler(FIELD_UNIFORM_2D<VECTOR_4D<double>>& q, const MRA_GRID_INT_2D& mra_grid,
                                  const FIELD_UNIFORM_2D<bool>& mask_from) const {
         using awcm_api_detail::check;
-        const int Nx = mra_grid.Nx, Ny = mra_grid.Ny;
-        if (mask_from.ghost_width != 3 || mask_from.i_start != 0 || mask_from.j_start != 0 ||
-            mask_from.i_res != Nx || mask_from.j_res != Ny)
-            throw std::invalid_argument(
-                "WAVELET_2D::InterpolateAllGridEuler: mask_from must span (0..Nx-1) x (0..Ny-1) with ghost width 3");
-        awcm_solver* h = awcm_api_detail::open_2d(*dev, mra_grid, q, "WAVELET_2D::InterpolateAllGridEuler");
-        const size_t NP = size_t(Nx + 6) * size_t(Ny + 6);
-        std::unique_ptr<uint8_t[]> m(new uint8_t[NP]);
-        for (size_t n = 0; n < NP; n++) m[n] = mask_from.arr.val[n] ? 1 : 0;
+        const char* who = "WAVELET_2D::InterpolateAllGridEuler";
+        awcm_api_detail::check_mask_2d(mra_grid, mask_from, who);
+        awcm_solver* h = awcm_api_detail::open_2d(*dev, mra_grid, q, who);
         check(awcm_upload_q(h, q.arr.val[0].values), "awcm_upload_q");
-        check(awcm_interpolate_all_grid(h, m.get()), "awcm_interpolate_all_grid");
+        check(awcm_interpolate_all_grid(h, awcm_api_detail::mask_bytes(mask_from).get()), "awcm_interpolate_all_grid");
         check(awcm_download_q(h, q.arr.val[0].values), "awcm_download_q");
     }
 
@@ -1326,6 +1369,57 @@ public:
         d10 = 3.0; d11 = -1.0 / 2.0; d12 = -2.0 / 3.0; d13 = 1.0 / 12.0;
         d20 = -3.0 / 2.0; d21 = 1.0; d22 = 0; d23 = -2.0 / 3.0; d24 = 1.0 / 12.0;
         d30 = 1.0 / 3.0; d31 = -1.0 / 6.0; d32 = 2.0 / 3.0; d33 = 0; d34 = -2.0 / 3.0; d35 = 1.0 / 12.0;
+    }
+
+    // AWCM_2D::Prepare (AWCM_2D.h:52-79) on the GPU: the scalar path.  mask0
+    // and the field go down; the field, mask0..mask3 and level0..level3 come
+    // back (mask_temp is not the scalar path's and stays as it was).  Throws
+    // std::runtime_error carrying the AWCM status on a device error.
+    void Prepare(FIELD_UNIFORM_2D<double>& q, const double& tol = 1e-6) {
+        using awcm_api_detail::check;
+        const int Nx = mra_grid.Nx, Ny = mra_grid.Ny;
+        awcm_api_detail::open_2d(dev, mra_grid, q, "AWCM_2D::Prepare");
+        const size_t NP = size_t(Nx + 6) * size_t(Ny + 6);
+        awcm_api_detail::push_mask0(dev.h, mask0, NP);
+        check(awcm_upload_scalar(dev.h, q.arr.val), "awcm_upload_scalar");
+        check(awcm_prepare(dev.h, tol), "awcm_prepare");
+        check(awcm_download_scalar(dev.h, q.arr.val), "awcm_download_scalar");
+        FIELD_UNIFORM_2D<bool> keep(mask_temp);
+        FIELD_UNIFORM_2D<bool>* const ms[5] = {&mask_temp, &mask0, &mask1, &mask2, &mask3};
+        FIELD_UNIFORM_2D<int>* const ls[4] = {&level0, &level1, &level2, &level3};
+        awcm_api_detail::fetch_masks(dev.h, NP, size_t(Nx) * size_t(Ny), ms, ls);
+        mask_temp = keep;
+    }
+    // RemoveResidue over the finest grid minus its outer ring (AWCM_2D.h:81-84,
+    // AWCM_MANAGER_2D.h:257-266)
+    void PostProcess(FIELD_UNIFORM_2D<double>& q) {
+        ITERATION_GRID_INT_2D(mra_grid.grid_finest, -1) {
+            if (mask0(i, j) == false) q(i, j) = 0.0;
+        }
+    }
+    // AWCM_2D.h:85-93: the 2-wide band next to the walls is always active
+    void BandingBoundary(FIELD_UNIFORM_2D<bool>& mask) {
+        for (int j = mask.j_start; j <= mask.j_end; j++)
+            for (int i = mask.i_start; i <= mask.i_end; i++) {
+                if (2 <= i && i <= mask.i_end - 2 && j >= 2 && j <= mask.j_end - 2) continue;
+                mask(i, j) = true;
+            }
+    }
+    // the scalar forms of AWCM_2D.h:95-110
+    void Differentiate(const double& ll, const double& l, const double& c, const double& r, const double& rr,
+                       double& value) {
+        (void)c;
+        value = d1 * (r - l) + d2 * (rr - ll);
+    }
+    void DifferentiateAtLeftBoundary(const double& l, const double& c, const double& r, const double& rr,
+                                     double& value) {
+        (void)c;
+        value = d1 * (r - l) + d2 * (rr - l);
+    }
+    void DifferentiateAtRightBoundary(const double& ll, const double& l, const double& c, const double& r,
+                                      double& value) {
+        (void)c;
+        value = d1 * (r - l) + d2 * (r - ll);
     }
 
     // AWCM_2D::PrepareEuler (AWCM_2D.h:114-147) on the GPU.  mask0 and the
@@ -1433,12 +1527,16 @@ public:
 };
 
 // zero-order extrapolation into the ghosts at both ends
-// (PROBLEM_SHOCKTUBE.h:57-107, PROBLEM_ENTROPY_INTERACTING.h:52-102)
-class BOUNDARY_MANAGER {
+// (PROBLEM_SHOCKTUBE.h:57-107, PROBLEM_ENTROPY_INTERACTING.h:52-102).  The
+// reference names the Sod one BOUNDARY_MANAGER, as it names the periodic
+// scalar one of PROBLEM_BOX_ADVECTION.h; each is declared by its problem's
+// header (include/compat/Problems/...), as there, so the two names never meet.
+namespace awcm_api_detail {
+class Extrap1D {
 public:
     int i_start, i_start_g, i_end, i_end_g, ghost_width;
     FIELD_UNIFORM_1D<VECTOR_3D<double>>& Q;
-    BOUNDARY_MANAGER(FIELD_UNIFORM_1D<VECTOR_3D<double>>& Q_)
+    Extrap1D(FIELD_UNIFORM_1D<VECTOR_3D<double>>& Q_)
         : i_start(Q_.i_start), i_start_g(Q_.i_start_g), i_end(Q_.i_end), i_end_g(Q_.i_end_g),
           ghost_width(Q_.ghost_width), Q(Q_) {}
     void SetLeftBoundaryCondition() {
@@ -1452,10 +1550,11 @@ public:
         SetRightBoundaryCondition();
     }
 };
+}  // namespace awcm_api_detail
 // the reference's class name, spelling included
-class BOUNDARY_MANAGER_ENTROPY_INTERATING : public BOUNDARY_MANAGER {
+class BOUNDARY_MANAGER_ENTROPY_INTERATING : public awcm_api_detail::Extrap1D {
 public:
-    BOUNDARY_MANAGER_ENTROPY_INTERATING(FIELD_UNIFORM_1D<VECTOR_3D<double>>& Q_) : BOUNDARY_MANAGER(Q_) {}
+    BOUNDARY_MANAGER_ENTROPY_INTERATING(FIELD_UNIFORM_1D<VECTOR_3D<double>>& Q_) : Extrap1D(Q_) {}
 };
 
 // PROBLEM_QUADRANTS.h:15-81: [0, 1]^2, quadrants split at 0.8, t_final = 0.8
@@ -1492,10 +1591,37 @@ public:
     }
 };
 
+// PROBLEM_BOX_ADVECTION.h:5-56 (the scalar path's problem): a Gaussian
+// exp(-50 (x-1/2)^2 - 50 (y-1/2)^2) on the cell centres of [0, 1]^2, ghost
+// cells included, t_final = 2.  The cell-centre grid's y_max is written with
+// x_upper there; both are 1.
+class BOX_ADVECTION {
+public:
+    static void Initialize(const int& resolution_x, GRID_UNIFORM_2D& grid, GRID_UNIFORM_2D& grid_x,
+                           GRID_UNIFORM_2D& grid_y, GRID_UNIFORM_2D& grid_xy, FIELD_UNIFORM_2D<double>& Q,
+                           double& t_final) {
+        const double x_lower = 0, x_upper = 1, y_lower = 0, y_upper = 1;
+        t_final = 2;
+        grid_xy.Initialize(0, 0, resolution_x + 1, resolution_x + 1, x_lower, y_lower, x_upper, y_upper);
+        const double dx = grid_xy.dx, dy = grid_xy.dy;
+        grid_x.Initialize(0, 0, resolution_x + 1, resolution_x, x_lower, y_lower + 0.5 * dy, x_upper, y_upper - 0.5 * dy);
+        grid_y.Initialize(0, 0, resolution_x, resolution_x + 1, x_lower + 0.5 * dx, y_lower, x_upper - 0.5 * dx, y_upper);
+        grid.Initialize(0, 0, resolution_x, resolution_x, x_lower + 0.5 * dx, y_lower + 0.5 * dy, x_upper - 0.5 * dx,
+                        x_upper - 0.5 * dy);
+        Q.Initialize(grid, 3);
+        for (int j = grid.j_start - 3; j <= grid.j_end + 3; j++)
+            for (int i = grid.i_start - 3; i <= grid.i_end + 3; i++) {
+                const VECTOR_2D<double> c = grid(i, j);
+                Q(i, j) = std::exp(-50 * SQR(c.x - 0.5) - 50 * SQR(c.y - 0.5));
+            }
+    }
+};
+
 namespace awcm_api_detail {
 struct Bounds2 {
     int i_start, i_start_g, i_end, i_end_g, j_start, j_start_g, j_end, j_end_g, ghost_width;
-    explicit Bounds2(const FIELD_UNIFORM_2D<VECTOR_4D<double>>& Q)
+    template <class T>
+    explicit Bounds2(const FIELD_UNIFORM_2D<T>& Q)
         : i_start(Q.i_start), i_start_g(Q.i_start_g), i_end(Q.i_end), i_end_g(Q.i_end_g), j_start(Q.j_start),
           j_start_g(Q.j_start_g), j_end(Q.j_end), j_end_g(Q.j_end_g), ghost_width(Q.ghost_width) {}
 };
@@ -1621,3 +1747,68 @@ private:
         set(i, j, 8, 8 * 8.25 * sqrt3 * 0.5, -8 * 8.25 * 0.5, EOS.E(8, 8.25 * sqrt3 * 0.5, -8.25 * 0.5, 116.5));
     }
 };
+
+// PROBLEM_BOX_ADVECTION.h:59-197: periodic ghost fills of the scalar field
+// (declared as BOUNDARY_MANAGER by include/compat/Problems/2D/
+// PROBLEM_BOX_ADVECTION.h).  SetBoundaryCondition does the four sides; the
+// corner blocks are filled only by SetDiagonalBoundaryCondition, whose second
+// pass on the right side carries its j counter over from the first, as the
+// reference's does.
+namespace awcm_api_detail {
+class Periodic2D : public Bounds2 {
+public:
+    FIELD_UNIFORM_2D<double>& Q;
+    Periodic2D(FIELD_UNIFORM_2D<double>& Q_) : Bounds2(Q_), Q(Q_) {}
+    void Initialize(const int i_start_, const int i_start_g_, const int i_end_, const int i_end_g_, const int j_start_,
+                    const int j_start_g_, const int j_end_, const int j_end_g_, const int ghost_width_) {
+        i_start = i_start_; i_start_g = i_start_g_; i_end = i_end_; i_end_g = i_end_g_;
+        j_start = j_start_; j_start_g = j_start_g_; j_end = j_end_; j_end_g = j_end_g_;
+        ghost_width = ghost_width_;
+    }
+    void SetLeftBoundaryCondition() {
+        for (int j = j_start; j <= j_end; j++) {
+            int c = 0;
+            for (int i = i_start - 1; i >= i_start_g; i--, c++) Q(i, j) = Q(i_end - c, j);
+        }
+    }
+    void SetRightBoundaryCondition() {
+        for (int j = j_start; j <= j_end; j++) {
+            int c = 0;
+            for (int i = i_end + 1; i <= i_end_g; i++, c++) Q(i, j) = Q(i_start + c, j);
+        }
+    }
+    void SetBottomBoundaryCondition() {
+        for (int i = i_start; i <= i_end; i++) {
+            int c = 0;
+            for (int j = j_start - 1; j >= j_start_g; j--, c++) Q(i, j) = Q(i, j_end - c);
+        }
+    }
+    void SetTopBoundaryCondition() {
+        for (int i = i_start; i <= i_end; i++) {
+            int c = 0;
+            for (int j = j_end + 1; j <= j_end_g; j++, c++) Q(i, j) = Q(i, j_start + c);
+        }
+    }
+    void SetDiagonalBoundaryCondition() {
+        int c1 = 0;
+        for (int i = i_start - 1; i >= i_start_g; i--, c1++) {
+            int c2 = 0;
+            for (int j = j_start - 1; j >= j_start_g; j--, c2++) Q(i, j) = Q(i_end - c1, j_end - c2);
+            c2 = 0;
+            for (int j = j_end + 1; j <= j_end_g; j++, c2++) Q(i, j) = Q(i_end - c1, j_start + c2);
+        }
+        c1 = 0;
+        for (int i = i_end + 1; i <= i_end_g; i++, c1++) {
+            int c2 = 0;
+            for (int j = j_start - 1; j >= j_start_g; j--, c2++) Q(i, j) = Q(i_start + c1, j_end - c2);
+            for (int j = j_end + 1; j <= j_end_g; j++, c2++) Q(i, j) = Q(i_start + c1, j_start + c2);
+        }
+    }
+    void SetBoundaryCondition() {
+        SetLeftBoundaryCondition();
+        SetRightBoundaryCondition();
+        SetBottomBoundaryCondition();
+        SetTopBoundaryCondition();
+    }
+};
+}  // namespace awcm_api_detail

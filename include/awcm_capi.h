@@ -99,6 +99,37 @@ int awcm_prepare_euler(awcm_solver* h, double tol);
  * (AWCM_ESTATE on a 1-D handle).  Synchronous. */
 int awcm_interpolate_all_grid(awcm_solver* h, const uint8_t* mask_from);
 
+/* ---- the scalar path (2-D; AWCM_ESTATE on a 1-D handle) --------------------
+ * AWCM_2D::Prepare / PostProcess and WAVELET_2D::Interpolate act on a scalar
+ * field (FIELD_UNIFORM_2D<double>, ghost width 3, (Nx+6) x (Ny+6) doubles, i
+ * fastest).  On the device it is the first component of the handle's field;
+ * the masks and levels are the handle's. */
+int awcm_upload_scalar(awcm_solver* h, const double* q);
+int awcm_download_scalar(awcm_solver* h, double* q);
+
+/* AWCM_2D::Prepare (Wavelet/AWCM_2D.h:52-79): forward transform of the scalar
+ * field with threshold tol, banding, adjacent zone, reconstruction check,
+ * masks 1..3 with their levels, inverse transform.  mask_temp is left as it
+ * was. */
+int awcm_prepare(awcm_solver* h, double tol);
+
+/* AWCM_2D::PostProcess (AWCM_2D.h:81-84, AWCM_MANAGER_2D::RemoveResidue
+ * AWCM_MANAGER_2D.h:257-266): zero the field at interior points outside
+ * mask0 (every component of the handle's field). */
+int awcm_post_process(awcm_solver* h);
+
+/* WAVELET_2D::Interpolate(q, mra, mask_from, mask_to) (WAVELET_2D.h:360-473),
+ * as main_2d_advect.cpp:219 calls it: the scalar field is predicted from the
+ * coarser levels at the points of CheckReconstruction(mask_to minus
+ * mask_from) that are not in mask_from.  Masks ghosted uint8, reference
+ * layout.  Synchronous. */
+int awcm_interpolate(awcm_solver* h, const uint8_t* mask_from, const uint8_t* mask_to);
+
+/* WAVELET_2D::InterpolateEuler(q, mra, mask_from, mask_to)
+ * (WAVELET_2D.h:784-907): the same on momentum and energy (components 1..3)
+ * of the Euler field.  Synchronous. */
+int awcm_interpolate_euler(awcm_solver* h, const uint8_t* mask_from, const uint8_t* mask_to);
+
 /* AWCM_*::PostProcessEuler (AWCM_1D.h:122-125, AWCM_2D.h:149-152). */
 int awcm_post_process_euler(awcm_solver* h);
 

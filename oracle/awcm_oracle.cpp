@@ -983,9 +983,20 @@ struct Solver2D {
         check_recon(tmp);
         interp_passes(from, tmp, 0);
     }
-    // coarse-to-fine d1, d2, d3 predictions of components p0..3 at the points of
-    // `tmp` not in `from` (WAVELET_2D.h:796-905 and :922-1041)
-    void interp_passes(std::vector<uint8_t>& from, std::vector<uint8_t>& tmp, int p0) {
+    // WAVELET_2D::Interpolate(q, mra, from, to) (WAVELET_2D.h:360-473): the
+    // passes of InterpolateEuler on the scalar field (component 0 here)
+    void interp_scalar(std::vector<uint8_t>& from, std::vector<uint8_t>& to) {
+        const int ie = Nx - 1, je = Ny - 1;
+        std::vector<uint8_t> tmp = to;
+        for (int j = 0; j <= je; j++)
+            for (int i = 0; i <= ie; i++)
+                if (mk(from, i, j)) mk(tmp, i, j) = 0;
+        check_recon(tmp);
+        interp_passes(from, tmp, 0, 1);
+    }
+    // coarse-to-fine d1, d2, d3 predictions of components p0..p1-1 at the points
+    // of `tmp` not in `from` (WAVELET_2D.h:796-905, :922-1041 and :372-470)
+    void interp_passes(std::vector<uint8_t>& from, std::vector<uint8_t>& tmp, int p0, int p1 = 4) {
         const int ie = Nx - 1, je = Ny - 1;
         for (int lev = 0; lev < maxlev; lev++) {
             const int d = mra[lev], dh = d / 2;
@@ -996,7 +1007,7 @@ struct Solver2D {
                         if (mk(tmp, i, j) == 0 || mk(from, i, j) == 1) continue;
                         int is = i - d - dh, iend = i + d + dh, k = 1;
                         if (is < 0) is = 0, k = 0; else if (iend > ie) is = ie - 3 * d, k = 2;
-                        for (int p = p0; p < 4; p++) {
+                        for (int p = p0; p < p1; p++) {
                             double& v = q(Q, i, j)[p];
                             v = 0.0;
                             int mm = is;
@@ -1011,7 +1022,7 @@ struct Solver2D {
                         if (mk(tmp, i, j) == 0 || mk(from, i, j) == 1) continue;
                         int js = j - d - dh, jend = j + d + dh, k = 1;
                         if (js < 0) js = 0, k = 0; else if (jend > je) js = je - 3 * d, k = 2;
-                        for (int p = p0; p < 4; p++) {
+                        for (int p = p0; p < p1; p++) {
                             double& v = q(Q, i, j)[p];
                             v = 0.0;
                             int n = js;
@@ -1027,7 +1038,7 @@ struct Solver2D {
                         int is = i - d - dh, js = j - d - dh, iend = i + d + dh, jend = j + d + dh, k1 = 1, k2 = 1;
                         if (is < 0) is = 0, k1 = 0; else if (iend > ie) is = ie - 3 * d, k1 = 2;
                         if (js < 0) js = 0, k2 = 0; else if (jend > je) js = je - 3 * d, k2 = 2;
-                        for (int p = p0; p < 4; p++) {
+                        for (int p = p0; p < p1; p++) {
                             double& v = q(Q, i, j)[p];
                             v = 0.0;
                             int mm = is, n = js;
@@ -1058,6 +1069,21 @@ struct Solver2D {
         }
         iwt(m[3]);
         interp(mt, m[3]);
+    }
+    // AWCM_2D::Prepare (AWCM_2D.h:52-79) on the scalar field (component 0):
+    // PrepareEuler without the mask_temp copy and the interpolation
+    void prepare_scalar(double tol) {
+        fwt(m[0], tol);
+        band(m[0]);
+        adjacent(m[0]);
+        check_recon(m[0]);
+        for (int k = 1; k <= 3; k++) {
+            m[k] = m[k - 1];
+            level(lv[k - 1], m[k - 1]);
+            addpoint(m[k], lv[k - 1]);
+            check_recon(m[k]);
+        }
+        iwt(m[3]);
     }
     // AWCM_MANAGER_2D::RemoveResidueEuler (AWCM_MANAGER_2D.h:267-280): interior only
     void post() {
@@ -1392,6 +1418,27 @@ void orc2_get_masks(void* h, uint8_t* masks, int32_t* levels) {
 }
 void orc2_set_mask0(void* h, const uint8_t* m0) { auto* s = static_cast<Solver2D*>(h); std::memcpy(s->m[0].data(), m0, s->m[0].size()); }
 void orc2_prepare(void* h, double tol) { static_cast<Solver2D*>(h)->prepare(tol); }
+// the scalar field of AWCM_2D::Prepare / WAVELET_2D::Interpolate lives in
+// component 0 (reference layout, (Nx+6) x (Ny+6) doubles)
+void orc2_set_scalar(void* h, const double* q) {
+    auto* s = static_cast<Solver2D*>(h);
+    for (size_t k = 0; k < s->m[0].size(); k++) s->Q[4 * k] = q[k];
+}
+void orc2_get_scalar(void* h, double* q) {
+    auto* s = static_cast<Solver2D*>(h);
+    for (size_t k = 0; k < s->m[0].size(); k++) q[k] = s->Q[4 * k];
+}
+void orc2_prepare_scalar(void* h, double tol) { static_cast<Solver2D*>(h)->prepare_scalar(tol); }
+void orc2_interpolate_scalar(void* h, const uint8_t* mask_from, const uint8_t* mask_to) {
+    auto* s = static_cast<Solver2D*>(h);
+    std::vector<uint8_t> from(mask_from, mask_from + s->m[0].size()), to(mask_to, mask_to + s->m[0].size());
+    s->interp_scalar(from, to);
+}
+void orc2_interpolate_euler(void* h, const uint8_t* mask_from, const uint8_t* mask_to) {
+    auto* s = static_cast<Solver2D*>(h);
+    std::vector<uint8_t> from(mask_from, mask_from + s->m[0].size()), to(mask_to, mask_to + s->m[0].size());
+    s->interp(from, to);
+}
 void orc2_interpolate_all(void* h, const uint8_t* mask_from) {
     auto* s = static_cast<Solver2D*>(h);
     std::vector<uint8_t> from(mask_from, mask_from + s->m[0].size());

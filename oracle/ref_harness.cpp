@@ -147,6 +147,7 @@ struct Ref2D {
     AWCM_2D* awcm;
     GRID_UNIFORM_2D grid;
     FIELD_UNIFORM_2D<VECTOR_4D<double>> Q, Q_0, Q_1, Q_2;
+    FIELD_UNIFORM_2D<double> Qs;  // the scalar field of AWCM_2D::Prepare
     BOUNDARY_MANAGER_QUAD *bq, *bq0, *bq1, *bq2;
     BOUNDARY_MANAGER_DMACH *bd, *bd0, *bd1, *bd2;
     EULER_EQUATION_2D* euler_eqn;
@@ -158,6 +159,7 @@ struct Ref2D {
         awcm = new AWCM_2D(mra_grid);
         grid.Initialize(0, 0, mra_grid.Nx, mra_grid.Ny, xl, yl, xu, yu);
         Q.Initialize(grid, 3); Q_0.Initialize(grid, 3); Q_1.Initialize(grid, 3); Q_2.Initialize(grid, 3);
+        Qs.Initialize(grid, 3);
         bq = new BOUNDARY_MANAGER_QUAD(Q); bq0 = new BOUNDARY_MANAGER_QUAD(Q_0);
         bq1 = new BOUNDARY_MANAGER_QUAD(Q_1); bq2 = new BOUNDARY_MANAGER_QUAD(Q_2);
         bd = new BOUNDARY_MANAGER_DMACH(Q); bd0 = new BOUNDARY_MANAGER_DMACH(Q_0);
@@ -397,6 +399,32 @@ void ref2_set_mask0(void* h, const uint8_t* m0) {
     for (int k = 0; k < s->awcm->mask0.arr.ij_res; k++) s->awcm->mask0.arr.val[k] = m0[k] != 0;
 }
 void ref2_prepare(void* h, double tol) { auto* s = static_cast<Ref2D*>(h); s->awcm->PrepareEuler(s->Q, tol); }
+void ref2_set_scalar(void* h, const double* q) {
+    auto* s = static_cast<Ref2D*>(h);
+    for (int k = 0; k < s->Qs.arr.ij_res; k++) s->Qs.arr.val[k] = q[k];
+}
+void ref2_get_scalar(void* h, double* q) {
+    auto* s = static_cast<Ref2D*>(h);
+    for (int k = 0; k < s->Qs.arr.ij_res; k++) q[k] = s->Qs.arr.val[k];
+}
+void ref2_prepare_scalar(void* h, double tol) { auto* s = static_cast<Ref2D*>(h); s->awcm->Prepare(s->Qs, tol); }
+static void bool_field(FIELD_UNIFORM_2D<bool>& f, const uint8_t* m) {
+    for (int k = 0; k < f.arr.ij_res; k++) f.arr.val[k] = m[k] != 0;
+}
+void ref2_interpolate_scalar(void* h, const uint8_t* mask_from, const uint8_t* mask_to) {
+    auto* s = static_cast<Ref2D*>(h);
+    FIELD_UNIFORM_2D<bool> from(s->awcm->mask0), to(s->awcm->mask0);
+    bool_field(from, mask_from);
+    bool_field(to, mask_to);
+    s->awcm->wavelet.Interpolate(s->Qs, s->mra_grid, from, to);
+}
+void ref2_interpolate_euler(void* h, const uint8_t* mask_from, const uint8_t* mask_to) {
+    auto* s = static_cast<Ref2D*>(h);
+    FIELD_UNIFORM_2D<bool> from(s->awcm->mask0), to(s->awcm->mask0);
+    bool_field(from, mask_from);
+    bool_field(to, mask_to);
+    s->awcm->wavelet.InterpolateEuler(s->Q, s->mra_grid, from, to);
+}
 void ref2_interpolate_all(void* h, const uint8_t* mask_from) {
     auto* s = static_cast<Ref2D*>(h);
     FIELD_UNIFORM_2D<bool> from(s->awcm->mask0);

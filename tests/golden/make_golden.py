@@ -147,6 +147,34 @@ def all_grid2(kind, L, nxc, nyc, steps, seed):
     return out
 
 
+def scalar_box2(L, nxc, seed):
+    """AWCM_2D::Prepare on a scalar field, three rounds, the field moved
+    between them (shifted one column and damped: a stand-in for the driver's
+    stages), then WAVELET_2D::Interpolate from/to four mask pairs."""
+    n = (nxc - 1) * 2 ** (L - 1) + 1
+    s = O.Solver2D("ref", L, nxc, nxc, 1.4, 0, 0.0, 1.0, 0.0, 1.0)
+    x = (np.arange(-3, n + 3) + 0.5) / n
+    X, Y = np.meshgrid(x, x)
+    q = np.exp(-50 * (X - 0.5) ** 2 - 50 * (Y - 0.3) ** 2) + 0.5 * (np.abs(X - 0.6) + np.abs(Y - 0.6) < 0.2)
+    out = dict(levels=L, nx_coarse=nxc, q0=q.ravel())
+    cur = q.ravel()
+    for r in range(3):
+        s.set_scalar(cur)
+        s.prepare_scalar(1e-4)
+        m, lv = s.masks()
+        out[f"q_{r}"], out[f"masks_{r}"], out[f"levels_{r}"] = s.get_scalar(), m.astype(np.uint8), lv.astype(np.int8)
+        cur = (np.roll(s.get_scalar().reshape(n + 6, n + 6), 1, axis=1) * 0.999).ravel()
+    rng = np.random.default_rng(seed)
+    m = out["masks_2"]
+    pairs = {"m0_all": (m[1], np.ones_like(m[1])), "m0_m3": (m[1], m[4]), "m1_m2": (m[2], m[3]),
+             "random": ((rng.random(m[1].shape) < 0.3).astype(np.uint8), (rng.random(m[1].shape) < 0.6).astype(np.uint8))}
+    for name, (f, t) in pairs.items():
+        s.set_scalar(out["q_2"])
+        s.interpolate_scalar(f, t)
+        out[f"from_{name}"], out[f"to_{name}"], out[f"qi_{name}"] = f.astype(np.uint8), t.astype(np.uint8), s.get_scalar()
+    return out
+
+
 def main():
     if not O.have_ref():
         sys.exit("oracle/_ref/libawcm_ref.so is not built (make -C oracle)")
@@ -158,6 +186,7 @@ def main():
         "dmr_2d_L4.npz": lambda: traj2("dmr", 4, 17, 6, 1e-3, 0.4, 0.2, 21, [1, 6]),
         "all_grid_quad_2d_L4.npz": lambda: all_grid2("quad", 4, 5, 5, 6, 31),
         "all_grid_dmr_2d_L3.npz": lambda: all_grid2("dmr", 3, 17, 5, 6, 32),
+        "scalar_box_2d_L4.npz": lambda: scalar_box2(4, 5, 33),
     }
     for name in sys.argv[1:] or list(files):
         np.savez_compressed(os.path.join(HERE, name), **files[name]())

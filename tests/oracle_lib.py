@@ -63,6 +63,11 @@ def load(prefix="orc"):
         f"{p}2_set_mask0": (None, [V, _u8p]),
         f"{p}2_prepare": (None, [V, C.c_double]),
         f"{p}2_interpolate_all": (None, [V, _u8p]),
+        f"{p}2_interpolate_euler": (None, [V, _u8p, _u8p]),
+        f"{p}2_set_scalar": (None, [V, _dp]),
+        f"{p}2_get_scalar": (None, [V, _dp]),
+        f"{p}2_prepare_scalar": (None, [V, C.c_double]),
+        f"{p}2_interpolate_scalar": (None, [V, _u8p, _u8p]),
         f"{p}2_step": (C.c_double, [V, C.c_double, C.c_double, _dp, C.c_int, _ip, _dp]),
         f"{p}_flux1": (None, [C.c_double, _dp, C.c_double, _dp]),
         f"{p}_flux2": (None, [C.c_double, C.c_int, _dp, C.c_double, _dp]),
@@ -255,6 +260,35 @@ class Solver2D:
 
     def prepare(self, tol):
         getattr(self.lib, f"{self.p}2_prepare")(self.h, tol)
+
+    def set_scalar(self, q):
+        """The scalar field of AWCM_2D::Prepare ((ny+6)*(nx+6) doubles)."""
+        q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1)
+        assert q.size == self.npts
+        getattr(self.lib, f"{self.p}2_set_scalar")(self.h, dptr(q))
+
+    def get_scalar(self):
+        q = np.zeros(self.npts)
+        getattr(self.lib, f"{self.p}2_get_scalar")(self.h, dptr(q))
+        return q
+
+    def prepare_scalar(self, tol):
+        """AWCM_2D::Prepare on the scalar field."""
+        getattr(self.lib, f"{self.p}2_prepare_scalar")(self.h, tol)
+
+    def interpolate_scalar(self, mask_from, mask_to):
+        """WAVELET_2D::Interpolate(q, mra, mask_from, mask_to) on the scalar field."""
+        f = np.ascontiguousarray(mask_from, dtype=np.uint8).reshape(-1)
+        t = np.ascontiguousarray(mask_to, dtype=np.uint8).reshape(-1)
+        assert f.size == self.npts and t.size == self.npts
+        getattr(self.lib, f"{self.p}2_interpolate_scalar")(self.h, u8ptr(f), u8ptr(t))
+
+    def interpolate_euler(self, mask_from, mask_to):
+        """WAVELET_2D::InterpolateEuler(q, mra, mask_from, mask_to)."""
+        f = np.ascontiguousarray(mask_from, dtype=np.uint8).reshape(-1)
+        t = np.ascontiguousarray(mask_to, dtype=np.uint8).reshape(-1)
+        assert f.size == self.npts and t.size == self.npts
+        getattr(self.lib, f"{self.p}2_interpolate_euler")(self.h, u8ptr(f), u8ptr(t))
 
     def interpolate_all(self, mask_from):
         """WAVELET_2D::InterpolateAllGridEuler with mask_from ((ny+6, nx+6) bytes)."""

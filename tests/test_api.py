@@ -16,9 +16,12 @@ The GPU half (PrepareEuler and the drivers' time loop) is tests/test_gpu_api.py.
 
 Where the reference tree is present, the probe is also rebuilt from it and run
 live, and the shipped drivers (Wavelet/main_1d_euler.cpp, main_2d_euler.cpp,
-main_2d_euler(dmach).cpp, main_2d_euler(quad).cpp) are compiled and linked
-against this library with only `void main` -> `int main` and sprintf_s ->
-snprintf changed.
+main_2d_euler(dmach).cpp, main_2d_euler(quad).cpp and the scalar
+main_2d_advect.cpp) are compiled and linked against this library with only
+`void main` -> `int main` and sprintf_s -> snprintf changed.
+(Wavelet/main_1d_advect.cpp is not among them: it calls a six-argument
+AWCM_1D::DifferentiateAtLeftBoundary that the reference does not define, so
+it does not build against the reference either.)
 """
 import os
 import re
@@ -33,7 +36,8 @@ PROBE = os.path.join(NATIVE, "_build", "awcm_api_probe")
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 REF = "/root/reference"
 REF_PROBE = os.path.join(ROOT, "oracle", "_ref", "awcm_api_probe_ref")
-DRIVERS = ["main_1d_euler", "main_2d_euler", "main_2d_euler(dmach)", "main_2d_euler(quad)"]
+REF_PROBE_ADVECT = os.path.join(ROOT, "oracle", "_ref", "awcm_api_probe_advect_ref")
+DRIVERS = ["main_1d_euler", "main_2d_euler", "main_2d_euler(dmach)", "main_2d_euler(quad)", "main_2d_advect"]
 
 
 def have_ref():
@@ -89,6 +93,9 @@ def test_golden_is_what_the_reference_prints(tmp_path):
         out = subprocess.run([REF_PROBE, mode] + arg, capture_output=True, text=True, timeout=300)
         assert out.returncode == 0, out.stderr
         assert not diff_lines(out.stdout.splitlines(), golden(mode)), mode
+    out = subprocess.run([REF_PROBE_ADVECT, "40"], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    assert not diff_lines(out.stdout.splitlines(), golden("advect")), "advect"
 
 
 def patched_driver(name, dst):
@@ -123,7 +130,7 @@ def test_compat_headers_cover_driver_includes():
     want = ["OMP_MANAGER.h", "STOP_WATCH.h", "WENO.h", "WENOZ.h", "EULER_EQUATION.h", "ARRAY.h", "ARRAY_1D.h",
             "AWCM_2D.h", "Wavelet/AWCM_1D.h", "Problems/1D/PROBLEM_SHOCKTUBE.h",
             "Problems/1D/PROBLEM_ENTROPY_INTERACTING.h", "Problems/2D/PROBLEM_DOUBLEMACH.h",
-            "Problems/2D/PROBLEM_QUADRANTS.h"]
+            "Problems/2D/PROBLEM_QUADRANTS.h", "Problems/2D/PROBLEM_BOX_ADVECTION.h", "Wavelet/WAVELET_2D.h"]
     compat = os.path.join(ROOT, "include", "compat")
     missing = [w for w in want if not os.path.exists(os.path.join(compat, w))]
     assert not missing, missing

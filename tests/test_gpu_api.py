@@ -17,6 +17,12 @@ bit-identical, and t and dt identical at every step:
            481 points and in the driver's own 15361-point configuration) and of
            main_2d_euler(quad|dmach).cpp's loop, each 2-D loop ending with
            the whole grid from mask0 as main_2d_euler.cpp:334 does
+  advect   the scalar path: 40 steps of main_2d_advect.cpp's loop (box
+           advection, AWCM_2D::Prepare / PostProcess, WENO5 right
+           reconstructions) on 65x65 and 10 in the driver's own 449x449
+           configuration, each ending with WAVELET_2D::Interpolate from mask0
+           onto every point (main_2d_advect.cpp:219) and from mask1 onto mask3
+           (tests/native/_build/awcm_api_probe_advect)
 """
 import os
 import subprocess
@@ -27,16 +33,17 @@ pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PROBE = os.path.join(ROOT, "tests", "native", "_build", "awcm_api_probe")
+PROBE_ADVECT = os.path.join(ROOT, "tests", "native", "_build", "awcm_api_probe_advect")
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 
 
-def run(mode, *args):
+def run(mode, *args, exe=PROBE):
     import pyawcm
     pyawcm.lib()  # raises if the HIP library is not built
-    if not os.path.exists(PROBE):
-        pytest.fail("tests/native/_build/awcm_api_probe is not built (run __graft_entry__.build())")
-    print(f"api probe: {mode} {' '.join(args)}", flush=True)
-    out = subprocess.run([PROBE, mode, *args], capture_output=True, text=True, timeout=110)
+    if not os.path.exists(exe):
+        pytest.fail(f"{exe} is not built (run __graft_entry__.build())")
+    print(f"api probe: {os.path.basename(exe)} {mode} {' '.join(args)}", flush=True)
+    out = subprocess.run([exe, mode, *args], capture_output=True, text=True, timeout=110)
     assert out.returncode == 0, f"probe {mode} failed ({out.returncode}): {out.stderr[-2000:]}"
     return out.stdout.splitlines()
 
@@ -55,3 +62,7 @@ def test_prepare_euler_matches_reference():
 
 def test_driver_loop_matches_reference():
     compare(run("loop", "40"), "loop")
+
+
+def test_advect_loop_matches_reference():
+    compare(run("40", exe=PROBE_ADVECT), "advect")

@@ -326,17 +326,84 @@ def test_2d_interpolate_all_grid_matches_oracle(case):
         assert_same(qg, qo, mg2, mo2, lg2, lo2, what)
 
 
-def test_interpolate_all_grid_rejects_1d_handle():
-    """InterpolateAllGridEuler is a 2-D routine: a 1-D handle is refused
-    (AWCM_ESTATE) and its field is left as it was."""
+def test_2d_only_entry_points_reject_1d_handle():
+    """InterpolateAllGridEuler, Interpolate(Euler) and the scalar path are 2-D
+    routines: a 1-D handle is refused (AWCM_ESTATE) and its field is left as
+    it was."""
     P = _pyawcm()
     q, dx = sod_setup(5, 17)
     g = P.Solver(1, 5, 17, 1.4, dx)
     g.upload_q(q)
-    with pytest.raises(P.AwcmError) as e:
-        g.interpolate_all_grid(np.ones(g.npts, np.uint8))
-    assert e.value.code == -6  # AWCM_ESTATE
+    ones = np.ones(g.npts, np.uint8)
+    calls = [lambda: g.interpolate_all_grid(ones), lambda: g.interpolate(ones, ones),
+             lambda: g.interpolate_euler(ones, ones), lambda: g.prepare_scalar(1e-3), lambda: g.post_process_scalar(),
+             lambda: g.upload_scalar(np.zeros(g.npts)), lambda: g.download_scalar()]
+    for call in calls:
+        with pytest.raises(P.AwcmError) as e:
+            call()
+        assert e.value.code == -6  # AWCM_ESTATE
     assert np.array_equal(g.download_q(), q)
+
+
+@pytest.mark.parametrize("case", ["box", "dmr_density", "box_2049"])
+def test_2d_scalar_path_matches_oracle(case):
+    """The scalar path through the C ABI: AWCM_2D::Prepare (awcm_prepare) on a
+    scalar field, four rounds with the field moved between them (shifted one
+    column and damped, a stand-in for the driver's stages), then
+    WAVELET_2D::Interpolate (awcm_interpolate) and InterpolateEuler
+    (awcm_interpolate_euler) from/to five mask pairs.  Masks, levels and
+    fields bit-identical to the oracle; the scalar Prepare leaves mask_temp
+    alone.  The last case is a 2049 x 2049 grid (7 levels, 33 x 33 coarse)."""
+    P = _pyawcm()
+    if case == "dmr_density":
+        L, nxc, nyc, xu, rounds = 4, 33, 9, 4.0, 4
+    elif case == "box":
+        L, nxc, nyc, xu, rounds = 5, 9, 9, 1.0, 4
+    else:
+        L, nxc, nyc, xu, rounds = 7, 33, 33, 1.0, 2
+    nx, ny = (nxc - 1) * 2 ** (L - 1) + 1, (nyc - 1) * 2 ** (L - 1) + 1
+    dx = xu / (nx - 1)
+    if case == "dmr_density":
+        q = O.ic2(1, nx, ny, 0.0, 4.0, 0.0, 1.0, 1.4)[0].reshape(-1, 4)[:, 0].copy()
+    else:
+        x = (np.arange(-3, nx + 3) + 0.5) / nx
+        X, Y = np.meshgrid(x, x)
+        q = (np.exp(-50 * (X - 0.5) ** 2 - 50 * (Y - 0.5) ** 2) + 0.3 * (X > 0.7)).ravel()
+    g = P.Solver(2, L, nxc, 1.4, dx, ny_coarse=nyc, x0=0.0, bc_kind=0)
+    o = O.Solver2D("orc", L, nxc, nyc, 1.4, 0, 0.0, xu, 0.0, 1.0)
+    cur = q
+    for r in range(rounds):
+        print(f"scalar {case} prepare round {r}", flush=True)
+        g.upload_scalar(cur)
+        o.set_scalar(cur)
+        g.prepare_scalar(1e-4)
+        o.prepare_scalar(1e-4)
+        mg, lg = g.masks()
+        mo, lo = o.masks()
+        sg = g.download_scalar()
+        assert not mg[0].any(), "the scalar Prepare leaves mask_temp alone"
+        assert_same(sg, o.get_scalar(), mg, mo, lg, lo, f"scalar {case} prepare round {r}")
+        cur = (np.roll(sg.reshape(ny + 6, nx + 6), 1, axis=1) * 0.999).ravel()
+    rng = np.random.default_rng(21)
+    m = mg
+    qe = np.stack([q, 2 * q, q - 1, 3 * q + 1], axis=-1).ravel()
+    pairs = {"m0_all": (m[1], np.ones_like(m[1])), "m0_m3": (m[1], m[4]), "m1_m2": (m[2], m[3]),
+             "none_m3": (np.zeros_like(m[1]), m[4]),
+             "random": ((rng.random(m[1].shape) < 0.3).astype(np.uint8), (rng.random(m[1].shape) < 0.6).astype(np.uint8))}
+    for name, (f, t) in pairs.items():
+        print(f"scalar {case} interpolate {name}", flush=True)
+        g.upload_scalar(q)
+        o.set_scalar(q)
+        g.interpolate(f, t)
+        o.interpolate_scalar(f, t)
+        a, b = g.download_scalar(), o.get_scalar()
+        assert np.array_equal(a.view(np.uint64), b.view(np.uint64)), f"Interpolate {name}: {int((a != b).sum())} differ"
+        g.upload_q(qe)
+        o.set_q(qe)
+        g.interpolate_euler(f, t)
+        o.interpolate_euler(f, t)
+        a, b = g.download_q(), o.get_q()
+        assert np.array_equal(a.view(np.uint64), b.view(np.uint64)), f"InterpolateEuler {name}: {int((a != b).sum())} differ"
 
 
 @pytest.mark.parametrize("seed", [0, 1, 2, 3, 4, 5])

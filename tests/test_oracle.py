@@ -9,8 +9,9 @@
   limit and mirror symmetry, flux consistency F(Q,...,Q) = F(Q) (which needs
   R L = I), nested masks, the always-active band, eps = 0 reducing the AWCM
   step to a uniform-grid WENO-Z SSP-RK3 step, Sod against the exact
-  Riemann solution, and InterpolateAllGridEuler reproducing bicubic fields
-  and leaving mask_from's points alone.
+  Riemann solution, InterpolateAllGridEuler reproducing bicubic fields
+  and leaving mask_from's points alone, and the scalar path (AWCM_2D::Prepare,
+  WAVELET_2D::Interpolate) agreeing with the Euler path on component 0.
 """
 import os
 
@@ -113,6 +114,29 @@ def test_oracle_all_grid_matches_golden(name):
     m, lv = s.masks()
     assert np.array_equal(m, g["masks_after"].reshape(m.shape)) and np.array_equal(lv, g["levels_after"].reshape(lv.shape))
     assert np.array_equal(bits(s.get_q()), bits(g["q_after"])) and s.t[0] == g["t_after"]
+
+
+def test_oracle_scalar_path_matches_golden():
+    """AWCM_2D::Prepare on a scalar field (three rounds) and
+    WAVELET_2D::Interpolate from/to four mask pairs, against the reference
+    build's vectors."""
+    g = load("scalar_box_2d_L4.npz")
+    L, nxc = int(g["levels"]), int(g["nx_coarse"])
+    n = (nxc - 1) * 2 ** (L - 1) + 1
+    s = O.Solver2D("orc", L, nxc, nxc, 1.4, 0, 0.0, 1.0, 0.0, 1.0)
+    cur = g["q0"]
+    for r in range(3):
+        s.set_scalar(cur)
+        s.prepare_scalar(1e-4)
+        m, lv = s.masks()
+        assert np.array_equal(m, g[f"masks_{r}"].reshape(m.shape)), f"masks, round {r}"
+        assert np.array_equal(lv, g[f"levels_{r}"].reshape(lv.shape)), f"levels, round {r}"
+        assert np.array_equal(bits(s.get_scalar()), bits(g[f"q_{r}"])), f"field, round {r}"
+        cur = (np.roll(s.get_scalar().reshape(n + 6, n + 6), 1, axis=1) * 0.999).ravel()
+    for name in ("m0_all", "m0_m3", "m1_m2", "random"):
+        s.set_scalar(g["q_2"])
+        s.interpolate_scalar(g[f"from_{name}"], g[f"to_{name}"])
+        assert np.array_equal(bits(s.get_scalar()), bits(g[f"qi_{name}"])), name
 
 
 # ----------------------------------------------------- reference build ----
@@ -230,6 +254,52 @@ def test_oracle_all_grid_matches_reference(case):
         assert np.array_equal(ma, mb) and np.array_equal(la, lb)
         assert np.array_equal(bits(a.get_q()), bits(b.get_q()))
     assert O.load("orc").orc2_oob(a.h) == 0
+
+
+@needs_ref
+@pytest.mark.parametrize("case", ["box", "dmr_density"])
+def test_oracle_scalar_path_matches_reference(case):
+    """AWCM_2D::Prepare (four rounds, the field moved between them),
+    WAVELET_2D::Interpolate and InterpolateEuler from/to five mask pairs."""
+    if case == "box":
+        L, nxc, nyc, bc, xu = 5, 9, 9, 0, 1.0
+    else:
+        L, nxc, nyc, bc, xu = 4, 33, 9, 1, 4.0
+    nx, ny = (nxc - 1) * 2 ** (L - 1) + 1, (nyc - 1) * 2 ** (L - 1) + 1
+    a = O.Solver2D("orc", L, nxc, nyc, 1.4, bc, 0.0, xu, 0.0, 1.0)
+    b = O.Solver2D("ref", L, nxc, nyc, 1.4, bc, 0.0, xu, 0.0, 1.0)
+    if case == "box":
+        x = (np.arange(-3, nx + 3) + 0.5) / nx
+        X, Y = np.meshgrid(x, x)
+        q = (np.exp(-50 * (X - 0.5) ** 2 - 50 * (Y - 0.5) ** 2) + 0.3 * (X > 0.7)).ravel()
+    else:
+        q = O.ic2(1, nx, ny, 0.0, 4.0, 0.0, 1.0, 1.4)[0].reshape(-1, 4)[:, 0].copy()
+    cur = q
+    for r in range(4):
+        a.set_scalar(cur)
+        b.set_scalar(cur)
+        a.prepare_scalar(1e-4)
+        b.prepare_scalar(1e-4)
+        ma, la = a.masks()
+        mb, lb = b.masks()
+        assert np.array_equal(ma, mb) and np.array_equal(la, lb), f"round {r}"
+        assert np.array_equal(bits(a.get_scalar()), bits(b.get_scalar())), f"round {r}"
+        cur = (np.roll(a.get_scalar().reshape(ny + 6, nx + 6), 1, axis=1) * 0.999).ravel()
+    rng = np.random.default_rng(13)
+    m = ma
+    qe = np.stack([q, 2 * q, q - 1, 3 * q + 1], axis=-1).ravel()
+    for f, t in ((m[1], np.ones_like(m[1])), (m[1], m[4]), (m[2], m[3]), (np.zeros_like(m[1]), m[4]),
+                 ((rng.random(m[1].shape) < 0.3).astype(np.uint8), (rng.random(m[1].shape) < 0.6).astype(np.uint8))):
+        a.set_scalar(q)
+        b.set_scalar(q)
+        a.interpolate_scalar(f, t)
+        b.interpolate_scalar(f, t)
+        assert np.array_equal(bits(a.get_scalar()), bits(b.get_scalar()))
+        a.set_q(qe)
+        b.set_q(qe)
+        a.interpolate_euler(f, t)
+        b.interpolate_euler(f, t)
+        assert np.array_equal(bits(a.get_q()), bits(b.get_q()))
 
 
 # ----------------------------------------------- closed forms, invariants ----
@@ -386,6 +456,47 @@ def test_all_grid_leaves_mask_from_points():
         work[4:-4, 4:-4] = True
         assert not changed[~work].any()
         assert changed[work & ~keep].all()
+
+
+def test_scalar_path_is_the_euler_path_on_component_0():
+    """The reference's scalar transforms are its Euler transforms on
+    component 0 (WAVELET_2D.h:37-359 against :478-783): Prepare and
+    PrepareEuler give the same masks, levels and first component, and
+    Interpolate(from, all points) is InterpolateAllGridEuler's first
+    component; Interpolate(from, to) changes only points of
+    CheckReconstruction(to minus from) outside from."""
+    L, nxc, nyc = 4, 33, 9
+    nx, ny = (nxc - 1) * 8 + 1, (nyc - 1) * 8 + 1
+    q4 = O.ic2(1, nx, ny, 0.0, 4.0, 0.0, 1.0, 1.4)[0]
+    a = O.Solver2D("orc", L, nxc, nyc, 1.4, 1, 0.0, 4.0, 0.0, 1.0)
+    b = O.Solver2D("orc", L, nxc, nyc, 1.4, 1, 0.0, 4.0, 0.0, 1.0)
+    a.set_q(q4)
+    a.prepare(1e-3)
+    b.set_scalar(q4.reshape(-1, 4)[:, 0])
+    b.prepare_scalar(1e-3)
+    ma, la = a.masks()
+    mb, lb = b.masks()
+    assert np.array_equal(ma[1:], mb[1:]) and np.array_equal(la[:3], lb[:3])
+    assert not mb[0].any(), "the scalar Prepare leaves mask_temp alone"
+    assert np.array_equal(bits(a.get_q().reshape(-1, 4)[:, 0]), bits(b.get_scalar()))
+    q = a.get_q()
+    f = ma[1]
+    a.set_q(q)
+    a.interpolate_all(f)
+    b.set_scalar(q.reshape(-1, 4)[:, 0])
+    b.interpolate_scalar(f, np.ones_like(f))
+    assert np.array_equal(bits(a.get_q().reshape(-1, 4)[:, 0]), bits(b.get_scalar()))
+    # a `to` of one detail point far from the walls: only that point moves
+    to = np.zeros_like(f)
+    j, i = 3 + 8 * 3 + 4, 3 + 8 * 10 + 4  # a d3 point of the level of spacing 8
+    to[j, i] = 1
+    f2 = np.zeros_like(f)
+    b.set_scalar(q.reshape(-1, 4)[:, 0])
+    b.interpolate_scalar(f2, to)
+    moved = b.get_scalar().reshape(ny + 6, nx + 6) != q.reshape(ny + 6, nx + 6, 4)[:, :, 0]
+    assert moved[j, i] and moved.sum() >= 1
+    # the closure of one d3 point only adds the coarser points its stencils need
+    assert not moved[:, :3].any() and not moved[:, -3:].any()
 
 
 def _uniform_step_1d(q, dx, cfl, g=1.4):
