@@ -85,6 +85,7 @@ int awcm_adjoint_record(awcm_solver* h, double tol, double cfl, double t_final, 
     if (h->dim != 1) return AWCM_ESTATE;
     Batch1D& B = h->b1;
     awcm::Inst1D& in = B.h[0];
+    if (B.flux != AWCM_FLUX_WENOZ) return AWCM_ESTATE;  // the adjoint differentiates the WENO-Z flux pair
     delete h->adj;
     h->adj = new (std::nothrow) Adj1DHost();
     if (!h->adj) return AWCM_ENOMEM;

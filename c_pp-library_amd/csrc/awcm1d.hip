@@ -5,8 +5,9 @@
 // instances too long for one workgroup's LDS); the conserved state Q, the RK
 // copies and the face-flux buffer live in HBM (planes of Nx+6 doubles, SoA) and
 // stay L2-resident at these sizes.  A launch runs the reference's time loop
-// (Wavelet/main_1d_euler.cpp:61-260, WENO-Z flux pair in place of the shipped
-// WENO5 ComputedF) to t_final or a step budget without returning to the host,
+// (Wavelet/main_1d_euler.cpp:61-260, with the WENO-Z flux pair of the C1
+// configuration or, FLUX = kRecWeno5, the shipped driver's WENO5-JS ComputedF)
+// to t_final or a step budget without returning to the host,
 // so the plain solve is a single kernel and the batched form is the same kernel
 // over a work queue of instances (largest first), one workgroup each.
 //
@@ -16,9 +17,13 @@
 // reference evaluates the same function on the same six states twice
 // (main_1d_euler.cpp:109 via WENO.h:176-184), so results are identical.
 
+#include "awcm_capi.h"
 #include "solver1d_dev.h"
 
 namespace awcm {
+
+// the C ABI's flux kinds name the reconstructions of awcm/physics.h
+static_assert(AWCM_FLUX_WENOZ == kRecWenoZ && AWCM_FLUX_WENO5 == kRecWeno5, "flux kinds");
 
 // mode 0: PrepareEuler only; mode 2: PostProcessEuler only; mode 1: time steps until t >= t_final or max_steps
 // steps of this launch.  Instances are taken from `order` through the atomic
@@ -27,7 +32,7 @@ namespace awcm {
 // instances): dynamic LDS, or with kHbm -- for instances whose masks do not
 // fit one workgroup's LDS -- this workgroup's slice of `hbm` (gridDim.x slices
 // of 11 * nxp bytes; workgroup-local, ordered by the same barriers).
-template <bool kHbm>
+template <bool kHbm, int FLUX>
 __global__ void __launch_bounds__(1024) k1_solve(Inst1D* insts, int n_inst, const int* order, int* qctr, int mode,
                                                  int max_steps, int nxp, uint8_t* hbm) {
     AWCM_DYNAMIC_LDS uint8_t lds_masks[];
@@ -76,7 +81,7 @@ __global__ void __launch_bounds__(1024) k1_solve(Inst1D* insts, int n_inst, cons
             const int budget = c.rec ? min(max_steps, in.tape_cap - in.tape_n) : max_steps;
             for (int s = 0; s < budget; s++) {
                 if (!(st.t < st.t_final) || st.bad) break;
-                step(c, st, in.print_times);
+                step<FLUX>(c, st, in.print_times);
             }
         }
         __syncthreads();
@@ -93,7 +98,9 @@ __global__ void __launch_bounds__(1024) k1_solve(Inst1D* insts, int n_inst, cons
     }
 }
 
-template __global__ void k1_solve<false>(Inst1D*, int, const int*, int*, int, int, int, uint8_t*);
-template __global__ void k1_solve<true>(Inst1D*, int, const int*, int*, int, int, int, uint8_t*);
+template __global__ void k1_solve<false, kRecWenoZ>(Inst1D*, int, const int*, int*, int, int, int, uint8_t*);
+template __global__ void k1_solve<true, kRecWenoZ>(Inst1D*, int, const int*, int*, int, int, int, uint8_t*);
+template __global__ void k1_solve<false, kRecWeno5>(Inst1D*, int, const int*, int*, int, int, int, uint8_t*);
+template __global__ void k1_solve<true, kRecWeno5>(Inst1D*, int, const int*, int*, int, int, int, uint8_t*);
 
 }  // namespace awcm

@@ -137,6 +137,13 @@ int awcm_prepare_euler(awcm_solver* h, double tol) {
     return st ? st : flag_status(flag);
 }
 
+int awcm_set_flux(awcm_solver* h, int kind) {
+    if (!h || (kind != AWCM_FLUX_WENOZ && kind != AWCM_FLUX_WENO5)) return AWCM_EINVAL;
+    if (h->dim != 1) return AWCM_ESTATE;
+    h->b1.flux = kind;
+    return AWCM_OK;
+}
+
 int awcm_upload_scalar(awcm_solver* h, const double* q) {
     if (!h || !q) return AWCM_EINVAL;
     if (h->dim != 2) return AWCM_ESTATE;

@@ -18,6 +18,7 @@ struct Batch1D {
     int max_nx = 0;
     int n_print = 0;
     int threads = 256;
+    int flux = 0;                 // stage face fluxes of every instance: AWCM_FLUX_WENOZ / AWCM_FLUX_WENO5
     int workgroups = 0;
     std::vector<Inst1D> h;        // host mirror of the descriptors
     Inst1D* d_inst = nullptr;
@@ -31,7 +32,7 @@ struct Batch1D {
     // when its launch shape or step budget changes
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
-    int g_threads = 0, g_wgs = 0, g_steps = -1;
+    int g_threads = 0, g_wgs = 0, g_steps = -1, g_flux = 0;
     const uint8_t* g_hbm = nullptr;
     // masks and levels of instances too long for one workgroup's LDS: an
     // 11 * nxp-byte slice per workgroup in HBM (grown on demand, never shrunk)

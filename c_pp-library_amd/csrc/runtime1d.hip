@@ -15,9 +15,29 @@
 
 namespace awcm {
 
-template <bool kHbm>
+template <bool kHbm, int FLUX>
 __global__ void k1_solve(Inst1D* insts, int n_inst, const int* order, int* qctr, int mode, int max_steps, int nxp,
                          uint8_t* hbm);
+
+// The k1_solve instantiation for a mask placement and a stage flux.
+static void launch_solve(bool hbm_masks, int flux, dim3 grid, dim3 block, size_t lds, hipStream_t stream, Inst1D* insts,
+                         int n_inst, const int* order, int* qctr, int mode, int max_steps, int nxp, uint8_t* hbm) {
+    if (flux == AWCM_FLUX_WENO5) {
+        if (hbm_masks)
+            hipLaunchKernelGGL((k1_solve<true, AWCM_FLUX_WENO5>), grid, block, lds, stream, insts, n_inst, order, qctr, mode,
+                               max_steps, nxp, hbm);
+        else
+            hipLaunchKernelGGL((k1_solve<false, AWCM_FLUX_WENO5>), grid, block, lds, stream, insts, n_inst, order, qctr, mode,
+                               max_steps, nxp, hbm);
+    } else {
+        if (hbm_masks)
+            hipLaunchKernelGGL((k1_solve<true, AWCM_FLUX_WENOZ>), grid, block, lds, stream, insts, n_inst, order, qctr, mode,
+                               max_steps, nxp, hbm);
+        else
+            hipLaunchKernelGGL((k1_solve<false, AWCM_FLUX_WENOZ>), grid, block, lds, stream, insts, n_inst, order, qctr, mode,
+                               max_steps, nxp, hbm);
+    }
+}
 
 // Work-queue counter reset as a kernel, so the batch launch can be captured
 // into a graph without a memset node.
@@ -56,7 +76,7 @@ int Batch1D::init(int n_, const awcm_instance_1d* params, const double* pt, int 
         hipFuncAttributes fa;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&cap, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
-            hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k1_solve<false>)) != hipSuccess)
+            hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k1_solve<false, AWCM_FLUX_WENOZ>)) != hipSuccess)
             return AWCM_EDEVICE;
         lds_cap = size_t(cap) > fa.sharedSizeBytes ? size_t(cap) - fa.sharedSizeBytes : 0;
     }
@@ -162,12 +182,8 @@ int Batch1D::launch(int mode, int max_steps, int first, int count) {
     uint8_t* hbm = nullptr;
     if (int st = mask_space(wgs, nxp, &hbm)) return st;
     if (hipMemsetAsync(d_qctr, 0, sizeof(int), stream) != hipSuccess) return AWCM_EDEVICE;
-    if (hbm)
-        hipLaunchKernelGGL(k1_solve<true>, dim3(wgs), dim3(threads), 0, stream, d_inst + first, count,
-                           d_order_for(first, count), d_qctr, mode, max_steps, nxp, hbm);
-    else
-        hipLaunchKernelGGL(k1_solve<false>, dim3(wgs), dim3(threads), lds_bytes(nxp), stream, d_inst + first, count,
-                           d_order_for(first, count), d_qctr, mode, max_steps, nxp, hbm);
+    launch_solve(hbm != nullptr, flux, dim3(wgs), dim3(threads), hbm ? 0 : lds_bytes(nxp), stream, d_inst + first, count,
+                 d_order_for(first, count), d_qctr, mode, max_steps, nxp, hbm);
     return hipGetLastError() == hipSuccess ? AWCM_OK : AWCM_EDEVICE;
 }
 
@@ -179,23 +195,20 @@ int Batch1D::launch_graph(int max_steps) {
     const int wgs = workgroups > 0 ? std::min(workgroups, n) : n;
     uint8_t* hbm = nullptr;
     if (int st = mask_space(wgs, nxp, &hbm)) return st;
-    if (!gexec || g_threads != threads || g_wgs != wgs || g_steps != max_steps || g_hbm != hbm) {
+    if (!gexec || g_threads != threads || g_wgs != wgs || g_steps != max_steps || g_hbm != hbm || g_flux != flux) {
         if (gexec) { hipGraphExecDestroy(gexec); gexec = nullptr; }
         if (graph) { hipGraphDestroy(graph); graph = nullptr; }
         if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return AWCM_EDEVICE;
         hipLaunchKernelGGL(k1_reset_counter, dim3(1), dim3(64), 0, stream, d_qctr);
-        if (hbm)
-            hipLaunchKernelGGL(k1_solve<true>, dim3(wgs), dim3(threads), 0, stream, d_inst, n, d_order, d_qctr, 1,
-                               max_steps, nxp, hbm);
-        else
-            hipLaunchKernelGGL(k1_solve<false>, dim3(wgs), dim3(threads), lds_bytes(nxp), stream, d_inst, n, d_order,
-                               d_qctr, 1, max_steps, nxp, hbm);
+        launch_solve(hbm != nullptr, flux, dim3(wgs), dim3(threads), hbm ? 0 : lds_bytes(nxp), stream, d_inst, n,
+                     d_order, d_qctr, 1, max_steps, nxp, hbm);
         if (hipStreamEndCapture(stream, &graph) != hipSuccess) return AWCM_EDEVICE;
         if (hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0) != hipSuccess) return AWCM_EDEVICE;
         g_threads = threads;
         g_wgs = wgs;
         g_steps = max_steps;
         g_hbm = hbm;
+        g_flux = flux;
     }
     return hipGraphLaunch(gexec, stream) == hipSuccess ? AWCM_OK : AWCM_EDEVICE;
 }
@@ -302,6 +315,12 @@ int awcm_batch_set_launch(awcm_batch* b, int threads, int workgroups) {
     if (!b || threads < 64 || threads > 1024 || threads % 64 || workgroups < 0) return AWCM_EINVAL;
     b->b1.threads = threads;
     b->b1.workgroups = workgroups;
+    return AWCM_OK;
+}
+
+int awcm_batch_set_flux(awcm_batch* b, int kind) {
+    if (!b || (kind != AWCM_FLUX_WENOZ && kind != AWCM_FLUX_WENO5)) return AWCM_EINVAL;
+    b->b1.flux = kind;
     return AWCM_OK;
 }
 

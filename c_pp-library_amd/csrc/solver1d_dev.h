@@ -382,6 +382,9 @@ __device__ double lf_alpha(const Ctx1& c, const uint8_t* mask, const uint8_t* lv
 
 // One SSP-RK3 stage over `mask`/`lvl` (main_1d_euler.cpp:97-245).  stage 0:
 // Q -= h dF; stage 1: Q = 0.75 Qc + 0.25 (Q - h dF); stage 2: Q = (Qc + 2 (Q - h dF))/3.
+// FLUX: the face fluxes, WENO-Z (kRecWenoZ, the C1 configuration) or WENO5-JS
+// (kRecWeno5, the shipped driver's ComputedF, WENO.h:176-184).
+template <int FLUX = kRecWenoZ>
 __device__ void rk_stage(const Ctx1& c, int stage, const uint8_t* mask, const uint8_t* lvl, double alpha, double dt) {
     const int Nx = c.g.Nx, NP = c.NP, maxlev = c.g.maxlev;
     for (int i = threadIdx.x; i < 3 * NP; i += blockDim.x) {
@@ -398,7 +401,7 @@ __device__ void rk_stage(const Ctx1& c, int stage, const uint8_t* mask, const ui
         double s[6][3];
         for (int p = 0; p < 6; p++) load3(c, c.Qt, f - 3 + p, s[p]);
         double fl[3];
-        face_flux1(c.gamma, s, alpha, fl);
+        face_flux1<FLUX>(c.gamma, s, alpha, fl);
         for (int k = 0; k < 3; k++) c.F[k * (Nx + 1) + f] = fl[k];
     }
     __syncthreads();
@@ -459,6 +462,7 @@ __device__ void tape_masks(const Ctx1& c, int n) {
 // One iteration of the driver loop (main_1d_euler.cpp:61-260).  With a tape
 // attached, also records what the adjoint needs (masks, levels, dt, alphas,
 // periodic checkpoints); the arithmetic is the same either way.
+template <int FLUX = kRecWenoZ>
 __device__ void step(const Ctx1& c, LoopState& st, const double* print_times) {
     const int rn = c.rec ? c.rec->tape_n + st.steps : 0;
     if (c.rec) tape_checkpoint(c, rn);
@@ -493,15 +497,15 @@ __device__ void step(const Ctx1& c, LoopState& st, const double* print_times) {
     bc(c);
     double a = lf_alpha(c, c.m[3], c.lv[3]);
     if (c.rec && threadIdx.x == 0) { c.rec->tape_s[4 * rn] = dt; c.rec->tape_s[4 * rn + 1] = a; }
-    rk_stage(c, 0, c.m[2], c.lv[2], a, dt);
+    rk_stage<FLUX>(c, 0, c.m[2], c.lv[2], a, dt);
     bc(c);
     a = lf_alpha(c, c.m[2], c.lv[2]);
     if (c.rec && threadIdx.x == 0) c.rec->tape_s[4 * rn + 2] = a;
-    rk_stage(c, 1, c.m[1], c.lv[1], a, dt);
+    rk_stage<FLUX>(c, 1, c.m[1], c.lv[1], a, dt);
     bc(c);
     a = lf_alpha(c, c.m[1], c.lv[1]);
     if (c.rec && threadIdx.x == 0) c.rec->tape_s[4 * rn + 3] = a;
-    rk_stage(c, 2, c.m[0], c.lv[0], a, dt);
+    rk_stage<FLUX>(c, 2, c.m[0], c.lv[0], a, dt);
     post(c);
     int n0 = 0;
     for (int i = threadIdx.x; i < c.g.Nx; i += blockDim.x) n0 += c.m[0][i + G];

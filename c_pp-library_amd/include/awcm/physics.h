@@ -166,8 +166,43 @@ AWCM_HD double wenoz_left(double ll, double l, double c, double r, double rr) {
 }
 AWCM_HD double wenoz_right(double ll, double l, double c, double r, double rr) { return wenoz_left(rr, r, c, l, ll); }
 
-// ---- Roe-characteristic LF-split WENO-Z numerical flux at one face -------------
+// ---- WENO5-JS (WENO.h:71-99: a_k = d_k / (1e-6 + b_k)^2, ideal weights 0.1/0.6/0.3) --
+AWCM_HD double wenojs_left(double ll, double l, double c, double r, double rr) {
+    const double t0 = c - 2 * r + rr, s0 = 3 * c - 4 * r + rr;
+    const double t1 = l - 2 * c + r, s1 = l - r;
+    const double t2 = ll - 2 * l + c, s2 = ll - 4 * l + 3 * c;
+    const double b0 = 13.0 / 12.0 * (t0 * t0) + 0.25 * (s0 * s0);
+    const double b1 = 13.0 / 12.0 * (t1 * t1) + 0.25 * (s1 * s1);
+    const double b2 = 13.0 / 12.0 * (t2 * t2) + 0.25 * (s2 * s2);
+    const double e0 = 1e-6 + b0, e1 = 1e-6 + b1, e2 = 1e-6 + b2;
+    const double a0 = 0.1 / (e0 * e0), a1 = 0.6 / (e1 * e1), a2 = 0.3 / (e2 * e2);
+    const double as = a0 + a1 + a2;
+    const double w0 = a0 / as, w1 = a1 / as, w2 = a2 / as;
+    double out = w0 * (11.0 / 6.0 * c + -7.0 / 6.0 * r + 1.0 / 3.0 * rr);
+    out += w1 * (1.0 / 3.0 * l + 5.0 / 6.0 * c + -1.0 / 6.0 * r);
+    out += w2 * (-1.0 / 6.0 * ll + 5.0 / 6.0 * l + 1.0 / 3.0 * c);
+    return out;
+}
+AWCM_HD double wenojs_right(double ll, double l, double c, double r, double rr) { return wenojs_left(rr, r, c, l, ll); }
+
+// reconstruction of the characteristic flux splitting: WENO-Z5 (WENOZ.h, the
+// 2-D drivers and the C1 configuration) or WENO5-JS (WENO.h, the shipped 1-D
+// driver's ComputedF)
+enum { kRecWenoZ = 0, kRecWeno5 = 1 };
+template <int REC>
+AWCM_HD double rec_left(double ll, double l, double c, double r, double rr) {
+    return REC == kRecWeno5 ? wenojs_left(ll, l, c, r, rr) : wenoz_left(ll, l, c, r, rr);
+}
+template <int REC>
+AWCM_HD double rec_right(double ll, double l, double c, double r, double rr) {
+    return REC == kRecWeno5 ? wenojs_right(ll, l, c, r, rr) : wenoz_right(ll, l, c, r, rr);
+}
+
+// ---- Roe-characteristic LF-split WENO numerical flux at one face ----------------
 // s[p] (p = 0..5) are the states lll, ll, l, r, rr, rrr around the face l|r.
+// WENOZ_EULER_SOLVER_1D::ComputeFlux (WENOZ.h:160-205) with REC = kRecWenoZ,
+// WENO_EULER_SOLVER_1D::ComputeFlux (WENO.h:129-174) with REC = kRecWeno5.
+template <int REC = kRecWenoZ>
 AWCM_HD void face_flux1(double g, const double (*s)[3], double alpha, double* flux) {
     double fp[6][3], fm[6][3];
     const Roe1 a = roe_avg1(g, s[2], s[3]);
@@ -186,8 +221,8 @@ AWCM_HD void face_flux1(double g, const double (*s)[3], double alpha, double* fl
     double sk[3];
 #pragma unroll
     for (int k = 0; k < 3; k++)
-        sk[k] = wenoz_right(fp[0][k], fp[1][k], fp[2][k], fp[3][k], fp[4][k]) +
-                wenoz_left(fm[1][k], fm[2][k], fm[3][k], fm[4][k], fm[5][k]);
+        sk[k] = rec_right<REC>(fp[0][k], fp[1][k], fp[2][k], fp[3][k], fp[4][k]) +
+                rec_left<REC>(fm[1][k], fm[2][k], fm[3][k], fm[4][k], fm[5][k]);
     const double r1[3] = {1, a.u - a.c, a.H - a.u * a.c};
     const double r2[3] = {1, a.u, 0.5 * (a.u * a.u)};
     const double r3[3] = {1, a.u + a.c, a.H + a.u * a.c};
@@ -195,7 +230,9 @@ AWCM_HD void face_flux1(double g, const double (*s)[3], double alpha, double* fl
     for (int k = 0; k < 3; k++) flux[k] = sk[0] * r1[k] + sk[1] * r2[k] + sk[2] * r3[k];
 }
 
-template <int DIR>
+// WENOZ_EULER_SOLVER_2D::ComputeFluxX/Y (WENOZ.h:236-320) with REC = kRecWenoZ,
+// WENO_EULER_SOLVER_2D::ComputeFluxX/Y (WENO.h:216-311) with REC = kRecWeno5.
+template <int DIR, int REC = kRecWenoZ>
 AWCM_HD void face_flux2(double g, const double (*s)[4], double alpha, double* flux) {
     double fp[6][4], fm[6][4];
     const Roe2 a = roe_avg2(g, s[2], s[3]);
@@ -214,8 +251,8 @@ AWCM_HD void face_flux2(double g, const double (*s)[4], double alpha, double* fl
     double sk[4];
 #pragma unroll
     for (int k = 0; k < 4; k++)
-        sk[k] = wenoz_right(fp[0][k], fp[1][k], fp[2][k], fp[3][k], fp[4][k]) +
-                wenoz_left(fm[1][k], fm[2][k], fm[3][k], fm[4][k], fm[5][k]);
+        sk[k] = rec_right<REC>(fp[0][k], fp[1][k], fp[2][k], fp[3][k], fp[4][k]) +
+                rec_left<REC>(fm[1][k], fm[2][k], fm[3][k], fm[4][k], fm[5][k]);
     const double u = a.u, v = a.v, H = a.H, c = a.c;
     double r1[4], r2[4], r3[4], r4[4];
     if (DIR == 0) {

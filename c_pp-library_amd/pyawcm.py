@@ -14,6 +14,8 @@ LIB_PATH = os.path.join(HERE, "libawcm_hip.so")
 AWCM_OK = 0
 ERRORS = {-1: "EINVAL", -2: "EDEVICE", -3: "ENOMEM", -4: "ENONFINITE", -5: "ERANGE", -6: "ESTATE", -7: "EDISABLED"}
 BC_TRANSMISSIVE = 0
+FLUX_WENOZ = 0   # WENOZ_EULER_SOLVER_1D::ComputeFlux pair (the default)
+FLUX_WENO5 = 1   # WENO_EULER_SOLVER_1D::ComputedF, the shipped 1-D driver's
 BC_DOUBLE_MACH = 1
 
 
@@ -56,6 +58,8 @@ def lib():
         "awcm_prepare_euler": [V, D],
         "awcm_interpolate_all_grid": [V, _u8p],
         "awcm_upload_scalar": [V, _dp],
+        "awcm_set_flux": [V, I],
+        "awcm_batch_set_flux": [V, I],
         "awcm_download_scalar": [V, _dp],
         "awcm_prepare": [V, D],
         "awcm_post_process": [V],
@@ -167,6 +171,10 @@ class Solver:
 
     def prepare(self, tol):
         _check(lib().awcm_prepare_euler(self.h, tol), "awcm_prepare_euler")
+
+    def set_flux(self, kind):
+        """1-D stage face fluxes: FLUX_WENOZ or FLUX_WENO5 (awcm_set_flux)."""
+        _check(lib().awcm_set_flux(self.h, kind), "awcm_set_flux")
 
     # ---- the scalar path (2-D): AWCM_2D::Prepare / PostProcess, WAVELET_2D::Interpolate
     def upload_scalar(self, q):
@@ -357,6 +365,10 @@ class Batch1D:
 
     def __del__(self):
         self.close()
+
+    def set_flux(self, kind):
+        """Stage face fluxes of every instance: FLUX_WENOZ or FLUX_WENO5."""
+        _check(lib().awcm_batch_set_flux(self.h, kind), "awcm_batch_set_flux")
 
     def upload_q(self, idx, q):
         q = np.ascontiguousarray(q, dtype=np.float64)

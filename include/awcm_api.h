@@ -760,66 +760,19 @@ inline double combine_left(double a0, double a1, double a2, double ll, double l,
 }
 // WENO5-JS (WENO.h:71-99): a_k = d_k / (1e-6 + b_k)^2, d = (0.1, 0.6, 0.3)
 inline double wenojs_left(double ll, double l, double c, double r, double rr) {
-    const Beta3 s = beta3(ll, l, c, r, rr);
-    const double eps = 1e-6;
-    return combine_left(0.1 / SQR(eps + s.b0), 0.6 / SQR(eps + s.b1), 0.3 / SQR(eps + s.b2), ll, l, c, r, rr);
+    return awcm::wenojs_left(ll, l, c, r, rr);
 }
 inline double wenojs_right(double ll, double l, double c, double r, double rr) { return wenojs_left(rr, r, c, l, ll); }
 
 // Roe-characteristic Lax-Friedrichs split flux at the face s[2]|s[3] with
-// WENO5-JS reconstructions (WENO.h:129-174, 216-311): the same projection,
-// splitting and right eigenvectors as awcm::face_flux1/face_flux2.
+// WENO5-JS reconstructions (WENO.h:129-174, 216-311): the device's
+// awcm::face_flux1/face_flux2 with the WENO5-JS reconstruction.
 inline void face_flux1_js(double g, const double (*s)[3], double alpha, double* flux) {
-    double fp[6][3], fm[6][3];
-    const awcm::Roe1 a = awcm::roe_avg1(g, s[2], s[3]);
-    for (int p = 0; p < 6; p++) {
-        double fu[3], RQ[3], Rf[3];
-        awcm::flux_f1(g, s[p], fu);
-        awcm::roe_proj1(a, s[p], RQ);
-        awcm::roe_proj1(a, fu, Rf);
-        for (int k = 0; k < 3; k++) {
-            fp[p][k] = 0.5 * (Rf[k] + alpha * RQ[k]);
-            fm[p][k] = 0.5 * (Rf[k] - alpha * RQ[k]);
-        }
-    }
-    double sk[3];
-    for (int k = 0; k < 3; k++)
-        sk[k] = wenojs_right(fp[0][k], fp[1][k], fp[2][k], fp[3][k], fp[4][k]) +
-                wenojs_left(fm[1][k], fm[2][k], fm[3][k], fm[4][k], fm[5][k]);
-    const double r1[3] = {1, a.u - a.c, a.H - a.u * a.c};
-    const double r2[3] = {1, a.u, 0.5 * (a.u * a.u)};
-    const double r3[3] = {1, a.u + a.c, a.H + a.u * a.c};
-    for (int k = 0; k < 3; k++) flux[k] = sk[0] * r1[k] + sk[1] * r2[k] + sk[2] * r3[k];
+    awcm::face_flux1<awcm::kRecWeno5>(g, s, alpha, flux);
 }
 template <int DIR>
 inline void face_flux2_js(double g, const double (*s)[4], double alpha, double* flux) {
-    double fp[6][4], fm[6][4];
-    const awcm::Roe2 a = awcm::roe_avg2(g, s[2], s[3]);
-    for (int p = 0; p < 6; p++) {
-        double fu[4], RQ[4], Rf[4];
-        awcm::flux_f2<DIR>(g, s[p], fu);
-        awcm::roe_proj2<DIR>(a, s[p], RQ);
-        awcm::roe_proj2<DIR>(a, fu, Rf);
-        for (int k = 0; k < 4; k++) {
-            fp[p][k] = 0.5 * (Rf[k] + alpha * RQ[k]);
-            fm[p][k] = 0.5 * (Rf[k] - alpha * RQ[k]);
-        }
-    }
-    double sk[4];
-    for (int k = 0; k < 4; k++)
-        sk[k] = wenojs_right(fp[0][k], fp[1][k], fp[2][k], fp[3][k], fp[4][k]) +
-                wenojs_left(fm[1][k], fm[2][k], fm[3][k], fm[4][k], fm[5][k]);
-    const double u = a.u, v = a.v, H = a.H, c = a.c;
-    // right eigenvectors: R[n][k] is component n of column k
-    double R[4][4];
-    if (DIR == 0) {
-        const double m[4][4] = {{1, 0, 1, 1}, {u - c, 0, u, u + c}, {v, 1, v, v}, {H - u * c, v, 0.5 * (u * u + v * v), H + u * c}};
-        std::memcpy(R, m, sizeof R);
-    } else {
-        const double m[4][4] = {{1, 0, 1, 1}, {u, 1, u, u}, {v - c, 0, v, v + c}, {H - v * c, u, 0.5 * (u * u + v * v), H + v * c}};
-        std::memcpy(R, m, sizeof R);
-    }
-    for (int n = 0; n < 4; n++) flux[n] = sk[0] * R[n][0] + sk[1] * R[n][1] + sk[2] * R[n][2] + sk[3] * R[n][3];
+    awcm::face_flux2<DIR, awcm::kRecWeno5>(g, s, alpha, flux);
 }
 
 template <class V, int N>

@@ -99,6 +99,18 @@ int awcm_prepare_euler(awcm_solver* h, double tol);
  * (AWCM_ESTATE on a 1-D handle).  Synchronous. */
 int awcm_interpolate_all_grid(awcm_solver* h, const uint8_t* mask_from);
 
+/* ---- the 1-D stage flux ---------------------------------------------------
+ * The finest-level dF of the 1-D stages is the difference of two face fluxes.
+ * AWCM_FLUX_WENOZ: WENOZ_EULER_SOLVER_1D::ComputeFlux (WENOZ.h:160-205), the
+ * default.  AWCM_FLUX_WENO5: WENO_EULER_SOLVER_1D::ComputedF (WENO.h:129-184),
+ * the WENO5-JS flux pair the shipped driver calls (main_1d_euler.cpp:109, 163,
+ * 218).  awcm_set_flux applies to a 1-D handle (AWCM_ESTATE on a 2-D one,
+ * whose drivers use WENO-Z), awcm_batch_set_flux to every instance of a batch.
+ * The adjoint differentiates the WENO-Z pair: awcm_adjoint_record returns
+ * AWCM_ESTATE on a WENO5-JS handle. */
+enum { AWCM_FLUX_WENOZ = 0, AWCM_FLUX_WENO5 = 1 };
+int awcm_set_flux(awcm_solver* h, int kind);
+
 /* ---- the scalar path (2-D; AWCM_ESTATE on a 1-D handle) --------------------
  * AWCM_2D::Prepare / PostProcess and WAVELET_2D::Interpolate act on a scalar
  * field (FIELD_UNIFORM_2D<double>, ghost width 3, (Nx+6) x (Ny+6) doubles, i
@@ -263,6 +275,10 @@ int awcm_batch_set_launch(awcm_batch* b, int threads, int workgroups);
  * The launch (work-counter reset + solve kernel) is captured once into a HIP
  * graph per launch shape and step budget and replayed. */
 int awcm_batch_run(awcm_batch* b, int max_steps);
+
+/* Stage face fluxes of every instance (AWCM_FLUX_WENOZ or AWCM_FLUX_WENO5, as
+ * awcm_set_flux); applies from the next run. */
+int awcm_batch_set_flux(awcm_batch* b, int kind);
 /* Project autotuner for the batched launch: times every candidate shape
  * (workgroup size 256 / 512 / 1024 x one workgroup per instance or 1x / 2x
  * the CU count of persistent workers) on probe_steps steps of every instance

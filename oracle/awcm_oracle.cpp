@@ -36,6 +36,7 @@ constexpr int GW = 3;  // ghost width of every ghosted field (AWCM_1D.h:23-27)
 
 // MACROS.h:3,6 -- ternary ABS / MAX (NaN and -0.0 semantics matter)
 inline double ABSd(double x) { return x < 0 ? -x : x; }
+inline double SQRd(double x) { return x * x; }  // SQR (MACROS.h)
 inline double MAXd(double a, double b) { return a > b ? a : b; }
 
 // ---------------------------------------------------------------------------
@@ -191,8 +192,29 @@ inline double wenoz_left(double ll, double l, double c, double r, double rr) {
 }
 inline double wenoz_right(double ll, double l, double c, double r, double rr) { return wenoz_left(rr, r, c, l, ll); }
 
-// WENOZ_EULER_SOLVER_1D::ComputeFlux (WENOZ.h:160-205); s[0..5] = lll..rrr
-void flux1(double g, const double* const* s, double alpha, double* flux) {
+// WENO5::ReconstructLeft (WENO.h:71-99): a_k = d_k / SQR(eps + b_k), eps = 1e-6;
+// Right(a..e) = Left(e..a) (WENO.h:101-104)
+const double JS_EPS = 1e-6;
+inline double wenojs_left(double ll, double l, double c, double r, double rr) {
+    double d0 = 0.1, d1 = 0.6, d2 = 0.3;
+    double b0 = 13.0 / 12.0 * SQRd(c - 2 * r + rr) + 0.25 * SQRd(3 * c - 4 * r + rr);
+    double b1 = 13.0 / 12.0 * SQRd(l - 2 * c + r) + 0.25 * SQRd(l - r);
+    double b2 = 13.0 / 12.0 * SQRd(ll - 2 * l + c) + 0.25 * SQRd(ll - 4 * l + 3 * c);
+    double a0 = d0 / SQRd(JS_EPS + b0);
+    double a1 = d1 / SQRd(JS_EPS + b1);
+    double a2 = d2 / SQRd(JS_EPS + b2);
+    double as = a0 + a1 + a2;
+    double w0 = a0 / as, w1 = a1 / as, w2 = a2 / as;
+    double left = w0 * (CM1[0] * c + CM1[1] * r + CM1[2] * rr);
+    left += w1 * (C0[0] * l + C0[1] * c + C0[2] * r);
+    left += w2 * (C1[0] * ll + C1[1] * l + C1[2] * c);
+    return left;
+}
+inline double wenojs_right(double ll, double l, double c, double r, double rr) { return wenojs_left(rr, r, c, l, ll); }
+
+// WENOZ_EULER_SOLVER_1D::ComputeFlux (WENOZ.h:160-205), or with js
+// WENO_EULER_SOLVER_1D::ComputeFlux (WENO.h:129-174); s[0..5] = lll..rrr
+void flux1(double g, const double* const* s, double alpha, double* flux, bool js = false) {
     double fu[6][3], RQ[6][3], Rfu[6][3], fp[6][3], fm[6][3];
     for (int p = 0; p < 6; p++) Fu1(g, s[p], fu[p]);
     double u, H, c;
@@ -209,8 +231,10 @@ void flux1(double g, const double* const* s, double alpha, double* flux) {
         }
     double sk[3];
     for (int k = 0; k < 3; k++) {
-        double lr = wenoz_right(fp[0][k], fp[1][k], fp[2][k], fp[3][k], fp[4][k]);
-        double rl = wenoz_left(fm[1][k], fm[2][k], fm[3][k], fm[4][k], fm[5][k]);
+        double lr = js ? wenojs_right(fp[0][k], fp[1][k], fp[2][k], fp[3][k], fp[4][k])
+                       : wenoz_right(fp[0][k], fp[1][k], fp[2][k], fp[3][k], fp[4][k]);
+        double rl = js ? wenojs_left(fm[1][k], fm[2][k], fm[3][k], fm[4][k], fm[5][k])
+                       : wenoz_left(fm[1][k], fm[2][k], fm[3][k], fm[4][k], fm[5][k]);
         sk[k] = lr + rl;
     }
     for (int k = 0; k < 3; k++) flux[k] = sk[0] * r1[k] + sk[1] * r2[k] + sk[2] * r3[k];
@@ -272,6 +296,7 @@ struct Solver1D {
     std::vector<uint8_t> mt, m[4];
     std::vector<int> lv[4];
     long oob = 0;                  // stencil reads outside the ghost band
+    bool flux_js = false;          // WENO5-JS face fluxes (the shipped driver) in place of WENO-Z
 
     Solver1D(int L_, int Nxc_, double g, double dx_) : L(L_), Nxc(Nxc_), gamma(g), dx(dx_) {
         // MRA_GRID_INT_1D::Initialize (MRA_GRID_UNIFORM_1D.h:70-98)
@@ -471,8 +496,9 @@ struct Solver1D {
         }
         return a;
     }
-    // dF at point i of one stage (main_1d_euler.cpp:99-134); WENO-Z flux pair in
-    // place of WENO_EULER_SOLVER_1D::ComputedF (WENO.h:176-184).
+    // dF at point i of one stage (main_1d_euler.cpp:99-134): the WENO-Z flux
+    // pair, or with flux_js the shipped WENO_EULER_SOLVER_1D::ComputedF
+    // (WENO.h:176-184), the same pair of WENO5-JS fluxes.
     void dF(int i, int lev, double a, double* out) {
         const int d = dxl[lev];
         auto QT = [&](int ii) -> const double* {
@@ -483,8 +509,8 @@ struct Solver1D {
             const double* s0[6] = {QT(i - 3), QT(i - 2), QT(i - 1), QT(i), QT(i + 1), QT(i + 2)};
             const double* s1[6] = {QT(i - 2), QT(i - 1), QT(i), QT(i + 1), QT(i + 2), QT(i + 3)};
             double Fl[3], Fr[3];
-            flux1(gamma, s0, a, Fl);
-            flux1(gamma, s1, a, Fr);
+            flux1(gamma, s0, a, Fl, flux_js);
+            flux1(gamma, s1, a, Fr, flux_js);
             for (int k = 0; k < 3; k++) out[k] = Fr[k] - Fl[k];
             return;
         }
@@ -1361,6 +1387,8 @@ void orc1_get_masks(void* h, uint8_t* masks, int32_t* levels) {
 }
 void orc1_set_mask0(void* h, const uint8_t* m0) { auto* s = static_cast<Solver1D*>(h); std::memcpy(s->m[0].data(), m0, s->m[0].size()); }
 void orc1_prepare(void* h, double tol) { static_cast<Solver1D*>(h)->prepare(tol); }
+// face fluxes of the stages: 0 WENO-Z pair, 1 WENO5-JS pair (the shipped driver)
+void orc1_set_flux(void* h, int kind) { static_cast<Solver1D*>(h)->flux_js = kind == 1; }
 double orc1_step(void* h, double tol, double cfl, const double* pt, int npt, int* cnt, double* t) {
     return static_cast<Solver1D*>(h)->step(tol, cfl, pt, npt, cnt, t);
 }
@@ -1388,6 +1416,11 @@ void orc1_replay(void* h, int first, int count) {
 void orc_flux1(double gamma, const double* states, double alpha, double* flux) {
     const double* s[6] = {states, states + 3, states + 6, states + 9, states + 12, states + 15};
     flux1(gamma, s, alpha, flux);
+}
+// WENO_EULER_SOLVER_1D::ComputeFlux on 6 x 3 doubles
+void orc_flux1_js(double gamma, const double* states, double alpha, double* flux) {
+    const double* s[6] = {states, states + 3, states + 6, states + 9, states + 12, states + 15};
+    flux1(gamma, s, alpha, flux, true);
 }
 // WENOZ_EULER_SOLVER_2D::ComputeFluxX (dir 0) / ComputeFluxY (dir 1) on 6 x 4 doubles
 void orc_flux2(double gamma, int dir, const double* states, double alpha, double* flux) {

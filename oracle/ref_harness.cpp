@@ -23,6 +23,7 @@ void FIELD_UNIFORM_1D<bool>::AssignAllValues(const bool& v) {
     for (int i = 0; i < arr.i_res; i++) arr.val[i] = v;
 }
 #include "WENOZ.h"
+#include "WENO.h"
 #include "Problems/1D/PROBLEM_SHOCKTUBE.h"
 #include "Problems/1D/PROBLEM_ENTROPY_INTERACTING.h"
 #include "Wavelet/AWCM_1D.h"
@@ -41,6 +42,8 @@ struct Ref1D {
     FIELD_UNIFORM_1D<VECTOR_3D<double>> Q, Q_temp, Q_copy;
     BOUNDARY_MANAGER* bcm;
     WENOZ_EULER_SOLVER_1D* wenoz;
+    WENO_EULER_SOLVER_1D* weno;
+    bool js = false;  // the shipped driver's WENO5-JS ComputedF in place of the WENO-Z pair
     IDEAL_GAS_1D* EOS;
     double gamma, dx;
     int max_level;
@@ -50,15 +53,20 @@ struct Ref1D {
         Q.Initialize(0, mra.Nx, 0.0, 1.0, 3);
         bcm = new BOUNDARY_MANAGER(Q);
         wenoz = new WENOZ_EULER_SOLVER_1D(gamma);
+        weno = new WENO_EULER_SOLVER_1D(gamma);
         EOS = new IDEAL_GAS_1D(gamma);
         max_level = L - 1;
     }
-    ~Ref1D() { delete awcm; delete bcm; delete wenoz; delete EOS; }
+    ~Ref1D() { delete awcm; delete bcm; delete wenoz; delete weno; delete EOS; }
 
     void dF_at(int i, int lev, double alpha_x, VECTOR_3D<double>& dF) {
         const int dx_ = mra.dx[lev];
         const GRID_UNIFORM_1D& grid = Q.base_grid;
-        if (lev == max_level) {
+        if (lev == max_level && js) {
+            // main_1d_euler.cpp:109
+            weno->ComputedF(Q_temp(i - 3), Q_temp(i - 2), Q_temp(i - 1), Q_temp(i), Q_temp(i + 1), Q_temp(i + 2),
+                            Q_temp(i + 3), alpha_x, dF);
+        } else if (lev == max_level) {
             VECTOR_3D<double> F_l, F_r;
             wenoz->ComputeFlux(Q_temp(i - 3), Q_temp(i - 2), Q_temp(i - 1), Q_temp(i), Q_temp(i + 1), Q_temp(i + 2), alpha_x, F_l);
             wenoz->ComputeFlux(Q_temp(i - 2), Q_temp(i - 1), Q_temp(i), Q_temp(i + 1), Q_temp(i + 2), Q_temp(i + 3), alpha_x, F_r);
@@ -345,11 +353,19 @@ void ref1_set_mask0(void* h, const uint8_t* m0) {
     for (int k = 0; k < s->awcm->mask0.arr.i_res; k++) s->awcm->mask0.arr.val[k] = m0[k] != 0;
 }
 void ref1_prepare(void* h, double tol) { auto* s = static_cast<Ref1D*>(h); s->awcm->PrepareEuler(s->Q, tol); }
+void ref1_set_flux(void* h, int kind) { static_cast<Ref1D*>(h)->js = kind == 1; }
 double ref1_step(void* h, double tol, double cfl, const double* pt, int npt, int* cnt, double* t) {
     return static_cast<Ref1D*>(h)->step(tol, cfl, pt, npt, cnt, t);
 }
 void ref_flux1(double gamma, const double* st, double alpha, double* flux) {
     WENOZ_EULER_SOLVER_1D s(gamma);
+    VECTOR_3D<double> v[6], f;
+    for (int p = 0; p < 6; p++) v[p] = VECTOR_3D<double>(st[p * 3], st[p * 3 + 1], st[p * 3 + 2]);
+    s.ComputeFlux(v[0], v[1], v[2], v[3], v[4], v[5], alpha, f);
+    for (int c = 0; c < 3; c++) flux[c] = f[c];
+}
+void ref_flux1_js(double gamma, const double* st, double alpha, double* flux) {
+    WENO_EULER_SOLVER_1D s(gamma);
     VECTOR_3D<double> v[6], f;
     for (int p = 0; p < 6; p++) v[p] = VECTOR_3D<double>(st[p * 3], st[p * 3 + 1], st[p * 3 + 2]);
     s.ComputeFlux(v[0], v[1], v[2], v[3], v[4], v[5], alpha, f);

@@ -44,6 +44,11 @@ def kats(rng):
         st = np.ascontiguousarray(s1[k]).ravel()
         R.ref_flux1(1.4, O.dptr(st), a1[k], O.dptr(f1[k]))
     out.update(flux1_states=s1, flux1_alpha=a1, flux1_out=f1)
+    f1js = np.zeros((n, 3))
+    for k in range(n):
+        st = np.ascontiguousarray(s1[k]).ravel()
+        R.ref_flux1_js(1.4, O.dptr(st), a1[k], O.dptr(f1js[k]))
+    out.update(flux1_js_out=f1js)
     s2 = np.zeros((n, 6, 4))
     a2 = rng.uniform(0.5, 3.0, n)
     for k in range(n):

@@ -37,15 +37,23 @@ struct Emu {
 // with masks_in_hbm, k1_solve<true> keeps the masks in a heap slice instead
 // of the emulated LDS (the path of instances too long for one workgroup's LDS)
 bool masks_in_hbm = false;
+int stage_flux = kRecWenoZ;  // face fluxes of the stages (kRecWenoZ / kRecWeno5)
 
-void launch_solve(Emu& e, int mode, int max_steps) {
+template <int FLUX>
+void launch_solve_f(Emu& e, int mode, int max_steps) {
     int order = 0, qctr = 0;
     if (masks_in_hbm) {
         std::vector<uint8_t> hbm(size_t(11) * e.nxp(), 0xA5);
-        emu_launch(EMU_THREADS, [&] { k1_solve<true>(&e.in, 1, &order, &qctr, mode, max_steps, e.nxp(), hbm.data()); });
+        emu_launch(EMU_THREADS,
+                   [&] { k1_solve<true, FLUX>(&e.in, 1, &order, &qctr, mode, max_steps, e.nxp(), hbm.data()); });
     } else {
-        emu_launch(EMU_THREADS, [&] { k1_solve<false>(&e.in, 1, &order, &qctr, mode, max_steps, e.nxp(), nullptr); });
+        emu_launch(EMU_THREADS,
+                   [&] { k1_solve<false, FLUX>(&e.in, 1, &order, &qctr, mode, max_steps, e.nxp(), nullptr); });
     }
+}
+void launch_solve(Emu& e, int mode, int max_steps) {
+    if (stage_flux == kRecWeno5) launch_solve_f<kRecWeno5>(e, mode, max_steps);
+    else launch_solve_f<kRecWenoZ>(e, mode, max_steps);
 }
 
 }  // namespace
@@ -53,6 +61,7 @@ void launch_solve(Emu& e, int mode, int max_steps) {
 extern "C" {
 
 void emu_set_masks_in_hbm(int on) { masks_in_hbm = on != 0; }
+void emu_set_flux(int kind) { stage_flux = kind == 1 ? kRecWeno5 : kRecWenoZ; }
 
 void* emu_create(int L, int nxc, double gamma, double dx, const double* pt, int npt) {
     auto* e = new Emu();

@@ -1,7 +1,8 @@
 // Driver of the 1-D kernels' emulation with a workgroup of real threads
 // (tests/native/emu1d.cpp built against tests/native/emu_mt): a short Sod run
 // recorded on the adaptive grid (k1_solve), its frozen replay and the reverse
-// sweep (k1_adjoint, two recompute segment lengths).  Built with
+// sweep (k1_adjoint, two recompute segment lengths); with a third argument 1,
+// the recording alone with the WENO5-JS stage fluxes.  Built with
 // ThreadSanitizer, any access of two threads to the same memory between
 // barriers, without atomics, is reported.  The run also checks that the
 // replay reproduces the recording bit for bit and that the two segment
@@ -22,11 +23,14 @@ int emu_record(void* h, double tol, double cfl, double t_final, int max_steps, i
 void emu_replay(void* h);
 void emu_adjoint(void* h, const double* gf, int seg_steps, double* g0);
 int emu_status(void* h);
+void emu_set_flux(int kind);
 }
 
 int main(int argc, char** argv) {
     const int L = argc > 1 ? std::atoi(argv[1]) : 4;
     const int steps = argc > 2 ? std::atoi(argv[2]) : 8;
+    const int flux = argc > 3 ? std::atoi(argv[3]) : 0;  // 1: WENO5-JS stage fluxes (no adjoint)
+    emu_set_flux(flux);
     const int nxc = 17;
     const int Nx = (nxc - 1) * (1 << (L - 1)) + 1, NP = Nx + 6;
     const double dx = 1.0 / (Nx - 1), g = 1.4;
@@ -44,6 +48,15 @@ int main(int argc, char** argv) {
     void* e = emu_create(L, nxc, g, dx, pt, 2);
     emu_set_q(e, q.data());
     const int n = emu_record(e, 1e-4, 0.5, 0.2, steps, 4);
+    if (flux) {
+        // the replay and the adjoint are the WENO-Z pair's: the recording alone
+        const int st = emu_status(e);
+        emu_destroy(e);
+        std::printf("steps %d status %d\n", n, st);
+        if (n != steps || st != 0) return 1;
+        std::printf("ok\n");
+        return 0;
+    }
     std::vector<double> rec(3 * NP), rep(3 * NP), g8(3 * NP), g4(3 * NP), gf(3 * NP);
     emu_get_q(e, rec.data());
     emu_set_q(e, q.data());

@@ -53,6 +53,8 @@ def load(prefix="orc"):
         f"{p}1_get_masks": (None, [V, _u8p, _i32p]),
         f"{p}1_set_mask0": (None, [V, _u8p]),
         f"{p}1_prepare": (None, [V, C.c_double]),
+        f"{p}1_set_flux": (None, [V, C.c_int]),
+        f"{p}_flux1_js": (None, [C.c_double, _dp, C.c_double, _dp]),
         f"{p}1_step": (C.c_double, [V, C.c_double, C.c_double, _dp, C.c_int, _ip, _dp]),
         f"{p}2_nx": (C.c_int, [V]),
         f"{p}2_ny": (C.c_int, [V]),
@@ -183,6 +185,11 @@ class Solver1D:
 
     def prepare(self, tol):
         getattr(self.lib, f"{self.p}1_prepare")(self.h, tol)
+
+    def set_flux(self, kind):
+        """Face fluxes of the stages: 0 the WENO-Z pair, 1 the shipped driver's
+        WENO5-JS ComputedF (main_1d_euler.cpp:109)."""
+        getattr(self.lib, f"{self.p}1_set_flux")(self.h, kind)
 
     def step(self, tol, cfl, pts):
         pts = np.ascontiguousarray(pts, dtype=np.float64)

@@ -44,11 +44,14 @@ def exes(tmp_path_factory):
     return out, str(supp)
 
 
-@pytest.mark.parametrize("threads,levels", [(64, 5), (256, 4)])
-def test_1d_kernels_race_free(exes, threads, levels):
+@pytest.mark.parametrize("threads,levels,flux", [(64, 5, 0), (256, 4, 0), (64, 4, 1)])
+def test_1d_kernels_race_free(exes, threads, levels, flux):
+    """flux 1: the recording alone with the WENO5-JS stage fluxes (the
+    k1_solve instantiation of the shipped driver's flux)."""
     paths, supp = exes
     env = dict(os.environ, TSAN_OPTIONS=f"suppressions={supp} halt_on_error=0")
-    r = subprocess.run([paths[threads], str(levels), "8"], capture_output=True, text=True, timeout=600, env=env)
+    r = subprocess.run([paths[threads], str(levels), "8", str(flux)], capture_output=True, text=True, timeout=600,
+                       env=env)
     log = r.stdout + r.stderr
     assert "WARNING: ThreadSanitizer" not in log, log[-4000:]
     assert r.returncode == 0 and r.stdout.strip().endswith("ok"), log[-2000:]

@@ -32,6 +32,15 @@ def assert_same(qg, qo, mg, mo, lg, lo, what):
         f"{what}: fields not bit-identical (max abs diff {np.abs(qg - qo).max()})"
 
 
+def assert_within(qg, qo, mg, mo, lg, lo, what, tol=1e-12):
+    """Masks and levels bit-exact, fields within the stated FP64 tolerance
+    (relative L1 and relative max error <= tol)."""
+    assert np.array_equal(mg, mo), f"{what}: masks differ at {np.argwhere(mg != mo)[:5].tolist()}"
+    assert np.array_equal(lg, lo), f"{what}: levels differ at {np.argwhere(lg != lo)[:5].tolist()}"
+    assert rel_l1(qg, qo) <= tol, f"{what}: rel L1 {rel_l1(qg, qo)}"
+    assert np.abs(qg - qo).max() <= tol * max(1.0, np.abs(qo).max()), f"{what}: max error {np.abs(qg - qo).max()}"
+
+
 def sod_setup(L=7, nxc=17):
     nx = (nxc - 1) * 2 ** (L - 1) + 1
     q, dx, _ = O.ic1(0, nx, 0.0, 1.0, 0.5, O.SOD_L, O.SOD_R)
@@ -89,6 +98,104 @@ def test_1d_sod_full_run_matches_oracle():
     mg, lg = g.masks()
     mo, lo = o.masks()
     assert_same(g.download_q(), o.get_q(), mg, mo, lg, lo, "1-D Sod t=0.2")
+
+
+def test_1d_weno5_flux_matches_oracle():
+    """The shipped driver's stage flux (WENO_EULER_SOLVER_1D::ComputedF,
+    main_1d_euler.cpp:109) on the device, awcm_set_flux(AWCM_FLUX_WENO5): a
+    full Sod run (7 levels, 1025 points) and 200 steps in the driver's own
+    configuration (SHOCKTUBE on [0, 10], 11 levels, 16 coarse points, 15361
+    points: masks in HBM).  Masks, levels, step counts and t bit-identical to
+    the oracle's WENO5-JS run; fields within 1e-12.  (Step 406 of the Sod run
+    divides E + p = 0x1.6666666666663p-2 by rho = 0x1.ffffffffffffbp-4, a
+    quotient 0.4999999999999999 ulp from the correctly rounded
+    0x1.6666666666667p+1; the device's FP64 division returns the other
+    neighbour there, and the field then differs from the oracle's by ~1e-14.)
+    The adjoint differentiates the WENO-Z pair and refuses a WENO5-JS handle."""
+    P = _pyawcm()
+    q, dx = sod_setup()
+    pts = O.print_times(0.2, 41)
+    g = P.Solver(1, 7, 17, 1.4, dx)
+    g.set_flux(P.FLUX_WENO5)
+    g.set_schedule(pts)
+    g.upload_q(q)
+    steps, _ = g.run(1e-4, 0.5, 0.2, 100000)
+    o = O.Solver1D("orc", 7, 17, 1.4, dx)
+    o.set_flux(1)
+    o.set_q(q)
+    n = 0
+    while o.t[0] < 0.2:
+        o.step(1e-4, 0.5, pts)
+        n += 1
+    assert steps == n and g.t == o.t[0] == 0.2
+    mg, lg = g.masks()
+    mo, lo = o.masks()
+    assert_within(g.download_q(), o.get_q(), mg, mo, lg, lo, "1-D Sod WENO5-JS t=0.2")
+    with pytest.raises(P.AwcmError) as e:
+        g.adjoint_record(1e-4, 0.5, 0.3, 4)
+    assert e.value.code == -6  # AWCM_ESTATE
+    # main_1d_euler.cpp's configuration (:14-17)
+    L, nxc = 11, 16
+    nx = (nxc - 1) * 2 ** (L - 1) + 1
+    q, dx, _ = O.ic1(0, nx, 0.0, 10.0, 5.0, O.SOD_L, O.SOD_R)
+    pts = O.print_times(2.0, 41)
+    g = P.Solver(1, L, nxc, 1.4, dx)
+    g.set_flux(P.FLUX_WENO5)
+    g.set_schedule(pts)
+    g.upload_q(q)
+    o = O.Solver1D("orc", L, nxc, 1.4, dx)
+    o.set_flux(1)
+    o.set_q(q)
+    for chunk in range(4):
+        steps, _ = g.run(1e-6, 0.5, 2.0, 50)
+        for _ in range(50):
+            o.step(1e-6, 0.5, pts)
+        assert steps == 50 and g.t == o.t[0], f"chunk {chunk}"
+        mg, lg = g.masks()
+        mo, lo = o.masks()
+        assert_within(g.download_q(), o.get_q(), mg, mo, lg, lo, f"driver configuration, {50 * (chunk + 1)} steps")
+
+
+def test_batch_weno5_flux_matches_oracle():
+    """awcm_batch_set_flux(AWCM_FLUX_WENO5) on a batch of six Riemann
+    problems; then back to WENO-Z for a further run (the batch graph is
+    re-captured for the other kernel)."""
+    P = _pyawcm()
+    rng = np.random.default_rng(9)
+    params, qs = [], []
+    for k in range(6):
+        L = 6 + k % 2
+        nx = 16 * 2 ** (L - 1) + 1
+        rl, rr = rng.uniform(0.1, 1.0, 2)
+        pl, pr = rng.uniform(0.05, 1.0, 2)
+        q, dx, _ = O.ic1(0, nx, 0.0, 1.0, rng.uniform(0.3, 0.7), (rl, 0.0, pl), (rr, 0.0, pr))
+        params.append((L, 17, 1.4, dx, 1e-4, 0.5, 0.04))
+        qs.append(q)
+    pts = O.print_times(0.04, 5)
+    b = P.Batch1D(params, pts)
+    for k, q in enumerate(qs):
+        b.upload_q(k, q)
+    b.set_flux(P.FLUX_WENO5)
+    b.run(30)
+    b.set_flux(P.FLUX_WENOZ)
+    b.run(20)
+    for k, q in enumerate(qs):
+        L, nxc, gm, dx, tol, cfl, tf = params[k]
+        o = O.Solver1D("orc", L, nxc, gm, dx)
+        o.set_q(q)
+        o.set_flux(1)
+        for _ in range(30):
+            if o.t[0] < tf:
+                o.step(tol, cfl, pts)
+        o.set_flux(0)
+        for _ in range(20):
+            if o.t[0] < tf:
+                o.step(tol, cfl, pts)
+        st = b.state(k)
+        assert st["status"] == 0 and st["t"] == o.t[0], k
+        mg, lg = b.masks(k)
+        mo, lo = o.masks()
+        assert_within(b.download_q(k), o.get_q(), mg, mo, lg, lo, f"batch instance {k}")
 
 
 def quad_setup(L, nxc):

@@ -48,6 +48,8 @@ def test_oracle_matches_golden_kats():
         f = np.zeros(3)
         L.orc_flux1(1.4, O.dptr(st), g["flux1_alpha"][k], O.dptr(f))
         assert np.array_equal(bits(f), bits(g["flux1_out"][k]))
+        L.orc_flux1_js(1.4, O.dptr(st), g["flux1_alpha"][k], O.dptr(f))
+        assert np.array_equal(bits(f), bits(g["flux1_js_out"][k])), "WENO5-JS flux"
     for k in range(len(g["flux2_alpha"])):
         st = np.ascontiguousarray(g["flux2_states"][k]).ravel()
         for d in range(2):
@@ -189,6 +191,38 @@ def test_oracle_sod_full_run_matches_reference():
     assert np.array_equal(ma, mb) and np.array_equal(la, lb)
     assert np.array_equal(bits(a.get_q()), bits(b.get_q()))
     assert O.load("orc").orc1_oob(a.h) == 0
+
+
+@needs_ref
+def test_oracle_sod_weno5_full_run_matches_reference():
+    """The shipped 1-D driver's stage flux, WENO_EULER_SOLVER_1D::ComputedF
+    (main_1d_euler.cpp:109): per-face known answers on random states, and a
+    full Sod run (918 steps) bit for bit."""
+    A, B = O.load("orc"), O.load("ref")
+    rng = np.random.default_rng(17)
+    for _ in range(200):
+        rho, u, p = rng.uniform(0.1, 2, 6), rng.uniform(-1.5, 1.5, 6), rng.uniform(0.05, 2, 6)
+        st = np.stack([rho, rho * u, p / 0.4 + 0.5 * rho * u * u], 1).ravel()
+        a = rng.uniform(0.5, 3)
+        fa, fb = np.zeros(3), np.zeros(3)
+        A.orc_flux1_js(1.4, O.dptr(st), a, O.dptr(fa))
+        B.ref_flux1_js(1.4, O.dptr(st), a, O.dptr(fb))
+        assert np.array_equal(bits(fa), bits(fb))
+    L, nxc = 7, 17
+    nx = (nxc - 1) * 2 ** (L - 1) + 1
+    q, dx, _ = O.ic1(0, nx, 0.0, 1.0, 0.5, O.SOD_L, O.SOD_R)
+    a, b = O.Solver1D("orc", L, nxc, 1.4, dx), O.Solver1D("ref", L, nxc, 1.4, dx)
+    a.set_flux(1)
+    b.set_flux(1)
+    a.set_q(q)
+    b.set_q(q)
+    pts = O.print_times(0.2, 41)
+    while a.t[0] < 0.2:
+        assert a.step(1e-4, 0.5, pts) == b.step(1e-4, 0.5, pts)
+    ma, la = a.masks()
+    mb, lb = b.masks()
+    assert np.array_equal(ma, mb) and np.array_equal(la, lb)
+    assert np.array_equal(bits(a.get_q()), bits(b.get_q()))
 
 
 @needs_ref
@@ -355,8 +389,11 @@ def test_face_flux_consistency():
         q1 = np.array([rho, rho * u, p / 0.4 + 0.5 * rho * u * u])
         st = np.tile(q1, 6)
         f = np.zeros(3)
-        L.orc_flux1(1.4, O.dptr(st), rng.uniform(1, 3), O.dptr(f))
+        a = rng.uniform(1, 3)
+        L.orc_flux1(1.4, O.dptr(st), a, O.dptr(f))
         assert np.allclose(f, _phys_flux(q1, 1), rtol=1e-12, atol=1e-12)
+        L.orc_flux1_js(1.4, O.dptr(st), a, O.dptr(f))
+        assert np.allclose(f, _phys_flux(q1, 1), rtol=1e-12, atol=1e-12), "WENO5-JS flux"
         q2 = np.array([rho, rho * u, rho * v, p / 0.4 + 0.5 * rho * (u * u + v * v)])
         st = np.tile(q2, 6)
         for d in range(2):
