@@ -25,14 +25,18 @@ namespace awcm {
 // the C ABI's flux kinds name the reconstructions of awcm/physics.h
 static_assert(AWCM_FLUX_WENOZ == kRecWenoZ && AWCM_FLUX_WENO5 == kRecWeno5, "flux kinds");
 
-// mode 0: PrepareEuler only; mode 2: PostProcessEuler only; mode 1: time steps until t >= t_final or max_steps
+// mode 0: PrepareEuler only; mode 2: PostProcessEuler only; mode 1: time
+// steps until t >= t_final or max_steps.  The SCALAR instantiation (kept apart
+// so the time-step kernel is unchanged) runs mode 3, the scalar Prepare (plane
+// 0), and mode 4, the scalar Interpolate from mask_temp's slot onto mask3's
+// slot (the host puts the two masks there and restores them).
 // steps of this launch.  Instances are taken from `order` through the atomic
 // work counter *qctr (the host zeroes it).  The masks and levels of the
 // instance in hand occupy 11 * nxp bytes (nxp >= max(Nx) + 6 over the
 // instances): dynamic LDS, or with kHbm -- for instances whose masks do not
 // fit one workgroup's LDS -- this workgroup's slice of `hbm` (gridDim.x slices
 // of 11 * nxp bytes; workgroup-local, ordered by the same barriers).
-template <bool kHbm, int FLUX>
+template <bool kHbm, int FLUX, bool SCALAR>
 __global__ void __launch_bounds__(1024) k1_solve(Inst1D* insts, int n_inst, const int* order, int* qctr, int mode,
                                                  int max_steps, int nxp, uint8_t* hbm) {
     AWCM_DYNAMIC_LDS uint8_t lds_masks[];
@@ -73,7 +77,10 @@ __global__ void __launch_bounds__(1024) k1_solve(Inst1D* insts, int n_inst, cons
             st.sum_active0 = 0; st.cnt = in.cnt; st.steps = 0; st.n_print = in.n_print; st.bad = 0;
         }
         __syncthreads();
-        if (mode == 0) {
+        if constexpr (SCALAR) {
+            if (mode == 3) prepare_scalar(c, st.tol);
+            else if (mode == 4) interp<0, 1>(c, c.mt, c.m[3]);
+        } else if (mode == 0) {
             prepare(c, st.tol);
         } else if (mode == 2) {
             post(c);
@@ -98,9 +105,11 @@ __global__ void __launch_bounds__(1024) k1_solve(Inst1D* insts, int n_inst, cons
     }
 }
 
-template __global__ void k1_solve<false, kRecWenoZ>(Inst1D*, int, const int*, int*, int, int, int, uint8_t*);
-template __global__ void k1_solve<true, kRecWenoZ>(Inst1D*, int, const int*, int*, int, int, int, uint8_t*);
-template __global__ void k1_solve<false, kRecWeno5>(Inst1D*, int, const int*, int*, int, int, int, uint8_t*);
-template __global__ void k1_solve<true, kRecWeno5>(Inst1D*, int, const int*, int*, int, int, int, uint8_t*);
+template __global__ void k1_solve<false, kRecWenoZ, false>(Inst1D*, int, const int*, int*, int, int, int, uint8_t*);
+template __global__ void k1_solve<true, kRecWenoZ, false>(Inst1D*, int, const int*, int*, int, int, int, uint8_t*);
+template __global__ void k1_solve<false, kRecWeno5, false>(Inst1D*, int, const int*, int*, int, int, int, uint8_t*);
+template __global__ void k1_solve<true, kRecWeno5, false>(Inst1D*, int, const int*, int*, int, int, int, uint8_t*);
+template __global__ void k1_solve<false, kRecWenoZ, true>(Inst1D*, int, const int*, int*, int, int, int, uint8_t*);
+template __global__ void k1_solve<true, kRecWenoZ, true>(Inst1D*, int, const int*, int*, int, int, int, uint8_t*);
 
 }  // namespace awcm

@@ -146,19 +146,25 @@ int awcm_set_flux(awcm_solver* h, int kind) {
 
 int awcm_upload_scalar(awcm_solver* h, const double* q) {
     if (!h || !q) return AWCM_EINVAL;
-    if (h->dim != 2) return AWCM_ESTATE;
-    return h->s2->upload_scalar(q);
+    return h->dim == 1 ? awcm::upload_scalar1(h->b1, 0, q) : h->s2->upload_scalar(q);
 }
 
 int awcm_download_scalar(awcm_solver* h, double* q) {
     if (!h || !q) return AWCM_EINVAL;
-    if (h->dim != 2) return AWCM_ESTATE;
-    return h->s2->download_scalar(q);
+    return h->dim == 1 ? awcm::download_scalar1(h->b1, 0, q) : h->s2->download_scalar(q);
 }
 
 int awcm_prepare(awcm_solver* h, double tol) {
     if (!h) return AWCM_EINVAL;
-    if (h->dim != 2) return AWCM_ESTATE;
+    if (h->dim == 1) {
+        Batch1D& B = h->b1;
+        B.h[0].tol = tol;
+        int st = B.push();
+        if (!st) st = B.launch(3, 0, 0, 1);
+        if (!st) st = B.pull();
+        if (!st) st = B.check_status();
+        return st;
+    }
     Solver2D& S = *h->s2;
     S.tol_host = tol;
     int st = S.enqueue_prepare(true, false);
@@ -169,13 +175,15 @@ int awcm_prepare(awcm_solver* h, double tol) {
 
 int awcm_post_process(awcm_solver* h) {
     if (!h) return AWCM_EINVAL;
-    if (h->dim != 2) return AWCM_ESTATE;
     return awcm_post_process_euler(h);
 }
 
 int awcm_interpolate(awcm_solver* h, const uint8_t* mask_from, const uint8_t* mask_to) {
     if (!h || !mask_from || !mask_to) return AWCM_EINVAL;
-    if (h->dim != 2) return AWCM_ESTATE;
+    if (h->dim == 1) {
+        const int st = awcm::interpolate1(h->b1, 0, mask_from, mask_to);
+        return st ? st : h->b1.check_status();
+    }
     return h->s2->interpolate(mask_from, mask_to, false);
 }
 

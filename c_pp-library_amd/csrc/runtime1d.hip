@@ -15,27 +15,35 @@
 
 namespace awcm {
 
-template <bool kHbm, int FLUX>
+template <bool kHbm, int FLUX, bool SCALAR>
 __global__ void k1_solve(Inst1D* insts, int n_inst, const int* order, int* qctr, int mode, int max_steps, int nxp,
                          uint8_t* hbm);
 
-// The k1_solve instantiation for a mask placement and a stage flux.
+// The k1_solve instantiation for a mask placement and a stage flux; modes 3 and
+// 4 (the scalar path) have their own.
 static void launch_solve(bool hbm_masks, int flux, dim3 grid, dim3 block, size_t lds, hipStream_t stream, Inst1D* insts,
                          int n_inst, const int* order, int* qctr, int mode, int max_steps, int nxp, uint8_t* hbm) {
-    if (flux == AWCM_FLUX_WENO5) {
+    if (mode >= 3) {
         if (hbm_masks)
-            hipLaunchKernelGGL((k1_solve<true, AWCM_FLUX_WENO5>), grid, block, lds, stream, insts, n_inst, order, qctr, mode,
-                               max_steps, nxp, hbm);
+            hipLaunchKernelGGL((k1_solve<true, AWCM_FLUX_WENOZ, true>), grid, block, lds, stream, insts, n_inst, order,
+                               qctr, mode, max_steps, nxp, hbm);
         else
-            hipLaunchKernelGGL((k1_solve<false, AWCM_FLUX_WENO5>), grid, block, lds, stream, insts, n_inst, order, qctr, mode,
-                               max_steps, nxp, hbm);
+            hipLaunchKernelGGL((k1_solve<false, AWCM_FLUX_WENOZ, true>), grid, block, lds, stream, insts, n_inst, order,
+                               qctr, mode, max_steps, nxp, hbm);
+    } else if (flux == AWCM_FLUX_WENO5) {
+        if (hbm_masks)
+            hipLaunchKernelGGL((k1_solve<true, AWCM_FLUX_WENO5, false>), grid, block, lds, stream, insts, n_inst, order,
+                               qctr, mode, max_steps, nxp, hbm);
+        else
+            hipLaunchKernelGGL((k1_solve<false, AWCM_FLUX_WENO5, false>), grid, block, lds, stream, insts, n_inst, order,
+                               qctr, mode, max_steps, nxp, hbm);
     } else {
         if (hbm_masks)
-            hipLaunchKernelGGL((k1_solve<true, AWCM_FLUX_WENOZ>), grid, block, lds, stream, insts, n_inst, order, qctr, mode,
-                               max_steps, nxp, hbm);
+            hipLaunchKernelGGL((k1_solve<true, AWCM_FLUX_WENOZ, false>), grid, block, lds, stream, insts, n_inst, order,
+                               qctr, mode, max_steps, nxp, hbm);
         else
-            hipLaunchKernelGGL((k1_solve<false, AWCM_FLUX_WENOZ>), grid, block, lds, stream, insts, n_inst, order, qctr, mode,
-                               max_steps, nxp, hbm);
+            hipLaunchKernelGGL((k1_solve<false, AWCM_FLUX_WENOZ, false>), grid, block, lds, stream, insts, n_inst, order,
+                               qctr, mode, max_steps, nxp, hbm);
     }
 }
 
@@ -76,7 +84,7 @@ int Batch1D::init(int n_, const awcm_instance_1d* params, const double* pt, int 
         hipFuncAttributes fa;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&cap, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
-            hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k1_solve<false, AWCM_FLUX_WENOZ>)) != hipSuccess)
+            hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k1_solve<false, AWCM_FLUX_WENOZ, false>)) != hipSuccess)
             return AWCM_EDEVICE;
         lds_cap = size_t(cap) > fa.sharedSizeBytes ? size_t(cap) - fa.sharedSizeBytes : 0;
     }
@@ -444,6 +452,41 @@ int get_masks1(Batch1D& B, int idx, uint8_t* masks, int32_t* levels) {
         for (size_t k = 0; k < lv.size(); k++) levels[k] = lv[k];
     }
     return AWCM_OK;
+}
+
+// The scalar field of AWCM_1D::Prepare / WAVELET_1D::Interpolate: plane 0 of
+// the instance's field, (Nx+6) doubles.
+int upload_scalar1(Batch1D& B, int idx, const double* q) {
+    Inst1D& in = B.h[idx];
+    if (hipStreamSynchronize(B.stream) != hipSuccess) return AWCM_EDEVICE;
+    return hipMemcpy(in.Q, q, sizeof(double) * (in.Nx + 6), hipMemcpyHostToDevice) == hipSuccess ? AWCM_OK
+                                                                                                : AWCM_EDEVICE;
+}
+int download_scalar1(Batch1D& B, int idx, double* q) {
+    Inst1D& in = B.h[idx];
+    if (hipStreamSynchronize(B.stream) != hipSuccess) return AWCM_EDEVICE;
+    return hipMemcpy(q, in.Q, sizeof(double) * (in.Nx + 6), hipMemcpyDeviceToHost) == hipSuccess ? AWCM_OK
+                                                                                                : AWCM_EDEVICE;
+}
+// WAVELET_1D::Interpolate(q, mra, from, to) on instance idx: the kernel's
+// mode 4 reads from and to in the mask_temp and mask3 slots, so the instance's
+// own masks are saved around it and put back.
+int interpolate1(Batch1D& B, int idx, const uint8_t* from, const uint8_t* to) {
+    Inst1D& in = B.h[idx];
+    const size_t NP = size_t(in.Nx) + 6;
+    std::vector<uint8_t> saved(5 * NP);
+    if (hipStreamSynchronize(B.stream) != hipSuccess ||
+        hipMemcpy(saved.data(), in.masks, saved.size(), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(in.masks, from, NP, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(in.masks + 4 * NP, to, NP, hipMemcpyHostToDevice) != hipSuccess)
+        return AWCM_EDEVICE;
+    int st = B.push();
+    if (!st) st = B.launch(4, 0, idx, 1);
+    if (!st) st = B.pull();
+    if (hipStreamSynchronize(B.stream) != hipSuccess ||
+        hipMemcpy(in.masks, saved.data(), saved.size(), hipMemcpyHostToDevice) != hipSuccess)
+        st = st ? st : AWCM_EDEVICE;
+    return st;
 }
 
 int set_mask0_1(Batch1D& B, int idx, const uint8_t* m0) {

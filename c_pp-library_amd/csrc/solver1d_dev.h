@@ -244,6 +244,9 @@ __device__ void level(const Ctx1& c, uint8_t* lvl, const uint8_t* mask) {
 
 // WAVELET_1D::InterpolateEuler(q, mra, from, to) (WAVELET_1D.h:393-431): momentum
 // and energy at points of `to` that were not active in `from`.
+// Components P0..P1-1: 1..2 for InterpolateEuler (WAVELET_1D.h:393-431), 0
+// for the scalar Interpolate(q, mra, from, to) (WAVELET_1D.h:198-233).
+template <int P0 = 1, int P1 = 3>
 __device__ void interp(const Ctx1& c, const uint8_t* from, const uint8_t* to) {
     const int ie = c.g.Nx - 1;
     for (int i = threadIdx.x; i < c.g.Nx + 2 * G; i += blockDim.x) c.tmp[i] = to[i];
@@ -263,7 +266,7 @@ __device__ void interp(const Ctx1& c, const uint8_t* from, const uint8_t* to) {
             int is = i - d - dh, k = 1;
             if (is < 0) is = 0, k = 0;
             else if (i + d + dh > ie) is = ie - 3 * d, k = 2;
-            for (int p = 1; p < 3; p++) {
+            for (int p = P0; p < P1; p++) {
                 double* q = c.Q + p * c.NP;
                 double v = 0;
                 for (int cc = 0; cc < 4; cc++) v += c.W[cc][k + 1] * q[is + cc * d + G];
@@ -301,6 +304,24 @@ __device__ void prepare(const Ctx1& c, double tol) {
     level(c, c.lv[3], c.m[3]);
     iwt(c, c.m[3]);
     interp(c, c.mt, c.m[3]);
+}
+
+// AWCM_1D::Prepare (AWCM_1D.h:45-71) on the scalar field (plane 0): no
+// mask_temp, no AddAdjacentZone on masks 1..3, no interpolation.
+__device__ void prepare_scalar(const Ctx1& c, double tol) {
+    fwt(c, c.m[0], tol);
+    band(c, c.m[0]);
+    adjacent(c, c.m[0]);
+    check_recon(c, c.m[0]);
+    for (int k = 1; k <= 3; k++) {
+        copy_mask(c, c.m[k], c.m[k - 1]);
+        __syncthreads();
+        level(c, c.lv[k - 1], c.m[k - 1]);
+        addpoint(c, c.m[k], c.lv[k - 1]);
+        check_recon(c, c.m[k]);
+    }
+    level(c, c.lv[3], c.m[3]);
+    iwt(c, c.m[3]);
 }
 
 // Zero-order extrapolation into the ghosts (PROBLEM_SHOCKTUBE.h:86-107; the

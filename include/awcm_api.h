@@ -39,6 +39,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <vector>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -1068,8 +1069,73 @@ void push_mask0(awcm_solver* h, const MaskField& mask0, size_t n_masked) {
 }
 }  // namespace awcm_api_detail
 
+namespace awcm_api_detail {
+// the 1-D mask argument in the C ABI's layout, Nx+6 bytes (ghosts it lacks
+// zero); only its finest-grid points are read, as in the reference
+inline std::unique_ptr<uint8_t[]> mask_bytes(const FIELD_UNIFORM_1D<bool>& m, int Nx) {
+    if (m.i_start != 0 || m.i_res != Nx) throw std::invalid_argument("WAVELET_1D: masks must span 0..Nx-1");
+    const int g = m.ghost_width < 3 ? m.ghost_width : 3;
+    std::unique_ptr<uint8_t[]> b(new uint8_t[size_t(Nx) + 6]());
+    for (int i = -g; i <= Nx - 1 + g; i++) b[size_t(i + 3)] = m(i) ? 1 : 0;
+    return b;
+}
+}  // namespace awcm_api_detail
+
+// WAVELET_1D (Wavelet/WAVELET_1D.h:11-42): N and weight(i, j) as in WAVELET_2D;
+// the scalar interpolations (WAVELET_1D.h:198-275) run on the GPU, on the
+// owning AWCM_1D's device solver (a free-standing one has its own, created on
+// first use from the field's grid).
+class WAVELET_1D {
+public:
+    const int N;
+    ARRAY_2D<double> weight;
+
+    WAVELET_1D(const int& N_ = 2) : N(N_) {
+        if (N != 2) throw std::invalid_argument("WAVELET_1D: only N = 2 (the 4-point wavelet) is supported");
+        weight.Initialize(0, -1, 2 * N, 2 * N + 1);
+        for (int j = -1; j < 2 * N; j++)
+            for (int i = 0; i < 2 * N; i++) {
+                double l = 1.0;
+                for (int k = 0; k < 2 * N; k++)
+                    if (k != i) l *= double(j + 0.5 - k) / double(i - k);
+                weight(i, j) = l;
+            }
+    }
+    WAVELET_1D(const WAVELET_1D&) = delete;
+    WAVELET_1D& operator=(const WAVELET_1D&) = delete;
+
+    // WAVELET_1D::Interpolate(q, mra, mask_from, mask_to) (WAVELET_1D.h:198-233)
+    void Interpolate(FIELD_UNIFORM_1D<double>& q, const MRA_GRID_INT_1D& mra, const FIELD_UNIFORM_1D<bool>& mask_from,
+                     const FIELD_UNIFORM_1D<bool>& mask_to) const {
+        run(q, mra, awcm_api_detail::mask_bytes(mask_from, mra.Nx).get(), awcm_api_detail::mask_bytes(mask_to, mra.Nx).get());
+    }
+    // WAVELET_1D::Interpolate(q, mra, mask_from) (WAVELET_1D.h:235-275): every
+    // point not in mask_from
+    void Interpolate(FIELD_UNIFORM_1D<double>& q, const MRA_GRID_INT_1D& mra,
+                     const FIELD_UNIFORM_1D<bool>& mask_from) const {
+        std::unique_ptr<uint8_t[]> all(new uint8_t[size_t(mra.Nx) + 6]);
+        for (int i = 0; i < mra.Nx + 6; i++) all[size_t(i)] = 1;
+        run(q, mra, awcm_api_detail::mask_bytes(mask_from, mra.Nx).get(), all.get());
+    }
+
+private:
+    friend class AWCM_1D;
+    mutable awcm_api_detail::Handle own;
+    awcm_api_detail::Handle* dev = &own;
+    void run(FIELD_UNIFORM_1D<double>& q, const MRA_GRID_INT_1D& mra, const uint8_t* from, const uint8_t* to) const {
+        using awcm_api_detail::check;
+        if (q.ghost_width != 3 || q.i_start != 0 || q.i_res != mra.Nx)
+            throw std::invalid_argument("WAVELET_1D::Interpolate: the field must span 0..Nx-1 with ghost width 3");
+        if (!dev->h) check(awcm_create_1d(mra.num_levels, mra.Nx_coarse, 1.4, q.base_grid.dx, &dev->h), "awcm_create_1d");
+        check(awcm_upload_scalar(dev->h, q.arr.val), "awcm_upload_scalar");
+        check(awcm_interpolate(dev->h, from, to), "awcm_interpolate");
+        check(awcm_download_scalar(dev->h, q.arr.val), "awcm_download_scalar");
+    }
+};
+
 class AWCM_1D {
 public:
+    WAVELET_1D wavelet;
     MRA_GRID_INT_1D& mra;
     FIELD_UNIFORM_1D<bool> mask_temp, mask0, mask1, mask2, mask3;
     FIELD_UNIFORM_1D<int> level0, level1, level2, level3;
@@ -1081,6 +1147,7 @@ public:
     awcm_api_detail::Handle dev;
 
     AWCM_1D(MRA_GRID_INT_1D& mra_) : mra(mra_), d1(2.0 / 3.0), d2(-1.0 / 12.0) {
+        wavelet.dev = &dev;
         FIELD_UNIFORM_1D<bool>* ms[5] = {&mask_temp, &mask0, &mask1, &mask2, &mask3};
         FIELD_UNIFORM_1D<int>* ls[4] = {&level0, &level1, &level2, &level3};
         for (auto* m : ms) m->Initialize(0, mra.Nx, 0, mra.Nx, 3);
@@ -1090,6 +1157,42 @@ public:
         d10 = 3.0; d11 = -1.0 / 2.0; d12 = -2.0 / 3.0; d13 = 1.0 / 12.0;
         d20 = -3.0 / 2.0; d21 = 1.0; d22 = 0; d23 = -2.0 / 3.0; d24 = 1.0 / 12.0;
         d30 = 1.0 / 3.0; d31 = -1.0 / 6.0; d32 = 2.0 / 3.0; d33 = 0; d34 = -2.0 / 3.0; d35 = 1.0 / 12.0;
+    }
+
+    // AWCM_1D::Prepare (AWCM_1D.h:45-71) on the GPU: the scalar path.  mask0
+    // and the field go down; the field, mask0..mask3 and level0..level3 come
+    // back (mask_temp is not the scalar path's and stays as it was).
+    void Prepare(FIELD_UNIFORM_1D<double>& q, const double& tol = 1e-6) {
+        using awcm_api_detail::check;
+        if (q.ghost_width != 3 || q.i_start != 0 || q.i_res != mra.Nx)
+            throw std::invalid_argument("AWCM_1D::Prepare: the field must span 0..Nx-1 with ghost width 3");
+        if (!dev.h) check(awcm_create_1d(mra.num_levels, mra.Nx_coarse, 1.4, q.base_grid.dx, &dev.h), "awcm_create_1d");
+        const size_t NP = size_t(mra.Nx) + 6;
+        awcm_api_detail::push_mask0(dev.h, mask0, NP);
+        check(awcm_upload_scalar(dev.h, q.arr.val), "awcm_upload_scalar");
+        check(awcm_prepare(dev.h, tol), "awcm_prepare");
+        check(awcm_download_scalar(dev.h, q.arr.val), "awcm_download_scalar");
+        // (FIELD_UNIFORM_1D's copy constructor drops the ghosts, so the values
+        // are kept as bytes)
+        std::vector<uint8_t> keep(NP);
+        for (size_t n = 0; n < NP; n++) keep[n] = mask_temp.arr.val[n];
+        FIELD_UNIFORM_1D<bool>* const ms[5] = {&mask_temp, &mask0, &mask1, &mask2, &mask3};
+        FIELD_UNIFORM_1D<int>* const ls[4] = {&level0, &level1, &level2, &level3};
+        awcm_api_detail::fetch_masks(dev.h, NP, size_t(mra.Nx), ms, ls);
+        for (size_t n = 0; n < NP; n++) mask_temp.arr.val[n] = keep[n] != 0;
+    }
+    // AWCM_1D.h:73-80: the 3-wide band at both ends is always active
+    void BadingBoundary(FIELD_UNIFORM_1D<bool>& mask) {
+        for (int i = mask.i_start; i <= mask.i_end; i++) {
+            if (3 <= i && i <= mask.i_end - 3) continue;
+            mask(i) = true;
+        }
+    }
+    // RemoveResidue over the finest grid (AWCM_1D.h:81-84, AWCM_MANAGER_1D.h:146-157)
+    void PostProcess(FIELD_UNIFORM_1D<double>& q) {
+        ITERATION_GRID_INT_1D(mra.grid_finest, 0) {
+            if (mask0(i) == false) q(i) = 0;
+        }
     }
 
     // AWCM_1D::PrepareEuler (AWCM_1D.h:86-120) on the GPU.  mask0 (the last

@@ -111,35 +111,37 @@ int awcm_interpolate_all_grid(awcm_solver* h, const uint8_t* mask_from);
 enum { AWCM_FLUX_WENOZ = 0, AWCM_FLUX_WENO5 = 1 };
 int awcm_set_flux(awcm_solver* h, int kind);
 
-/* ---- the scalar path (2-D; AWCM_ESTATE on a 1-D handle) --------------------
- * AWCM_2D::Prepare / PostProcess and WAVELET_2D::Interpolate act on a scalar
- * field (FIELD_UNIFORM_2D<double>, ghost width 3, (Nx+6) x (Ny+6) doubles, i
- * fastest).  On the device it is the first component of the handle's field;
- * the masks and levels are the handle's. */
+/* ---- the scalar path ------------------------------------------------------
+ * AWCM_1D/2D::Prepare / PostProcess and WAVELET_1D/2D::Interpolate act on a
+ * scalar field (FIELD_UNIFORM_1D/2D<double>, ghost width 3: Nx+6 doubles, or
+ * (Nx+6) x (Ny+6) with i fastest).  On the device it is the first component
+ * of the handle's field; the masks and levels are the handle's. */
 int awcm_upload_scalar(awcm_solver* h, const double* q);
 int awcm_download_scalar(awcm_solver* h, double* q);
 
-/* AWCM_2D::Prepare (Wavelet/AWCM_2D.h:52-79): forward transform of the scalar
- * field with threshold tol, banding, adjacent zone, reconstruction check,
- * masks 1..3 with their levels, inverse transform.  mask_temp is left as it
- * was. */
+/* AWCM_2D::Prepare (Wavelet/AWCM_2D.h:52-79) / AWCM_1D::Prepare
+ * (Wavelet/AWCM_1D.h:45-71): forward transform of the scalar field with
+ * threshold tol, banding, adjacent zone, reconstruction check, masks 1..3
+ * with their levels, inverse transform.  mask_temp is left as it was. */
 int awcm_prepare(awcm_solver* h, double tol);
 
 /* AWCM_2D::PostProcess (AWCM_2D.h:81-84, AWCM_MANAGER_2D::RemoveResidue
- * AWCM_MANAGER_2D.h:257-266): zero the field at interior points outside
- * mask0 (every component of the handle's field). */
+ * AWCM_MANAGER_2D.h:257-266) / AWCM_1D::PostProcess (AWCM_1D.h:81-84,
+ * AWCM_MANAGER_1D.h:146-157): zero the field at the points outside mask0 (the
+ * interior in 2-D; every component of the handle's field). */
 int awcm_post_process(awcm_solver* h);
 
 /* WAVELET_2D::Interpolate(q, mra, mask_from, mask_to) (WAVELET_2D.h:360-473),
- * as main_2d_advect.cpp:219 calls it: the scalar field is predicted from the
- * coarser levels at the points of CheckReconstruction(mask_to minus
- * mask_from) that are not in mask_from.  Masks ghosted uint8, reference
- * layout.  Synchronous. */
+ * as main_2d_advect.cpp:219 calls it, and WAVELET_1D::Interpolate
+ * (WAVELET_1D.h:198-275; its one-mask form is mask_to = every point): the
+ * scalar field is predicted from the coarser levels at the points of
+ * CheckReconstruction(mask_to minus mask_from) that are not in mask_from.
+ * Masks ghosted uint8, reference layout.  Synchronous. */
 int awcm_interpolate(awcm_solver* h, const uint8_t* mask_from, const uint8_t* mask_to);
 
 /* WAVELET_2D::InterpolateEuler(q, mra, mask_from, mask_to)
  * (WAVELET_2D.h:784-907): the same on momentum and energy (components 1..3)
- * of the Euler field.  Synchronous. */
+ * of the Euler field.  2-D (AWCM_ESTATE on a 1-D handle).  Synchronous. */
 int awcm_interpolate_euler(awcm_solver* h, const uint8_t* mask_from, const uint8_t* mask_to);
 
 /* AWCM_*::PostProcessEuler (AWCM_1D.h:122-125, AWCM_2D.h:149-152). */

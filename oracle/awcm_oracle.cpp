@@ -385,7 +385,9 @@ struct Solver1D {
         }
     }
     // WAVELET_1D::InterpolateEuler(q, mra, from, to) (WAVELET_1D.h:393-431)
-    void interp(std::vector<uint8_t>& from, std::vector<uint8_t>& to) {
+    // components p0..p1-1: 1..2 for InterpolateEuler (WAVELET_1D.h:393-431), 0 for the
+    // scalar Interpolate(q, mra, from, to) (WAVELET_1D.h:198-233)
+    void interp(std::vector<uint8_t>& from, std::vector<uint8_t>& to, int p0 = 1, int p1 = 3) {
         const int ie = Nx - 1;
         std::vector<uint8_t> tmp = to;
         for (int i = 0; i <= ie; i++) if (mk(from, i)) mk(tmp, i) = 0;
@@ -398,7 +400,7 @@ struct Solver1D {
                 int is = i - d - dh, iend = i + d + dh, k = 1;
                 if (is < 0) is = 0, k = 0;
                 else if (iend > ie) is = ie - 3 * d, k = 2;
-                for (int p = 1; p < 3; p++) {
+                for (int p = p0; p < p1; p++) {
                     q(Q, i)[p] = 0;
                     for (int c = 0; c < 4; c++) q(Q, i)[p] += w(c, k) * q(Q, is + c * d)[p];
                 }
@@ -474,6 +476,22 @@ struct Solver1D {
         level(lv[3], m[3]);
         iwt(m[3]);
         interp(mt, m[3]);
+    }
+    // AWCM_1D::Prepare (AWCM_1D.h:45-71) on the scalar field (component 0): no
+    // mask_temp, no AddAdjacentZone on masks 1..3, no interpolation
+    void prepare_scalar(double tol) {
+        fwt(m[0], tol);
+        band(m[0]);
+        adjacent(m[0]);
+        check_recon(m[0]);
+        for (int k = 1; k <= 3; k++) {
+            m[k] = m[k - 1];
+            level(lv[k - 1], m[k - 1]);
+            addpoint(m[k], lv[k - 1]);
+            check_recon(m[k]);
+        }
+        level(lv[3], m[3]);
+        iwt(m[3]);
     }
     // AWCM_MANAGER_1D::RemoveResidue (AWCM_MANAGER_1D.h:146-157)
     void post() {
@@ -1389,6 +1407,25 @@ void orc1_set_mask0(void* h, const uint8_t* m0) { auto* s = static_cast<Solver1D
 void orc1_prepare(void* h, double tol) { static_cast<Solver1D*>(h)->prepare(tol); }
 // face fluxes of the stages: 0 WENO-Z pair, 1 WENO5-JS pair (the shipped driver)
 void orc1_set_flux(void* h, int kind) { static_cast<Solver1D*>(h)->flux_js = kind == 1; }
+// the scalar path: the field is component 0 ((Nx+6) doubles)
+void orc1_set_scalar(void* h, const double* q) {
+    auto* s = static_cast<Solver1D*>(h);
+    for (int i = 0; i < s->Nx + 6; i++) s->Q[3 * i] = q[i];
+}
+void orc1_get_scalar(void* h, double* q) {
+    auto* s = static_cast<Solver1D*>(h);
+    for (int i = 0; i < s->Nx + 6; i++) q[i] = s->Q[3 * i];
+}
+void orc1_prepare_scalar(void* h, double tol) { static_cast<Solver1D*>(h)->prepare_scalar(tol); }
+// WAVELET_1D::Interpolate(q, mra, from, to) (WAVELET_1D.h:198-233); a null `to`
+// is Interpolate(q, mra, from) (:235-275): every point
+void orc1_interpolate_scalar(void* h, const uint8_t* mask_from, const uint8_t* mask_to) {
+    auto* s = static_cast<Solver1D*>(h);
+    const size_t n = size_t(s->Nx) + 6;
+    std::vector<uint8_t> from(mask_from, mask_from + n), to(n, 1);
+    if (mask_to) to.assign(mask_to, mask_to + n);
+    s->interp(from, to, 0, 1);
+}
 double orc1_step(void* h, double tol, double cfl, const double* pt, int npt, int* cnt, double* t) {
     return static_cast<Solver1D*>(h)->step(tol, cfl, pt, npt, cnt, t);
 }

@@ -40,6 +40,7 @@ struct Ref1D {
     MRA_GRID_INT_1D mra;
     AWCM_1D* awcm;
     FIELD_UNIFORM_1D<VECTOR_3D<double>> Q, Q_temp, Q_copy;
+    FIELD_UNIFORM_1D<double> Qs;  // the scalar field of AWCM_1D::Prepare
     BOUNDARY_MANAGER* bcm;
     WENOZ_EULER_SOLVER_1D* wenoz;
     WENO_EULER_SOLVER_1D* weno;
@@ -51,6 +52,7 @@ struct Ref1D {
         mra.Initialize(L, nxc);
         awcm = new AWCM_1D(mra);
         Q.Initialize(0, mra.Nx, 0.0, 1.0, 3);
+        Qs.Initialize(0, mra.Nx, 0.0, 1.0, 3);
         bcm = new BOUNDARY_MANAGER(Q);
         wenoz = new WENOZ_EULER_SOLVER_1D(gamma);
         weno = new WENO_EULER_SOLVER_1D(gamma);
@@ -354,6 +356,30 @@ void ref1_set_mask0(void* h, const uint8_t* m0) {
 }
 void ref1_prepare(void* h, double tol) { auto* s = static_cast<Ref1D*>(h); s->awcm->PrepareEuler(s->Q, tol); }
 void ref1_set_flux(void* h, int kind) { static_cast<Ref1D*>(h)->js = kind == 1; }
+void ref1_set_scalar(void* h, const double* q) {
+    auto* s = static_cast<Ref1D*>(h);
+    for (int k = 0; k < s->Qs.arr.i_res; k++) s->Qs.arr.val[k] = q[k];
+}
+void ref1_get_scalar(void* h, double* q) {
+    auto* s = static_cast<Ref1D*>(h);
+    for (int k = 0; k < s->Qs.arr.i_res; k++) q[k] = s->Qs.arr.val[k];
+}
+void ref1_prepare_scalar(void* h, double tol) { auto* s = static_cast<Ref1D*>(h); s->awcm->Prepare(s->Qs, tol); }
+void ref1_interpolate_scalar(void* h, const uint8_t* mask_from, const uint8_t* mask_to) {
+    auto* s = static_cast<Ref1D*>(h);
+    // ghost width 3, as the masks arrive (FIELD_UNIFORM_1D's copy constructor
+    // would drop the ghosts: FIELD_UNIFORM_1D.h:31-39)
+    FIELD_UNIFORM_1D<bool> from, to;
+    from.Initialize(0, s->mra.Nx, 0.0, 1.0, 3);
+    to.Initialize(0, s->mra.Nx, 0.0, 1.0, 3);
+    for (int k = 0; k < from.arr.i_res; k++) from.arr.val[k] = mask_from[k] != 0;
+    if (!mask_to) {
+        s->awcm->wavelet.Interpolate(s->Qs, s->mra, from);
+        return;
+    }
+    for (int k = 0; k < to.arr.i_res; k++) to.arr.val[k] = mask_to[k] != 0;
+    s->awcm->wavelet.Interpolate(s->Qs, s->mra, from, to);
+}
 double ref1_step(void* h, double tol, double cfl, const double* pt, int npt, int* cnt, double* t) {
     return static_cast<Ref1D*>(h)->step(tol, cfl, pt, npt, cnt, t);
 }

@@ -180,6 +180,35 @@ def scalar_box2(L, nxc, seed):
     return out
 
 
+def scalar_1d(L, nxc, seed):
+    """AWCM_1D::Prepare on a scalar field (three rounds, the field moved
+    between them) and WAVELET_1D::Interpolate from/to four mask pairs (None:
+    the one-mask form, every point)."""
+    nx = (nxc - 1) * 2 ** (L - 1) + 1
+    s = O.Solver1D("ref", L, nxc, 1.4, 1.0 / nx)
+    x = (np.arange(-3, nx + 3) + 0.5) / nx
+    q = np.exp(-200 * (x - 0.4) ** 2) + 0.5 * (np.abs(x - 0.7) < 0.1)
+    out = dict(levels=L, nx_coarse=nxc, q0=q)
+    cur = q
+    for r in range(3):
+        s.set_scalar(cur)
+        s.prepare_scalar(1e-4)
+        m, lv = s.masks()
+        out[f"q_{r}"], out[f"masks_{r}"], out[f"levels_{r}"] = s.get_scalar(), m.astype(np.uint8), lv.astype(np.int8)
+        cur = np.roll(s.get_scalar(), 2) * 0.999
+    rng = np.random.default_rng(seed)
+    m = out["masks_2"]
+    pairs = {"m0_all": (m[1], None), "m0_m3": (m[1], m[4]), "m1_m2": (m[2], m[3]),
+             "random": ((rng.random(m[1].shape) < 0.3).astype(np.uint8), (rng.random(m[1].shape) < 0.6).astype(np.uint8))}
+    for name, (f, t) in pairs.items():
+        s.set_scalar(out["q_2"])
+        s.interpolate_scalar(f, t)
+        out[f"from_{name}"], out[f"qi_{name}"] = f.astype(np.uint8), s.get_scalar()
+        if t is not None:
+            out[f"to_{name}"] = t.astype(np.uint8)
+    return out
+
+
 def main():
     if not O.have_ref():
         sys.exit("oracle/_ref/libawcm_ref.so is not built (make -C oracle)")
@@ -192,6 +221,7 @@ def main():
         "all_grid_quad_2d_L4.npz": lambda: all_grid2("quad", 4, 5, 5, 6, 31),
         "all_grid_dmr_2d_L3.npz": lambda: all_grid2("dmr", 3, 17, 5, 6, 32),
         "scalar_box_2d_L4.npz": lambda: scalar_box2(4, 5, 33),
+        "scalar_1d_L7.npz": lambda: scalar_1d(7, 17, 34),
     }
     for name in sys.argv[1:] or list(files):
         np.savez_compressed(os.path.join(HERE, name), **files[name]())

@@ -1,7 +1,8 @@
 // Probe of the reference's scalar AWCM path: AWCM_2D::Prepare / PostProcess
 // and WAVELET_2D::Interpolate, driven by the time loop of
 // Wavelet/main_2d_advect.cpp (box advection, WENO5 right reconstructions at
-// the finest level, AWCM central differences below it, SSP-RK3).
+// the finest level, AWCM central differences below it, SSP-RK3); then
+// AWCM_1D::Prepare / PostProcess and both WAVELET_1D::Interpolate forms.
 // TEST INFRASTRUCTURE.  One source, built twice, as api_probe.cpp:
 //   * against include/compat/ (this library)          -> tests/native/_build/awcm_api_probe_advect
 //   * against the reference's own headers, in place    -> oracle/_ref/awcm_api_probe_advect_ref
@@ -19,8 +20,19 @@
 #include <cstring>
 #include <string>
 
+#include "FIELD_UNIFORM_1D.h"
+#ifdef AWCM_PROBE_REFERENCE
+// FIELD_UNIFORM_1D<T>::AssignAllValues forwards to ARRAY_1D::AssignAllValue,
+// which the reference does not define (FIELD_UNIFORM_1D.h:65-68); AWCM_1D's
+// constructor needs it for bool.
+template <>
+void FIELD_UNIFORM_1D<bool>::AssignAllValues(const bool& v) {
+    for (int i = 0; i < arr.i_res; i++) arr.val[i] = v;
+}
+#endif
 #include "Problems/2D/PROBLEM_BOX_ADVECTION.h"
 #include "WENO.h"
+#include "Wavelet/AWCM_1D.h"
 #include "Wavelet/AWCM_2D.h"
 
 namespace {
@@ -159,6 +171,49 @@ void run(const Cfg& c, int steps) {
     std::fflush(stdout);
 }
 
+// AWCM_1D::Prepare on a Gaussian plus a plateau (three rounds, the field moved
+// two points and damped between them as a stand-in for a driver's stages,
+// PostProcess after each), then WAVELET_1D::Interpolate from mask0 onto every
+// point and from mask1 onto mask3
+void run1(int L, int nxc) {
+    MRA_GRID_INT_1D mra;
+    mra.Initialize(L, nxc);
+    const int Nx = mra.Nx;
+    FIELD_UNIFORM_1D<double> q(0, Nx, 0.0, 1.0, 3);
+    for (int i = -3; i <= Nx + 2; i++) {
+        const double x = (i + 0.5) / Nx;
+        q(i) = std::exp(-200 * (x - 0.4) * (x - 0.4)) + (std::fabs(x - 0.7) < 0.1 ? 0.5 : 0.0);
+    }
+    AWCM_1D awcm(mra);
+    char tag[64];
+    for (int r = 0; r < 3; r++) {
+        awcm.Prepare(q, 1e-4);
+        Hash hq, hm, hl;
+        for (int i = -3; i <= Nx + 2; i++) hq.f(q(i));
+        const FIELD_UNIFORM_1D<bool>* ms[5] = {&awcm.mask_temp, &awcm.mask0, &awcm.mask1, &awcm.mask2, &awcm.mask3};
+        for (const auto* m : ms)
+            for (int i = -3; i <= Nx + 2; i++) hm.byte((*m)(i) ? 1 : 0);
+        const FIELD_UNIFORM_1D<int>* ls[4] = {&awcm.level0, &awcm.level1, &awcm.level2, &awcm.level3};
+        for (const auto* l : ls)
+            for (int i = 0; i < Nx; i++) hl.i((*l)(i));
+        std::printf("scalar1d.L%d.%d.prepare Q=%016" PRIx64 " masks=%016" PRIx64 " levels=%016" PRIx64 "\n", L, r, hq.h,
+                    hm.h, hl.h);
+        awcm.PostProcess(q);
+        FIELD_UNIFORM_1D<double> moved(0, Nx, 0.0, 1.0, 3);
+        for (int i = -3; i <= Nx + 2; i++) moved(i) = q(i - 2 < -3 ? Nx + 2 - (-3 - (i - 2)) + 1 : i - 2) * 0.999;
+        for (int i = -3; i <= Nx + 2; i++) q(i) = moved(i);
+    }
+    FIELD_UNIFORM_1D<double> qb(0, Nx, 0.0, 1.0, 3);
+    for (int i = -3; i <= Nx + 2; i++) qb(i) = q(i);
+    awcm.wavelet.Interpolate(q, mra, awcm.mask0);
+    awcm.wavelet.Interpolate(qb, mra, awcm.mask1, awcm.mask3);
+    Hash ha, hb;
+    for (int i = -3; i <= Nx + 2; i++) ha.f(q(i)), hb.f(qb(i));
+    std::snprintf(tag, sizeof tag, "scalar1d.L%d", L);
+    std::printf("%s.interpolate.all Q=%016" PRIx64 "\n%s.interpolate.m1m3 Q=%016" PRIx64 "\n", tag, ha.h, tag, hb.h);
+    std::fflush(stdout);
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -169,5 +224,7 @@ int main(int argc, char** argv) {
     const int steps = std::atoi(argv[1]);
     run(kCfg[0], steps);
     run(kCfg[1], steps < 10 ? steps : 10);
+    run1(7, 17);
+    run1(11, 16);
     return 0;
 }

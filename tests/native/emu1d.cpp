@@ -45,10 +45,10 @@ void launch_solve_f(Emu& e, int mode, int max_steps) {
     if (masks_in_hbm) {
         std::vector<uint8_t> hbm(size_t(11) * e.nxp(), 0xA5);
         emu_launch(EMU_THREADS,
-                   [&] { k1_solve<true, FLUX>(&e.in, 1, &order, &qctr, mode, max_steps, e.nxp(), hbm.data()); });
+                   [&] { k1_solve<true, FLUX, false>(&e.in, 1, &order, &qctr, mode, max_steps, e.nxp(), hbm.data()); });
     } else {
         emu_launch(EMU_THREADS,
-                   [&] { k1_solve<false, FLUX>(&e.in, 1, &order, &qctr, mode, max_steps, e.nxp(), nullptr); });
+                   [&] { k1_solve<false, FLUX, false>(&e.in, 1, &order, &qctr, mode, max_steps, e.nxp(), nullptr); });
     }
 }
 void launch_solve(Emu& e, int mode, int max_steps) {

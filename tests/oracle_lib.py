@@ -54,6 +54,10 @@ def load(prefix="orc"):
         f"{p}1_set_mask0": (None, [V, _u8p]),
         f"{p}1_prepare": (None, [V, C.c_double]),
         f"{p}1_set_flux": (None, [V, C.c_int]),
+        f"{p}1_set_scalar": (None, [V, _dp]),
+        f"{p}1_get_scalar": (None, [V, _dp]),
+        f"{p}1_prepare_scalar": (None, [V, C.c_double]),
+        f"{p}1_interpolate_scalar": (None, [V, _u8p, _u8p]),
         f"{p}_flux1_js": (None, [C.c_double, _dp, C.c_double, _dp]),
         f"{p}1_step": (C.c_double, [V, C.c_double, C.c_double, _dp, C.c_int, _ip, _dp]),
         f"{p}2_nx": (C.c_int, [V]),
@@ -185,6 +189,25 @@ class Solver1D:
 
     def prepare(self, tol):
         getattr(self.lib, f"{self.p}1_prepare")(self.h, tol)
+
+    def set_scalar(self, q):
+        """The scalar field of AWCM_1D::Prepare ((nx+6) doubles)."""
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        getattr(self.lib, f"{self.p}1_set_scalar")(self.h, dptr(q))
+
+    def get_scalar(self):
+        q = np.zeros(self.nx + 6)
+        getattr(self.lib, f"{self.p}1_get_scalar")(self.h, dptr(q))
+        return q
+
+    def prepare_scalar(self, tol):
+        getattr(self.lib, f"{self.p}1_prepare_scalar")(self.h, tol)
+
+    def interpolate_scalar(self, mask_from, mask_to=None):
+        """WAVELET_1D::Interpolate(q, mra, from[, to]); no `to`: every point."""
+        f = np.ascontiguousarray(mask_from, dtype=np.uint8)
+        t = None if mask_to is None else np.ascontiguousarray(mask_to, dtype=np.uint8)
+        getattr(self.lib, f"{self.p}1_interpolate_scalar")(self.h, u8ptr(f), None if t is None else u8ptr(t))
 
     def set_flux(self, kind):
         """Face fluxes of the stages: 0 the WENO-Z pair, 1 the shipped driver's

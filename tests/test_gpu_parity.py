@@ -434,22 +434,64 @@ def test_2d_interpolate_all_grid_matches_oracle(case):
 
 
 def test_2d_only_entry_points_reject_1d_handle():
-    """InterpolateAllGridEuler, Interpolate(Euler) and the scalar path are 2-D
-    routines: a 1-D handle is refused (AWCM_ESTATE) and its field is left as
-    it was."""
+    """InterpolateAllGridEuler and InterpolateEuler(from, to) are 2-D routines:
+    a 1-D handle is refused (AWCM_ESTATE) and its field is left as it was."""
     P = _pyawcm()
     q, dx = sod_setup(5, 17)
     g = P.Solver(1, 5, 17, 1.4, dx)
     g.upload_q(q)
     ones = np.ones(g.npts, np.uint8)
-    calls = [lambda: g.interpolate_all_grid(ones), lambda: g.interpolate(ones, ones),
-             lambda: g.interpolate_euler(ones, ones), lambda: g.prepare_scalar(1e-3), lambda: g.post_process_scalar(),
-             lambda: g.upload_scalar(np.zeros(g.npts)), lambda: g.download_scalar()]
+    calls = [lambda: g.interpolate_all_grid(ones), lambda: g.interpolate_euler(ones, ones)]
     for call in calls:
         with pytest.raises(P.AwcmError) as e:
             call()
         assert e.value.code == -6  # AWCM_ESTATE
     assert np.array_equal(g.download_q(), q)
+
+
+@pytest.mark.parametrize("L,nxc", [(7, 17), (11, 16)])
+def test_1d_scalar_path_matches_oracle(L, nxc):
+    """The 1-D scalar path through the C ABI: AWCM_1D::Prepare (awcm_prepare)
+    over four rounds with the field moved between them, then
+    WAVELET_1D::Interpolate (awcm_interpolate; the one-mask form is `to` =
+    every point) from/to five mask pairs; the instance's own masks are the
+    same afterwards.  Masks, levels and fields bit-identical to the oracle.
+    The second grid (15361 points) keeps its masks in HBM."""
+    P = _pyawcm()
+    nx = (nxc - 1) * 2 ** (L - 1) + 1
+    x = (np.arange(-3, nx + 3) + 0.5) / nx
+    q = np.exp(-200 * (x - 0.4) ** 2) + 0.5 * (np.abs(x - 0.7) < 0.1)
+    g = P.Solver(1, L, nxc, 1.4, 1.0 / nx)
+    o = O.Solver1D("orc", L, nxc, 1.4, 1.0 / nx)
+    cur = q
+    for r in range(4):
+        print(f"scalar 1-D L{L} prepare round {r}", flush=True)
+        g.upload_scalar(cur)
+        o.set_scalar(cur)
+        g.prepare_scalar(1e-4)
+        o.prepare_scalar(1e-4)
+        mg, lg = g.masks()
+        mo, lo = o.masks()
+        sg = g.download_scalar()
+        assert not mg[0].any(), "the scalar Prepare leaves mask_temp alone"
+        assert_same(sg, o.get_scalar(), mg, mo, lg, lo, f"1-D scalar prepare round {r}")
+        cur = np.roll(sg, 2) * 0.999
+    rng = np.random.default_rng(4)
+    m = mg
+    ones = np.ones_like(m[1])
+    for name, (f, t) in {"m0_all": (m[1], None), "m0_m3": (m[1], m[4]), "m1_m2": (m[2], m[3]),
+                         "random": ((rng.random(m[1].shape) < 0.3).astype(np.uint8),
+                                    (rng.random(m[1].shape) < 0.6).astype(np.uint8)),
+                         "sparse_all": ((rng.random(m[1].shape) < 0.2).astype(np.uint8), None)}.items():
+        print(f"scalar 1-D L{L} interpolate {name}", flush=True)
+        g.upload_scalar(q)
+        o.set_scalar(q)
+        g.interpolate(f, ones if t is None else t)
+        o.interpolate_scalar(f, t)
+        a, b = g.download_scalar(), o.get_scalar()
+        assert np.array_equal(a.view(np.uint64), b.view(np.uint64)), f"Interpolate {name}: {int((a != b).sum())} differ"
+        mg2, lg2 = g.masks()
+        assert np.array_equal(mg2, mg) and np.array_equal(lg2, lg), f"Interpolate {name} changed the masks"
 
 
 @pytest.mark.parametrize("case", ["box", "dmr_density", "box_2049"])

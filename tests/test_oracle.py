@@ -141,6 +141,49 @@ def test_oracle_scalar_path_matches_golden():
         assert np.array_equal(bits(s.get_scalar()), bits(g[f"qi_{name}"])), name
 
 
+def test_oracle_scalar_1d_matches_golden():
+    """AWCM_1D::Prepare on a scalar field (three rounds) and
+    WAVELET_1D::Interpolate (both forms) against the reference build's
+    vectors."""
+    g = load("scalar_1d_L7.npz")
+    L, nxc = int(g["levels"]), int(g["nx_coarse"])
+    nx = (nxc - 1) * 2 ** (L - 1) + 1
+    s = O.Solver1D("orc", L, nxc, 1.4, 1.0 / nx)
+    cur = g["q0"]
+    for r in range(3):
+        s.set_scalar(cur)
+        s.prepare_scalar(1e-4)
+        m, lv = s.masks()
+        assert np.array_equal(m, g[f"masks_{r}"].reshape(m.shape)), f"masks, round {r}"
+        assert np.array_equal(lv, g[f"levels_{r}"].reshape(lv.shape)), f"levels, round {r}"
+        assert np.array_equal(bits(s.get_scalar()), bits(g[f"q_{r}"])), f"field, round {r}"
+        cur = np.roll(s.get_scalar(), 2) * 0.999
+    for name in ("m0_all", "m0_m3", "m1_m2", "random"):
+        s.set_scalar(g["q_2"])
+        s.interpolate_scalar(g[f"from_{name}"], g[f"to_{name}"] if f"to_{name}" in g.files else None)
+        assert np.array_equal(bits(s.get_scalar()), bits(g[f"qi_{name}"])), name
+
+
+def test_scalar_1d_interpolation_reproduces_cubics():
+    """WAVELET_1D::Interpolate(q, mra, from) with from = the coarsest grid and
+    both ends (the points no lattice reaches): a cubic comes back everywhere
+    to rounding from garbage."""
+    L, nxc = 6, 9
+    nx = (nxc - 1) * 2 ** (L - 1) + 1
+    x = np.arange(-3, nx + 3) / (nx - 1)
+    exact = 0.3 - 1.2 * x + 2.5 * x ** 2 - 1.7 * x ** 3
+    f = np.zeros(nx + 6, np.uint8)
+    f[3:-3:2 ** (L - 1)] = 1
+    f[:4] = 1
+    f[-4:] = 1
+    rng = np.random.default_rng(3)
+    q = np.where(f == 1, exact, rng.uniform(-20, 20, exact.shape))
+    s = O.Solver1D("orc", L, nxc, 1.4, 1.0 / nx)
+    s.set_scalar(q)
+    s.interpolate_scalar(f)
+    assert np.abs(s.get_scalar()[3:-3] - exact[3:-3]).max() < 1e-12
+
+
 # ----------------------------------------------------- reference build ----
 @needs_ref
 def test_oracle_ics_match_reference():
@@ -334,6 +377,38 @@ def test_oracle_scalar_path_matches_reference(case):
         a.interpolate_euler(f, t)
         b.interpolate_euler(f, t)
         assert np.array_equal(bits(a.get_q()), bits(b.get_q()))
+
+
+@needs_ref
+def test_oracle_scalar_1d_matches_reference():
+    """AWCM_1D::Prepare (four rounds) and WAVELET_1D::Interpolate (both forms)
+    against the reference build, on two grids."""
+    for L, nxc in ((7, 17), (9, 16)):
+        nx = (nxc - 1) * 2 ** (L - 1) + 1
+        x = (np.arange(-3, nx + 3) + 0.5) / nx
+        q = np.exp(-200 * (x - 0.4) ** 2) + 0.5 * (np.abs(x - 0.7) < 0.1)
+        a, b = O.Solver1D("orc", L, nxc, 1.4, 1.0 / nx), O.Solver1D("ref", L, nxc, 1.4, 1.0 / nx)
+        cur = q
+        for r in range(4):
+            a.set_scalar(cur)
+            b.set_scalar(cur)
+            a.prepare_scalar(1e-4)
+            b.prepare_scalar(1e-4)
+            ma, la = a.masks()
+            mb, lb = b.masks()
+            assert np.array_equal(ma, mb) and np.array_equal(la, lb), (L, r)
+            assert np.array_equal(bits(a.get_scalar()), bits(b.get_scalar())), (L, r)
+            cur = np.roll(a.get_scalar(), 2) * 0.999
+        rng = np.random.default_rng(2)
+        m = ma
+        for f, t in ((m[1], None), (m[1], m[4]), (m[2], m[3]),
+                     ((rng.random(m[1].shape) < 0.3).astype(np.uint8), (rng.random(m[1].shape) < 0.6).astype(np.uint8)),
+                     ((rng.random(m[1].shape) < 0.2).astype(np.uint8), None)):
+            a.set_scalar(q)
+            b.set_scalar(q)
+            a.interpolate_scalar(f, t)
+            b.interpolate_scalar(f, t)
+            assert np.array_equal(bits(a.get_scalar()), bits(b.get_scalar()))
 
 
 # ----------------------------------------------- closed forms, invariants ----
