@@ -28,8 +28,10 @@ static_assert(AWCM_FLUX_WENOZ == kRecWenoZ && AWCM_FLUX_WENO5 == kRecWeno5, "flu
 // mode 0: PrepareEuler only; mode 2: PostProcessEuler only; mode 1: time
 // steps until t >= t_final or max_steps.  The SCALAR instantiation (kept apart
 // so the time-step kernel is unchanged) runs mode 3, the scalar Prepare (plane
-// 0), and mode 4, the scalar Interpolate from mask_temp's slot onto mask3's
-// slot (the host puts the two masks there and restores them).
+// 0), and the interpolations from mask_temp's slot onto mask3's slot (the host
+// puts the two masks there and restores them): mode 4 the scalar one (plane
+// 0), mode 5 InterpolateEuler(from, to) (planes 1..2), mode 6 the one-mask
+// InterpolateEuler (all three planes, WAVELET_1D.h:433-473).
 // steps of this launch.  Instances are taken from `order` through the atomic
 // work counter *qctr (the host zeroes it).  The masks and levels of the
 // instance in hand occupy 11 * nxp bytes (nxp >= max(Nx) + 6 over the
@@ -80,6 +82,8 @@ __global__ void __launch_bounds__(1024) k1_solve(Inst1D* insts, int n_inst, cons
         if constexpr (SCALAR) {
             if (mode == 3) prepare_scalar(c, st.tol);
             else if (mode == 4) interp<0, 1>(c, c.mt, c.m[3]);
+            else if (mode == 5) interp<1, 3>(c, c.mt, c.m[3]);
+            else if (mode == 6) interp<0, 3>(c, c.mt, c.m[3]);
         } else if (mode == 0) {
             prepare(c, st.tol);
         } else if (mode == 2) {

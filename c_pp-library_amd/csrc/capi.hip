@@ -181,7 +181,7 @@ int awcm_post_process(awcm_solver* h) {
 int awcm_interpolate(awcm_solver* h, const uint8_t* mask_from, const uint8_t* mask_to) {
     if (!h || !mask_from || !mask_to) return AWCM_EINVAL;
     if (h->dim == 1) {
-        const int st = awcm::interpolate1(h->b1, 0, mask_from, mask_to);
+        const int st = awcm::interpolate1(h->b1, 0, mask_from, mask_to, 4);
         return st ? st : h->b1.check_status();
     }
     return h->s2->interpolate(mask_from, mask_to, false);
@@ -189,13 +189,20 @@ int awcm_interpolate(awcm_solver* h, const uint8_t* mask_from, const uint8_t* ma
 
 int awcm_interpolate_euler(awcm_solver* h, const uint8_t* mask_from, const uint8_t* mask_to) {
     if (!h || !mask_from || !mask_to) return AWCM_EINVAL;
-    if (h->dim != 2) return AWCM_ESTATE;
+    if (h->dim == 1) {
+        const int st = awcm::interpolate1(h->b1, 0, mask_from, mask_to, 5);
+        return st ? st : h->b1.check_status();
+    }
     return h->s2->interpolate(mask_from, mask_to, true);
 }
 
 int awcm_interpolate_all_grid(awcm_solver* h, const uint8_t* mask_from) {
     if (!h || !mask_from) return AWCM_EINVAL;
-    if (h->dim != 2) return AWCM_ESTATE;
+    if (h->dim == 1) {
+        std::vector<uint8_t> all(size_t(h->b1.h[0].Nx) + 6, 1);
+        const int st = awcm::interpolate1(h->b1, 0, mask_from, all.data(), 6);
+        return st ? st : h->b1.check_status();
+    }
     return h->s2->interpolate_all(mask_from);
 }
 

@@ -63,7 +63,7 @@ int download_q1(Batch1D& B, int idx, double* q);
 int get_masks1(Batch1D& B, int idx, uint8_t* masks, int32_t* levels);
 int upload_scalar1(Batch1D& B, int idx, const double* q);
 int download_scalar1(Batch1D& B, int idx, double* q);
-int interpolate1(Batch1D& B, int idx, const uint8_t* from, const uint8_t* to);
+int interpolate1(Batch1D& B, int idx, const uint8_t* from, const uint8_t* to, int mode);
 int set_mask0_1(Batch1D& B, int idx, const uint8_t* m0);
 int set_schedule1(Batch1D& B, const double* pt, int npt);
 

@@ -468,10 +468,10 @@ int download_scalar1(Batch1D& B, int idx, double* q) {
     return hipMemcpy(q, in.Q, sizeof(double) * (in.Nx + 6), hipMemcpyDeviceToHost) == hipSuccess ? AWCM_OK
                                                                                                 : AWCM_EDEVICE;
 }
-// WAVELET_1D::Interpolate(q, mra, from, to) on instance idx: the kernel's
-// mode 4 reads from and to in the mask_temp and mask3 slots, so the instance's
-// own masks are saved around it and put back.
-int interpolate1(Batch1D& B, int idx, const uint8_t* from, const uint8_t* to) {
+// WAVELET_1D::Interpolate / InterpolateEuler on instance idx (k1_solve modes
+// 4, 5, 6): the kernel reads from and to in the mask_temp and mask3 slots, so
+// the instance's own masks are saved around it and put back.
+int interpolate1(Batch1D& B, int idx, const uint8_t* from, const uint8_t* to, int mode) {
     Inst1D& in = B.h[idx];
     const size_t NP = size_t(in.Nx) + 6;
     std::vector<uint8_t> saved(5 * NP);
@@ -481,7 +481,7 @@ int interpolate1(Batch1D& B, int idx, const uint8_t* from, const uint8_t* to) {
         hipMemcpy(in.masks + 4 * NP, to, NP, hipMemcpyHostToDevice) != hipSuccess)
         return AWCM_EDEVICE;
     int st = B.push();
-    if (!st) st = B.launch(4, 0, idx, 1);
+    if (!st) st = B.launch(mode, 0, idx, 1);
     if (!st) st = B.pull();
     if (hipStreamSynchronize(B.stream) != hipSuccess ||
         hipMemcpy(in.masks, saved.data(), saved.size(), hipMemcpyHostToDevice) != hipSuccess)

@@ -1082,7 +1082,7 @@ inline std::unique_ptr<uint8_t[]> mask_bytes(const FIELD_UNIFORM_1D<bool>& m, in
 }  // namespace awcm_api_detail
 
 // WAVELET_1D (Wavelet/WAVELET_1D.h:11-42): N and weight(i, j) as in WAVELET_2D;
-// the scalar interpolations (WAVELET_1D.h:198-275) run on the GPU, on the
+// the interpolations (WAVELET_1D.h:198-275, 393-473) run on the GPU, on the
 // owning AWCM_1D's device solver (a free-standing one has its own, created on
 // first use from the field's grid).
 class WAVELET_1D {
@@ -1118,18 +1118,47 @@ public:
         run(q, mra, awcm_api_detail::mask_bytes(mask_from, mra.Nx).get(), all.get());
     }
 
+    // WAVELET_1D::InterpolateEuler(q, mra, mask_from, mask_to)
+    // (WAVELET_1D.h:393-431): momentum and energy
+    void InterpolateEuler(FIELD_UNIFORM_1D<VECTOR_3D<double>>& q, const MRA_GRID_INT_1D& mra,
+                          const FIELD_UNIFORM_1D<bool>& mask_from, const FIELD_UNIFORM_1D<bool>& mask_to) const {
+        using awcm_api_detail::check;
+        awcm_solver* h = open(q.ghost_width, q.i_start, q.i_res, q.base_grid.dx, mra);
+        check(awcm_upload_q(h, q.arr.val[0].values), "awcm_upload_q");
+        check(awcm_interpolate_euler(h, awcm_api_detail::mask_bytes(mask_from, mra.Nx).get(),
+                                     awcm_api_detail::mask_bytes(mask_to, mra.Nx).get()),
+              "awcm_interpolate_euler");
+        check(awcm_download_q(h, q.arr.val[0].values), "awcm_download_q");
+    }
+    // WAVELET_1D::InterpolateEuler(q, mra, mask_from) (WAVELET_1D.h:433-473):
+    // every point not in mask_from, all three components
+    void InterpolateEuler(FIELD_UNIFORM_1D<VECTOR_3D<double>>& q, const MRA_GRID_INT_1D& mra,
+                          const FIELD_UNIFORM_1D<bool>& mask_from) const {
+        using awcm_api_detail::check;
+        awcm_solver* h = open(q.ghost_width, q.i_start, q.i_res, q.base_grid.dx, mra);
+        check(awcm_upload_q(h, q.arr.val[0].values), "awcm_upload_q");
+        check(awcm_interpolate_all_grid(h, awcm_api_detail::mask_bytes(mask_from, mra.Nx).get()),
+              "awcm_interpolate_all_grid");
+        check(awcm_download_q(h, q.arr.val[0].values), "awcm_download_q");
+    }
+
 private:
     friend class AWCM_1D;
     mutable awcm_api_detail::Handle own;
     awcm_api_detail::Handle* dev = &own;
+    awcm_solver* open(int ghost, int i_start, int i_res, double dx, const MRA_GRID_INT_1D& mra) const {
+        using awcm_api_detail::check;
+        if (ghost != 3 || i_start != 0 || i_res != mra.Nx)
+            throw std::invalid_argument("WAVELET_1D: the field must span 0..Nx-1 with ghost width 3");
+        if (!dev->h) check(awcm_create_1d(mra.num_levels, mra.Nx_coarse, 1.4, dx, &dev->h), "awcm_create_1d");
+        return dev->h;
+    }
     void run(FIELD_UNIFORM_1D<double>& q, const MRA_GRID_INT_1D& mra, const uint8_t* from, const uint8_t* to) const {
         using awcm_api_detail::check;
-        if (q.ghost_width != 3 || q.i_start != 0 || q.i_res != mra.Nx)
-            throw std::invalid_argument("WAVELET_1D::Interpolate: the field must span 0..Nx-1 with ghost width 3");
-        if (!dev->h) check(awcm_create_1d(mra.num_levels, mra.Nx_coarse, 1.4, q.base_grid.dx, &dev->h), "awcm_create_1d");
-        check(awcm_upload_scalar(dev->h, q.arr.val), "awcm_upload_scalar");
-        check(awcm_interpolate(dev->h, from, to), "awcm_interpolate");
-        check(awcm_download_scalar(dev->h, q.arr.val), "awcm_download_scalar");
+        awcm_solver* h = open(q.ghost_width, q.i_start, q.i_res, q.base_grid.dx, mra);
+        check(awcm_upload_scalar(h, q.arr.val), "awcm_upload_scalar");
+        check(awcm_interpolate(h, from, to), "awcm_interpolate");
+        check(awcm_download_scalar(h, q.arr.val), "awcm_download_scalar");
     }
 };
 

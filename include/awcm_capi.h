@@ -95,8 +95,9 @@ int awcm_prepare_euler(awcm_solver* h, double tol);
  * (Wavelet/WAVELET_2D.h:912-1044), called by Wavelet/main_2d_euler.cpp:334
  * after the time loop: every finest-grid point not in mask_from (ghosted
  * uint8, reference layout) gets its interpolating-wavelet prediction from the
- * coarser levels, all four components, on the device state.  2-D only
- * (AWCM_ESTATE on a 1-D handle).  Synchronous. */
+ * coarser levels, all four components, on the device state.  On a 1-D handle
+ * its counterpart, the one-mask WAVELET_1D::InterpolateEuler(q, mra,
+ * mask_from) (WAVELET_1D.h:433-473), all three components.  Synchronous. */
 int awcm_interpolate_all_grid(awcm_solver* h, const uint8_t* mask_from);
 
 /* ---- the 1-D stage flux ---------------------------------------------------
@@ -140,8 +141,9 @@ int awcm_post_process(awcm_solver* h);
 int awcm_interpolate(awcm_solver* h, const uint8_t* mask_from, const uint8_t* mask_to);
 
 /* WAVELET_2D::InterpolateEuler(q, mra, mask_from, mask_to)
- * (WAVELET_2D.h:784-907): the same on momentum and energy (components 1..3)
- * of the Euler field.  2-D (AWCM_ESTATE on a 1-D handle).  Synchronous. */
+ * (WAVELET_2D.h:784-907) / WAVELET_1D::InterpolateEuler(q, mra, from, to)
+ * (WAVELET_1D.h:393-431): the same on momentum and energy of the Euler field.
+ * Synchronous. */
 int awcm_interpolate_euler(awcm_solver* h, const uint8_t* mask_from, const uint8_t* mask_to);
 
 /* AWCM_*::PostProcessEuler (AWCM_1D.h:122-125, AWCM_2D.h:149-152). */

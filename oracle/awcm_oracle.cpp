@@ -1417,6 +1417,20 @@ void orc1_get_scalar(void* h, double* q) {
     for (int i = 0; i < s->Nx + 6; i++) q[i] = s->Q[3 * i];
 }
 void orc1_prepare_scalar(void* h, double tol) { static_cast<Solver1D*>(h)->prepare_scalar(tol); }
+// WAVELET_1D::InterpolateEuler(q, mra, from, to) (WAVELET_1D.h:393-431: momentum
+// and energy); a null `to` is InterpolateEuler(q, mra, from) (:433-473): every
+// point, all three components
+void orc1_interpolate_euler(void* h, const uint8_t* mask_from, const uint8_t* mask_to) {
+    auto* s = static_cast<Solver1D*>(h);
+    const size_t n = size_t(s->Nx) + 6;
+    std::vector<uint8_t> from(mask_from, mask_from + n), to(n, 1);
+    if (mask_to) {
+        to.assign(mask_to, mask_to + n);
+        s->interp(from, to, 1, 3);
+    } else {
+        s->interp(from, to, 0, 3);
+    }
+}
 // WAVELET_1D::Interpolate(q, mra, from, to) (WAVELET_1D.h:198-233); a null `to`
 // is Interpolate(q, mra, from) (:235-275): every point
 void orc1_interpolate_scalar(void* h, const uint8_t* mask_from, const uint8_t* mask_to) {

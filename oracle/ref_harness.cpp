@@ -365,6 +365,19 @@ void ref1_get_scalar(void* h, double* q) {
     for (int k = 0; k < s->Qs.arr.i_res; k++) q[k] = s->Qs.arr.val[k];
 }
 void ref1_prepare_scalar(void* h, double tol) { auto* s = static_cast<Ref1D*>(h); s->awcm->Prepare(s->Qs, tol); }
+void ref1_interpolate_euler(void* h, const uint8_t* mask_from, const uint8_t* mask_to) {
+    auto* s = static_cast<Ref1D*>(h);
+    FIELD_UNIFORM_1D<bool> from, to;
+    from.Initialize(0, s->mra.Nx, 0.0, 1.0, 3);
+    to.Initialize(0, s->mra.Nx, 0.0, 1.0, 3);
+    for (int k = 0; k < from.arr.i_res; k++) from.arr.val[k] = mask_from[k] != 0;
+    if (!mask_to) {
+        s->awcm->wavelet.InterpolateEuler(s->Q, s->mra, from);
+        return;
+    }
+    for (int k = 0; k < to.arr.i_res; k++) to.arr.val[k] = mask_to[k] != 0;
+    s->awcm->wavelet.InterpolateEuler(s->Q, s->mra, from, to);
+}
 void ref1_interpolate_scalar(void* h, const uint8_t* mask_from, const uint8_t* mask_to) {
     auto* s = static_cast<Ref1D*>(h);
     // ghost width 3, as the masks arrive (FIELD_UNIFORM_1D's copy constructor

@@ -209,6 +209,30 @@ def scalar_1d(L, nxc, seed):
     return out
 
 
+def euler_interp_1d(L, nxc, steps, seed):
+    """WAVELET_1D::InterpolateEuler, both forms, on a Sod state after `steps`
+    steps, from four mask pairs (None: the one-mask form)."""
+    nx = (nxc - 1) * 2 ** (L - 1) + 1
+    q, dx, _ = O.ic1(0, nx, 0.0, 1.0, 0.5, O.SOD_L, O.SOD_R)
+    s = O.Solver1D("ref", L, nxc, 1.4, dx)
+    s.set_q(q)
+    pts = O.print_times(0.2, 41)
+    for _ in range(steps):
+        s.step(1e-4, 0.5, pts)
+    q_in, m = s.get_q(), s.masks()[0]
+    rng = np.random.default_rng(seed)
+    out = dict(levels=L, nx_coarse=nxc, dx=dx, q_in=q_in)
+    pairs = {"m0_all": (m[1], None), "m0_m3": (m[1], m[4]), "m1_m2": (m[2], m[3]),
+             "random_all": ((rng.random(m[1].shape) < 0.3).astype(np.uint8), None)}
+    for name, (f, t) in pairs.items():
+        s.set_q(q_in)
+        s.interpolate_euler(f, t)
+        out[f"from_{name}"], out[f"q_{name}"] = f.astype(np.uint8), s.get_q()
+        if t is not None:
+            out[f"to_{name}"] = t.astype(np.uint8)
+    return out
+
+
 def main():
     if not O.have_ref():
         sys.exit("oracle/_ref/libawcm_ref.so is not built (make -C oracle)")
@@ -222,6 +246,7 @@ def main():
         "all_grid_dmr_2d_L3.npz": lambda: all_grid2("dmr", 3, 17, 5, 6, 32),
         "scalar_box_2d_L4.npz": lambda: scalar_box2(4, 5, 33),
         "scalar_1d_L7.npz": lambda: scalar_1d(7, 17, 34),
+        "euler_interp_1d_L6.npz": lambda: euler_interp_1d(6, 17, 20, 35),
     }
     for name in sys.argv[1:] or list(files):
         np.savez_compressed(os.path.join(HERE, name), **files[name]())

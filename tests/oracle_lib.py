@@ -58,6 +58,7 @@ def load(prefix="orc"):
         f"{p}1_get_scalar": (None, [V, _dp]),
         f"{p}1_prepare_scalar": (None, [V, C.c_double]),
         f"{p}1_interpolate_scalar": (None, [V, _u8p, _u8p]),
+        f"{p}1_interpolate_euler": (None, [V, _u8p, _u8p]),
         f"{p}_flux1_js": (None, [C.c_double, _dp, C.c_double, _dp]),
         f"{p}1_step": (C.c_double, [V, C.c_double, C.c_double, _dp, C.c_int, _ip, _dp]),
         f"{p}2_nx": (C.c_int, [V]),
@@ -208,6 +209,13 @@ class Solver1D:
         f = np.ascontiguousarray(mask_from, dtype=np.uint8)
         t = None if mask_to is None else np.ascontiguousarray(mask_to, dtype=np.uint8)
         getattr(self.lib, f"{self.p}1_interpolate_scalar")(self.h, u8ptr(f), None if t is None else u8ptr(t))
+
+    def interpolate_euler(self, mask_from, mask_to=None):
+        """WAVELET_1D::InterpolateEuler(q, mra, from[, to]); no `to`: every
+        point, all three components."""
+        f = np.ascontiguousarray(mask_from, dtype=np.uint8)
+        t = None if mask_to is None else np.ascontiguousarray(mask_to, dtype=np.uint8)
+        getattr(self.lib, f"{self.p}1_interpolate_euler")(self.h, u8ptr(f), None if t is None else u8ptr(t))
 
     def set_flux(self, kind):
         """Face fluxes of the stages: 0 the WENO-Z pair, 1 the shipped driver's

@@ -433,20 +433,53 @@ def test_2d_interpolate_all_grid_matches_oracle(case):
         assert_same(qg, qo, mg2, mo2, lg2, lo2, what)
 
 
-def test_2d_only_entry_points_reject_1d_handle():
-    """InterpolateAllGridEuler and InterpolateEuler(from, to) are 2-D routines:
-    a 1-D handle is refused (AWCM_ESTATE) and its field is left as it was."""
+def test_set_flux_rejects_2d_handle():
+    """The stage-flux switch is the 1-D solver's (the 2-D drivers use WENO-Z):
+    a 2-D handle is refused with AWCM_ESTATE."""
     P = _pyawcm()
-    q, dx = sod_setup(5, 17)
-    g = P.Solver(1, 5, 17, 1.4, dx)
+    q, dx = dmr_setup(4, 33, 9)
+    g = P.Solver(2, 4, 33, 1.4, dx, ny_coarse=9, x0=0.0, bc_kind=1)
+    with pytest.raises(P.AwcmError) as e:
+        g.set_flux(P.FLUX_WENO5)
+    assert e.value.code == -6  # AWCM_ESTATE
+
+
+def test_1d_interpolate_euler_matches_oracle():
+    """WAVELET_1D::InterpolateEuler through the C ABI: the two-mask form
+    (awcm_interpolate_euler: momentum and energy) and the one-mask form
+    (awcm_interpolate_all_grid on a 1-D handle: every point, all three
+    components), on a Sod state after 20 steps; fields bit-identical to the
+    oracle and the instance's masks unchanged."""
+    P = _pyawcm()
+    q, dx = sod_setup()
+    pts = O.print_times(0.2, 41)
+    g = P.Solver(1, 7, 17, 1.4, dx)
+    g.set_schedule(pts)
     g.upload_q(q)
-    ones = np.ones(g.npts, np.uint8)
-    calls = [lambda: g.interpolate_all_grid(ones), lambda: g.interpolate_euler(ones, ones)]
-    for call in calls:
-        with pytest.raises(P.AwcmError) as e:
-            call()
-        assert e.value.code == -6  # AWCM_ESTATE
-    assert np.array_equal(g.download_q(), q)
+    g.run(1e-4, 0.5, 0.2, 20)
+    o = O.Solver1D("orc", 7, 17, 1.4, dx)
+    o.set_q(q)
+    for _ in range(20):
+        o.step(1e-4, 0.5, pts)
+    qs = g.download_q()
+    mg, lg = g.masks()
+    mo, lo = o.masks()
+    assert_same(qs, o.get_q(), mg, mo, lg, lo, "1-D after 20 steps")
+    rng = np.random.default_rng(6)
+    m = mg
+    for name, (f, t) in {"m0_all": (m[1], None), "m0_m3": (m[1], m[4]), "m1_m2": (m[2], m[3]),
+                         "random_all": ((rng.random(m[1].shape) < 0.3).astype(np.uint8), None)}.items():
+        g.upload_q(qs)
+        o.set_q(qs)
+        if t is None:
+            g.interpolate_all_grid(f)
+        else:
+            g.interpolate_euler(f, t)
+        o.interpolate_euler(f, t)
+        a, b = g.download_q(), o.get_q()
+        assert np.array_equal(a.view(np.uint64), b.view(np.uint64)), f"InterpolateEuler {name}: {int((a != b).sum())} differ"
+        mg2, lg2 = g.masks()
+        assert np.array_equal(mg2, mg) and np.array_equal(lg2, lg), f"InterpolateEuler {name} changed the masks"
 
 
 @pytest.mark.parametrize("L,nxc", [(7, 17), (11, 16)])

@@ -164,6 +164,17 @@ def test_oracle_scalar_1d_matches_golden():
         assert np.array_equal(bits(s.get_scalar()), bits(g[f"qi_{name}"])), name
 
 
+def test_oracle_euler_interp_1d_matches_golden():
+    """WAVELET_1D::InterpolateEuler, both forms, against the reference build's
+    vectors."""
+    g = load("euler_interp_1d_L6.npz")
+    s = O.Solver1D("orc", int(g["levels"]), int(g["nx_coarse"]), 1.4, float(g["dx"]))
+    for name in ("m0_all", "m0_m3", "m1_m2", "random_all"):
+        s.set_q(g["q_in"])
+        s.interpolate_euler(g[f"from_{name}"], g[f"to_{name}"] if f"to_{name}" in g.files else None)
+        assert np.array_equal(bits(s.get_q()), bits(g[f"q_{name}"])), name
+
+
 def test_scalar_1d_interpolation_reproduces_cubics():
     """WAVELET_1D::Interpolate(q, mra, from) with from = the coarsest grid and
     both ends (the points no lattice reaches): a cubic comes back everywhere
