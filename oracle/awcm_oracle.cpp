@@ -13,9 +13,16 @@
 //     on 1-D and 2-D trajectories (masks, levels, fields, dt, t);
 //   * committed golden vectors (tests/golden/) made from that build;
 //   * closed forms and invariants: dyadic wavelet weights, WENO-Z ideal-weight
-//     limit (2,-13,47,27,-3)/60, flux consistency F(Q,..,Q)=F(Q), Roe
-//     eigenvector inversion R*L=I, nested masks, band always active,
-//     eps=0 reduces to the uniform-grid WENO-Z solver.
+//     limit (2,-13,47,27,-3)/60, flux consistency F(Q,..,Q)=F(Q) (WENO-Z and
+//     WENO5-JS), Roe eigenvector inversion R*L=I, nested masks, band always
+//     active, eps=0 reduces to the uniform-grid WENO-Z solver, the
+//     interpolations reproduce cubics / bicubics from the coarsest grid and
+//     leave mask_from's points alone, the scalar path equals the Euler path
+//     on component 0.
+// Entry points beyond the time step: PrepareEuler, the scalar Prepare (1-D
+// and 2-D), Interpolate / InterpolateEuler (both dimensions, both forms),
+// InterpolateAllGridEuler, and the WENO5-JS stage flux of the shipped 1-D
+// driver; each is pinned by the same three means.
 //
 // Layouts (same as the reference API):
 //   1-D state  AoS, ghost 3:  q[(i+3)*3 + c],             i = -3..Nx+2
