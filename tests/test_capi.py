@@ -85,6 +85,12 @@ def test_invalid_arguments_rejected_before_device_work():
     assert lib.awcm_create_2d(7, 4, 17, 1.4, 0.001, 0.0, 1, C.byref(h)) == -1
     assert lib.awcm_create_2d(7, 65, 17, 1.4, 0.001, 0.0, 9, C.byref(h)) == -1  # unknown BC kind
     assert lib.awcm_create_2d(14, 65, 17, 1.4, 0.001, 0.0, 1, C.byref(h)) == -1  # grid too large
+    # list entries pack a grid index in 16 bits: 65535 finest points is the
+    # largest grid, 65537 the next one these parameters reach
+    assert lib.awcm_create_1d(2, 32769, 1.4, 0.001, C.byref(h)) == -1
+    assert lib.awcm_create_2d(2, 32769, 5, 1.4, 0.001, 0.0, 1, C.byref(h)) == -1
+    assert lib.awcm_create_2d(2, 5, 32769, 1.4, 0.001, 0.0, 1, C.byref(h)) == -1
+    assert lib.awcm_create_1d(15, 6, 1.4, 0.001, C.byref(h)) == -1      # more than 14 levels
     assert lib.awcm_run(None, 1e-3, 0.4, 0.2, 1, None, None, None, None) == -1
     assert lib.awcm_interpolate_all_grid(None, None) == -1
     assert lib.awcm_ic_double_mach_2d(1, 1.4, 1.4, None, None) == -1

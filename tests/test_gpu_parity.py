@@ -156,6 +156,33 @@ def test_1d_weno5_flux_matches_oracle():
         assert_within(g.download_q(), o.get_q(), mg, mo, lg, lo, f"driver configuration, {50 * (chunk + 1)} steps")
 
 
+def test_1d_largest_instance_matches_oracle():
+    """The largest 1-D instance the C ABI accepts with the most levels: 14
+    levels over 8 coarse points, 57345 finest points (list entries pack the
+    index in 16 bits, so a grid stops at 65535 points; masks in HBM).  A Sod
+    tube on [0, 1], three runs of 10 steps, masks, levels, t and fields
+    bit-identical to the oracle after each."""
+    P = _pyawcm()
+    L, nxc = 14, 8
+    nx = (nxc - 1) * 2 ** (L - 1) + 1
+    assert nx == 57345
+    q, dx, _ = O.ic1(0, nx, 0.0, 1.0, 0.5, O.SOD_L, O.SOD_R)
+    pts = O.print_times(0.2, 41)
+    g = P.Solver(1, L, nxc, 1.4, dx)
+    g.set_schedule(pts)
+    g.upload_q(q)
+    o = O.Solver1D("orc", L, nxc, 1.4, dx)
+    o.set_q(q)
+    for chunk in range(3):
+        steps, _ = g.run(1e-4, 0.5, 0.2, 10)
+        for _ in range(10):
+            o.step(1e-4, 0.5, pts)
+        assert steps == 10 and g.t == o.t[0], f"chunk {chunk}"
+        mg, lg = g.masks()
+        mo, lo = o.masks()
+        assert_same(g.download_q(), o.get_q(), mg, mo, lg, lo, f"57345 points, {10 * (chunk + 1)} steps")
+
+
 def test_batch_weno5_flux_matches_oracle():
     """awcm_batch_set_flux(AWCM_FLUX_WENO5) on a batch of six Riemann
     problems; then back to WENO-Z for a further run (the batch graph is

@@ -768,3 +768,28 @@ def test_batched_workload_instance_that_breaks_down():
         assert np.isfinite(o.t[0]), n
         o.step(eps, 0.5, pts)
     assert not np.isfinite(o.t[0])
+
+
+@needs_ref
+def test_oracle_matches_reference_on_largest_1d_grid():
+    """14 levels over 8 coarse points (57345 finest points, the most levels
+    and nearly the largest grid the C ABI accepts): 30 steps of a Sod tube,
+    oracle and reference build bit-identical in t, masks, levels and state."""
+    L, nxc = 14, 8
+    nx = (nxc - 1) * 2 ** (L - 1) + 1
+    q, dx, _ = O.ic1(0, nx, 0.0, 1.0, 0.5, O.SOD_L, O.SOD_R)
+    pts = O.print_times(0.2, 41)
+    o = O.Solver1D("orc", L, nxc, 1.4, dx)
+    r = O.Solver1D("ref", L, nxc, 1.4, dx)
+    o.set_q(q)
+    r.set_q(q)
+    for _ in range(30):
+        o.step(1e-4, 0.5, pts)
+        r.step(1e-4, 0.5, pts)
+    assert o.t[0] == r.t[0] > 0
+    mo, lo = o.masks()
+    mr, lr = r.masks()
+    assert all(np.array_equal(a, b) for a, b in zip(mo, mr)) and np.array_equal(lo, lr)
+    assert np.array_equal(o.get_q().view(np.uint64), r.get_q().view(np.uint64))
+    # the discontinuities are resolved down to the finest of the 14 levels
+    assert lo.max() == L - 1 and len(np.unique(lo)) >= L - 2
