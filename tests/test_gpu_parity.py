@@ -288,6 +288,35 @@ def test_2d_steps_match_oracle(case, tail_cells):
         assert_same(g.download_q(), o.get_q(), mg, mo, lg, lo, f"2-D {case} step {n}")
 
 
+@pytest.mark.parametrize("shape", [(4096, 5), (5, 4096)])
+def test_2d_index_limit_strip_matches_oracle(shape):
+    """Rows and columns at the 16-bit limit of the packed list entries: a
+    65521 x 65 strip and its transpose (5 levels), square cells, the quadrant
+    problem with transmissive boundaries.  Two steps, masks, levels, t and
+    fields bit-identical to the oracle after each (on the CPU the oracle
+    matches the reference build on the same two steps, both ways round)."""
+    P = _pyawcm()
+    L, (nxc, nyc) = 5, shape
+    nx, ny = (nxc - 1) * 16 + 1, (nyc - 1) * 16 + 1
+    assert max(nx, ny) == 65521
+    h = 1.0 / (max(nx, ny) - 1)
+    xu, yu = (nx - 1) * h, (ny - 1) * h
+    q, dx, _ = O.ic2(0, nx, ny, 0.0, xu, 0.0, yu, 0.5)
+    pts = O.print_times(0.25, 9)
+    g = P.Solver(2, L, nxc, 1.4, dx, ny_coarse=nyc, x0=0.0, bc_kind=0)
+    g.set_schedule(pts)
+    o = O.Solver2D("orc", L, nxc, nyc, 1.4, 0, 0.0, xu, 0.0, yu)
+    g.upload_q(q)
+    o.set_q(q)
+    for n in range(2):
+        o.step(1e-3, 0.4, pts)
+        g.run(1e-3, 0.4, 0.25, 1)
+        assert g.t == o.t[0] and g.cnt == o.cnt.value, f"step {n}"
+        mg, lg = g.masks()
+        mo, lo = o.masks()
+        assert_same(g.download_q(), o.get_q(), mg, mo, lg, lo, f"{nx} x {ny} strip, step {n}")
+
+
 @pytest.mark.parametrize("tail_cells", [4096, 256, 0])
 def test_2d_multistep_run_matches_oracle(tail_cells):
     """Several steps queued in one awcm_run call (graph replays, no host round

@@ -793,3 +793,30 @@ def test_oracle_matches_reference_on_largest_1d_grid():
     assert np.array_equal(o.get_q().view(np.uint64), r.get_q().view(np.uint64))
     # the discontinuities are resolved down to the finest of the 14 levels
     assert lo.max() == L - 1 and len(np.unique(lo)) >= L - 2
+
+
+@needs_ref
+@pytest.mark.parametrize("shape", [(4096, 5), (5, 4096)])
+def test_oracle_matches_reference_on_index_limit_strip(shape):
+    """A 65521 x 65 strip and its transpose (5 levels, square cells, the
+    quadrant problem): rows and columns at the 16-bit limit of the device's
+    packed list entries.  Two steps, bit-identical in t, masks, levels and
+    state."""
+    L, (nxc, nyc) = 5, shape
+    nx, ny = (nxc - 1) * 16 + 1, (nyc - 1) * 16 + 1
+    h = 1.0 / (max(nx, ny) - 1)
+    xu, yu = (nx - 1) * h, (ny - 1) * h
+    q, _, _ = O.ic2(0, nx, ny, 0.0, xu, 0.0, yu, 0.5)
+    pts = O.print_times(0.25, 9)
+    o = O.Solver2D("orc", L, nxc, nyc, 1.4, 0, 0.0, xu, 0.0, yu)
+    r = O.Solver2D("ref", L, nxc, nyc, 1.4, 0, 0.0, xu, 0.0, yu)
+    o.set_q(q)
+    r.set_q(q)
+    for _ in range(2):
+        o.step(1e-3, 0.4, pts)
+        r.step(1e-3, 0.4, pts)
+    assert o.t[0] == r.t[0] > 0
+    mo, lo = o.masks()
+    mr, lr = r.masks()
+    assert all(np.array_equal(a, b) for a, b in zip(mo, mr)) and np.array_equal(lo, lr)
+    assert np.array_equal(o.get_q().view(np.uint64), r.get_q().view(np.uint64))
