@@ -81,7 +81,7 @@ extern "C" {
 
 int awcm_adjoint_record(awcm_solver* h, double tol, double cfl, double t_final, int max_steps, int checkpoint_every,
                         double* t, int* cnt, int* steps_done) {
-    if (!h || !t || !cnt || max_steps <= 0 || checkpoint_every <= 0 || !(cfl > 0)) return AWCM_EINVAL;
+    if (!h || !t || !cnt || *cnt < 0 || max_steps <= 0 || checkpoint_every <= 0 || !(cfl > 0)) return AWCM_EINVAL;
     if (h->dim != 1) return AWCM_ESTATE;
     Batch1D& B = h->b1;
     awcm::Inst1D& in = B.h[0];

@@ -222,7 +222,8 @@ int awcm_post_process_euler(awcm_solver* h) {
 
 int awcm_run(awcm_solver* h, double tol, double cfl, double t_final, int max_steps, double* t, int* cnt,
              int* steps_done, long long* updates) {
-    if (!h || !t || !cnt || max_steps < 0 || !(cfl > 0)) return AWCM_EINVAL;
+    // cnt indexes the print schedule on the device: a negative one is refused
+    if (!h || !t || !cnt || *cnt < 0 || max_steps < 0 || !(cfl > 0)) return AWCM_EINVAL;
     if (h->dim == 1) {
         Batch1D& B = h->b1;
         awcm::Inst1D& in = B.h[0];
@@ -267,7 +268,7 @@ int awcm_run(awcm_solver* h, double tol, double cfl, double t_final, int max_ste
 
 int awcm_run_timed(awcm_solver* h, double tol, double cfl, double t_final, int n_steps, double* t, int* cnt,
                    int* steps_done, long long* updates, double* elapsed_ms) {
-    if (!h || !t || !cnt || n_steps < 0 || !(cfl > 0)) return AWCM_EINVAL;
+    if (!h || !t || !cnt || *cnt < 0 || n_steps < 0 || !(cfl > 0)) return AWCM_EINVAL;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return AWCM_EDEVICE;
     int st = AWCM_OK;

@@ -34,6 +34,31 @@ struct Emu {
     int nxp() const { return (NP() + 15) / 16 * 16; }
 };
 
+// Poison: buffers the device side leaves uninitialised (hipMalloc) are filled
+// with a NaN pattern, and the emulated LDS with 0xA5 bytes before every
+// launch, so a read of anything not written first shows up in the results.
+bool poison = false;
+double nan_pattern() {
+    const uint64_t b = 0x7ff4dead7ff4deadULL;
+    double d;
+    std::memcpy(&d, &b, 8);
+    return d;
+}
+void poison_lds() {
+    if (!poison) return;
+    std::memset(sm, 0xA5, sizeof(sm));
+    std::memset(lds_masks, 0xA5, sizeof(lds_masks));
+}
+template <class T>
+void fill_new(std::vector<T>& v, size_t n, T zero) {
+    if (poison) {
+        if constexpr (sizeof(T) == 8) v.assign(n, nan_pattern());
+        else v.assign(n, T(0xA5));
+    } else {
+        v.assign(n, zero);
+    }
+}
+
 // with masks_in_hbm, k1_solve<true> keeps the masks in a heap slice instead
 // of the emulated LDS (the path of instances too long for one workgroup's LDS)
 bool masks_in_hbm = false;
@@ -42,6 +67,7 @@ int stage_flux = kRecWenoZ;  // face fluxes of the stages (kRecWenoZ / kRecWeno5
 template <int FLUX>
 void launch_solve_f(Emu& e, int mode, int max_steps) {
     int order = 0, qctr = 0;
+    poison_lds();
     if (masks_in_hbm) {
         std::vector<uint8_t> hbm(size_t(11) * e.nxp(), 0xA5);
         emu_launch(EMU_THREADS,
@@ -76,7 +102,7 @@ void* emu_create(int L, int nxc, double gamma, double dx, const double* pt, int 
     e->Q.assign(3 * NP, 0.0);
     e->Qc = e->Q;
     e->Qt = e->Q;
-    e->F.assign(3 * (in.Nx + 1), 0.0);
+    fill_new(e->F, 3 * size_t(in.Nx + 1), 0.0);  // not cleared by the device runtime either
     e->masks.assign(5 * NP, 0);
     std::fill(e->masks.begin() + NP, e->masks.begin() + 2 * NP, 1);
     e->levels.assign(4 * in.Nx, 0);
@@ -103,21 +129,17 @@ void emu_get_q(void* h, double* q) {
         for (int c = 0; c < 3; c++) q[i * 3 + c] = e->Q[c * NP + i];
 }
 
-int emu_record(void* h, double tol, double cfl, double t_final, int max_steps, int K) {
+// A recording from the instance's current state, masks and levels, as
+// awcm_adjoint_record does: t and the print counter are the caller's.
+int emu_record_from(void* h, double tol, double cfl, double t_final, int max_steps, int K, double t0, int cnt0) {
     auto* e = static_cast<Emu*>(h);
     Inst1D& in = e->in;
-    // each recording starts a fresh run: t = 0, print counter 1, the masks and
-    // levels of a new solver (mask0 all ones, AWCM_1D.h:23-34)
-    const int NPr = e->NP();
-    in.t = 0.0; in.cnt = 1; in.steps = 0; in.sum_active0 = 0; in.status = 0;
-    std::fill(e->masks.begin(), e->masks.end(), 0);
-    std::fill(e->masks.begin() + NPr, e->masks.begin() + 2 * NPr, 1);
-    std::fill(e->levels.begin(), e->levels.end(), 0);
+    in.t = t0; in.cnt = cnt0; in.steps = 0; in.sum_active0 = 0; in.status = 0;
     const long long NP = e->NP();
     const long long stride = (5 * NP + 3LL * in.Nx + 255) / 256 * 256;
-    e->tape_m.assign(size_t(stride) * max_steps, 0);
-    e->tape_s.assign(4 * size_t(max_steps), 0.0);
-    e->ckpt.assign(3 * NP * size_t(max_steps / K + 1), 0.0);
+    fill_new(e->tape_m, size_t(stride) * max_steps, uint8_t(0));
+    fill_new(e->tape_s, 4 * size_t(max_steps), 0.0);
+    fill_new(e->ckpt, 3 * NP * size_t(max_steps / K + 1), 0.0);
     e->K = K;
     in.tol = tol; in.cfl = cfl; in.t_final = t_final;
     in.tape_m = e->tape_m.data(); in.tape_s = e->tape_s.data(); in.ckpt = e->ckpt.data();
@@ -125,8 +147,22 @@ int emu_record(void* h, double tol, double cfl, double t_final, int max_steps, i
     launch_solve(*e, 1, max_steps);
     e->steps = in.tape_n;
     in.tape_m = nullptr; in.tape_s = nullptr; in.ckpt = nullptr;
+    in.tape_n = 0; in.tape_cap = 0;
     return e->steps;
 }
+
+// Each recording of a fresh run: t = 0, print counter 1, the masks and levels
+// of a new solver (mask0 all ones, AWCM_1D.h:23-34).
+int emu_record(void* h, double tol, double cfl, double t_final, int max_steps, int K) {
+    auto* e = static_cast<Emu*>(h);
+    const int NPr = e->NP();
+    std::fill(e->masks.begin(), e->masks.end(), 0);
+    std::fill(e->masks.begin() + NPr, e->masks.begin() + 2 * NPr, 1);
+    std::fill(e->levels.begin(), e->levels.end(), 0);
+    return emu_record_from(h, tol, cfl, t_final, max_steps, K, 0.0, 1);
+}
+
+void emu_set_poison(int on) { poison = on != 0; }
 
 static Adj1DArgs args(Emu& e, int seg_steps) {
     Adj1DArgs A{};
@@ -138,12 +174,12 @@ static Adj1DArgs args(Emu& e, int seg_steps) {
     A.n_steps = e.steps;
     A.ckpt_every = e.K;
     A.seg_steps = seg_steps;
-    e.seg.assign(9 * NP * size_t(seg_steps), 0.0);
-    e.Qb.assign(3 * NP, 0.0);
-    e.Xb.assign(3 * NP, 0.0);
-    e.Cb.assign(3 * NP, 0.0);
-    e.Gb.assign(3 * NP, 0.0);
-    e.FV.assign(18 * size_t(e.in.Nx + 1), 0.0);
+    fill_new(e.seg, 9 * NP * size_t(seg_steps), 0.0);
+    fill_new(e.Qb, 3 * NP, 0.0);
+    fill_new(e.Xb, 3 * NP, 0.0);
+    fill_new(e.Cb, 3 * NP, 0.0);
+    fill_new(e.Gb, 3 * NP, 0.0);
+    fill_new(e.FV, 18 * size_t(e.in.Nx + 1), 0.0);
     A.seg = e.seg.data(); A.Qb = e.Qb.data(); A.Xb = e.Xb.data(); A.Cb = e.Cb.data();
     A.Gb = e.Gb.data(); A.FV = e.FV.data();
     return A;
@@ -152,6 +188,7 @@ static Adj1DArgs args(Emu& e, int seg_steps) {
 void emu_replay(void* h) {
     auto* e = static_cast<Emu*>(h);
     const Adj1DArgs A = args(*e, e->K);
+    poison_lds();
     emu_launch(EMU_THREADS, [&] { k1_adjoint(&e->in, A, 0, e->nxp()); });
 }
 
@@ -160,8 +197,11 @@ void emu_adjoint(void* h, const double* gf, int seg_steps, double* g0) {
     const int NP = e->NP();
     std::vector<double> Qsave = e->Q;
     Adj1DArgs A = args(*e, seg_steps ? seg_steps : e->K);
+    // the C ABI uploads the whole adjoint state, ghosts as zeros
+    std::fill(e->Qb.begin(), e->Qb.end(), 0.0);
     for (int i = 3; i < NP - 3; i++)
         for (int c = 0; c < 3; c++) e->Qb[c * NP + i] = gf[i * 3 + c];
+    poison_lds();
     emu_launch(EMU_THREADS, [&] { k1_adjoint(&e->in, A, 1, e->nxp()); });
     e->Q = Qsave;
     e->in.Q = e->Q.data();

@@ -92,6 +92,15 @@ def test_invalid_arguments_rejected_before_device_work():
     assert lib.awcm_create_2d(2, 5, 32769, 1.4, 0.001, 0.0, 1, C.byref(h)) == -1
     assert lib.awcm_create_1d(15, 6, 1.4, 0.001, C.byref(h)) == -1      # more than 14 levels
     assert lib.awcm_run(None, 1e-3, 0.4, 0.2, 1, None, None, None, None) == -1
+    # a negative print counter would index the device's print schedule below
+    # its start: refused before the handle is touched (a dummy handle here)
+    dummy = C.create_string_buffer(64)
+    t, cnt = np.zeros(1), C.c_int(-1)
+    td = t.ctypes.data_as(C.POINTER(C.c_double))
+    assert lib.awcm_run(C.cast(dummy, C.c_void_p), 1e-3, 0.4, 0.2, 1, td, C.byref(cnt), None, None) == -1
+    assert lib.awcm_run_timed(C.cast(dummy, C.c_void_p), 1e-3, 0.4, 0.2, 1, td, C.byref(cnt), None, None,
+                              None) == -1
+    assert lib.awcm_adjoint_record(C.cast(dummy, C.c_void_p), 1e-3, 0.4, 0.2, 1, 1, td, C.byref(cnt), None) == -1
     assert lib.awcm_interpolate_all_grid(None, None) == -1
     assert lib.awcm_ic_double_mach_2d(1, 1.4, 1.4, None, None) == -1
 
