@@ -15,7 +15,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SRC = ["csrc/awcm1d.hip", "csrc/adjoint1d.hip", "csrc/runtime1d.hip", "csrc/awcm2d.hip", "csrc/capi.hip", "csrc/problems.cpp"]
+SRC = ["csrc/awcm1d.hip", "csrc/adjoint1d.hip", "csrc/runtime1d.hip", "csrc/awcm2d.hip", "csrc/capi.hip", "csrc/noalloc1d.hip", "csrc/problems.cpp"]
 OUT = os.path.join(HERE, "libawcm_hip.so")
 ARCH = os.environ.get("AWCM_OFFLOAD_ARCH", "gfx950")
 

@@ -297,6 +297,32 @@ int awcm_batch_run_sequential(awcm_batch* b, int max_steps);
 int awcm_batch_state(awcm_batch* b, int idx, double* t, int* cnt, int* steps, long long* updates, int* status);
 int awcm_batch_synchronize(awcm_batch* b);
 
+/* ---- the 1-D solve without allocation ------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): awcm_run's 1-D time loop (same kernel, bit-identical
+ * results) on caller-owned device buffers, allocating nothing.  The library's
+ * working memory (descriptor, schedule, work counter, RK copy, stage input,
+ * face fluxes) is one fixed device array per GPU, about 0.6 MB, held until the
+ * process ends.  Limits: Nx = (nx_coarse-1)*2^(num_levels-1)+1 <= 8193 and
+ * n_print <= 64; larger requests return AWCM_EINVAL.  Calls that overlap on
+ * one GPU share that array and race; each call runs on the null stream and
+ * returns after the solve.
+ *   p            levels, coarse points, gamma, dx, tol, cfl, t_final
+ *   flux         AWCM_FLUX_WENOZ or AWCM_FLUX_WENO5
+ *   print_times  host array of n_print output times (the schedule)
+ *   q            device, in/out: 3 planes (rho, rho u, E) of Nx+6 doubles,
+ *                q[c*(Nx+6) + i+3] for i = -3 .. Nx+2
+ *   masks        device, in/out: mask_temp, mask0..mask3, each Nx+6 bytes
+ *                (ghosted); a new solve starts from mask0 all 1, the others 0
+ *   levels       device, in/out: level0..level3, each Nx bytes
+ *   t, cnt, steps_done, updates   as awcm_run */
+#define AWCM_NOALLOC_MAX_NX 8193
+#define AWCM_NOALLOC_MAX_PRINT 64
+int awcm_run_1d_noalloc(const awcm_instance_1d* p, int flux, const double* print_times, int n_print, double* q,
+                        uint8_t* masks, uint8_t* levels, int max_steps, double* t, int* cnt, int* steps_done,
+                        long long* updates);
+
 #ifdef __cplusplus
 }
 #endif
