@@ -1,8 +1,9 @@
 // Host side of the 1-D discrete adjoint: tape allocation and the C ABI entry
 // points awcm_adjoint_record / awcm_adjoint_replay / awcm_adjoint.  The device
-// code (k1_adjoint) is in adjoint1d_dev.h.
+// code is in adjoint1d_dev.h (frozen replay and segment recompute, one
+// workgroup) and adjoint1d_rev.h (the reverse sweep, grid-wide kernels).
 
-#include "adjoint1d_dev.h"
+#include "adjoint1d_rev.h"
 
 #include <algorithm>
 #include <cstring>
@@ -15,6 +16,8 @@
 namespace awcm {
 
 Adj1DHost::~Adj1DHost() {
+    if (gexec) hipGraphExecDestroy(gexec);
+    if (graph) hipGraphDestroy(graph);
     for (void* p : allocs) hipFree(p);
 }
 
@@ -41,11 +44,7 @@ Adj1DArgs args_of(const Adj1DHost& H) {
     A.ckpt_every = H.K;
     A.seg_steps = H.seg_steps;
     A.seg = H.seg;
-    A.Qb = H.Qb;
-    A.Xb = H.Xb;
-    A.Cb = H.Cb;
-    A.Gb = H.Gb;
-    A.FV = H.FV;
+    A.step = H.d_step;
     return A;
 }
 
@@ -91,7 +90,7 @@ int awcm_adjoint_record(awcm_solver* h, double tol, double cfl, double t_final, 
     if (!h->adj) return AWCM_ENOMEM;
     Adj1DHost& H = *h->adj;
     const long long NP = in.Nx + 2 * awcm::G;
-    H.stride = (5 * NP + 3LL * in.Nx + 255) / 256 * 256;
+    H.stride = awcm::adj_tape_stride(in.Nx);
     H.K = checkpoint_every;
     H.cap = max_steps;
     int st = H.dalloc(&H.tape_m, size_t(H.stride) * max_steps);
@@ -126,21 +125,18 @@ int awcm_adjoint_replay(awcm_solver* h) {
     return st;
 }
 
-// The backward launch is switched off: its first device run ended in a GPU
-// memory fault that is not yet explained (see the header).
-constexpr bool kAdjointBackwardEnabled = false;
-
 int awcm_adjoint(awcm_solver* h, const double* dJ_dQfinal, int checkpoint_every, double* dJ_dQ0) {
     if (!h || !dJ_dQfinal || !dJ_dQ0 || checkpoint_every < 0) return AWCM_EINVAL;
     if (h->dim != 1 || !h->adj) return AWCM_ESTATE;
-    if (!kAdjointBackwardEnabled) return AWCM_EDISABLED;
     Adj1DHost& H = *h->adj;
     const int K = checkpoint_every ? checkpoint_every : H.K;
     if (K % H.K) return AWCM_EINVAL;
     Batch1D& B = h->b1;
     awcm::Inst1D& in = B.h[0];
     const size_t NP = size_t(in.Nx) + 2 * awcm::G;
-    int st = AWCM_OK;
+    const int nxp = awcm::nxp_of(in);
+    int st = awcm::adjoint_fits(nxp);
+    if (st) return st;
     if (!H.Qb) {
         st = H.dalloc(&H.Qb, sizeof(double) * 3 * NP);
         if (!st) st = H.dalloc(&H.Xb, sizeof(double) * 3 * NP);
@@ -148,28 +144,65 @@ int awcm_adjoint(awcm_solver* h, const double* dJ_dQfinal, int checkpoint_every,
         if (!st) st = H.dalloc(&H.Gb, sizeof(double) * 3 * NP);
         if (!st) st = H.dalloc(&H.FV, sizeof(double) * 18 * (size_t(in.Nx) + 1));
         if (!st) st = H.dalloc(&H.Qsave, sizeof(double) * 3 * NP);
+        if (!st) st = H.dalloc(&H.d_step, sizeof(int));
         if (st) return st;
     }
-    if (H.seg_steps < K) {
+    if (H.seg_cap < K) {
         if ((st = H.dalloc(&H.seg, sizeof(double) * 9 * NP * size_t(K)))) return st;
-        H.seg_steps = K;
+        H.seg_cap = K;
     }
-    // the kernel reads seg_steps as the segment length
-    Adj1DHost Hk = H;
-    Hk.allocs.clear();
-    Hk.seg_steps = K;
+    H.seg_steps = K;
+    if (H.steps == 0) {  // nothing recorded: the gradient passes through unchanged
+        for (size_t i = 0; i < NP; i++)
+            for (int c = 0; c < 3; c++)
+                dJ_dQ0[i * 3 + c] = (i < size_t(awcm::G) || i >= NP - awcm::G) ? 0.0 : dJ_dQfinal[i * 3 + c];
+        return AWCM_OK;
+    }
+    // one reversed step, captured once per segment length and replayed for
+    // every step: recompute the segment if the step ends it, reverse the step,
+    // move the step counter back
+    if (!H.gexec || H.g_K != K || H.g_seg != H.seg) {
+        if (H.gexec) { hipGraphExecDestroy(H.gexec); H.gexec = nullptr; }
+        if (H.graph) { hipGraphDestroy(H.graph); H.graph = nullptr; }
+        awcm::AdjRev R;
+        std::memset(&R, 0, sizeof(R));
+        R.Nx = in.Nx; R.L = in.L; R.NP = int(NP); R.K = K;
+        R.gamma = in.gamma; R.dx = in.dx;
+        R.tape_m = H.tape_m; R.stride = H.stride; R.tape_s = H.tape_s; R.seg = H.seg;
+        R.Qb = H.Qb; R.Xb = H.Xb; R.Cb = H.Cb; R.Gb = H.Gb; R.FV = H.FV;
+        R.step = H.d_step;
+        const awcm::Adj1DArgs A = awcm::args_of(H);
+        hipStream_t stream = B.stream;
+        if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return AWCM_EDEVICE;
+        hipLaunchKernelGGL(awcm::k1_adjoint, dim3(1), dim3(1024), size_t(11) * nxp, stream, B.d_inst, A, 2, nxp);
+        awcm::launch_reverse_step(R, stream);
+        hipLaunchKernelGGL(awcm::k1r_next, dim3(1), dim3(64), 0, stream, H.d_step);
+        if (hipStreamEndCapture(stream, &H.graph) != hipSuccess) return AWCM_EDEVICE;
+        if (hipGraphInstantiate(&H.gexec, H.graph, nullptr, nullptr, 0) != hipSuccess) return AWCM_EDEVICE;
+        H.g_K = K;
+        H.g_seg = H.seg;
+    }
     std::vector<double> soa(3 * NP, 0.0);
     for (size_t i = awcm::G; i < NP - awcm::G; i++)
         for (int c = 0; c < 3; c++) soa[c * NP + i] = dJ_dQfinal[i * 3 + c];
+    const int last = H.steps - 1;
     if (hipStreamSynchronize(B.stream) != hipSuccess) return AWCM_EDEVICE;
-    if (hipMemcpy(H.Qb, soa.data(), sizeof(double) * 3 * NP, hipMemcpyHostToDevice) != hipSuccess) return AWCM_EDEVICE;
-    if (hipMemcpy(H.Qsave, in.Q, sizeof(double) * 3 * NP, hipMemcpyDeviceToDevice) != hipSuccess) return AWCM_EDEVICE;
+    if (hipMemcpyAsync(H.Qb, soa.data(), sizeof(double) * 3 * NP, hipMemcpyHostToDevice, B.stream) != hipSuccess ||
+        hipMemcpyAsync(H.Qsave, in.Q, sizeof(double) * 3 * NP, hipMemcpyDeviceToDevice, B.stream) != hipSuccess ||
+        hipMemcpyAsync(H.d_step, &last, sizeof(int), hipMemcpyHostToDevice, B.stream) != hipSuccess)
+        return AWCM_EDEVICE;
     in.status = 0;
     st = B.push();
-    if (!st) st = awcm::launch_adjoint(B, Hk, 1);
-    if (hipMemcpy(in.Q, H.Qsave, sizeof(double) * 3 * NP, hipMemcpyDeviceToDevice) != hipSuccess && !st) st = AWCM_EDEVICE;
+    for (int n = 0; n < H.steps && !st; n++)
+        if (hipGraphLaunch(H.gexec, B.stream) != hipSuccess) st = AWCM_EDEVICE;
+    if (hipMemcpyAsync(in.Q, H.Qsave, sizeof(double) * 3 * NP, hipMemcpyDeviceToDevice, B.stream) != hipSuccess && !st)
+        st = AWCM_EDEVICE;
+    if (!st && hipMemcpyAsync(soa.data(), H.Qb, sizeof(double) * 3 * NP, hipMemcpyDeviceToHost, B.stream) != hipSuccess)
+        st = AWCM_EDEVICE;
+    const int pst = B.pull();  // synchronises the stream
+    if (!st) st = pst;
+    if (!st && B.h[0].status) st = B.h[0].status == -5 ? AWCM_ERANGE : AWCM_EDEVICE;
     if (st) return st;
-    if (hipMemcpy(soa.data(), H.Qb, sizeof(double) * 3 * NP, hipMemcpyDeviceToHost) != hipSuccess) return AWCM_EDEVICE;
     for (size_t i = 0; i < NP; i++)
         for (int c = 0; c < 3; c++)
             dJ_dQ0[i * 3 + c] = (i < size_t(awcm::G) || i >= NP - awcm::G) ? 0.0 : soa[c * NP + i];

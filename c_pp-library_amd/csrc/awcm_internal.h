@@ -27,10 +27,12 @@ struct Inst1D {
     int pad;
     // Trajectory tape for the discrete adjoint (null when not recording).  Step
     // n of the recording stores, at tape_m + n * tape_stride, the masks
-    // mask_temp, mask0..mask3 (5 x (Nx+6) bytes) and level0..level2 (3 x Nx
-    // bytes) that the step used, at tape_s + 4n its dt and the three stage
-    // alphas, and every ckpt_every steps the state Q at the step's start
-    // (3 x (Nx+6) doubles) at ckpt + (n / ckpt_every) * 3 * (Nx+6).
+    // mask_temp, mask0..mask3 (5 x (Nx+6) bytes), level0..level2 (3 x Nx
+    // bytes) and the interpolation set of its PrepareEuler (Nx+6 bytes: the
+    // points of CheckReconstruction(mask3 minus mask_temp)) that the step
+    // used, at tape_s + 4n its dt and the three stage alphas, and every
+    // ckpt_every steps the state Q at the step's start (3 x (Nx+6) doubles) at
+    // ckpt + (n / ckpt_every) * 3 * (Nx+6).
     uint8_t* tape_m;
     double* tape_s;
     double* ckpt;
@@ -53,11 +55,14 @@ struct Adj1DArgs {
     int seg_steps;    // steps per recompute segment (a multiple of ckpt_every)
     int pad;
     double* seg;      // seg_steps x 3 stage inputs x 3 planes x (Nx+6)
-    double* Qb;       // 3 planes x (Nx+6): adjoint state
-    double* Xb;       // 3 planes x (Nx+6)
-    double* Cb;       // 3 planes x (Nx+6): adjoint of the RK copy
-    double* Gb;       // 3 planes x (Nx+6): adjoint of dF per point
-    double* FV;       // (Nx+1) x 18: face-flux products
+    const int* step;  // the step being reversed (device scalar)
 };
+
+// Byte size of one step of the adjoint tape for Nx finest points (masks,
+// levels and interpolation set, rounded up to 256 bytes).
+inline long long adj_tape_stride(int Nx) {
+    const long long NP = Nx + 6;
+    return (6 * NP + 3LL * Nx + 255) / 256 * 256;
+}
 
 }  // namespace awcm

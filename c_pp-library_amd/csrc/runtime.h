@@ -76,8 +76,16 @@ struct Adj1DHost {
     long long stride = 0;
     int cap = 0, steps = 0, K = 0;
     double* seg = nullptr;
-    int seg_steps = 0;
+    int seg_cap = 0;      // steps the segment buffer holds
+    int seg_steps = 0;    // steps per recompute segment of the current sweep
     double *Qb = nullptr, *Xb = nullptr, *Cb = nullptr, *Gb = nullptr, *FV = nullptr, *Qsave = nullptr;
+    int* d_step = nullptr;  // the step being reversed (device)
+    // graph of one reversed step (segment recompute, reverse kernels, step
+    // counter), for segment length g_K over segment buffer g_seg
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int g_K = 0;
+    const double* g_seg = nullptr;
     ~Adj1DHost();
     int dalloc(void* pp, size_t bytes);
 };

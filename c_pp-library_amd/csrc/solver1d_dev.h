@@ -478,6 +478,8 @@ __device__ void tape_masks(const Ctx1& c, int n) {
     }
     for (int i = threadIdx.x; i < c.g.Nx; i += blockDim.x)
         for (int k = 0; k < 3; k++) dst[5 * c.NP + k * c.g.Nx + i] = c.lv[k][i];
+    // the interpolation set PrepareEuler's InterpolateEuler just built (c.tmp)
+    for (int i = threadIdx.x; i < c.NP; i += blockDim.x) dst[5 * c.NP + 3 * c.g.Nx + i] = c.tmp[i];
 }
 
 // One iteration of the driver loop (main_1d_euler.cpp:61-260).  With a tape

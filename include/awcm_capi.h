@@ -170,10 +170,11 @@ int awcm_run(awcm_solver* h, double tol, double cfl, double t_final, int max_ste
  * and records for every step its masks mask_temp/mask0..3, levels level0..2,
  * dt and the three stage alphas, plus the state at the start of every
  * checkpoint_every-th step.  Replaces any earlier recording.  max_steps also
- * sizes the tape (about 8 (Nx+6) bytes per step plus 24 (Nx+6) bytes per
- * checkpoint).  Replay and reverse keep the masks in LDS, so they return
- * AWCM_EINVAL for instances longer than one workgroup's LDS holds (about
- * 14 800 finest points on gfx950); recording has no such limit. */
+ * sizes the tape (about 9 (Nx+6) bytes per step plus 24 (Nx+6) bytes per
+ * checkpoint).  Replay and the segment recompute keep the masks in LDS, so
+ * replay and awcm_adjoint return AWCM_EINVAL for instances longer than one
+ * workgroup's LDS holds (about 14 800 finest points on gfx950); recording has
+ * no such limit. */
 int awcm_adjoint_record(awcm_solver* h, double tol, double cfl, double t_final, int max_steps, int checkpoint_every,
                         double* t, int* cnt, int* steps_done);
 
@@ -187,15 +188,13 @@ int awcm_adjoint_replay(awcm_solver* h);
  * masks, levels, dt and alphas), by checkpoint-and-recompute: the steps are
  * reversed in segments of checkpoint_every steps (a positive multiple of the
  * recording's spacing; 0 = the recording's), each recomputed from its
- * checkpoint.  Both arrays use the layout of awcm_upload_q; ghost entries of
- * dJ_dQfinal are ignored and those of dJ_dQ0 are returned as 0.  The device
- * state Q is left unchanged.  Deterministic: the result does not depend on
- * the segment length and is identical between calls.
- * Returns AWCM_EDISABLED in this build: the backward kernel's first device
- * run ended in a GPU memory fault whose cause is not yet identified (its
- * device code runs clean, and matches finite differences, in the host
- * emulation of tests/native/emu1d.cpp, including under AddressSanitizer with
- * 1024 emulated threads), so it is not launched on a GPU until it is. */
+ * checkpoint (one workgroup, the frozen replay) with its stage inputs kept in
+ * HBM, then reversed step by step by grid-wide gather kernels (one reversed
+ * step is a HIP graph replayed for every step).  Both arrays use the layout of
+ * awcm_upload_q; ghost entries of dJ_dQfinal are ignored and those of dJ_dQ0
+ * are returned as 0.  The device state Q is left unchanged.  Deterministic:
+ * the result does not depend on the segment length and is identical between
+ * calls.  Synchronous. */
 int awcm_adjoint(awcm_solver* h, const double* dJ_dQfinal, int checkpoint_every, double* dJ_dQ0);
 
 /* As awcm_run, but all n_steps are queued with no host synchronisation in

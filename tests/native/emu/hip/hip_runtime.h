@@ -25,6 +25,8 @@ inline EmuDim3 blockIdx;
 inline EmuDim3 blockDim{1, 1, 1};
 inline EmuDim3 gridDim{1, 1, 1};
 
+typedef void* hipStream_t;
+
 inline void __syncthreads() {}
 inline double __shfl_xor(double v, int, int = 64) { return v; }
 inline long long __double_as_longlong(double v) {

@@ -12,8 +12,18 @@
 #define EMU_THREADS 1
 #endif
 
+// the reverse sweep's grid launches become loops over the work items, one
+// emulated thread per workgroup of one item
+#define AWCM_REV_LAUNCH(kern, items, ...)                                       \
+    do {                                                                         \
+        blockDim.x = 1;                                                          \
+        threadIdx.x = 0;                                                         \
+        for (blockIdx.x = 0; blockIdx.x < unsigned(items); blockIdx.x++) kern(__VA_ARGS__); \
+        blockIdx.x = 0;                                                          \
+    } while (0)
+
 #include "awcm1d.hip"
-#include "adjoint1d_dev.h"
+#include "adjoint1d_rev.h"
 
 namespace awcm {
 uint8_t sm[1 << 21];         // dynamic LDS of the emulated workgroup (k1_adjoint)
@@ -30,6 +40,7 @@ struct Emu {
     std::vector<uint8_t> masks, levels, tape_m;
     std::vector<double> tape_s, ckpt, seg, Qb, Xb, Cb, Gb, FV;
     int steps = 0, K = 1;
+    int step = 0;  // the step being reversed
     int NP() const { return in.Nx + 6; }
     int nxp() const { return (NP() + 15) / 16 * 16; }
 };
@@ -136,7 +147,7 @@ int emu_record_from(void* h, double tol, double cfl, double t_final, int max_ste
     Inst1D& in = e->in;
     in.t = t0; in.cnt = cnt0; in.steps = 0; in.sum_active0 = 0; in.status = 0;
     const long long NP = e->NP();
-    const long long stride = (5 * NP + 3LL * in.Nx + 255) / 256 * 256;
+    const long long stride = adj_tape_stride(in.Nx);
     fill_new(e->tape_m, size_t(stride) * max_steps, uint8_t(0));
     fill_new(e->tape_s, 4 * size_t(max_steps), 0.0);
     fill_new(e->ckpt, 3 * NP * size_t(max_steps / K + 1), 0.0);
@@ -170,7 +181,7 @@ static Adj1DArgs args(Emu& e, int seg_steps) {
     A.tape_m = e.tape_m.data();
     A.tape_s = e.tape_s.data();
     A.ckpt = e.ckpt.data();
-    A.tape_stride = (5 * NP + 3LL * e.in.Nx + 255) / 256 * 256;
+    A.tape_stride = adj_tape_stride(e.in.Nx);
     A.n_steps = e.steps;
     A.ckpt_every = e.K;
     A.seg_steps = seg_steps;
@@ -180,8 +191,8 @@ static Adj1DArgs args(Emu& e, int seg_steps) {
     fill_new(e.Cb, 3 * NP, 0.0);
     fill_new(e.Gb, 3 * NP, 0.0);
     fill_new(e.FV, 18 * size_t(e.in.Nx + 1), 0.0);
-    A.seg = e.seg.data(); A.Qb = e.Qb.data(); A.Xb = e.Xb.data(); A.Cb = e.Cb.data();
-    A.Gb = e.Gb.data(); A.FV = e.FV.data();
+    A.seg = e.seg.data();
+    A.step = &e.step;
     return A;
 }
 
@@ -192,21 +203,46 @@ void emu_replay(void* h) {
     emu_launch(EMU_THREADS, [&] { k1_adjoint(&e->in, A, 0, e->nxp()); });
 }
 
+// awcm_adjoint's sequence: for every step from the last, the segment
+// recompute (one workgroup, when the step ends its segment), the reverse
+// sweep's grid kernels, and the step counter moved back.
 void emu_adjoint(void* h, const double* gf, int seg_steps, double* g0) {
     auto* e = static_cast<Emu*>(h);
     const int NP = e->NP();
     std::vector<double> Qsave = e->Q;
-    Adj1DArgs A = args(*e, seg_steps ? seg_steps : e->K);
+    const int K = seg_steps ? seg_steps : e->K;
+    const Adj1DArgs A = args(*e, K);
     // the C ABI uploads the whole adjoint state, ghosts as zeros
     std::fill(e->Qb.begin(), e->Qb.end(), 0.0);
     for (int i = 3; i < NP - 3; i++)
         for (int c = 0; c < 3; c++) e->Qb[c * NP + i] = gf[i * 3 + c];
-    poison_lds();
-    emu_launch(EMU_THREADS, [&] { k1_adjoint(&e->in, A, 1, e->nxp()); });
+    AdjRev R{};
+    R.Nx = e->in.Nx; R.L = e->in.L; R.NP = NP; R.K = K;
+    R.gamma = e->in.gamma; R.dx = e->in.dx;
+    R.tape_m = e->tape_m.data(); R.stride = adj_tape_stride(e->in.Nx); R.tape_s = e->tape_s.data();
+    R.seg = e->seg.data();
+    R.Qb = e->Qb.data(); R.Xb = e->Xb.data(); R.Cb = e->Cb.data(); R.Gb = e->Gb.data(); R.FV = e->FV.data();
+    R.step = &e->step;
+    for (e->step = e->steps - 1; e->step >= 0;) {
+        poison_lds();
+        emu_launch(EMU_THREADS, [&] { k1_adjoint(&e->in, A, 2, e->nxp()); });
+        launch_reverse_step(R, nullptr);
+        AWCM_REV_LAUNCH(k1r_next, 1, &e->step);
+    }
     e->Q = Qsave;
     e->in.Q = e->Q.data();
     for (int i = 0; i < NP; i++)
         for (int c = 0; c < 3; c++) g0[i * 3 + c] = (i < 3 || i >= NP - 3) ? 0.0 : e->Qb[c * NP + i];
+}
+
+// The literal lifting weights of the reverse sweep equal wavelet_weights'.
+int emu_check_weights() {
+    double W[4][5];
+    wavelet_weights(W);
+    for (int cc = 0; cc < 4; cc++)
+        for (int kc = 1; kc <= 3; kc++)
+            if (W[cc][kc] != wlift(cc, kc)) return 0;
+    return 1;
 }
 
 int emu_status(void* h) { return static_cast<Emu*>(h)->in.status; }

@@ -148,26 +148,13 @@ def test_oracle_frozen_replay_reproduces_forward():
 
 @pytest.fixture(scope="module")
 def emu(tmp_path_factory):
-    out = str(tmp_path_factory.mktemp("emu") / "emu1d.so")
-    pkg = os.path.join(ROOT, "c_pp-library_amd")
-    subprocess.run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-fPIC", "-shared",
-                    "-I", os.path.join(ROOT, "tests", "native", "emu"), "-I", os.path.join(pkg, "include"),
-                    "-I", os.path.join(pkg, "csrc"), "-I", os.path.join(ROOT, "include"), "-x", "c++",
-                    os.path.join(ROOT, "tests", "native", "emu1d.cpp"), "-o", out], check=True)
-    E = C.CDLL(out)
-    V = C.c_void_p
-    E.emu_create.restype = V
-    E.emu_create.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, _dp, C.c_int]
-    E.emu_destroy.argtypes = [V]
-    E.emu_set_q.argtypes = [V, _dp]
-    E.emu_get_q.argtypes = [V, _dp]
-    E.emu_record.argtypes = [V, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int]
-    E.emu_record.restype = C.c_int
-    E.emu_replay.argtypes = [V]
-    E.emu_adjoint.argtypes = [V, _dp, C.c_int, _dp]
-    E.emu_status.argtypes = [V]
-    E.emu_set_masks_in_hbm.argtypes = [C.c_int]
-    return E
+    import emu_lib
+    return emu_lib.build(str(tmp_path_factory.mktemp("emu") / "emu1d.so"))
+
+
+def test_reverse_sweep_weights_are_the_wavelet_weights(emu):
+    """The reverse sweep's literal lifting weights equal wavelet_weights' rows."""
+    assert emu.emu_check_weights() == 1
 
 
 def test_emulated_solve_with_masks_in_hbm_matches_lds(emu):
@@ -202,8 +189,12 @@ def test_emulated_solve_with_masks_in_hbm_matches_lds(emu):
     assert np.array_equal(q1.view(np.uint64), o.get_q().view(np.uint64))
 
 
-def test_emulated_kernels_forward_replay_and_adjoint(emu):
-    """Shu-Osher at 513 points to t = 0.06 (about 30 steps)."""
+@pytest.mark.parametrize("poison", [0, 1])
+def test_emulated_kernels_forward_replay_and_adjoint(emu, poison):
+    """Shu-Osher at 513 points to t = 0.06 (about 30 steps).  With poison, every
+    buffer the device leaves uninitialised (and the LDS before each launch)
+    holds a NaN pattern, so a read of anything not written first would show."""
+    emu.emu_set_poison(poison)
     L, T = 6, 0.06
     nx = 16 * 2 ** (L - 1) + 1
     q, dx, x0 = O.ic1(1, nx, -5.0, 5.0, -4.0, O.SHU_L, O.SHU_L)
@@ -259,6 +250,7 @@ def test_emulated_kernels_forward_replay_and_adjoint(emu):
         assert np.array_equal(g0.view(np.uint64), g1.view(np.uint64))
     finally:
         emu.emu_destroy(h)
+        emu.emu_set_poison(0)
 
 
 class EmuSolver:

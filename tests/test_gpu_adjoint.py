@@ -1,24 +1,18 @@
 """GPU discrete adjoint of the 1-D AWCM step (frozen masks, levels, dt, alphas).
 
-The backward entry point awcm_adjoint is switched off on the GPU (it returns
-AWCM_EDISABLED): its first device run ended in a GPU memory fault whose cause
-is not identified.  Its tests below are kept and skipped with that reason; the
-same device code is checked on the host, through the emulation of
-tests/native/emu1d.cpp, by tests/test_adjoint_cpu.py.  Recording and the
-frozen replay are enabled and tested here.
-
 What is checked, through the C ABI:
 * recording does not change the forward (bit-identical to awcm_run);
 * the frozen replay reproduces the recorded run bit for bit, and from a
   perturbed state is bit-identical to the oracle's frozen replay -- so finite
   differences of either are finite differences of the same map;
-* the adjoint is deterministic and independent of the recompute segment;
+* the adjoint is deterministic and independent of the recompute segment, and
+  bit-identical to the host emulation of the same device code;
 * the adjoint against central finite differences of the frozen map, on seeded
   smooth random density directions: on a short horizon to 1e-6 relative; on
-  the full Shu-Osher run (4097 points, t = 1.8) within the finite
-  differences' own resolution, measured as the spread between steps h = 1e-6
-  and h = 5e-7 (WENO-Z's |b0 - b2| kinks make the frozen map only piecewise
-  smooth, and over 6700 steps FD at h = 1e-6 is itself uncertain at ~1e-5).
+  the full Shu-Osher run (4097 points, t = 1.8, 7090 steps) within the finite
+  differences' own resolution, measured as the spread between steps h = 1e-8
+  and h = 1e-9 (WENO-Z's |b0 - b2| kinks make the frozen map only piecewise
+  smooth, so larger steps measure secants across kinks; see FD_H below);
 
 Why density directions: in the Shu-Osher state the left region is uniform and
 the right region has u = 0, p = 1 exactly, where the two acoustic
@@ -32,6 +26,7 @@ the frozen map is differentiable.
 * the same gradient against finite differences of the CPU oracle's replay;
 * a torch.autograd.Function wrapper (test harness only) around replay/adjoint.
 """
+import ctypes as C
 import time
 
 import numpy as np
@@ -40,10 +35,6 @@ import pytest
 import oracle_lib as O
 
 pytestmark = pytest.mark.gpu
-
-BACKWARD_OFF = pytest.mark.skip(
-    reason="awcm_adjoint returns AWCM_EDISABLED: its first GPU run faulted and the cause is not identified; "
-           "its device code is checked on the host by tests/test_adjoint_cpu.py")
 
 SHU_AMP = 0.2
 SHORT_T = 0.04  # about 40 steps at 1025 points
@@ -168,24 +159,48 @@ def test_replay_is_the_oracle_frozen_map(short):
     assert np.array_equal(short.s.download_q().view(np.uint64), o.get_q().view(np.uint64))
 
 
-def test_backward_switched_off(short):
-    P = _pyawcm()
-    with pytest.raises(P.AwcmError) as e:
-        short.grad()
-    assert e.value.code == -7
-
-
-@BACKWARD_OFF
 def test_adjoint_deterministic_and_segment_independent(short):
     g8 = short.grad(8)
+    print("adjoint gradient: max |g| %.3e" % np.abs(g8).max(), flush=True)
+    assert np.all(np.isfinite(g8))
     assert np.array_equal(g8.view(np.uint64), short.grad(8).view(np.uint64))
     assert np.array_equal(g8.view(np.uint64), short.grad(16).view(np.uint64))
     assert np.array_equal(g8.view(np.uint64), short.grad(64).view(np.uint64))
     assert np.all(g8.reshape(-1, 3)[:3] == 0) and np.all(g8.reshape(-1, 3)[-3:] == 0)
-    assert np.abs(g8).max() > 0
+    assert 0 < np.abs(g8).max() < 1e3
+    # one finite-difference direction already pins the scale (a wrong sweep
+    # that is merely deterministic fails here)
+    v = smooth_directions(short.nx, short.x0, short.dx, 1, seed=5)[0]
+    fd = short.fd(v, 1e-6)
+    assert abs(float(np.dot(g8, v)) - fd) <= 1e-6 * abs(fd), (float(np.dot(g8, v)), fd)
 
 
-@BACKWARD_OFF
+def test_adjoint_matches_host_emulation(short, tmp_path):
+    """The same device code compiled for the host (tests/native/emu1d.cpp)
+    records the same trajectory and gives the same gradient: identical, or
+    within a few ulp where a device FP64 division rounds a halfway case the
+    other way."""
+    import emu_lib
+    E = emu_lib.build(str(tmp_path / "emu1d.so"))
+    _dp = C.POINTER(C.c_double)
+    pts = O.print_times(SHORT_T, 2)
+    h = E.emu_create(7, 17, 1.4, short.dx, pts.ctypes.data_as(_dp), len(pts))
+    try:
+        q0 = short.q0.copy()
+        E.emu_set_q(h, q0.ctypes.data_as(_dp))
+        assert E.emu_record(h, 1e-4, 0.5, SHORT_T, 20000, 8) == short.steps
+        gf = objective_grad(short.qf, short.rho_t)
+        ge = np.zeros_like(gf)
+        E.emu_adjoint(h, gf.ctypes.data_as(_dp), 0, ge.ctypes.data_as(_dp))
+    finally:
+        E.emu_destroy(h)
+    g = short.grad()
+    same = np.array_equal(g.view(np.uint64), ge.view(np.uint64))
+    err = np.abs(g - ge).max() / np.abs(ge).max()
+    print("GPU vs host-emulated adjoint: bit-identical %s, max rel diff %.3e" % (same, err))
+    assert err <= 1e-13
+
+
 def test_adjoint_matches_fd_short_horizon(short):
     g = short.grad()
     errs = []
@@ -197,7 +212,6 @@ def test_adjoint_matches_fd_short_horizon(short):
     assert max(errs) <= 1e-6, errs
 
 
-@BACKWARD_OFF
 def test_torch_autograd_wrapper(short):
     torch = pytest.importorskip("torch")
 
@@ -223,7 +237,9 @@ def test_torch_autograd_wrapper(short):
     ad = float(torch.dot(q0.grad, torch.from_numpy(v)))
     fd = short.fd(v, 1e-6)
     assert abs(ad - fd) <= 1e-6 * abs(fd)
-    assert abs(float(J) - objective(short.qf, short.rho_t)) == 0.0
+    # torch's sum and numpy's dot add the squares in different orders
+    Jn = objective(short.qf, short.rho_t)
+    assert abs(float(J.detach()) - Jn) <= 1e-12 * abs(Jn)
 
 
 @pytest.fixture(scope="module")
@@ -256,9 +272,10 @@ def test_shu_osher_record_and_replay(shu_osher):
     assert np.array_equal(F.s.download_q().view(np.uint64), F.qf.view(np.uint64))
 
 
-def test_shu_osher_gpu_frozen_map_is_the_oracle_one(shu_osher):
-    """Finite differences of the GPU replay and of the oracle's frozen replay
-    (the CPU reference of the gradient) are the same numbers."""
+@pytest.fixture(scope="module")
+def shu_oracle(shu_osher):
+    """The CPU oracle's recording of the same run (bit-identical tape) and its
+    frozen map J(q0)."""
     F = shu_osher
     o = O.Solver1D("orc", 9, 17, 1.4, F.dx)
     o.set_q(F.q0)
@@ -273,12 +290,34 @@ def test_shu_osher_gpu_frozen_map_is_the_oracle_one(shu_osher):
         o.replay()
         return objective(o.get_q(), F.rho_t)
 
+    return J
+
+
+def test_shu_osher_gpu_frozen_map_is_the_oracle_one(shu_osher, shu_oracle):
+    """Finite differences of the GPU replay and of the oracle's frozen replay
+    (the CPU reference of the gradient) are the same numbers."""
+    F, J = shu_osher, shu_oracle
     v = smooth_directions(F.nx, F.x0, F.dx, 1, seed=99)[0]
     fd_cpu = (J(F.q0 + 1e-6 * v) - J(F.q0 - 1e-6 * v)) / 2e-6
     assert fd_cpu == F.fd(v, 1e-6)
 
 
-@BACKWARD_OFF
+# Over the full horizon the frozen map is only piecewise smooth (the |b0 - b2|
+# kinks of WENO-Z are crossed by a perturbation of size h somewhere in 7090
+# steps x 4097 points x 3 stages), so a central difference at h = 1e-6 is a
+# secant across kinks (off by up to ~3e-2 from the derivative at q0), and
+# differences converge to the adjoint only as h -> 0: measured on this case,
+# the relative gap is ~1e-2 at h = 1e-6, ~1e-4..5e-3 at 1e-7, ~3e-6..4e-4 at
+# 1e-8 and ~2e-6..1e-4 at 1e-9.  The adjoint is therefore held to the finite
+# differences' own resolution: |ad - FD(1e-9)| <= 4 |FD(1e-8) - FD(1e-9)| +
+# 1e-6 |FD(1e-9)|.
+FD_H = (1e-8, 1e-9)
+
+
+def _fd_pair(F, v):
+    return F.fd(v, FD_H[0]), F.fd(v, FD_H[1])
+
+
 def test_shu_osher_adjoint_vs_fd(shu_osher):
     F = shu_osher
     backward(F)
@@ -286,44 +325,31 @@ def test_shu_osher_adjoint_vs_fd(shu_osher):
     rows = []
     for v in smooth_directions(F.nx, F.x0, F.dx, 16, seed=1234):
         ad = float(np.dot(F.g, v))
-        f1 = F.fd(v, 1e-6)
-        f2 = F.fd(v, 5e-7)
-        rows.append((ad, f1, f2))
-    errs = [abs(a - f1) / abs(f1) for a, f1, _ in rows]
-    spread = [abs(f1 - f2) / abs(f1) for _, f1, f2 in rows]
-    print("full-horizon adjoint vs FD(1e-6): max rel err %.3e, median %.3e; FD spread h=1e-6 vs 5e-7: max %.3e"
-          % (max(errs), float(np.median(errs)), max(spread)))
-    for (a, f1, f2), e in zip(rows, errs):
-        assert e <= 1e-6 or abs(a - f1) <= 4 * abs(f1 - f2) + 1e-6 * abs(f1), (a, f1, f2)
+        f8, f9 = _fd_pair(F, v)
+        rows.append((ad, f8, f9))
+    errs = [abs(a - f9) / abs(f9) for a, _, f9 in rows]
+    spread = [abs(f8 - f9) / abs(f9) for _, f8, f9 in rows]
+    print("full-horizon adjoint vs FD(1e-9) on 16 directions: max rel gap %.3e, median %.3e; "
+          "FD(1e-8) vs FD(1e-9): max %.3e" % (max(errs), float(np.median(errs)), max(spread)))
+    for a, f8, f9 in rows:
+        assert abs(a - f9) <= 4 * abs(f8 - f9) + 1e-6 * abs(f9), (a, f8, f9)
+    assert float(np.median(errs)) <= 1e-4
 
 
-@BACKWARD_OFF
-def test_shu_osher_adjoint_vs_oracle_fd(shu_osher):
-    F = shu_osher
+def test_shu_osher_adjoint_vs_oracle_fd(shu_osher, shu_oracle):
+    """Finite differences of the CPU oracle's frozen replay are those of the
+    GPU replay, bit for bit, and the adjoint meets them as above."""
+    F, J = shu_osher, shu_oracle
     backward(F)
-    o = O.Solver1D("orc", 9, 17, 1.4, F.dx)
-    o.set_q(F.q0)
-    o.record()
-    pts = O.print_times(1.8, 2)
-    while o.t[0] < 1.8:
-        o.step(1e-4, 0.5, pts)
-    assert o.tape_steps() == F.steps
-
-    def J(q0):
-        o.set_q(q0)
-        o.replay()
-        return objective(o.get_q(), F.rho_t)
-
-    for v in smooth_directions(F.nx, F.x0, F.dx, 2, seed=99):
-        fd_cpu = (J(F.q0 + 1e-6 * v) - J(F.q0 - 1e-6 * v)) / 2e-6
-        fd_gpu = F.fd(v, 1e-6)
-        assert fd_cpu == fd_gpu  # same frozen map, bit-identical replays
-        ad = float(np.dot(F.g, v))
-        f2 = F.fd(v, 5e-7)
-        assert abs(ad - fd_cpu) <= 1e-6 * abs(fd_cpu) or abs(ad - fd_cpu) <= 4 * abs(fd_cpu - f2) + 1e-6 * abs(fd_cpu)
+    v = smooth_directions(F.nx, F.x0, F.dx, 1, seed=99)[0]
+    h = FD_H[1]
+    fd_cpu = (J(F.q0 + h * v) - J(F.q0 - h * v)) / (2 * h)
+    f8, f9 = _fd_pair(F, v)
+    assert fd_cpu == f9  # same frozen map, bit-identical replays
+    ad = float(np.dot(F.g, v))
+    assert abs(ad - fd_cpu) <= 4 * abs(f8 - fd_cpu) + 1e-6 * abs(fd_cpu), (ad, f8, fd_cpu)
 
 
-@BACKWARD_OFF
 def test_shu_osher_adjoint_cost(shu_osher):
     """Adjoint (recompute from checkpoints + reverse sweep) against the plain forward run."""
     F = shu_osher
