@@ -31,7 +31,9 @@ static_assert(AWCM_FLUX_WENOZ == kRecWenoZ && AWCM_FLUX_WENO5 == kRecWeno5, "flu
 // 0), and the interpolations from mask_temp's slot onto mask3's slot (the host
 // puts the two masks there and restores them): mode 4 the scalar one (plane
 // 0), mode 5 InterpolateEuler(from, to) (planes 1..2), mode 6 the one-mask
-// InterpolateEuler (all three planes, WAVELET_1D.h:433-473).
+// InterpolateEuler (all three planes, WAVELET_1D.h:433-473); modes 7..10
+// AddAdjacentZone, CheckReconstruction, AddPointForDifferentiation and
+// ComputeLevel (AWCM_MANAGER_1D.h:14-133) on mask3's and level3's slots.
 // steps of this launch.  Instances are taken from `order` through the atomic
 // work counter *qctr (the host zeroes it).  The masks and levels of the
 // instance in hand occupy 11 * nxp bytes (nxp >= max(Nx) + 6 over the
@@ -84,6 +86,12 @@ __global__ void __launch_bounds__(1024) k1_solve(Inst1D* insts, int n_inst, cons
             else if (mode == 4) interp<0, 1>(c, c.mt, c.m[3]);
             else if (mode == 5) interp<1, 3>(c, c.mt, c.m[3]);
             else if (mode == 6) interp<0, 3>(c, c.mt, c.m[3]);
+            // AWCM_MANAGER_1D's statics on the mask in mask3's slot (and the
+            // level field in level3's), which the host fills and restores
+            else if (mode == 7) adjacent(c, c.m[3]);
+            else if (mode == 8) check_recon(c, c.m[3]);
+            else if (mode == 9) addpoint(c, c.m[3], c.lv[3]);
+            else if (mode == 10) level(c, c.lv[3], c.m[3]);
         } else if (mode == 0) {
             prepare(c, st.tol);
         } else if (mode == 2) {

@@ -1260,6 +1260,68 @@ __global__ void k2_widen_levels(Dims2 D, const uint8_t* __restrict__ src, int32_
     dst[t] = src[gidx(D, i, j)];
 }
 
+// ---------------------------------- AWCM_MANAGER_2D statics on one mask ----
+// AWCM_MANAGER_2D::ComputeLevel (AWCM_MANAGER_2D.h:215-241) at every finest
+// point: 0 where the mask is off; else the finest level with an active
+// 4-neighbour at that level's spacing; unchanged when there is none.  A
+// neighbour outside the ghosted domain sets the error flag.
+__global__ void k2_compute_level(Dims2 D, const uint8_t* __restrict__ mask, uint8_t* __restrict__ lvl, Scal2* sc) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)D.Nx * D.Ny) return;
+    const int i = int(t % D.Nx), j = int(t / D.Nx);
+    const long long p = gidx(D, i, j);
+    if (!mask[p]) { lvl[p] = 0; return; }
+    int d = 1;
+    for (int l = D.maxlev; l >= 0; l--) {
+        if (i - d < -G || i + d > D.Nx - 1 + G || j - d < -G || j + d > D.Ny - 1 + G) { sc->flag |= 1; return; }
+        if (mask[p - d] || mask[p + d] || mask[p - (long long)d * D.P] || mask[p + (long long)d * D.P]) {
+            lvl[p] = (uint8_t)l;
+            return;
+        }
+        d *= 2;
+    }
+}
+
+// AWCM_MANAGER_2D::AddPointForDifferentiation (AWCM_MANAGER_2D.h:160-213) at
+// every interior point of the snapshot, with the caller's levels.
+__global__ void k2_add_point(Dims2 D, const uint8_t* __restrict__ snap, uint8_t* __restrict__ mask,
+                             const uint8_t* __restrict__ lvl, Scal2* sc) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int ni = D.Nx - 2, nj = D.Ny - 2;
+    if (t >= (long long)ni * nj) return;
+    const int i = 1 + int(t % ni), j = 1 + int(t / ni);
+    const long long p = gidx(D, i, j);
+    if (!snap[p]) return;
+    const int lev = lvl[p];
+    const int dd = 1 << (D.L - 1 - lev);  // grid[lev].dx
+    if (lev == D.maxlev) {
+        for (int c = -3; c <= 3; c++) { mset(D, mask, i + c, j, sc); mset(D, mask, i, j + c, sc); }
+        return;
+    }
+    int lo, hi;
+    if (i - 2 * dd < 0) lo = -1, hi = 2;
+    else if (i + 2 * dd > D.Nx - 1) lo = -2, hi = 1;
+    else lo = -2, hi = 2;
+    for (int c = lo; c <= hi; c++) mset(D, mask, i + dd * c, j, sc);
+    if (j - 2 * dd < 0) lo = -1, hi = 2;
+    else if (j + 2 * dd > D.Ny - 1) lo = -2, hi = 1;
+    else lo = -2, hi = 2;
+    for (int c = lo; c <= hi; c++) mset(D, mask, i, j + dd * c, sc);
+}
+
+// AWCM_MANAGER_2D::RemoveResidue / RemoveResidueEuler (AWCM_MANAGER_2D.h:
+// 257-280) on an AoS field of ncomp components in the reference layout (row
+// pitch PR): every component zeroed at the interior points outside the mask.
+__global__ void k2_residue_aos(Dims2 D, double* __restrict__ q, int ncomp, const uint8_t* __restrict__ mask) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int ni = D.Nx - 2, nj = D.Ny - 2;
+    if (t >= (long long)ni * nj) return;
+    const int i = 1 + int(t % ni), j = 1 + int(t / ni);
+    if (mask[gidx(D, i, j)]) return;
+    double* v = q + ((long long)(i + G) + (long long)D.PR * (j + G)) * ncomp;
+    for (int c = 0; c < ncomp; c++) v[c] = 0.0;
+}
+
 inline unsigned nblk(long long n, int b) { return unsigned((n + b - 1) / b); }
 
 // Host mirror of lattice(): thread counts of the lattice kernels.
@@ -1720,6 +1782,78 @@ int Solver2D::download_scalar(double* q) {
     if (hipStreamSynchronize(stream) != hipSuccess) return AWCM_EDEVICE;
     return hipMemcpy2D(q, sizeof(double) * D.PR, Q, sizeof(double) * D.P, sizeof(double) * D.PR, D.Ny + 2 * G,
                        hipMemcpyDeviceToHost) == hipSuccess ? AWCM_OK : AWCM_EDEVICE;
+}
+
+// AWCM_MANAGER_2D's static mask algebra on a caller mask (reference layout,
+// in/out) held in `tmp`, with the caller's levels (int32 without ghosts, in
+// `work` as bytes in the masks' layout) where the operation reads or writes
+// them.  The step scalars are kept; the error flag of the operation is
+// returned as AWCM_ERANGE.
+int Solver2D::mask_op(int op, uint8_t* mask, int32_t* levels) {
+    const long long nl = (long long)D.Nx * D.Ny;
+    if (hipStreamSynchronize(stream) != hipSuccess) return AWCM_EDEVICE;
+    if (hipMemcpy2D(tmp, D.P, mask, D.PR, D.PR, D.Ny + 2 * G, hipMemcpyHostToDevice) != hipSuccess)
+        return AWCM_EDEVICE;
+    if (levels) {
+        std::vector<uint8_t> lb(size_t(D.plane), uint8_t(D.maxlev));
+        for (long long t = 0; t < nl; t++) {
+            const int v = levels[t];
+            if (v < 0 || v > D.maxlev) return AWCM_EINVAL;
+            lb[size_t((t % D.Nx) + G + (long long)D.P * (t / D.Nx + G))] = uint8_t(v);
+        }
+        if (hipMemcpy(work, lb.data(), lb.size(), hipMemcpyHostToDevice) != hipSuccess) return AWCM_EDEVICE;
+    }
+    Scal2 s0;
+    if (hipMemcpy(&s0, sc, sizeof(s0), hipMemcpyDeviceToHost) != hipSuccess) return AWCM_EDEVICE;
+    Scal2 s1 = s0;
+    s1.skip = 0;  // the mask passes test `skip`, which a finished run leaves set
+    s1.flag = 0;
+    if (hipMemcpy(sc, &s1, sizeof(s1), hipMemcpyHostToDevice) != hipSuccess) return AWCM_EDEVICE;
+    int st = AWCM_OK;
+    const long long ninner = (long long)(D.Nx - 2) * (D.Ny - 2);
+    if (op == MASK_OP_ADJACENT || op == MASK_OP_ADD_POINT) {
+        if (hipMemcpyAsync(snap, tmp, size_t(D.plane), hipMemcpyDeviceToDevice, stream) != hipSuccess) st = AWCM_EDEVICE;
+        if (!st && op == MASK_OP_ADJACENT) {
+            const long long n = (long long)(D.P / kMaskB) * (D.Ny + 2 * G);
+            AWCM_LAUNCH((k2_adjacent<kMaskB>), dim3(nblk(n, 256)), dim3(256), 0, stream, D, snap, tmp, sc);
+        } else if (!st) {
+            AWCM_LAUNCH(k2_add_point, dim3(nblk(ninner, 256)), dim3(256), 0, stream, D, snap, tmp, work, sc);
+        }
+    } else if (op == MASK_OP_CHECK) {
+        st = check_recon(*this, tmp);
+    } else if (op == MASK_OP_LEVEL) {
+        AWCM_LAUNCH(k2_compute_level, dim3(nblk(nl, 256)), dim3(256), 0, stream, D, tmp, work, sc);
+    } else {
+        st = AWCM_EINVAL;
+    }
+    if (hipStreamSynchronize(stream) != hipSuccess && !st) st = AWCM_EDEVICE;
+    Scal2 s2;
+    if (!st && hipMemcpy(&s2, sc, sizeof(s2), hipMemcpyDeviceToHost) != hipSuccess) st = AWCM_EDEVICE;
+    if (hipMemcpy(sc, &s0, sizeof(s0), hipMemcpyHostToDevice) != hipSuccess && !st) st = AWCM_EDEVICE;
+    if (st) return st;
+    if (s2.flag & 1) return AWCM_ERANGE;
+    if (op == MASK_OP_LEVEL) {
+        std::vector<uint8_t> lb(size_t(D.plane));
+        if (hipMemcpy(lb.data(), work, lb.size(), hipMemcpyDeviceToHost) != hipSuccess) return AWCM_EDEVICE;
+        for (long long t = 0; t < nl; t++) levels[t] = lb[size_t((t % D.Nx) + G + (long long)D.P * (t / D.Nx + G))];
+        return AWCM_OK;
+    }
+    return hipMemcpy2D(mask, D.PR, tmp, D.P, D.PR, D.Ny + 2 * G, hipMemcpyDeviceToHost) == hipSuccess ? AWCM_OK
+                                                                                                    : AWCM_EDEVICE;
+}
+
+// RemoveResidue / RemoveResidueEuler on a caller field (AoS, reference layout,
+// ncomp components), staged in Q2; the mask is held in `tmp`.
+int Solver2D::remove_residue(double* q, int ncomp, const uint8_t* mask) {
+    const size_t n = size_t(D.PR) * (D.Ny + 2 * G) * size_t(ncomp);
+    if (hipStreamSynchronize(stream) != hipSuccess) return AWCM_EDEVICE;
+    if (hipMemcpy2D(tmp, D.P, mask, D.PR, D.PR, D.Ny + 2 * G, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(Q2, q, sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess)
+        return AWCM_EDEVICE;
+    const long long ninner = (long long)(D.Nx - 2) * (D.Ny - 2);
+    hipLaunchKernelGGL(k2_residue_aos, dim3(nblk(ninner, 256)), dim3(256), 0, stream, D, Q2, ncomp, tmp);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(stream) != hipSuccess) return AWCM_EDEVICE;
+    return hipMemcpy(q, Q2, sizeof(double) * n, hipMemcpyDeviceToHost) == hipSuccess ? AWCM_OK : AWCM_EDEVICE;
 }
 
 int Solver2D::enqueue_step() {

@@ -220,6 +220,38 @@ int awcm_post_process_euler(awcm_solver* h) {
     return st;
 }
 
+// AWCM_MANAGER_1D / AWCM_MANAGER_2D statics on caller arrays
+int awcm_add_adjacent_zone(awcm_solver* h, uint8_t* mask) {
+    if (!h || !mask) return AWCM_EINVAL;
+    return h->dim == 1 ? awcm::mask_op1(h->b1, 0, 7, mask, nullptr)
+                       : h->s2->mask_op(Solver2D::MASK_OP_ADJACENT, mask, nullptr);
+}
+
+int awcm_check_reconstruction(awcm_solver* h, uint8_t* mask) {
+    if (!h || !mask) return AWCM_EINVAL;
+    return h->dim == 1 ? awcm::mask_op1(h->b1, 0, 8, mask, nullptr)
+                       : h->s2->mask_op(Solver2D::MASK_OP_CHECK, mask, nullptr);
+}
+
+int awcm_add_point_for_differentiation(awcm_solver* h, uint8_t* mask, const int32_t* level) {
+    if (!h || !mask || !level) return AWCM_EINVAL;
+    int32_t* lv = const_cast<int32_t*>(level);  // read only: the ADD_POINT path does not write it
+    return h->dim == 1 ? awcm::mask_op1(h->b1, 0, 9, mask, lv)
+                       : h->s2->mask_op(Solver2D::MASK_OP_ADD_POINT, mask, lv);
+}
+
+int awcm_compute_level(awcm_solver* h, int32_t* level, const uint8_t* mask) {
+    if (!h || !mask || !level) return AWCM_EINVAL;
+    uint8_t* m = const_cast<uint8_t*>(mask);  // read only: the LEVEL path does not write it
+    return h->dim == 1 ? awcm::mask_op1(h->b1, 0, 10, m, level) : h->s2->mask_op(Solver2D::MASK_OP_LEVEL, m, level);
+}
+
+int awcm_remove_residue(awcm_solver* h, double* q, int ncomp, const uint8_t* mask) {
+    if (!h || !q || !mask) return AWCM_EINVAL;
+    if (ncomp != 1 && ncomp != (h->dim == 1 ? 3 : 4)) return AWCM_EINVAL;
+    return h->dim == 1 ? awcm::remove_residue1(h->b1, 0, q, ncomp, mask) : h->s2->remove_residue(q, ncomp, mask);
+}
+
 int awcm_run(awcm_solver* h, double tol, double cfl, double t_final, int max_steps, double* t, int* cnt,
              int* steps_done, long long* updates) {
     // cnt indexes the print schedule on the device: a negative one is refused

@@ -66,6 +66,8 @@ int download_scalar1(Batch1D& B, int idx, double* q);
 int interpolate1(Batch1D& B, int idx, const uint8_t* from, const uint8_t* to, int mode);
 int set_mask0_1(Batch1D& B, int idx, const uint8_t* m0);
 int set_schedule1(Batch1D& B, const double* pt, int npt);
+int mask_op1(Batch1D& B, int idx, int mode, uint8_t* mask, int32_t* levels);
+int remove_residue1(Batch1D& B, int idx, double* q, int ncomp, const uint8_t* mask);
 
 // Tape and work buffers of the 1-D discrete adjoint (one per handle).
 struct Adj1DHost {

@@ -489,6 +489,78 @@ int interpolate1(Batch1D& B, int idx, const uint8_t* from, const uint8_t* to, in
     return st;
 }
 
+// AWCM_MANAGER_1D's statics on a caller mask (and level field) for instance
+// idx (k1_solve modes 7..10): the kernel works on the mask3 and level3 slots,
+// so the instance's own masks and levels are saved around it and put back.
+// mode 10 (ComputeLevel) returns the level field, the others the mask.
+int mask_op1(Batch1D& B, int idx, int mode, uint8_t* mask, int32_t* levels) {
+    Inst1D& in = B.h[idx];
+    const size_t NP = size_t(in.Nx) + 6, nl = size_t(in.Nx);
+    std::vector<uint8_t> saved_m(5 * NP), saved_l(4 * nl), lb(nl, 0);
+    if (levels)
+        for (size_t i = 0; i < nl; i++) {
+            if (levels[i] < 0 || levels[i] > in.L - 1) return AWCM_EINVAL;
+            lb[i] = uint8_t(levels[i]);
+        }
+    if (hipStreamSynchronize(B.stream) != hipSuccess ||
+        hipMemcpy(saved_m.data(), in.masks, saved_m.size(), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(saved_l.data(), in.levels, saved_l.size(), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(in.masks + 4 * NP, mask, NP, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(in.levels + 3 * nl, lb.data(), nl, hipMemcpyHostToDevice) != hipSuccess)
+        return AWCM_EDEVICE;
+    in.status = 0;
+    int st = B.push();
+    if (!st) st = B.launch(mode, 0, idx, 1);
+    if (!st) st = B.pull();
+    if (!st && in.status) st = in.status == -5 ? AWCM_ERANGE : AWCM_EDEVICE;
+    if (!st) {
+        if (mode == 10) {
+            if (hipMemcpy(lb.data(), in.levels + 3 * nl, nl, hipMemcpyDeviceToHost) != hipSuccess) st = AWCM_EDEVICE;
+            else
+                for (size_t i = 0; i < nl; i++) levels[i] = lb[i];
+        } else if (hipMemcpy(mask, in.masks + 4 * NP, NP, hipMemcpyDeviceToHost) != hipSuccess) {
+            st = AWCM_EDEVICE;
+        }
+    }
+    in.status = 0;
+    if (hipMemcpy(in.masks, saved_m.data(), saved_m.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(in.levels, saved_l.data(), saved_l.size(), hipMemcpyHostToDevice) != hipSuccess)
+        st = st ? st : AWCM_EDEVICE;
+    return st;
+}
+
+// AWCM_MANAGER_1D::RemoveResidue (AWCM_MANAGER_1D.h:135-157) on an AoS field
+// of ncomp components, (Nx+6) points: every component zeroed at the finest
+// points outside the mask.
+__global__ void k1_residue_aos(double* q, const uint8_t* mask, int Nx, int ncomp) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Nx || mask[i + G]) return;
+    for (int c = 0; c < ncomp; c++) q[(long long)(i + G) * ncomp + c] = 0.0;
+}
+
+int remove_residue1(Batch1D& B, int idx, double* q, int ncomp, const uint8_t* mask) {
+    const Inst1D& in = B.h[idx];
+    const size_t NP = size_t(in.Nx) + 6;
+    double* dq = nullptr;
+    uint8_t* dm = nullptr;
+    if (hipMalloc(&dq, sizeof(double) * NP * ncomp) != hipSuccess) return AWCM_ENOMEM;
+    if (hipMalloc(&dm, NP) != hipSuccess) { hipFree(dq); return AWCM_ENOMEM; }
+    int st = AWCM_OK;
+    if (hipStreamSynchronize(B.stream) != hipSuccess ||
+        hipMemcpy(dq, q, sizeof(double) * NP * ncomp, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dm, mask, NP, hipMemcpyHostToDevice) != hipSuccess)
+        st = AWCM_EDEVICE;
+    if (!st) {
+        hipLaunchKernelGGL(k1_residue_aos, dim3((in.Nx + 255) / 256), dim3(256), 0, B.stream, dq, dm, in.Nx, ncomp);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(B.stream) != hipSuccess ||
+            hipMemcpy(q, dq, sizeof(double) * NP * ncomp, hipMemcpyDeviceToHost) != hipSuccess)
+            st = AWCM_EDEVICE;
+    }
+    hipFree(dq);
+    hipFree(dm);
+    return st;
+}
+
 int set_mask0_1(Batch1D& B, int idx, const uint8_t* m0) {
     Inst1D& in = B.h[idx];
     const int NP = in.Nx + 2 * 3;

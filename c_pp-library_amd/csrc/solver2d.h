@@ -103,6 +103,10 @@ struct Solver2D {
     int interpolate(const uint8_t* mask_from, const uint8_t* mask_to, bool euler);  // synchronous
     int upload_scalar(const double* q);
     int download_scalar(double* q);
+    // AWCM_MANAGER_2D statics on caller arrays (synchronous; see mask_op)
+    enum { MASK_OP_ADJACENT = 0, MASK_OP_CHECK = 1, MASK_OP_ADD_POINT = 2, MASK_OP_LEVEL = 3 };
+    int mask_op(int op, uint8_t* mask, int32_t* levels);
+    int remove_residue(double* q, int ncomp, const uint8_t* mask);
 
     // kernel sequences (enqueued on `stream`, no host synchronisation)
     int tail = -1;            // levels 0..tail of each cascade run in one workgroup

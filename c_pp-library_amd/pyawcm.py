@@ -83,6 +83,11 @@ def lib():
         "awcm_profile_enable": [V, I],
         "awcm_set_tuning": [V, I, I],
         "awcm_profile_read": [V, I, _dp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)],
+        "awcm_add_adjacent_zone": [V, _u8p],
+        "awcm_check_reconstruction": [V, _u8p],
+        "awcm_add_point_for_differentiation": [V, _u8p, _i32p],
+        "awcm_compute_level": [V, _i32p, _u8p],
+        "awcm_remove_residue": [V, _dp, I, _u8p],
         "awcm_adjoint_record": [V, D, D, D, I, I, _dp, C.POINTER(I), C.POINTER(I)],
         "awcm_adjoint_replay": [V],
         "awcm_adjoint": [V, _dp, I, _dp],
@@ -262,6 +267,35 @@ class Solver:
 
     def synchronize(self):
         _check(lib().awcm_synchronize(self.h), "awcm_synchronize")
+
+    # ---- AWCM_MANAGER_1D / AWCM_MANAGER_2D statics on caller arrays ----
+    MASK_OPS = ("awcm_add_adjacent_zone", "awcm_check_reconstruction", "awcm_add_point_for_differentiation",
+                "awcm_compute_level")
+
+    def mask_op(self, op, mask, level=None):
+        """op 0 AddAdjacentZone, 1 CheckReconstruction, 2 AddPointForDifferentiation
+        (level read), 3 ComputeLevel (level in/out), on copies; returns (mask, level)."""
+        m = np.ascontiguousarray(mask, dtype=np.uint8).reshape(-1).copy()
+        nl = self.nx * self.ny
+        lv = np.ascontiguousarray(level if level is not None else np.zeros(nl), dtype=np.int32).reshape(-1).copy()
+        L = lib()
+        mp, lp = m.ctypes.data_as(_u8p), lv.ctypes.data_as(_i32p)
+        if op == 0:
+            st = L.awcm_add_adjacent_zone(self.h, mp)
+        elif op == 1:
+            st = L.awcm_check_reconstruction(self.h, mp)
+        elif op == 2:
+            st = L.awcm_add_point_for_differentiation(self.h, mp, lp)
+        else:
+            st = L.awcm_compute_level(self.h, lp, mp)
+        _check(st, self.MASK_OPS[op])
+        return m, lv
+
+    def remove_residue(self, q, ncomp, mask):
+        q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1).copy()
+        m = np.ascontiguousarray(mask, dtype=np.uint8).reshape(-1)
+        _check(lib().awcm_remove_residue(self.h, _d(q), ncomp, m.ctypes.data_as(_u8p)), "awcm_remove_residue")
+        return q
 
     # ---- discrete adjoint (1-D) ----
     def adjoint_record(self, tol, cfl, t_final, max_steps, checkpoint_every=32):

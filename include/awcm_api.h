@@ -36,7 +36,9 @@
 #include <ctime>
 #include <fstream>
 #include <iostream>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -1004,11 +1006,80 @@ public:
     void TocAndReadDuration(const char* name_work) { std::printf("%2.3f seconds in %s", Toc(), name_work); }
 };
 
-/* ---- AWCM managers: output writers (AWCM_MANAGER_1D.h:159-186,
- *      AWCM_MANAGER_2D.h:243-255): "x q" per active point, default stream
- *      precision in 1-D and 15 digits in 2-D ----------------------------------- */
+/* ---- AWCM managers (AWCM_MANAGER_1D.h, AWCM_MANAGER_2D.h): the static mask
+ *      algebra on the GPU, and the output writers --------------------------- */
+namespace awcm_api_detail {
+// The static members run on a device solver of the MRA grid's geometry, one
+// per geometry, created on first use and held until the process ends; calls
+// on it are serialised.
+template <class F>
+void on_grid(int dim, int L, int nxc, int nyc, F&& f) {
+    static std::mutex mu;
+    static auto* cache = new std::map<std::vector<int>, awcm_solver*>();
+    std::lock_guard<std::mutex> lock(mu);
+    const std::vector<int> key{dim, L, nxc, nyc};
+    auto it = cache->find(key);
+    awcm_solver* h = it == cache->end() ? nullptr : it->second;
+    if (!h) {
+        const int st = dim == 1 ? awcm_create_1d(L, nxc, 1.4, 1.0, &h)
+                                : awcm_create_2d(L, nxc, nyc, 1.4, 1.0, 0.0, AWCM_BC_TRANSMISSIVE, &h);
+        if (st != AWCM_OK)
+            throw std::runtime_error("AWCM_MANAGER: device solver creation failed with AWCM status " + std::to_string(st));
+        (*cache)[key] = h;
+    }
+    f(h);
+}
+inline void manager_check(int st, const char* what) {
+    if (st != AWCM_OK) throw std::runtime_error(std::string(what) + " failed with AWCM status " + std::to_string(st));
+}
+// A mask or field argument spans the finest grid with any ghost width; it
+// crosses the C ABI with ghost width 3 (ghosts it lacks zero) and comes back
+// over its own extent (points the GPU set beyond it, where the reference
+// would write out of bounds, are dropped).
+template <class F>
+void span_check_1d(const F& f, int Nx, const char* who) {
+    if (f.i_start != 0 || f.i_res != Nx) throw std::invalid_argument(std::string(who) + ": fields must span 0..Nx-1");
+}
+template <class F>
+void span_check_2d(const F& f, int Nx, int Ny, const char* who) {
+    if (f.i_start != 0 || f.j_start != 0 || f.i_res != Nx || f.j_res != Ny)
+        throw std::invalid_argument(std::string(who) + ": fields must span (0..Nx-1) x (0..Ny-1)");
+}
+template <class F>
+int ghosts_in(const F& f) { return f.ghost_width < 3 ? f.ghost_width : 3; }
+}  // namespace awcm_api_detail
+
 class AWCM_MANAGER_1D {
 public:
+    // AddAdjacentZone (AWCM_MANAGER_1D.h:14-43)
+    static void AddAdjacentZone(FIELD_UNIFORM_1D<bool>& mask, const MRA_GRID_INT_1D& mra) {
+        mask_op(0, mask, nullptr, mra, "AWCM_MANAGER_1D::AddAdjacentZone");
+    }
+    // CheckReconstruction (AWCM_MANAGER_1D.h:45-71)
+    static void CheckReconstruction(FIELD_UNIFORM_1D<bool>& mask, const MRA_GRID_INT_1D& mra) {
+        mask_op(1, mask, nullptr, mra, "AWCM_MANAGER_1D::CheckReconstruction");
+    }
+    // AddPointForDifferentiation (AWCM_MANAGER_1D.h:73-107)
+    static void AddPointForDifferentiation(FIELD_UNIFORM_1D<bool>& mask, const MRA_GRID_INT_1D& mra,
+                                           const FIELD_UNIFORM_1D<int>& level) {
+        mask_op(2, mask, const_cast<FIELD_UNIFORM_1D<int>*>(&level), mra, "AWCM_MANAGER_1D::AddPointForDifferentiation");
+    }
+    // ComputeLevel (AWCM_MANAGER_1D.h:109-133)
+    static void ComputeLevel(FIELD_UNIFORM_1D<int>& level, const FIELD_UNIFORM_1D<bool>& mask,
+                             const MRA_GRID_INT_1D& mra) {
+        mask_op(3, const_cast<FIELD_UNIFORM_1D<bool>&>(mask), &level, mra, "AWCM_MANAGER_1D::ComputeLevel");
+    }
+    // RemoveResidue (AWCM_MANAGER_1D.h:135-144, 146-157)
+    static void RemoveResidue(FIELD_UNIFORM_1D<double>& q, const FIELD_UNIFORM_1D<bool>& mask,
+                              const MRA_GRID_INT_1D& mra) {
+        residue(q, 1, [](FIELD_UNIFORM_1D<double>& f, int i, int) -> double& { return f(i); }, mask, mra);
+    }
+    static void RemoveResidue(FIELD_UNIFORM_1D<VECTOR_3D<double>>& q, const FIELD_UNIFORM_1D<bool>& mask,
+                              const MRA_GRID_INT_1D& mra) {
+        residue(q, 3, [](FIELD_UNIFORM_1D<VECTOR_3D<double>>& f, int i, int c) -> double& { return f(i)[c]; }, mask,
+                mra);
+    }
+
     template <class T>
     static void Print(const FIELD_UNIFORM_1D<T>& q, const FIELD_UNIFORM_1D<bool>& mask, const char* filename) {
         std::ofstream f(filename);
@@ -1017,10 +1088,90 @@ public:
             if (mask(i) == true) f << grid(i) << " " << q(i) << std::endl;
         }
     }
+
+private:
+    static std::vector<uint8_t> mask_in(const FIELD_UNIFORM_1D<bool>& m, int Nx) {
+        std::vector<uint8_t> b(size_t(Nx) + 6, 0);
+        const int g = awcm_api_detail::ghosts_in(m);
+        for (int i = -g; i <= Nx - 1 + g; i++) b[size_t(i + 3)] = m(i) ? 1 : 0;
+        return b;
+    }
+    static void mask_op(int op, FIELD_UNIFORM_1D<bool>& mask, FIELD_UNIFORM_1D<int>* level,
+                        const MRA_GRID_INT_1D& mra, const char* who) {
+        const int Nx = mra.Nx;
+        awcm_api_detail::span_check_1d(mask, Nx, who);
+        if (level) awcm_api_detail::span_check_1d(*level, Nx, who);
+        std::vector<uint8_t> b = mask_in(mask, Nx);
+        std::vector<int32_t> lv(size_t(Nx), 0);
+        if (level)
+            for (int i = 0; i < Nx; i++) lv[size_t(i)] = (*level)(i);
+        awcm_api_detail::on_grid(1, mra.num_levels, mra.Nx_coarse, 0, [&](awcm_solver* h) {
+            int st = AWCM_OK;
+            if (op == 0) st = awcm_add_adjacent_zone(h, b.data());
+            else if (op == 1) st = awcm_check_reconstruction(h, b.data());
+            else if (op == 2) st = awcm_add_point_for_differentiation(h, b.data(), lv.data());
+            else st = awcm_compute_level(h, lv.data(), b.data());
+            awcm_api_detail::manager_check(st, who);
+        });
+        if (op == 3) {
+            for (int i = 0; i < Nx; i++) (*level)(i) = lv[size_t(i)];
+            return;
+        }
+        const int g = awcm_api_detail::ghosts_in(mask);
+        for (int i = -g; i <= Nx - 1 + g; i++) mask(i) = b[size_t(i + 3)] != 0;
+    }
+    template <class Field, class At>
+    static void residue(Field& q, int nc, At at, const FIELD_UNIFORM_1D<bool>& mask, const MRA_GRID_INT_1D& mra) {
+        const char* who = "AWCM_MANAGER_1D::RemoveResidue";
+        const int Nx = mra.Nx, g = awcm_api_detail::ghosts_in(q);
+        awcm_api_detail::span_check_1d(q, Nx, who);
+        awcm_api_detail::span_check_1d(mask, Nx, who);
+        std::vector<double> v((size_t(Nx) + 6) * nc, 0.0);
+        for (int i = -g; i <= Nx - 1 + g; i++)
+            for (int c = 0; c < nc; c++) v[size_t(i + 3) * nc + c] = at(q, i, c);
+        std::vector<uint8_t> b = mask_in(mask, Nx);
+        awcm_api_detail::on_grid(1, mra.num_levels, mra.Nx_coarse, 0, [&](awcm_solver* h) {
+            awcm_api_detail::manager_check(awcm_remove_residue(h, v.data(), nc, b.data()), who);
+        });
+        for (int i = -g; i <= Nx - 1 + g; i++)
+            for (int c = 0; c < nc; c++) at(q, i, c) = v[size_t(i + 3) * nc + c];
+    }
 };
 
 class AWCM_MANAGER_2D {
 public:
+    // AddAdjacentZone (AWCM_MANAGER_2D.h:14-61)
+    static void AddAdjacentZone(FIELD_UNIFORM_2D<bool>& mask, const MRA_GRID_INT_2D& mra_grid) {
+        mask_op(0, mask, nullptr, mra_grid, "AWCM_MANAGER_2D::AddAdjacentZone");
+    }
+    // CheckReconstruction (AWCM_MANAGER_2D.h:63-158)
+    static void CheckReconstruction(FIELD_UNIFORM_2D<bool>& mask, const MRA_GRID_INT_2D& mra_grid) {
+        mask_op(1, mask, nullptr, mra_grid, "AWCM_MANAGER_2D::CheckReconstruction");
+    }
+    // AddPointForDifferentiation (AWCM_MANAGER_2D.h:160-213)
+    static void AddPointForDifferentiation(FIELD_UNIFORM_2D<bool>& mask, const MRA_GRID_INT_2D& mra_grid,
+                                           const FIELD_UNIFORM_2D<int>& level) {
+        mask_op(2, mask, const_cast<FIELD_UNIFORM_2D<int>*>(&level), mra_grid,
+                "AWCM_MANAGER_2D::AddPointForDifferentiation");
+    }
+    // ComputeLevel (AWCM_MANAGER_2D.h:215-241)
+    static void ComputeLevel(FIELD_UNIFORM_2D<int>& level, const FIELD_UNIFORM_2D<bool>& mask,
+                             const MRA_GRID_INT_2D& mra_grid) {
+        mask_op(3, const_cast<FIELD_UNIFORM_2D<bool>&>(mask), &level, mra_grid, "AWCM_MANAGER_2D::ComputeLevel");
+    }
+    // RemoveResidue (AWCM_MANAGER_2D.h:257-266)
+    static void RemoveResidue(FIELD_UNIFORM_2D<double>& q, const FIELD_UNIFORM_2D<bool>& mask,
+                              const MRA_GRID_INT_2D& mra_grid) {
+        residue(q, 1, [](FIELD_UNIFORM_2D<double>& f, int i, int j, int) -> double& { return f(i, j); }, mask,
+                mra_grid, "AWCM_MANAGER_2D::RemoveResidue");
+    }
+    // RemoveResidueEuler (AWCM_MANAGER_2D.h:267-280)
+    static void RemoveResidueEuler(FIELD_UNIFORM_2D<VECTOR_4D<double>>& q, const FIELD_UNIFORM_2D<bool>& mask,
+                                   const MRA_GRID_INT_2D& mra_grid) {
+        residue(q, 4, [](FIELD_UNIFORM_2D<VECTOR_4D<double>>& f, int i, int j, int c) -> double& { return f(i, j)[c]; },
+                mask, mra_grid, "AWCM_MANAGER_2D::RemoveResidueEuler");
+    }
+
     template <class T>
     static void Print(const FIELD_UNIFORM_2D<T>& q, const FIELD_UNIFORM_2D<bool>& mask, const char* filename) {
         std::ofstream f(filename);
@@ -1030,6 +1181,63 @@ public:
             if (mask(i, j) == false) continue;
             f << grid(i, j).i << " " << grid(i, j).j << " " << q(i, j) << std::endl;
         }
+    }
+
+private:
+    static std::vector<uint8_t> mask_in(const FIELD_UNIFORM_2D<bool>& m, int Nx, int Ny) {
+        const size_t P = size_t(Nx) + 6;
+        std::vector<uint8_t> b(P * (size_t(Ny) + 6), 0);
+        const int g = awcm_api_detail::ghosts_in(m);
+        for (int j = -g; j <= Ny - 1 + g; j++)
+            for (int i = -g; i <= Nx - 1 + g; i++) b[size_t(i + 3) + P * size_t(j + 3)] = m(i, j) ? 1 : 0;
+        return b;
+    }
+    static void mask_op(int op, FIELD_UNIFORM_2D<bool>& mask, FIELD_UNIFORM_2D<int>* level,
+                        const MRA_GRID_INT_2D& mra_grid, const char* who) {
+        const int Nx = mra_grid.Nx, Ny = mra_grid.Ny;
+        awcm_api_detail::span_check_2d(mask, Nx, Ny, who);
+        if (level) awcm_api_detail::span_check_2d(*level, Nx, Ny, who);
+        std::vector<uint8_t> b = mask_in(mask, Nx, Ny);
+        std::vector<int32_t> lv(size_t(Nx) * Ny, 0);
+        if (level)
+            for (int j = 0; j < Ny; j++)
+                for (int i = 0; i < Nx; i++) lv[size_t(i) + size_t(Nx) * j] = (*level)(i, j);
+        awcm_api_detail::on_grid(2, mra_grid.num_levels, mra_grid.Nx_coarse, mra_grid.Ny_coarse, [&](awcm_solver* h) {
+            int st = AWCM_OK;
+            if (op == 0) st = awcm_add_adjacent_zone(h, b.data());
+            else if (op == 1) st = awcm_check_reconstruction(h, b.data());
+            else if (op == 2) st = awcm_add_point_for_differentiation(h, b.data(), lv.data());
+            else st = awcm_compute_level(h, lv.data(), b.data());
+            awcm_api_detail::manager_check(st, who);
+        });
+        if (op == 3) {
+            for (int j = 0; j < Ny; j++)
+                for (int i = 0; i < Nx; i++) (*level)(i, j) = lv[size_t(i) + size_t(Nx) * j];
+            return;
+        }
+        const size_t P = size_t(Nx) + 6;
+        const int g = awcm_api_detail::ghosts_in(mask);
+        for (int j = -g; j <= Ny - 1 + g; j++)
+            for (int i = -g; i <= Nx - 1 + g; i++) mask(i, j) = b[size_t(i + 3) + P * size_t(j + 3)] != 0;
+    }
+    template <class Field, class At>
+    static void residue(Field& q, int nc, At at, const FIELD_UNIFORM_2D<bool>& mask, const MRA_GRID_INT_2D& mra_grid,
+                        const char* who) {
+        const int Nx = mra_grid.Nx, Ny = mra_grid.Ny, g = awcm_api_detail::ghosts_in(q);
+        awcm_api_detail::span_check_2d(q, Nx, Ny, who);
+        awcm_api_detail::span_check_2d(mask, Nx, Ny, who);
+        const size_t P = size_t(Nx) + 6;
+        std::vector<double> v(P * (size_t(Ny) + 6) * nc, 0.0);
+        for (int j = -g; j <= Ny - 1 + g; j++)
+            for (int i = -g; i <= Nx - 1 + g; i++)
+                for (int c = 0; c < nc; c++) v[(size_t(i + 3) + P * size_t(j + 3)) * nc + c] = at(q, i, j, c);
+        std::vector<uint8_t> b = mask_in(mask, Nx, Ny);
+        awcm_api_detail::on_grid(2, mra_grid.num_levels, mra_grid.Nx_coarse, mra_grid.Ny_coarse, [&](awcm_solver* h) {
+            awcm_api_detail::manager_check(awcm_remove_residue(h, v.data(), nc, b.data()), who);
+        });
+        for (int j = -g; j <= Ny - 1 + g; j++)
+            for (int i = -g; i <= Nx - 1 + g; i++)
+                for (int c = 0; c < nc; c++) at(q, i, j, c) = v[(size_t(i + 3) + P * size_t(j + 3)) * nc + c];
     }
 };
 

@@ -149,6 +149,33 @@ int awcm_interpolate_euler(awcm_solver* h, const uint8_t* mask_from, const uint8
 /* AWCM_*::PostProcessEuler (AWCM_1D.h:122-125, AWCM_2D.h:149-152). */
 int awcm_post_process_euler(awcm_solver* h);
 
+/* ---- AWCM_MANAGER_1D / AWCM_MANAGER_2D mask algebra on caller arrays ------
+ * The static members of AWCM_MANAGER_1D (Wavelet/AWCM_MANAGER_1D.h:14-157)
+ * and AWCM_MANAGER_2D (Wavelet/AWCM_MANAGER_2D.h:14-280), on the GPU, for
+ * masks, levels and fields the caller owns, on the handle's MRA grid (only its
+ * geometry is used; the handle's own state is left unchanged).  Layouts are
+ * the reference's: a mask is ghosted uint8 ((Nx+6) or (Nx+6)(Ny+6) bytes, in
+ * and out), a level field int32 without ghosts (Nx or Nx*Ny entries, each in
+ * 0 .. num_levels-1), a field AoS ghosted with ncomp components.  A mask
+ * update that would leave the ghosted domain returns AWCM_ERANGE (the
+ * reference writes out of bounds there).  Synchronous. */
+/* AddAdjacentZone (AWCM_MANAGER_1D.h:14-43, AWCM_MANAGER_2D.h:14-61) */
+int awcm_add_adjacent_zone(awcm_solver* h, uint8_t* mask);
+/* CheckReconstruction (AWCM_MANAGER_1D.h:45-71, AWCM_MANAGER_2D.h:63-158) */
+int awcm_check_reconstruction(awcm_solver* h, uint8_t* mask);
+/* AddPointForDifferentiation (AWCM_MANAGER_1D.h:73-107, AWCM_MANAGER_2D.h:160-213) */
+int awcm_add_point_for_differentiation(awcm_solver* h, uint8_t* mask, const int32_t* level);
+/* ComputeLevel (AWCM_MANAGER_1D.h:109-133, AWCM_MANAGER_2D.h:215-241): level is
+ * in/out (a point with no active neighbour keeps its value, as in the
+ * reference) */
+int awcm_compute_level(awcm_solver* h, int32_t* level, const uint8_t* mask);
+/* RemoveResidue (AWCM_MANAGER_1D.h:135-157, both overloads; AWCM_MANAGER_2D.h:
+ * 257-266) and RemoveResidueEuler (AWCM_MANAGER_2D.h:267-280): zero every
+ * component outside mask -- 1-D at the finest-grid points, 2-D at the interior
+ * points (the outer ring is left alone, as in the reference).  ncomp: 1
+ * (scalar field) or the Euler field's 3 (1-D) / 4 (2-D). */
+int awcm_remove_residue(awcm_solver* h, double* q, int ncomp, const uint8_t* mask);
+
 /* Output-time schedule (arr_printing_times of the drivers,
  * main_1d_euler.cpp:43-48): dt is clipped so steps land on these times. */
 int awcm_set_schedule(awcm_solver* h, const double* print_times, int n_print);

@@ -1540,4 +1540,46 @@ double orc2_step(void* h, double tol, double cfl, const double* pt, int npt, int
 }
 long orc2_oob(void* h) { return static_cast<Solver2D*>(h)->oob; }
 
+// AWCM_MANAGER_1D / AWCM_MANAGER_2D statics on caller arrays: op 0
+// AddAdjacentZone, 1 CheckReconstruction, 2 AddPointForDifferentiation (level
+// read), 3 ComputeLevel (level in/out).  Ghosted masks, unghosted int levels.
+void orc1_mask_op(void* h, int op, uint8_t* mask, int32_t* level) {
+    auto* s = static_cast<Solver1D*>(h);
+    std::vector<uint8_t> m(mask, mask + s->Nx + 2 * GW);
+    std::vector<int> lv(size_t(s->Nx), 0);
+    if (level) for (int i = 0; i < s->Nx; i++) lv[i] = level[i];
+    if (op == 0) s->adjacent(m);
+    else if (op == 1) s->check_recon(m);
+    else if (op == 2) s->addpoint(m, lv);
+    else s->level(lv, m);
+    std::memcpy(mask, m.data(), m.size());
+    if (op == 3) for (int i = 0; i < s->Nx; i++) level[i] = lv[i];
+}
+void orc2_mask_op(void* h, int op, uint8_t* mask, int32_t* level) {
+    auto* s = static_cast<Solver2D*>(h);
+    const size_t np = size_t(s->P) * (s->Ny + 2 * GW), nl = size_t(s->Nx) * s->Ny;
+    std::vector<uint8_t> m(mask, mask + np);
+    std::vector<int> lv(nl, 0);
+    if (level) for (size_t t = 0; t < nl; t++) lv[t] = level[t];
+    if (op == 0) s->adjacent(m);
+    else if (op == 1) s->check_recon(m);
+    else if (op == 2) s->addpoint(m, lv);
+    else s->level(lv, m);
+    std::memcpy(mask, m.data(), np);
+    if (op == 3) for (size_t t = 0; t < nl; t++) level[t] = lv[t];
+}
+// AWCM_MANAGER_1D::RemoveResidue (AWCM_MANAGER_1D.h:135-157): finest points
+void orc1_remove_residue(void* h, double* q, int ncomp, const uint8_t* mask) {
+    auto* s = static_cast<Solver1D*>(h);
+    for (int i = 0; i <= s->Nx - 1; i++)
+        if (!mask[i + GW]) for (int c = 0; c < ncomp; c++) q[size_t(i + GW) * ncomp + c] = 0.0;
+}
+// AWCM_MANAGER_2D::RemoveResidue / RemoveResidueEuler (AWCM_MANAGER_2D.h:257-280): interior points
+void orc2_remove_residue(void* h, double* q, int ncomp, const uint8_t* mask) {
+    auto* s = static_cast<Solver2D*>(h);
+    for (int j = 1; j <= s->Ny - 2; j++)
+        for (int i = 1; i <= s->Nx - 2; i++)
+            if (!mask[s->id(i, j)]) for (int c = 0; c < ncomp; c++) q[s->id(i, j) * ncomp + c] = 0.0;
+}
+
 }  // extern "C"

@@ -32,6 +32,7 @@ void FIELD_UNIFORM_1D<bool>::AssignAllValues(const bool& v) {
 #include "Wavelet/AWCM_2D.h"
 
 #include <cstdint>
+#include <vector>
 #include <cstring>
 
 namespace {
@@ -488,6 +489,89 @@ void ref2_interpolate_all(void* h, const uint8_t* mask_from) {
 }
 double ref2_step(void* h, double tol, double cfl, const double* pt, int npt, int* cnt, double* t) {
     return static_cast<Ref2D*>(h)->step(tol, cfl, pt, npt, cnt, t);
+}
+
+// AWCM_MANAGER_1D / AWCM_MANAGER_2D statics on caller arrays, the reference's
+// own functions (op 0 AddAdjacentZone, 1 CheckReconstruction, 2
+// AddPointForDifferentiation, 3 ComputeLevel): the AWCM object's mask3 and
+// level3 fields (the reference's own extents) carry the arguments and are
+// restored afterwards.
+void ref1_mask_op(void* h, int op, uint8_t* mask, int32_t* level) {
+    auto* s = static_cast<Ref1D*>(h);
+    FIELD_UNIFORM_1D<bool>& m = s->awcm->mask3;
+    FIELD_UNIFORM_1D<int>& lv = s->awcm->level3;
+    const int n = m.arr.i_res, N = s->mra.Nx;
+    std::vector<bool> keep_m(n);
+    std::vector<int> keep_l(N);
+    for (int k = 0; k < n; k++) { keep_m[k] = m.arr.val[k]; m.arr.val[k] = mask[k] != 0; }
+    for (int i = 0; i < N; i++) { keep_l[i] = lv(i); lv(i) = level ? level[i] : 0; }
+    if (op == 0) AWCM_MANAGER_1D::AddAdjacentZone(m, s->mra);
+    else if (op == 1) AWCM_MANAGER_1D::CheckReconstruction(m, s->mra);
+    else if (op == 2) AWCM_MANAGER_1D::AddPointForDifferentiation(m, s->mra, lv);
+    else AWCM_MANAGER_1D::ComputeLevel(lv, m, s->mra);
+    for (int k = 0; k < n; k++) { mask[k] = m.arr.val[k] ? 1 : 0; m.arr.val[k] = keep_m[k]; }
+    for (int i = 0; i < N; i++) { if (op == 3) level[i] = lv(i); lv(i) = keep_l[i]; }
+}
+void ref2_mask_op(void* h, int op, uint8_t* mask, int32_t* level) {
+    auto* s = static_cast<Ref2D*>(h);
+    FIELD_UNIFORM_2D<bool>& m = s->awcm->mask3;
+    FIELD_UNIFORM_2D<int>& lv = s->awcm->level3;
+    const size_t n = m.arr.ij_res, nl = size_t(s->mra_grid.Nx) * s->mra_grid.Ny;
+    std::vector<bool> keep_m(n);
+    std::vector<int> keep_l(nl);
+    for (size_t k = 0; k < n; k++) { keep_m[k] = m.arr.val[k]; m.arr.val[k] = mask[k] != 0; }
+    for (size_t k = 0; k < nl; k++) { keep_l[k] = lv.arr.val[k]; lv.arr.val[k] = level ? level[k] : 0; }
+    if (op == 0) AWCM_MANAGER_2D::AddAdjacentZone(m, s->mra_grid);
+    else if (op == 1) AWCM_MANAGER_2D::CheckReconstruction(m, s->mra_grid);
+    else if (op == 2) AWCM_MANAGER_2D::AddPointForDifferentiation(m, s->mra_grid, lv);
+    else AWCM_MANAGER_2D::ComputeLevel(lv, m, s->mra_grid);
+    for (size_t k = 0; k < n; k++) { mask[k] = m.arr.val[k] ? 1 : 0; m.arr.val[k] = keep_m[k]; }
+    for (size_t k = 0; k < nl; k++) { if (op == 3) level[k] = lv.arr.val[k]; lv.arr.val[k] = keep_l[k]; }
+}
+// RemoveResidue (AWCM_MANAGER_1D.h:135-157, both overloads) on the AWCM
+// object's fields, restored afterwards
+void ref1_remove_residue(void* h, double* q, int ncomp, const uint8_t* mask) {
+    auto* s = static_cast<Ref1D*>(h);
+    FIELD_UNIFORM_1D<bool>& m = s->awcm->mask3;
+    const int n = m.arr.i_res;
+    std::vector<bool> keep_m(n);
+    for (int k = 0; k < n; k++) { keep_m[k] = m.arr.val[k]; m.arr.val[k] = mask[k] != 0; }
+    if (ncomp == 1) {
+        FIELD_UNIFORM_1D<double>& f = s->Qs;
+        std::vector<double> keep(f.arr.val, f.arr.val + f.arr.i_res);
+        for (int k = 0; k < f.arr.i_res; k++) f.arr.val[k] = q[k];
+        AWCM_MANAGER_1D::RemoveResidue(f, m, s->mra);
+        for (int k = 0; k < f.arr.i_res; k++) { q[k] = f.arr.val[k]; f.arr.val[k] = keep[k]; }
+    } else {
+        FIELD_UNIFORM_1D<VECTOR_3D<double>>& f = s->Q;
+        std::vector<VECTOR_3D<double>> keep(f.arr.val, f.arr.val + f.arr.i_res);
+        for (int k = 0; k < f.arr.i_res; k++) for (int c = 0; c < 3; c++) f.arr.val[k][c] = q[k * 3 + c];
+        AWCM_MANAGER_1D::RemoveResidue(f, m, s->mra);
+        for (int k = 0; k < f.arr.i_res; k++) { for (int c = 0; c < 3; c++) q[k * 3 + c] = f.arr.val[k][c]; f.arr.val[k] = keep[k]; }
+    }
+    for (int k = 0; k < n; k++) m.arr.val[k] = keep_m[k];
+}
+// RemoveResidue / RemoveResidueEuler (AWCM_MANAGER_2D.h:257-280)
+void ref2_remove_residue(void* h, double* q, int ncomp, const uint8_t* mask) {
+    auto* s = static_cast<Ref2D*>(h);
+    FIELD_UNIFORM_2D<bool>& m = s->awcm->mask3;
+    const size_t n = m.arr.ij_res;
+    std::vector<bool> keep_m(n);
+    for (size_t k = 0; k < n; k++) { keep_m[k] = m.arr.val[k]; m.arr.val[k] = mask[k] != 0; }
+    if (ncomp == 1) {
+        FIELD_UNIFORM_2D<double>& f = s->Qs;
+        std::vector<double> keep(f.arr.val, f.arr.val + f.arr.ij_res);
+        for (int k = 0; k < f.arr.ij_res; k++) f.arr.val[k] = q[k];
+        AWCM_MANAGER_2D::RemoveResidue(f, m, s->mra_grid);
+        for (int k = 0; k < f.arr.ij_res; k++) { q[k] = f.arr.val[k]; f.arr.val[k] = keep[k]; }
+    } else {
+        FIELD_UNIFORM_2D<VECTOR_4D<double>>& f = s->Q;
+        std::vector<VECTOR_4D<double>> keep(f.arr.val, f.arr.val + f.arr.ij_res);
+        for (int k = 0; k < f.arr.ij_res; k++) for (int c = 0; c < 4; c++) f.arr.val[k][c] = q[k * 4 + c];
+        AWCM_MANAGER_2D::RemoveResidueEuler(f, m, s->mra_grid);
+        for (int k = 0; k < f.arr.ij_res; k++) { for (int c = 0; c < 4; c++) q[k * 4 + c] = f.arr.val[k][c]; f.arr.val[k] = keep[k]; }
+    }
+    for (size_t k = 0; k < n; k++) m.arr.val[k] = keep_m[k];
 }
 
 }  // extern "C"

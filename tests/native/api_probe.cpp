@@ -10,6 +10,7 @@
 //
 //   awcm_api_probe host <dir>     host-side classes; writes Print() files in <dir>
 //   awcm_api_probe prepare        AWCM_1D/2D::PrepareEuler on five problems
+//   awcm_api_probe manager        AWCM_MANAGER_1D/2D static members
 //   awcm_api_probe loop <steps>   the drivers' time loop through the class API
 #include <cinttypes>
 #include <cstdint>
@@ -635,6 +636,114 @@ void run_prepare() {
     }
 }
 
+// AWCM_MANAGER_1D / AWCM_MANAGER_2D static members on copies of an AWCM
+// object's masks after two PrepareEuler calls: AddAdjacentZone and
+// CheckReconstruction of mask0, AddPointForDifferentiation of mask1 with
+// level1, ComputeLevel of mask2 into a copy of level0, RemoveResidue of the
+// density (scalar) and of the whole field outside mask1.
+template <class F>
+void hash_mask1(const F& m, Hash& h) {
+    for (int i = m.i_start_g; i <= m.i_end_g; i++) h.byte(m(i));
+}
+template <class F>
+void hash_mask2(const F& m, Hash& h) {
+    for (int j = m.j_start_g; j <= m.j_end_g; j++)
+        for (int i = m.i_start_g; i <= m.i_end_g; i++) h.byte(m(i, j));
+}
+void run_manager() {
+    for (const Cfg1& c : kCfg1) {
+        if (c.L > 8) continue;
+        MRA_GRID_INT_1D mra;
+        GRID_UNIFORM_1D g, gx;
+        FIELD_UNIFORM_1D<V3> Q;
+        double tf, gamma;
+        init1(c, mra, g, gx, Q, tf, gamma);
+        AWCM_1D awcm(mra);
+        awcm.PrepareEuler(Q, c.tol);
+        awcm.PrepareEuler(Q, c.tol);
+        const int Nx = mra.Nx;
+        auto copy_mask = [&](const FIELD_UNIFORM_1D<bool>& src) {
+            FIELD_UNIFORM_1D<bool> m(0, Nx, 0, Nx, 3);
+            for (int i = -3; i <= Nx + 2; i++) m(i) = src(i);
+            return m;
+        };
+        auto copy_level = [&](const FIELD_UNIFORM_1D<int>& src) {
+            FIELD_UNIFORM_1D<int> l(0, Nx, 0, Nx);
+            for (int i = 0; i < Nx; i++) l(i) = src(i);
+            return l;
+        };
+        Hash h;
+        FIELD_UNIFORM_1D<bool> a = copy_mask(awcm.mask0);
+        AWCM_MANAGER_1D::AddAdjacentZone(a, mra);
+        hash_mask1(a, h);
+        FIELD_UNIFORM_1D<bool> b = copy_mask(awcm.mask0);
+        AWCM_MANAGER_1D::CheckReconstruction(b, mra);
+        hash_mask1(b, h);
+        FIELD_UNIFORM_1D<bool> d = copy_mask(awcm.mask1);
+        AWCM_MANAGER_1D::AddPointForDifferentiation(d, mra, awcm.level1);
+        hash_mask1(d, h);
+        FIELD_UNIFORM_1D<int> l = copy_level(awcm.level0);
+        AWCM_MANAGER_1D::ComputeLevel(l, awcm.mask2, mra);
+        for (int i = 0; i < Nx; i++) h.i(l(i));
+        FIELD_UNIFORM_1D<double> rho(0, Nx, 0.0, 1.0, 3);
+        for (int i = -3; i <= Nx + 2; i++) rho(i) = Q(i)[0];
+        AWCM_MANAGER_1D::RemoveResidue(rho, awcm.mask1, mra);
+        FIELD_UNIFORM_1D<V3> Qc(0, Nx, 0.0, 1.0, 3);
+        for (int i = -3; i <= Nx + 2; i++) Qc(i) = Q(i);
+        AWCM_MANAGER_1D::RemoveResidue(Qc, awcm.mask1, mra);
+        for (int i = -3; i <= Nx + 2; i++) h.f(rho(i)), h.v3(Qc(i));
+        char tag[64];
+        std::snprintf(tag, sizeof tag, "manager.%s", c.name);
+        line(tag, h);
+        std::fflush(stdout);
+    }
+    for (const Cfg2& c : kCfg2) {
+        MRA_GRID_INT_2D mra;
+        GRID_UNIFORM_2D g;
+        FIELD_UNIFORM_2D<V4> Q;
+        double tf, gamma;
+        init2(c, mra, g, Q, tf, gamma);
+        AWCM_2D awcm(mra);
+        awcm.PrepareEuler(Q, c.tol);
+        awcm.PrepareEuler(Q, c.tol);
+        const int Nx = mra.Nx, Ny = mra.Ny;
+        auto copy_mask = [&](const FIELD_UNIFORM_2D<bool>& src) {
+            FIELD_UNIFORM_2D<bool> m(0, 0, Nx, Ny, 0, 0, Nx, Ny, 3);
+            for (int j = -3; j <= Ny + 2; j++)
+                for (int i = -3; i <= Nx + 2; i++) m(i, j) = src(i, j);
+            return m;
+        };
+        Hash h;
+        FIELD_UNIFORM_2D<bool> a = copy_mask(awcm.mask0);
+        AWCM_MANAGER_2D::AddAdjacentZone(a, mra);
+        hash_mask2(a, h);
+        FIELD_UNIFORM_2D<bool> b = copy_mask(awcm.mask0);
+        AWCM_MANAGER_2D::CheckReconstruction(b, mra);
+        hash_mask2(b, h);
+        FIELD_UNIFORM_2D<bool> d = copy_mask(awcm.mask1);
+        AWCM_MANAGER_2D::AddPointForDifferentiation(d, mra, awcm.level1);
+        hash_mask2(d, h);
+        FIELD_UNIFORM_2D<int> l(0, 0, Nx, Ny, 0, 0, Nx, Ny);
+        for (int j = 0; j < Ny; j++)
+            for (int i = 0; i < Nx; i++) l(i, j) = awcm.level0(i, j);
+        AWCM_MANAGER_2D::ComputeLevel(l, awcm.mask2, mra);
+        for (int j = 0; j < Ny; j++)
+            for (int i = 0; i < Nx; i++) h.i(l(i, j));
+        FIELD_UNIFORM_2D<double> rho(0, 0, Nx, Ny, 0, 0, Nx, Ny, 3);
+        FIELD_UNIFORM_2D<V4> Qc(0, 0, Nx, Ny, 0, 0, Nx, Ny, 3);
+        for (int j = -3; j <= Ny + 2; j++)
+            for (int i = -3; i <= Nx + 2; i++) rho(i, j) = Q(i, j)[0], Qc(i, j) = Q(i, j);
+        AWCM_MANAGER_2D::RemoveResidue(rho, awcm.mask1, mra);
+        AWCM_MANAGER_2D::RemoveResidueEuler(Qc, awcm.mask1, mra);
+        for (int j = -3; j <= Ny + 2; j++)
+            for (int i = -3; i <= Nx + 2; i++) h.f(rho(i, j)), h.v4(Qc(i, j));
+        char tag[64];
+        std::snprintf(tag, sizeof tag, "manager.%s", c.name);
+        line(tag, h);
+        std::fflush(stdout);
+    }
+}
+
 // The 1-D time loop of Wavelet/main_1d_euler.cpp through the class API.  With
 // use_js the finest-level difference is WENO_EULER_SOLVER_1D::ComputedF, as
 // the shipped driver has it; otherwise the WENO-Z face-flux pair.
@@ -880,6 +989,10 @@ int main(int argc, char** argv) {
         run_prepare();
         return 0;
     }
+    if (mode == "manager") {
+        run_manager();
+        return 0;
+    }
     if (mode == "loop" && argc > 2) {
         const int steps = std::atoi(argv[2]);
         loop1(kCfg1[0], steps, false);
@@ -890,6 +1003,6 @@ int main(int argc, char** argv) {
         loop2<BOUNDARY_MANAGER_DMACH>(kCfg2[1], steps);
         return 0;
     }
-    std::fprintf(stderr, "usage: %s host <dir> | prepare | loop <steps>\n", argv[0]);
+    std::fprintf(stderr, "usage: %s host <dir> | prepare | manager | loop <steps>\n", argv[0]);
     return 2;
 }

@@ -76,6 +76,10 @@ def load(prefix="orc"):
         f"{p}2_prepare_scalar": (None, [V, C.c_double]),
         f"{p}2_interpolate_scalar": (None, [V, _u8p, _u8p]),
         f"{p}2_step": (C.c_double, [V, C.c_double, C.c_double, _dp, C.c_int, _ip, _dp]),
+        f"{p}1_mask_op": (None, [V, C.c_int, _u8p, _i32p]),
+        f"{p}2_mask_op": (None, [V, C.c_int, _u8p, _i32p]),
+        f"{p}1_remove_residue": (None, [V, _dp, C.c_int, _u8p]),
+        f"{p}2_remove_residue": (None, [V, _dp, C.c_int, _u8p]),
         f"{p}_flux1": (None, [C.c_double, _dp, C.c_double, _dp]),
         f"{p}_flux2": (None, [C.c_double, C.c_int, _dp, C.c_double, _dp]),
         f"{p}_wenoz_left": (C.c_double, [C.c_double] * 5),
@@ -191,6 +195,20 @@ class Solver1D:
     def prepare(self, tol):
         getattr(self.lib, f"{self.p}1_prepare")(self.h, tol)
 
+    def mask_op(self, op, mask, level=None):
+        """AWCM_MANAGER_1D statics (op 0 AddAdjacentZone, 1 CheckReconstruction,
+        2 AddPointForDifferentiation, 3 ComputeLevel) on copies; returns (mask, level)."""
+        m = np.ascontiguousarray(mask, dtype=np.uint8).copy()
+        lv = np.ascontiguousarray(level if level is not None else np.zeros(self.nx), dtype=np.int32).copy()
+        getattr(self.lib, f"{self.p}1_mask_op")(self.h, op, u8ptr(m), i32ptr(lv))
+        return m, lv
+
+    def remove_residue(self, q, ncomp, mask):
+        q = np.ascontiguousarray(q, dtype=np.float64).copy()
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+        getattr(self.lib, f"{self.p}1_remove_residue")(self.h, dptr(q), ncomp, u8ptr(m))
+        return q
+
     def set_scalar(self, q):
         """The scalar field of AWCM_1D::Prepare ((nx+6) doubles)."""
         q = np.ascontiguousarray(q, dtype=np.float64)
@@ -298,6 +316,20 @@ class Solver2D:
 
     def prepare(self, tol):
         getattr(self.lib, f"{self.p}2_prepare")(self.h, tol)
+
+    def mask_op(self, op, mask, level=None):
+        """AWCM_MANAGER_2D statics (op 0 AddAdjacentZone, 1 CheckReconstruction,
+        2 AddPointForDifferentiation, 3 ComputeLevel) on copies; returns (mask, level)."""
+        m = np.ascontiguousarray(mask, dtype=np.uint8).copy()
+        lv = np.ascontiguousarray(level if level is not None else np.zeros(self.nx * self.ny), dtype=np.int32).copy()
+        getattr(self.lib, f"{self.p}2_mask_op")(self.h, op, u8ptr(m), i32ptr(lv))
+        return m, lv
+
+    def remove_residue(self, q, ncomp, mask):
+        q = np.ascontiguousarray(q, dtype=np.float64).copy()
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+        getattr(self.lib, f"{self.p}2_remove_residue")(self.h, dptr(q), ncomp, u8ptr(m))
+        return q
 
     def set_scalar(self, q):
         """The scalar field of AWCM_2D::Prepare ((ny+6)*(nx+6) doubles)."""

@@ -89,7 +89,7 @@ def test_golden_is_what_the_reference_prints(tmp_path):
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref-probe"], check=True)
     got = run_host(REF_PROBE, tmp_path)
     assert not diff_lines(got, golden("host"))
-    for mode, arg in (("prepare", []), ("loop", ["40"])):
+    for mode, arg in (("prepare", []), ("manager", []), ("loop", ["40"])):
         out = subprocess.run([REF_PROBE, mode] + arg, capture_output=True, text=True, timeout=300)
         assert out.returncode == 0, out.stderr
         assert not diff_lines(out.stdout.splitlines(), golden(mode)), mode

@@ -60,6 +60,11 @@ def test_prepare_euler_matches_reference():
     compare(run("prepare"), "prepare")
 
 
+def test_manager_statics_match_reference():
+    """AWCM_MANAGER_1D/2D's static mask algebra through the class API."""
+    compare(run("manager"), "manager")
+
+
 def test_driver_loop_matches_reference():
     compare(run("loop", "40"), "loop")
 

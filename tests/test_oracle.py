@@ -820,3 +820,90 @@ def test_oracle_matches_reference_on_index_limit_strip(shape):
     mr, lr = r.masks()
     assert all(np.array_equal(a, b) for a, b in zip(mo, mr)) and np.array_equal(lo, lr)
     assert np.array_equal(o.get_q().view(np.uint64), r.get_q().view(np.uint64))
+
+
+def mask_cases_2d(S, rng):
+    """Masks from a run (mask0, mask1) and a seeded random one with the 2-wide
+    band every AWCM mask carries, which at 4 levels keeps every ComputeLevel
+    search inside the ghosts."""
+    m, lv = S.masks()
+    ny, nx = m.shape[1] - 6, m.shape[2] - 6
+    r = (rng.random((ny + 6, nx + 6)) < 0.05).astype(np.uint8)
+    r[:, :5] = 1
+    r[:, -5:] = 1
+    r[:5, :] = 1
+    r[-5:, :] = 1
+    return [m[1].reshape(-1), m[2].reshape(-1), r.reshape(-1)], lv[0].reshape(-1)
+
+
+def mask_cases_1d(S, rng, levels):
+    """As mask_cases_2d.  ComputeLevel tests the two points at exactly +-d, d =
+    1, 2, .., 2^(levels-1); a point within that reach of an end must find an
+    active one before it leaves the ghosts, so the band spans 2^(levels-1)
+    points at each end (then the neighbour at d = 1 is active)."""
+    m, lv = S.masks()
+    r = (rng.random(m.shape[1]) < 0.1).astype(np.uint8)
+    w = 3 + 2 ** (levels - 1)
+    r[:w] = 1
+    r[-w:] = 1
+    return [m[1], m[2], r], lv[0]
+
+
+def run_mask_algebra(A, B, cases, lv0, ncomp, q):
+    """AWCM_MANAGER statics of A and B on every case: op 0 AddAdjacentZone, 1
+    CheckReconstruction, 2 AddPointForDifferentiation (with ComputeLevel's
+    levels of the same mask), 3 ComputeLevel (from level0), RemoveResidue on
+    the scalar and the Euler field."""
+    for mk in cases:
+        lv_mk = B.mask_op(3, mk, lv0)[1]
+        for op in range(4):
+            lvl = lv_mk if op == 2 else (lv0 if op == 3 else None)
+            a, b = A.mask_op(op, mk, lvl), B.mask_op(op, mk, lvl)
+            assert np.array_equal(a[0], b[0]), op
+            if op == 3:
+                assert np.array_equal(a[1], b[1])
+            if op != 3:
+                assert np.all(a[0] >= mk)  # these only add points
+        for nc in (1, ncomp):
+            qin = q if nc == ncomp else q.reshape(-1, ncomp)[:, 0].copy()
+            a, b = A.remove_residue(qin, nc, mk), B.remove_residue(qin, nc, mk)
+            assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
+
+
+@needs_ref
+@pytest.mark.parametrize("kind", ["quad", "dmr"])
+def test_oracle_mask_algebra_matches_reference_2d(kind):
+    """AWCM_MANAGER_2D's static members, oracle against the reference build, on
+    masks of a 5-step run and a random banded mask."""
+    L, nxc, nyc = (4, 9, 9) if kind == "quad" else (4, 17, 5)
+    nx, ny = (nxc - 1) * 2 ** (L - 1) + 1, (nyc - 1) * 2 ** (L - 1) + 1
+    if kind == "quad":
+        q, dx, _ = O.ic2(0, nx, ny, 0.0, 1.0, 0.0, 1.0, 0.5)
+        bc, xu, tf, npr = 0, 1.0, 0.25, 9
+    else:
+        q, dx, _ = O.ic2(1, nx, ny, 0.0, 4.0, 0.0, 1.0, 1.4)
+        bc, xu, tf, npr = 1, 4.0, 0.2, 21
+    S = {p: O.Solver2D(p, L, nxc, nyc, 1.4, bc, 0.0, xu, 0.0, 1.0) for p in ("orc", "ref")}
+    pts = O.print_times(tf, npr)
+    for s in S.values():
+        s.set_q(q)
+        for _ in range(5):
+            s.step(1e-3, 0.4, pts)
+    cases, lv0 = mask_cases_2d(S["ref"], np.random.default_rng(41))
+    run_mask_algebra(S["orc"], S["ref"], cases, lv0, 4, S["ref"].get_q())
+
+
+@needs_ref
+def test_oracle_mask_algebra_matches_reference_1d():
+    """AWCM_MANAGER_1D's static members, oracle against the reference build."""
+    L = 7
+    nx = 16 * 2 ** (L - 1) + 1
+    q, dx, _ = O.ic1(0, nx, 0.0, 1.0, 0.5, O.SOD_L, O.SOD_R)
+    pts = O.print_times(0.2, 41)
+    S = {p: O.Solver1D(p, L, 17, 1.4, dx) for p in ("orc", "ref")}
+    for s in S.values():
+        s.set_q(q)
+        for _ in range(20):
+            s.step(1e-4, 0.5, pts)
+    cases, lv0 = mask_cases_1d(S["ref"], np.random.default_rng(42), L)
+    run_mask_algebra(S["orc"], S["ref"], cases, lv0, 3, S["ref"].get_q())
