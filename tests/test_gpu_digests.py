@@ -1,0 +1,98 @@
+"""Long-horizon parity on the GPU against digests of the reference build.
+
+tests/golden/digests_*.json hold, from the reference's own classes
+(tests/golden/make_digests.py), SHA-256 digests of the state bits, masks and
+levels, with t, the step count and the print counter:
+
+* at every output time of the 2-D double Mach reflection (1025 x 257, 5
+  levels, 20 output times to t = 0.2, ~2600 steps) and of the four-quadrant
+  problem (513 x 513, 5 levels, 8 output times to t = 0.25);
+* at the first output times of the full-size runs: double Mach 4097 x 1025 at
+  t = 0.01 and 0.02, the quadrants 2049 x 2049 at t = 0.03125;
+* at the end of every one of the 512 batched shock tubes.
+
+The device runs each case through awcm_run (or the batch) and must reproduce
+every digest: bit-identical fields, masks and levels at every output time.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from digest_lib import digest, same
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _pyawcm():
+    import pyawcm
+    pyawcm.lib()
+    return pyawcm
+
+
+def load(name):
+    path = os.path.join(GOLDEN, f"digests_{name}.json")
+    if not os.path.exists(path):
+        pytest.skip(f"{path} not generated")
+    with open(path) as f:
+        return json.load(f)
+
+
+@pytest.mark.parametrize("name", ["quad_L5", "dmr_L5", "dmr_full", "quad_full"])
+def test_2d_output_times_match_reference(name):
+    P = _pyawcm()
+    d = load(name)
+    L, nxc, nyc = d["levels"], d["nx_coarse"], d["ny_coarse"]
+    ny = (nyc - 1) * 2 ** (L - 1) + 1
+    if d["problem"] == "quad":
+        q, dx = P.ic_quadrants_2d(ny, 0.5)
+        bc = P.BC_TRANSMISSIVE
+    else:
+        q, dx = P.ic_double_mach_2d(ny, 1.4)
+        bc = P.BC_DOUBLE_MACH
+    S = P.Solver(2, L, nxc, 1.4, dx, ny_coarse=nyc, x0=0.0, bc_kind=bc)
+    pts = P.print_times(d["t_final"], d["num_print"])
+    S.set_schedule(pts)
+    S.upload_q(q)
+    steps = 0
+    for want in d["outputs"]:
+        k = want["k"]
+        while S.t < pts[k]:
+            n, _ = S.run(d["tol"], d["cfl"], pts[k], 4096)
+            steps += n
+            assert n > 0
+        m, lv = S.masks()
+        got = digest(S.download_q(), m, lv)
+        print(f"{name} output {k}: t={S.t} steps={steps} mask0={got['n_mask0']}", flush=True)
+        assert float(S.t).hex() == want["t"] and steps == want["steps"] and S.cnt == want["cnt"], (k, S.t, steps)
+        assert not same(got, want), (k, same(got, want))
+
+
+def test_c4_every_instance_matches_reference():
+    """All 512 instances: 511 end at t = 0.15 bit-identical to the reference;
+    the one whose scheme breaks down (448) stops after 1558 steps, at the
+    PrepareEuler whose wave speed is not finite, with the reference's state
+    there (the reference itself goes on until its time is NaN, at step 1657)."""
+    P = _pyawcm()
+    import workloads as W
+    d = load("c4")
+    assert d["breakdown"] == [448] and d["instances"][448]["steps"] == 1657 and d["instances"][448]["t"] == "nan"
+    b, inst, params, states = W.shock_tube_batch(512)
+    b.run()
+    bad = []
+    for k in range(512):
+        st = b.state(k)
+        m, lv = b.masks(k)
+        got = digest(b.download_q(k), m, lv)
+        if k == 448:
+            want = d["stop_448_1558"]
+            ok = st["status"] == -4 and st["steps"] == 1558 and float(st["t"]).hex() == want["t"]
+        else:
+            want = d["instances"][k]
+            ok = st["status"] == 0 and st["steps"] == want["steps"] and float(st["t"]).hex() == want["t"]
+        if not ok or same(got, want):
+            bad.append((k, st, same(got, want)))
+    assert not bad, bad[:5]

@@ -907,3 +907,42 @@ def test_oracle_mask_algebra_matches_reference_1d():
             s.step(1e-4, 0.5, pts)
     cases, lv0 = mask_cases_1d(S["ref"], np.random.default_rng(42), L)
     run_mask_algebra(S["orc"], S["ref"], cases, lv0, 3, S["ref"].get_q())
+
+
+@needs_ref
+def test_reference_breakdown_of_batch_instance_448():
+    """Instance 448 of the batched workload breaks down in the reference
+    scheme itself: the reference build runs 1657 steps until its time turns
+    NaN (its MAX drops NaN wave speeds depending on operand order), and after
+    1558 steps the next PrepareEuler leaves a non-finite wave speed, where the
+    GPU stops the instance.  Both states are the committed digests, and the
+    oracle reproduces the second."""
+    import json
+    import workloads as W
+    from digest_lib import digest
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "digests_c4.json")) as f:
+        d = json.load(f)
+    L, rl, rr, pl, pr, u, x0, eps = W.shock_tube_instances(512)[448]
+    nx = 16 * 2 ** (L - 1) + 1
+    q, dx, _ = O.ic1(0, nx, 0.0, 1.0, x0, (rl, u, pl), (rr, u, pr))
+    pts = O.print_times(W.C4_T_FINAL, W.C4_NUM_PRINT)
+    r = O.Solver1D("ref", L, 17, 1.4, dx)
+    o = O.Solver1D("orc", L, 17, 1.4, dx)
+    r.set_q(q)
+    o.set_q(q)
+    for _ in range(1558):
+        r.step(eps, W.C4_CFL, pts)
+        o.step(eps, W.C4_CFL, pts)
+    o.prepare(eps)
+    m, lv = o.masks()
+    want = d["stop_448_1558"]
+    got = digest(o.get_q(), m, lv)
+    assert all(got[k] == want[k] for k in ("q", "masks", "levels")) and float(o.t[0]).hex() == want["t"]
+    n = 1558
+    while r.t[0] < W.C4_T_FINAL:
+        r.step(eps, W.C4_CFL, pts)
+        n += 1
+    assert n == 1657 and np.isnan(r.t[0])
+    m, lv = r.masks()
+    fin = digest(r.get_q(), m, lv)
+    assert all(fin[k] == d["instances"][448][k] for k in ("q", "masks", "levels"))
