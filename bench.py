@@ -7,23 +7,35 @@ coarse), pre-shock state (1.4, 0, 0, 1), eps = 1e-3, WENO-Z5 + Roe, SSP-RK3 at
 CFL 0.4, output schedule of 21 times to t = 0.2, FP64.
 
 A step is one full AWCM time step (PrepareEuler + CFL dt + three SSP-RK3
-stages + post-processing) over the device-resident state.  The solution is
-first advanced to --advance (untimed) so the measured steps see a developed
-flow, then --warmup steps run untimed and --steps steps are timed with HIP
-events on the solver's stream (no host synchronisation inside the window).
+stages + post-processing) over the device-resident state, replayed as one HIP
+graph.  The run:
+
+1. the whole DMR run from t = 0 to t = 0.2 on the GPU, timed with HIP events
+   on the solver's stream in chunks of steps (full_run: updates/s over the
+   run), with the state snapshotted as it passes t = 0.02, 0.1 and 0.19;
+2. from each snapshot, --warmup untimed steps, then exactly --steps timed
+   steps (windows); the t ~ 0.1 window is `value`.  Each window is replayed
+   twice more from the same state, bit-identically: with HIP events around
+   every kernel class (breakdown, roofline of k2_stage_faces) and step by step
+   with the masks read back (the algorithmic HBM bytes of each step);
+3. the four-quadrant problem at full size (2049 x 2049), a window at its first
+   output time;
+4. the batched 1-D form and the adjoint;
+5. the reference's CPU path (oracle/_ref, the reference's own headers) on a
+   bounded sample of the headline window, with all host threads and with one.
 
 value = active-point RK3 updates per second = sum over the timed steps of
-|mask0| / device time.  The CPU baseline resumes the reference build (or the
-oracle port) from the exact state the timed window started from and times a
-bounded number of the same steps on the host cores.
+|mask0| / device time.  Prints one JSON line.
 
-Prints one JSON line.
+    python bench.py [--steps K] [--warmup W] [--only T] [--no-full] [--no-cpu] [--no-batch] [--no-c2]
+    python bench.py --hbm-copy     # a plain device copy of 1 GiB (HBM counter calibration)
 """
 import argparse
 import json
 import os
 import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -40,6 +52,12 @@ FP64_PEAK_TFLOPS = 78.6
 LEVELS, NXC, NYC = 7, 65, 17
 RES = (NYC - 1) * 2 ** (LEVELS - 1) + 1  # 1025
 TOL, CFL, T_FINAL, NUM_PRINT, RHO_PRE = 1e-3, 0.4, 0.2, 21, 1.4
+WINDOWS = (0.02, 0.1, 0.19)
+HEADLINE = 0.1
+# four-quadrant configuration (full size)
+Q_LEVELS, Q_NXC = 7, 33
+Q_RES = (Q_NXC - 1) * 2 ** (Q_LEVELS - 1) + 1  # 2049
+Q_TOL, Q_CFL, Q_T_FINAL, Q_NUM_PRINT = 1e-3, 0.4, 0.25, 9
 
 # Algorithmic cost of the dominant kernel, k2_stage_faces (one WENO-Z/Roe face
 # flux per thread, each face of the stage once).  FP64 operations of one face
@@ -49,11 +67,11 @@ FLOPS_PER_FACE = 1202
 # Bytes per face: the face's own state (32 B, neighbours' states are the
 # neighbouring faces' own reads), its 4 B list entry, and the 32 B flux written.
 BYTES_PER_FACE = 32 + 4 + 32
-# Measured on this workload at t = 0.02 with rocprofv3 (profiles/
-# k2_stage_faces_dmr_pmc.md): HBM-side bytes per k2_stage_faces launch =
-# FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a 1 GiB device copy), and
-# issued FP64 lane operations per face (ADD + MUL + TRANS + 2 FMA, x 64 lanes).
-TRAFFIC_BYTES_PER_LAUNCH = 94705466
+# Measured with rocprofv3 on the t ~ 0.1 window (profiles/
+# k2_stage_faces_dmr_pmc.md): HBM bytes per face of a k2_stage_faces launch,
+# FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from `bench.py --hbm-copy`), and
+# issued FP64 lane operations per face.  None until measured.
+TRAFFIC_BYTES_PER_FACE = None
 ISSUED_FP64_OPS_PER_FACE = 2143
 
 
@@ -64,21 +82,17 @@ def ensure_built():
     return pyawcm
 
 
-def cpu_baseline(q, m0, t, cnt, budget_s):
-    """Time the same steps on the host from the GPU state (reference build if present)."""
-    try:
-        import oracle_lib as O
-    except Exception as e:  # pragma: no cover
-        return {"error": f"oracle unavailable: {e}"}
+def cpu_sample(path, budget_s):
+    """Time DMR steps of the reference build (or the oracle port) on the host
+    from the state saved in `path`; OpenMP threads from OMP_NUM_THREADS."""
+    import oracle_lib as O
+    z = np.load(path)
     kind, prefix = ("reference", "ref") if O.have_ref() else ("port", "orc")
-    try:
-        s = O.Solver2D(prefix, LEVELS, NXC, NYC, 1.4, 1, 0.0, 4.0, 0.0, 1.0)
-    except FileNotFoundError as e:
-        return {"error": str(e)}
-    s.set_q(q)
-    s.set_mask0(m0)
-    s.t[0] = t
-    s.cnt.value = cnt
+    s = O.Solver2D(prefix, LEVELS, NXC, NYC, 1.4, 1, 0.0, 4.0, 0.0, 1.0)
+    s.set_q(z["q"])
+    s.set_mask0(z["m0"])
+    s.t[0] = float(z["t"])
+    s.cnt.value = int(z["cnt"])
     pts = O.print_times(T_FINAL, NUM_PRINT)
     upd, elapsed, steps = 0, 0.0, 0
     while elapsed < budget_s and steps < 50:
@@ -88,10 +102,38 @@ def cpu_baseline(q, m0, t, cnt, budget_s):
         steps += 1
         m, _ = s.masks()
         upd += int(m[1, 3:-3, 3:-3].sum())
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": upd / elapsed, "unit": "updates/s", "cores": threads, "kind": kind,
-            "sample": f"{steps} DMR steps resumed from the GPU state at t={t:.6f} "
-                      f"({elapsed:.1f} s wall, OpenMP {threads} threads, {prefix} build)"}
+    return {"value": upd / elapsed, "kind": kind, "steps": steps, "elapsed_s": elapsed, "prefix": prefix}
+
+
+def cpu_baseline(snap, budget_s, threads):
+    """The CPU sample in a child process with `threads` OpenMP threads."""
+    q, m0, t, cnt = snap
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "state.npz")
+        np.savez(path, q=q, m0=m0, t=t, cnt=cnt)
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close")
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-sample", path, "--cpu-budget",
+                            str(budget_s)], capture_output=True, text=True, env=env, timeout=budget_s * 6 + 600)
+    if r.returncode != 0:
+        return {"error": r.stderr[-400:]}
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    return {"value": res["value"], "unit": "updates/s", "cores": threads, "kind": res["kind"],
+            "sample": f"{res['steps']} DMR steps of the headline window, resumed from the GPU state at t={t:.6f} "
+                      f"({res['elapsed_s']:.1f} s wall, OpenMP {threads} thread(s), {res['prefix']} build)"}
+
+
+def hbm_copy():
+    """A plain device copy of known size, for scaling the FETCH_SIZE/WRITE_SIZE
+    counters: 1 GiB filled, then copied device to device, three times."""
+    import torch
+    n = (1 << 30) // 8
+    a = torch.empty(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"hbm_copy_bytes": n * 8, "copies": 3}))
 
 
 def batched_section(P, cmp_steps):
@@ -177,66 +219,202 @@ def adjoint_section(P):
             "adjoint_over_forward": bwd / fwd, "record_plus_backward_over_forward": (rec + bwd) / fwd}
 
 
+def dmr_solver(P):
+    q0, dx = P.ic_double_mach_2d(RES, RHO_PRE)
+    S = P.Solver(2, LEVELS, NXC, 1.4, dx, ny_coarse=NYC, x0=0.0, bc_kind=P.BC_DOUBLE_MACH)
+    S.set_schedule(P.print_times(T_FINAL, NUM_PRINT))
+    S.upload_q(q0)
+    return S
+
+
+def snapshot(S):
+    S.synchronize()
+    return (S.download_q(), S.masks()[0][1].copy(), S.t, S.cnt)
+
+
+def restore(S, snap):
+    q, m0, t, cnt = snap
+    S.upload_q(q)
+    S.set_mask0(m0)
+    S.t, S.cnt = t, cnt
+
+
+def full_run(S, t_final, marks, chunk=32):
+    """The run to t_final in timed chunks of graph-replayed steps, with a
+    snapshot as t passes each mark (between chunks, outside the timed spans)."""
+    ms = upd = steps = 0
+    snaps = {}
+    while S.t < t_final:
+        for w in marks:
+            if w not in snaps and S.t >= w:
+                snaps[w] = snapshot(S)
+        n, u, dt_ms = S.run_timed(TOL, CFL, t_final, chunk)
+        ms, upd, steps = ms + dt_ms, upd + u, steps + n
+        if n == 0:
+            break
+    return {"steps": steps, "updates": upd, "device_s": ms / 1e3, "updates_per_s": upd / (ms / 1e3),
+            "ms_per_step": ms / max(steps, 1), "t_end": S.t}, snaps
+
+
+def step_bytes(m, prev_m0):
+    """Algorithmic HBM bytes of one 2-D step from its masks (interior points):
+    s(2|M2| + 3|M1| + 3|M0| + |M3| - |M0|) RK stages and stencil halos,
+    16(|P| + |M3|) density transforms, 8(nc-1)|M3 minus P| interpolation,
+    5(|M0|+|M1|+|M2|+|M3|) list entries and levels; s = 32 B, nc = 4."""
+    M = [int(m[k, 3:-3, 3:-3].sum()) for k in range(1, 5)]
+    P0 = int(prev_m0[3:-3, 3:-3].sum())
+    new = int((m[4, 3:-3, 3:-3] & (1 - prev_m0[3:-3, 3:-3])).sum())
+    s = 32
+    return s * (2 * M[2] + 3 * M[1] + 3 * M[0] + M[3] - M[0]) + 16 * (P0 + M[3]) + 8 * 3 * new + 5 * sum(M)
+
+
+def window(P, S, snap, K, W, detail=True):
+    """W warm-up steps from snap, then K timed steps (graph replay); with
+    detail, the same K steps again with per-kernel events and step by step with
+    the masks read back (both bit-identical to the timed replay)."""
+    restore(S, snap)
+    S.run(TOL, CFL, T_FINAL, W)
+    start = snapshot(S)
+    steps, upd, ms = S.run_timed(TOL, CFL, T_FINAL, K)
+    t_end = S.t
+    res = {"t_start": start[2], "t_end": t_end, "steps": steps, "updates": upd, "device_ms": ms,
+           "ms_per_step": ms / max(steps, 1), "updates_per_s": upd / (ms / 1e3),
+           "mean_active_points": upd / max(steps, 1)}
+    if not detail:
+        return res, start
+    restore(S, start)
+    S.profile(True)
+    steps_p, upd_p, ms_p = S.run_timed(TOL, CFL, T_FINAL, K)
+    w_ms, w_launch, w_pts = S.profile_read(P.KERNEL_STAGE_WENO)
+    c_ms, c_launch, c_pts = S.profile_read(P.KERNEL_STAGE_COARSE)
+    p_ms, p_launch, _ = S.profile_read(P.KERNEL_PREPARE)
+    S.profile(False)
+    assert steps_p == steps and upd_p == upd and S.t == t_end, "profiled replay diverged"
+    res["eager_ms_per_step"] = ms_p / max(steps, 1)
+    res["breakdown_ms_per_step"] = {"stage_faces": w_ms / max(steps, 1), "stage_update": c_ms / max(steps, 1),
+                                    "prepare_euler": p_ms / max(steps, 1),
+                                    "other": (ms_p - w_ms - c_ms - p_ms) / max(steps, 1)}
+    res["faces"] = {"launches": w_launch, "faces": w_pts, "ms": w_ms}
+    # algorithmic bytes, step by step
+    restore(S, start)
+    prev = S.masks()[0][1].copy()
+    nbytes = 0
+    for _ in range(K):
+        S.run(TOL, CFL, T_FINAL, 1)
+        m, _ = S.masks()
+        nbytes += step_bytes(m, prev)
+        prev = m[1].copy()
+    assert S.t == t_end, "step-by-step replay diverged"
+    gbs = nbytes / (ms / 1e3) / 1e9
+    res["algorithmic_bytes_per_step"] = nbytes / max(steps, 1)
+    res["algorithmic_gbs"] = gbs
+    res["hbm_frac"] = gbs / HBM_PEAK_GBS
+    return res, start
+
+
+def faces_roofline(win):
+    f = win["faces"]
+    avg_launch_s = (f["ms"] / f["launches"]) / 1e3
+    per_launch = f["faces"] / f["launches"]
+    achieved = per_launch * FLOPS_PER_FACE / avg_launch_s / 1e12
+    gbs = per_launch * BYTES_PER_FACE / avg_launch_s / 1e9
+    traffic = per_launch * TRAFFIC_BYTES_PER_FACE if TRAFFIC_BYTES_PER_FACE else None
+    return {
+        "bound": "fp64-valu",
+        "compute": "FP64 vector ALU: k2_stage_faces issues no MFMA; MI355X's dense FP64 peak is 78.6 TFLOP/s, and "
+                   "bit-exact parity forbids FMA contraction",
+        "kernel": "k2_stage_faces (WENO-Z/Roe face fluxes, each face of a stage once), headline window",
+        "achieved": achieved,
+        "peak": FP64_PEAK_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": achieved / FP64_PEAK_TFLOPS,
+        "traffic": traffic,
+        "traffic_over_algorithmic": traffic / (per_launch * BYTES_PER_FACE) if traffic else None,
+        "issued_fp64_tflops": per_launch * ISSUED_FP64_OPS_PER_FACE / avg_launch_s / 1e12,
+        "hbm_algorithmic_gbs": gbs,
+        "avg_launch_ms": avg_launch_s * 1e3,
+        "faces_per_launch": per_launch,
+        "flops_per_face": FLOPS_PER_FACE,
+        "bytes_per_face": BYTES_PER_FACE,
+    }
+
+
+def quadrants_section(P, K, W):
+    """The four-quadrant problem at full size (2049 x 2049, 7 levels, 33 x 33
+    coarse, eps 1e-3, CFL 0.4): advanced to its first output time, then a
+    window of K steps."""
+    q, dx = P.ic_quadrants_2d(Q_RES, 0.5)
+    S = P.Solver(2, Q_LEVELS, Q_NXC, 1.4, dx, ny_coarse=Q_NXC, x0=0.0, bc_kind=P.BC_TRANSMISSIVE)
+    pts = P.print_times(Q_T_FINAL, Q_NUM_PRINT)
+    S.set_schedule(pts)
+    S.upload_q(q)
+    t0 = time.perf_counter()
+    adv = 0
+    while S.t < pts[1]:
+        n, _ = S.run(Q_TOL, Q_CFL, pts[1], 256)
+        adv += n
+        if n == 0:
+            break
+    adv_s = time.perf_counter() - t0
+    S.run(Q_TOL, Q_CFL, Q_T_FINAL, W)
+    t_start = S.t
+    steps, upd, ms = S.run_timed(Q_TOL, Q_CFL, Q_T_FINAL, K)
+    return {"workload": "2-D four-quadrant Riemann problem AWCM, 2049x2049 finest (7 levels, 33x33 coarse), eps=1e-3, "
+                        "WENO-Z5+Roe, SSP-RK3 CFL 0.4, FP64",
+            "advance_steps": adv, "advance_wall_s": adv_s, "t_start": t_start, "t_end": S.t, "steps": steps,
+            "updates": upd, "ms_per_step": ms / max(steps, 1), "updates_per_s": upd / (ms / 1e3),
+            "mean_active_points": upd / max(steps, 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--advance", type=float, default=0.02, help="untimed advance to this time first")
-    ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU-baseline work")
+    ap.add_argument("--only", type=float, default=None,
+                    help="advance untimed to this time and time one window there (for profiling)")
+    ap.add_argument("--no-full", action="store_true", help="advance untimed to the windows instead of the timed run")
+    ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of each CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-batch", action="store_true", help="skip the batched 1-D and adjoint-forward sections")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched 1-D and adjoint sections")
+    ap.add_argument("--no-c2", action="store_true", help="skip the four-quadrant window")
     ap.add_argument("--batch-steps", type=int, default=40,
                     help="steps per instance of the batched-vs-sequential comparison")
+    ap.add_argument("--cpu-sample", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--hbm-copy", action="store_true")
     args = ap.parse_args()
+    if args.cpu_sample:
+        print(json.dumps(cpu_sample(args.cpu_sample, args.cpu_budget)))
+        return
+    if args.hbm_copy:
+        hbm_copy()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build runs every solve on a single MI355X); "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
 
     P = ensure_built()
-    q0, dx = P.ic_double_mach_2d(RES, RHO_PRE)
-    S = P.Solver(2, LEVELS, NXC, 1.4, dx, ny_coarse=NYC, x0=0.0, bc_kind=P.BC_DOUBLE_MACH)
-    S.set_schedule(P.print_times(T_FINAL, NUM_PRINT))
-    S.upload_q(q0)
-    t_adv0 = time.perf_counter()
-    adv_steps = 0
-    while S.t < args.advance:
-        n, _ = S.run(TOL, CFL, args.advance, 256)
-        adv_steps += n
-        if n == 0:
-            break
-    adv_s = time.perf_counter() - t_adv0
-    S.run(TOL, CFL, T_FINAL, args.warmup)
-    S.synchronize()
-    # state at the start of the timed window: replayed for the per-kernel timing
-    # pass and handed to the CPU baseline
-    t_start, cnt_start = S.t, S.cnt
-    q_start = S.download_q()
-    m_start = S.masks()[0][1].copy()
-
-    # timed window: whole-step HIP graph replays, no host synchronisation
-    steps, updates, ms = S.run_timed(TOL, CFL, T_FINAL, args.steps)
-    t_end = S.t
-
-    # per-kernel HIP-event timing over the same window (state restored bit-exactly)
-    S.upload_q(q_start)
-    S.set_mask0(m_start)
-    S.t, S.cnt = t_start, cnt_start
-    S.profile(True)
-    steps_p, updates_p, ms_p = S.run_timed(TOL, CFL, T_FINAL, args.steps)
-    w_ms, w_launch, w_pts = S.profile_read(P.KERNEL_STAGE_WENO)
-    c_ms, c_launch, c_pts = S.profile_read(P.KERNEL_STAGE_COARSE)
-    p_ms, p_launch, _ = S.profile_read(P.KERNEL_PREPARE)
-    S.profile(False)
-    assert steps_p == steps and updates_p == updates and S.t == t_end, "profiled replay diverged"
-
-    value = updates / (ms / 1e3) if ms > 0 else 0.0
-    # dominant kernel: WENO-Z/Roe face fluxes of a stage
-    avg_launch_s = (w_ms / w_launch) / 1e3 if w_launch else float("nan")
-    faces_per_launch = w_pts / w_launch if w_launch else 0.0
-    achieved_tflops = faces_per_launch * FLOPS_PER_FACE / avg_launch_s / 1e12 if w_launch else 0.0
-    achieved_gbs = faces_per_launch * BYTES_PER_FACE / avg_launch_s / 1e9 if w_launch else 0.0
+    S = dmr_solver(P)
+    marks = (args.only,) if args.only is not None else WINDOWS
+    headline = args.only if args.only is not None else HEADLINE
+    full = None
+    if args.only is None and not args.no_full:
+        full, snaps = full_run(S, T_FINAL, marks)
+    else:
+        snaps = {}
+        for w in marks:
+            while S.t < w:
+                n, _ = S.run(TOL, CFL, w, 256)
+                if n == 0:
+                    break
+            snaps[w] = snapshot(S)
+    wins, starts = {}, {}
+    for w in marks:
+        wins[w], starts[w] = window(P, S, snaps[w], args.steps, args.warmup)
+    head = wins[headline]
+    steps, ms = head["steps"], head["device_ms"]
+    value = head["updates_per_s"]
 
     out = {
         "metric": METRIC,
@@ -252,44 +430,34 @@ def main():
         "dtype": "f64",
         "data": "synthetic (double Mach reflection initial condition, advanced on the GPU)",
         "config": {"workload": "2-D double Mach reflection AWCM, 4097x1025 finest (7 levels, 65x17 coarse), "
-                               "eps=1e-3, WENO-Z5+Roe, SSP-RK3 CFL 0.4, FP64",
-                   "t_start": t_start, "t_end": t_end, "eager_ms_per_step": ms_p / max(steps, 1), "advance_steps": adv_steps,
-                   "advance_wall_s": round(adv_s, 3), "updates": updates,
-                   "mean_active_points": updates / max(steps, 1)},
-        "roofline": {
-            "bound": "mfma",
-            "compute": "FP64 vector ALU: the kernel issues no MFMA; MI355X's FP64 vector and matrix peaks are both "
-                       "78.6 TFLOP/s, and bit-exact parity forbids FMA contraction",
-            "kernel": "k2_stage_faces (WENO-Z/Roe face fluxes, each face of a stage once)",
-            "achieved": achieved_tflops,
-            "peak": FP64_PEAK_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": achieved_tflops / FP64_PEAK_TFLOPS,
-            "traffic": TRAFFIC_BYTES_PER_LAUNCH,
-            "traffic_over_algorithmic": TRAFFIC_BYTES_PER_LAUNCH / (faces_per_launch * BYTES_PER_FACE)
-            if faces_per_launch else None,
-            "issued_fp64_tflops": faces_per_launch * ISSUED_FP64_OPS_PER_FACE / avg_launch_s / 1e12 if w_launch else 0.0,
-            "hbm_achieved_gbs": achieved_gbs,
-            "hbm_frac": achieved_gbs / HBM_PEAK_GBS,
-            "avg_launch_ms": avg_launch_s * 1e3,
-            "faces_per_launch": faces_per_launch,
-            "flops_per_face": FLOPS_PER_FACE,
-            "bytes_per_face": BYTES_PER_FACE,
-        },
-        "breakdown_ms_per_step": {
-            "stage_faces": w_ms / max(steps, 1),
-            "stage_update": c_ms / max(steps, 1),
-            "prepare_euler": p_ms / max(steps, 1),
-        },
+                               "eps=1e-3, WENO-Z5+Roe, SSP-RK3 CFL 0.4, FP64; value = the window of --steps steps "
+                               f"after --warmup steps from t~{headline}",
+                   "t_start": head["t_start"], "t_end": head["t_end"], "updates": head["updates"],
+                   "mean_active_points": head["mean_active_points"]},
+        "roofline": faces_roofline(head),
+        "step_hbm": {"algorithmic_bytes_per_step": head["algorithmic_bytes_per_step"],
+                     "achieved_gbs": head["algorithmic_gbs"], "peak_gbs": HBM_PEAK_GBS, "frac": head["hbm_frac"]},
+        "breakdown_ms_per_step": head["breakdown_ms_per_step"],
+        "windows": {f"t~{w}": wins[w] for w in marks},
     }
-    if not args.no_batch:
+    if full:
+        out["full_run"] = dict(full, workload="the whole DMR run, t = 0 .. 0.2, HIP-event device time over "
+                                              "chunks of 32 graph-replayed steps")
+    if not args.no_c2 and args.only is None:
+        out["quadrants"] = quadrants_section(P, args.steps, args.warmup)
+    if not args.no_batch and args.only is None:
         out["batched"] = batched_section(P, args.batch_steps)
         out["adjoint"] = adjoint_section(P)
     if not args.no_cpu:
-        cb = cpu_baseline(q_start, m_start, t_start, cnt_start, args.cpu_budget)
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        cb = cpu_baseline(starts[headline], args.cpu_budget, threads)
         out["cpu_baseline"] = cb
         if "value" in cb and cb["value"] > 0:
             out["gpu_over_cpu"] = value / cb["value"]
+        c1 = cpu_baseline(starts[headline], args.cpu_budget, 1)
+        out["cpu_baseline_1core"] = c1
+        if "value" in c1 and c1["value"] > 0:
+            out["gpu_over_cpu_1core"] = value / c1["value"]
     print(json.dumps(out))
 
 
