@@ -20,7 +20,8 @@ graph.  The run:
    with the masks read back (the algorithmic HBM bytes of each step);
 3. the four-quadrant problem at full size (2049 x 2049), a window at its first
    output time;
-4. the batched 1-D form and the adjoint;
+4. the batched 1-D form, the adjoint and the scalar box-advection driver's
+   loop (main_2d_advect.cpp) on the device;
 5. the reference's CPU path (oracle/_ref, the reference's own headers) on a
    bounded sample of the headline window, with all host threads and with one.
 
@@ -366,6 +367,42 @@ def quadrants_section(P, K, W):
             "mean_active_points": upd / max(steps, 1)}
 
 
+def advect_section(P, iters=100, cpu_iters=5):
+    """The scalar box-advection driver (Wavelet/main_2d_advect.cpp) in its own
+    configuration -- 449 x 449 (6 levels, 15 x 15 coarse), eps 1e-4, dt = 0.4
+    dx -- for its 100 iterations in one awcm_run_scalar call (host wall clock
+    around the synchronous call, after a warm-up on another handle), with the
+    oracle port's host time per iteration from the same start beside it."""
+    import advect_lib as A
+    import oracle_lib as O
+    L, nxc, eps, _, res, q, dx, dt = A.setup("driver")
+
+    def fresh():
+        S = P.Solver(2, L, nxc, 1.4, dx, ny_coarse=nxc, x0=0.0, bc_kind=P.BC_TRANSMISSIVE)
+        S.upload_scalar(q)
+        return S
+
+    fresh().run_scalar(eps, dt, 2)
+    S = fresh()
+    t0 = time.perf_counter()
+    S.run_scalar(eps, dt, iters)
+    wall = time.perf_counter() - t0
+    m, _ = S.masks()
+    xl, xu, _ = O.box_grid(res)
+    R = O.Solver2D("orc", L, nxc, nxc, 1.4, 0, xl, xu, xl, xu)
+    R.set_scalar(q)
+    t0 = time.perf_counter()
+    R.advect(eps, dt, cpu_iters)
+    cpu = (time.perf_counter() - t0) / cpu_iters
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"workload": "main_2d_advect.cpp's loop: box advection 449x449 (6 levels, 15x15 coarse), eps 1e-4, "
+                        "dt = 0.4 dx, WENO5 right reconstructions + AWCM central differences, SSP-RK3, FP64",
+            "iterations": iters, "wall_s": wall, "ms_per_iteration": 1e3 * wall / iters,
+            "active_points_end": int(m[1, 3:-3, 3:-3].sum()),
+            "cpu_port_ms_per_iteration": 1e3 * cpu, "cpu_port_iterations": cpu_iters, "cpu_port_threads": threads,
+            "gpu_over_cpu_port": cpu / (wall / iters)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -448,6 +485,7 @@ def main():
     if not args.no_batch and args.only is None:
         out["batched"] = batched_section(P, args.batch_steps)
         out["adjoint"] = adjoint_section(P)
+        out["scalar_advect"] = advect_section(P)
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         cb = cpu_baseline(starts[headline], args.cpu_budget, threads)

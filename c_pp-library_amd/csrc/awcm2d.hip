@@ -533,15 +533,20 @@ __device__ __forceinline__ void st16(uint8_t* p, const V16& x) { *reinterpret_ca
 // the step's masks touch, unless `dirty` says Q may be non-zero elsewhere; a
 // PrepareEuler on its own (prepare_only) runs no residue pass but drops points
 // from mask0, so it leaves `dirty` set for the next pass.
+// Every thread decides the skip itself, from t and t_final (which only earlier
+// launches write): a skipped step leaves mask_temp and masks 1..3 as the last
+// step left them.
 __global__ void k2_begin_copy(Dims2 D, const uint8_t* __restrict__ src, uint8_t* __restrict__ dst, double* alpha,
                               uint8_t* z1, uint8_t* z2, uint8_t* z3, Scal2* sc, int force, int prepare_only) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int skip = (!force && !(sc->t < sc->t_final)) ? 1 : 0;
     if (t == 0) {
-        sc->skip = (!force && !(sc->t < sc->t_final)) ? 1 : 0;
-        if (!sc->skip) { sc->residue_full = sc->dirty; sc->dirty = prepare_only ? 1 : 0; }
+        sc->skip = skip;
+        if (!skip) { sc->residue_full = sc->dirty; sc->dirty = prepare_only ? 1 : 0; }
         sc->vmax = 0ull;
         for (int k = 0; k < 3; k++) { sc->cntW[k] = 0; sc->cntC[k] = 0; sc->cntXS[k] = 0; sc->cntYS[k] = 0; }
     }
+    if (skip) return;
     const int na = 3 * (D.Nx + D.Ny);
     if (t < na) alpha[t] = 0.0;
     Seg s;
@@ -1237,6 +1242,91 @@ __global__ void k2_combine(Dims2 D, const uint8_t* __restrict__ m0, const uint8_
     }
 }
 
+// -------------------------------------------------- scalar box advection ----
+// The scalar driver's iteration after its Prepare (Wavelet/main_2d_advect.cpp:
+// 50-198) on plane 0 of Q, velocity (1, 1), with the driver's three stage
+// fields A2, A1, A0 (one plane each, kept between iterations as the driver
+// keeps Q_2, Q_1, Q_0).
+
+// BOUNDARY_MANAGER::SetBoundaryCondition of PROBLEM_BOX_ADVECTION.h:78-151:
+// periodic x ghosts of rows 0..Ny-1 and y ghosts of columns 0..Nx-1 (corners
+// untouched).  Every source is an interior point, so no ghost reads another.
+__global__ void k2_periodic(Dims2 D, double* __restrict__ F) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nxs = G * D.Ny, nys = G * D.Nx;
+    if (t < 2 * nxs) {
+        const int r = t % nxs, j = r / G, c = r % G;
+        if (t < nxs) F[gidx(D, -1 - c, j)] = F[gidx(D, D.Nx - 1 - c, j)];
+        else F[gidx(D, D.Nx + c, j)] = F[gidx(D, c, j)];
+    } else if (t < 2 * nxs + 2 * nys) {
+        const int u = t - 2 * nxs, r = u % nys, i = r / G, c = r % G;
+        if (u < nys) F[gidx(D, i, -1 - c)] = F[gidx(D, i, D.Ny - 1 - c)];
+        else F[gidx(D, i, D.Ny + c)] = F[gidx(D, i, c)];
+    }
+}
+
+// One stage over the mask's points (finest points dense per row, one thread
+// each): dF + dG from WENO5 right reconstructions on the finest level
+// (main_2d_advect.cpp:63-73) or AWCM_2D::Differentiate and its one-sided forms
+// at spacing d below it (:74-103, AWCM_2D.h:95-110), then
+//   S = 1: out = F - dt/h (dF + dG)                          (:104, mask2)
+//   S = 2: out = 0.75 Qa + 0.25 (F - dt/h (dF + dG))         (:149, mask1)
+//   S = 3: out = F - dt/h (dF + dG)                          (:194, mask0)
+// with h = dx * d.  A coarse stencil that would leave the ghosted grid (the
+// reference reads outside its arrays there) sets the range flag.
+template <int S>
+__global__ void __launch_bounds__(256) k2_advect_stage(Dims2 D, const uint8_t* __restrict__ mask,
+                                                       const uint8_t* __restrict__ lvl, const double* __restrict__ Qa,
+                                                       const double* __restrict__ F, double* __restrict__ out, double dt,
+                                                       double gdx, Scal2* sc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= D.Nx) return;
+    const long long p = gidx(D, i, j);
+    if (!mask[p]) return;
+    const int lev = lvl[p];
+    const int d = 1 << (D.L - 1 - lev);
+    double r;
+    if (lev == D.maxlev) {
+        const long long P = D.P;
+        double a = wenojs_right(F[p - 3], F[p - 2], F[p - 1], F[p], F[p + 1]);
+        double b = wenojs_right(F[p - 2], F[p - 1], F[p], F[p + 1], F[p + 2]);
+        const double dF = b - a;
+        a = wenojs_right(F[p - 3 * P], F[p - 2 * P], F[p - P], F[p], F[p + P]);
+        b = wenojs_right(F[p - 2 * P], F[p - P], F[p], F[p + P], F[p + 2 * P]);
+        const double dG = b - a;
+        r = dF + dG;
+    } else {
+        const bool lx = i <= d, rx = !lx && i >= D.Nx - 1 - d;
+        const bool ly = j <= d, ry = !ly && j >= D.Ny - 1 - d;
+        const bool okx = i - (lx ? d : 2 * d) >= -G && i + (rx ? d : 2 * d) <= D.Nx - 1 + G;
+        const bool oky = j - (ly ? d : 2 * d) >= -G && j + (ry ? d : 2 * d) <= D.Ny - 1 + G;
+        if (!okx || !oky) { sc->flag |= 1; return; }
+        const long long sx = d, sy = (long long)d * D.P;
+        double dF, dG;
+        if (lx) dF = kD1 * (F[p + sx] - F[p - sx]) + kD2 * (F[p + 2 * sx] - F[p - sx]);
+        else if (rx) dF = kD1 * (F[p + sx] - F[p - sx]) + kD2 * (F[p + sx] - F[p - 2 * sx]);
+        else dF = kD1 * (F[p + sx] - F[p - sx]) + kD2 * (F[p + 2 * sx] - F[p - 2 * sx]);
+        if (ly) dG = kD1 * (F[p + sy] - F[p - sy]) + kD2 * (F[p + 2 * sy] - F[p - sy]);
+        else if (ry) dG = kD1 * (F[p + sy] - F[p - sy]) + kD2 * (F[p + sy] - F[p - 2 * sy]);
+        else dG = kD1 * (F[p + sy] - F[p - sy]) + kD2 * (F[p + 2 * sy] - F[p - 2 * sy]);
+        r = dF + dG;
+    }
+    const double h = gdx * d;
+    if (S == 2) out[p] = 0.75 * Qa[p] + 0.25 * (F[p] - dt / h * r);
+    else out[p] = F[p] - dt / h * r;
+}
+
+// The combine on mask0 (main_2d_advect.cpp:190-197) and PostProcess:
+// AWCM_MANAGER_2D::RemoveResidue on the interior (AWCM_MANAGER_2D.h:257-266).
+__global__ void k2_advect_combine(Dims2 D, const uint8_t* __restrict__ m0, double* __restrict__ Q,
+                                  const double* __restrict__ A0) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= D.Nx) return;
+    const long long p = gidx(D, i, j);
+    if (m0[p]) Q[p] = (1.0 / 3.0 * Q[p] + 2.0 / 3.0 * A0[p]);
+    else if (i >= 1 && i <= D.Nx - 2 && j >= 1 && j <= D.Ny - 2) Q[p] = 0.0;
+}
+
 // AoS, reference pitch (Nx+6) <-> SoA planes, device pitch P
 __global__ void k2_aos_to_soa(const double* __restrict__ aos, double* __restrict__ soa, long long n, int PR, int P,
                               long long plane) {
@@ -1920,6 +2010,43 @@ int Solver2D::launch_step() {
         graph_tol = tol_host;
     }
     return hipGraphLaunch(gexec, stream) == hipSuccess ? AWCM_OK : AWCM_EDEVICE;
+}
+
+// n_steps iterations of Wavelet/main_2d_advect.cpp:46-199 on plane 0 of Q:
+// AWCM_2D::Prepare, the periodic ghost fill, the three stages on mask2,
+// mask1, mask0 (reading Q, A2, A1), the combine and PostProcess.
+int Solver2D::advect(double tol, double dt, int n_steps) {
+    if (n_steps < 0) return AWCM_EINVAL;
+    const size_t pl = sizeof(double) * size_t(D.plane);
+    if (!adv) {
+        if (int st = dalloc(&adv, 3 * pl)) return st;
+        if (hipMemset(adv, 0, 3 * pl) != hipSuccess) return AWCM_EDEVICE;
+    }
+    double* A2 = adv;
+    double* A1 = adv + D.plane;
+    double* A0 = adv + 2 * D.plane;
+    Scal2 s0;
+    if (hipStreamSynchronize(stream) != hipSuccess || hipMemcpy(&s0, sc, sizeof(s0), hipMemcpyDeviceToHost) != hipSuccess)
+        return AWCM_EDEVICE;
+    s0.flag = 0;
+    if (hipMemcpy(sc, &s0, sizeof(s0), hipMemcpyHostToDevice) != hipSuccess) return AWCM_EDEVICE;
+    tol_host = tol;
+    const dim3 pg(nblk(D.Nx, 256), D.Ny), pb(256);
+    const unsigned nper = nblk(2 * G * (D.Nx + D.Ny), 256);
+    for (int k = 0; k < n_steps; k++) {
+        if (int st = enqueue_prepare(true, false)) return st;
+        AWCM_LAUNCH(k2_periodic, dim3(nper), dim3(256), 0, stream, D, Q);
+        AWCM_LAUNCH(k2_advect_stage<1>, pg, pb, 0, stream, D, m[2], lv[2], Q, Q, A2, dt, dx, sc);
+        AWCM_LAUNCH(k2_periodic, dim3(nper), dim3(256), 0, stream, D, A2);
+        AWCM_LAUNCH(k2_advect_stage<2>, pg, pb, 0, stream, D, m[1], lv[1], Q, A2, A1, dt, dx, sc);
+        AWCM_LAUNCH(k2_periodic, dim3(nper), dim3(256), 0, stream, D, A1);
+        AWCM_LAUNCH(k2_advect_stage<3>, pg, pb, 0, stream, D, m[0], lv[0], Q, A1, A0, dt, dx, sc);
+        AWCM_LAUNCH(k2_advect_combine, pg, pb, 0, stream, D, m[0], Q, A0);
+    }
+    if (hipStreamSynchronize(stream) != hipSuccess) return AWCM_EDEVICE;
+    Scal2 s1;
+    if (hipMemcpy(&s1, sc, sizeof(s1), hipMemcpyDeviceToHost) != hipSuccess) return AWCM_EDEVICE;
+    return (s1.flag & 1) ? AWCM_ERANGE : AWCM_OK;
 }
 
 int Solver2D::enqueue_post() {

@@ -206,6 +206,12 @@ int awcm_interpolate_all_grid(awcm_solver* h, const uint8_t* mask_from) {
     return h->s2->interpolate_all(mask_from);
 }
 
+int awcm_run_scalar(awcm_solver* h, double tol, double dt, int n_steps) {
+    if (!h || n_steps < 0 || !(dt >= 0)) return AWCM_EINVAL;
+    if (h->dim != 2) return AWCM_ESTATE;
+    return h->s2->advect(tol, dt, n_steps);
+}
+
 int awcm_post_process_euler(awcm_solver* h) {
     if (!h) return AWCM_EINVAL;
     if (h->dim == 1) {

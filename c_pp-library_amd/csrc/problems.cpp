@@ -5,6 +5,7 @@
 //   awcm_ic_shu_osher_1d   ENTROPY_INTERACTING::Initialize  (PROBLEM_ENTROPY_INTERACTING.h:14-49)
 //   awcm_ic_quadrants_2d   QUADRANTS::Initialize            (PROBLEM_QUADRANTS.h:15-81)
 //   awcm_ic_double_mach_2d DOUBLEMACH::Initialize           (PROBLEM_DOUBLEMACH.h:13-63)
+//   awcm_ic_box_advection_2d BOX_ADVECTION::Initialize      (PROBLEM_BOX_ADVECTION.h:15-47)
 // Coordinates and conserved variables are formed with the same expressions so
 // the fields are bit-identical to the reference's for the same parameters.
 
@@ -67,6 +68,22 @@ int awcm_ic_shu_osher_1d(int res, double xl, double xu, double split, const doub
     }
     if (dx_out) *dx_out = dx;
     if (x0_out) *x0_out = xmin;
+    return AWCM_OK;
+}
+
+int awcm_ic_box_advection_2d(int res, double* q, double* dx_out) {
+    if (res < 2 || !q) return AWCM_EINVAL;
+    // grid_xy: res+1 points on [0, 1]; the cell grid from dx'/2 to 1 - dx'/2
+    const double dxy = (1.0 - 0.0) / ((res + 1) - 1);
+    const double xmin = 0.0 + 0.5 * dxy, xmax = 1.0 - 0.5 * dxy;
+    const double dx = (xmax - xmin) / (res - 1);
+    const int P = res + 2 * G;
+    for (int j = -G; j <= res - 1 + G; j++)
+        for (int i = -G; i <= res - 1 + G; i++) {
+            const double x = xmin + (i - 0) * dx, y = xmin + (j - 0) * dx;
+            q[size_t(i + G) + size_t(P) * (j + G)] = std::exp(-50 * ((x - 0.5) * (x - 0.5)) - 50 * ((y - 0.5) * (y - 0.5)));
+        }
+    if (dx_out) *dx_out = dx;
     return AWCM_OK;
 }
 

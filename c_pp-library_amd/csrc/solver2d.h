@@ -107,6 +107,10 @@ struct Solver2D {
     enum { MASK_OP_ADJACENT = 0, MASK_OP_CHECK = 1, MASK_OP_ADD_POINT = 2, MASK_OP_LEVEL = 3 };
     int mask_op(int op, uint8_t* mask, int32_t* levels);
     int remove_residue(double* q, int ncomp, const uint8_t* mask);
+    // the scalar box-advection driver's loop (synchronous); its stage planes
+    // A2, A1, A0 are allocated, zeroed, on first use and kept
+    double* adv = nullptr;
+    int advect(double tol, double dt, int n_steps);
 
     // kernel sequences (enqueued on `stream`, no host synchronisation)
     int tail = -1;            // levels 0..tail of each cascade run in one workgroup

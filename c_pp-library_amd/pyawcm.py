@@ -63,6 +63,7 @@ def lib():
         "awcm_download_scalar": [V, _dp],
         "awcm_prepare": [V, D],
         "awcm_post_process": [V],
+        "awcm_run_scalar": [V, D, D, I],
         "awcm_interpolate": [V, _u8p, _u8p],
         "awcm_interpolate_euler": [V, _u8p, _u8p],
         "awcm_post_process_euler": [V],
@@ -95,6 +96,7 @@ def lib():
         "awcm_ic_shu_osher_1d": [I, D, D, D, _dp, D, D, _dp, _dp, _dp],
         "awcm_ic_quadrants_2d": [I, D, _dp, D, _dp, _dp],
         "awcm_ic_double_mach_2d": [I, D, D, _dp, _dp],
+        "awcm_ic_box_advection_2d": [I, _dp, _dp],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -197,6 +199,11 @@ class Solver:
 
     def post_process_scalar(self):
         _check(lib().awcm_post_process(self.h), "awcm_post_process")
+
+    def run_scalar(self, tol, dt, n_steps):
+        """n_steps iterations of the box-advection driver (main_2d_advect.cpp) on
+        the device (awcm_run_scalar)."""
+        _check(lib().awcm_run_scalar(self.h, tol, dt, n_steps), "awcm_run_scalar")
 
     def interpolate(self, mask_from, mask_to):
         """WAVELET_2D::Interpolate(q, mra, mask_from, mask_to) on the scalar field."""
@@ -370,6 +377,14 @@ def ic_double_mach_2d(res, rho_pre=1.4, gamma=1.4):
     q = np.empty((nx + 6) * (res + 6) * 4)
     dx = np.zeros(1)
     _check(lib().awcm_ic_double_mach_2d(res, rho_pre, gamma, _d(q), _d(dx)), "awcm_ic_double_mach_2d")
+    return q, float(dx[0])
+
+
+def ic_box_advection_2d(res):
+    """BOX_ADVECTION::Initialize: the scalar field ((res+6)^2 doubles) and dx."""
+    q = np.empty((res + 6) * (res + 6))
+    dx = np.zeros(1)
+    _check(lib().awcm_ic_box_advection_2d(res, _d(q), _d(dx)), "awcm_ic_box_advection_2d")
     return q, float(dx[0])
 
 

@@ -149,6 +149,19 @@ int awcm_interpolate_euler(awcm_solver* h, const uint8_t* mask_from, const uint8
 /* AWCM_*::PostProcessEuler (AWCM_1D.h:122-125, AWCM_2D.h:149-152). */
 int awcm_post_process_euler(awcm_solver* h);
 
+/* The scalar driver's time loop, Wavelet/main_2d_advect.cpp:46-199, on the
+ * device (2-D handles): n_steps iterations of AWCM_2D::Prepare(q, tol), the
+ * periodic ghost fill of PROBLEM_BOX_ADVECTION.h's BOUNDARY_MANAGER (sides
+ * only), three SSP-RK3 stages on mask2, mask1, mask0 with velocity (1, 1) --
+ * WENO5 right reconstructions on the finest level, AWCM_2D::Differentiate and
+ * its one-sided forms at the point's spacing below it, h = dx * spacing with
+ * the handle's dx -- the combine on mask0 and AWCM_2D::PostProcess.  The
+ * field is the handle's scalar field (awcm_upload_scalar).  The three stage
+ * fields are kept in the handle between calls (zero when first used), as the
+ * driver keeps Q_2, Q_1, Q_0.  AWCM_ERANGE: a coarse stencil would leave the
+ * ghosted grid (the reference reads outside its arrays there).  Synchronous. */
+int awcm_run_scalar(awcm_solver* h, double tol, double dt, int n_steps);
+
 /* ---- AWCM_MANAGER_1D / AWCM_MANAGER_2D mask algebra on caller arrays ------
  * The static members of AWCM_MANAGER_1D (Wavelet/AWCM_MANAGER_1D.h:14-157)
  * and AWCM_MANAGER_2D (Wavelet/AWCM_MANAGER_2D.h:14-280), on the GPU, for
@@ -273,6 +286,10 @@ int awcm_ic_quadrants_2d(int res, double split, const double* states, double gam
 /* DOUBLEMACH::Initialize (PROBLEM_DOUBLEMACH.h:13-63): ((res-1)*4+1) x res
  * points on [0,4]x[0,1], pre-shock density rho_pre. */
 int awcm_ic_double_mach_2d(int res, double rho_pre, double gamma, double* q, double* dx);
+/* BOX_ADVECTION::Initialize (PROBLEM_BOX_ADVECTION.h:15-47): the scalar field
+ * exp(-50 (x-1/2)^2 - 50 (y-1/2)^2) on the res x res cell centres of [0,1]^2,
+ * ghosts included ((res+6)^2 doubles); dx of that grid. */
+int awcm_ic_box_advection_2d(int res, double* q, double* dx);
 
 /* ---- batched 1-D form (no reference counterpart: replaces N sequential
  *      main_1d_euler.cpp runs) ------------------------------------------- */

@@ -1143,6 +1143,106 @@ struct Solver2D {
             for (int i = 1; i <= Nx - 2; i++)
                 if (mk(m[0], i, j) == 0) { double* v = q(Q, i, j); v[0] = v[1] = v[2] = v[3] = 0.0; }
     }
+
+    // ---- the scalar driver's stages (Wavelet/main_2d_advect.cpp:55-200) on
+    // component 0: box advection with velocity (1, 1).  Q2 / Q1 / Q0 keep
+    // their values between iterations, as the driver's fields do.
+    // PROBLEM_BOX_ADVECTION.h:78-151: periodic sides (rows 0..Ny-1 for the x
+    // ghosts, columns 0..Nx-1 for the y ghosts); the corners are not written.
+    void periodic(std::vector<double>& F) {
+        for (int j = 0; j <= Ny - 1; j++)
+            for (int c = 0; c < GW; c++) {
+                q(F, -1 - c, j)[0] = q(F, Nx - 1 - c, j)[0];
+                q(F, Nx + c, j)[0] = q(F, c, j)[0];
+            }
+        for (int i = 0; i <= Nx - 1; i++)
+            for (int c = 0; c < GW; c++) {
+                q(F, i, -1 - c)[0] = q(F, i, Ny - 1 - c)[0];
+                q(F, i, Ny + c)[0] = q(F, i, c)[0];
+            }
+    }
+    // dF + dG at (i, j): WENO5 right reconstructions on the finest level
+    // (main_2d_advect.cpp:63-73), AWCM_2D::Differentiate and its one-sided
+    // forms at spacing d below it (:74-103, AWCM_2D.h:95-110)
+    double adv_rhs(std::vector<double>& F, int i, int j, int lev, long& oobs) {
+        auto f = [&](int a, int b) { return q(F, a, b)[0]; };
+        if (lev == maxlev) {
+            double a = wenojs_right(f(i - 3, j), f(i - 2, j), f(i - 1, j), f(i, j), f(i + 1, j));
+            double b = wenojs_right(f(i - 2, j), f(i - 1, j), f(i, j), f(i + 1, j), f(i + 2, j));
+            const double dF = b - a;
+            a = wenojs_right(f(i, j - 3), f(i, j - 2), f(i, j - 1), f(i, j), f(i, j + 1));
+            b = wenojs_right(f(i, j - 2), f(i, j - 1), f(i, j), f(i, j + 1), f(i, j + 2));
+            const double dG = b - a;
+            return dF + dG;
+        }
+        const int d = grid[lev].dx;
+        // the reference reads outside its arrays here; such a point is counted
+        auto in = [&](int a, int b) { return a >= -GW && a <= Nx - 1 + GW && b >= -GW && b <= Ny - 1 + GW; };
+        double dF, dG;
+        if (i <= d) {
+            if (!in(i - d, j) || !in(i + 2 * d, j)) { oobs++; return 0; }
+            dF = D1 * (f(i + d, j) - f(i - d, j)) + D2 * (f(i + 2 * d, j) - f(i - d, j));
+        } else if (i >= Nx - 1 - d) {
+            if (!in(i - 2 * d, j) || !in(i + d, j)) { oobs++; return 0; }
+            dF = D1 * (f(i + d, j) - f(i - d, j)) + D2 * (f(i + d, j) - f(i - 2 * d, j));
+        } else {
+            if (!in(i - 2 * d, j) || !in(i + 2 * d, j)) { oobs++; return 0; }
+            dF = D1 * (f(i + d, j) - f(i - d, j)) + D2 * (f(i + 2 * d, j) - f(i - 2 * d, j));
+        }
+        if (j <= d) {
+            if (!in(i, j - d) || !in(i, j + 2 * d)) { oobs++; return 0; }
+            dG = D1 * (f(i, j + d) - f(i, j - d)) + D2 * (f(i, j + 2 * d) - f(i, j - d));
+        } else if (j >= Ny - 1 - d) {
+            if (!in(i, j - 2 * d) || !in(i, j + d)) { oobs++; return 0; }
+            dG = D1 * (f(i, j + d) - f(i, j - d)) + D2 * (f(i, j + d) - f(i, j - 2 * d));
+        } else {
+            if (!in(i, j - 2 * d) || !in(i, j + 2 * d)) { oobs++; return 0; }
+            dG = D1 * (f(i, j + d) - f(i, j - d)) + D2 * (f(i, j + 2 * d) - f(i, j - 2 * d));
+        }
+        return dF + dG;
+    }
+    // everything of one driver iteration after awcm.Prepare (main_2d_advect.cpp:50-198)
+    void advect_stages(double dt) {
+        long oobs = 0;
+        periodic(Q);
+#pragma omp parallel for reduction(+:oobs)
+        for (int j = 0; j <= Ny - 1; j++)
+            for (int i = 0; i <= Nx - 1; i++) {
+                if (mk(m[2], i, j) == 0) continue;
+                const int lev = lv[2][i + size_t(Nx) * j];
+                const double h = dx * grid[lev].dx;
+                const double r = adv_rhs(Q, i, j, lev, oobs);
+                q(Q2, i, j)[0] = q(Q, i, j)[0] - dt / h * r;
+            }
+        periodic(Q2);
+#pragma omp parallel for reduction(+:oobs)
+        for (int j = 0; j <= Ny - 1; j++)
+            for (int i = 0; i <= Nx - 1; i++) {
+                if (mk(m[1], i, j) == 0) continue;
+                const int lev = lv[1][i + size_t(Nx) * j];
+                const double h = dx * grid[lev].dx;
+                const double r = adv_rhs(Q2, i, j, lev, oobs);
+                q(Q1, i, j)[0] = 0.75 * q(Q, i, j)[0] + 0.25 * (q(Q2, i, j)[0] - dt / h * r);
+            }
+        periodic(Q1);
+#pragma omp parallel for reduction(+:oobs)
+        for (int j = 0; j <= Ny - 1; j++)
+            for (int i = 0; i <= Nx - 1; i++) {
+                if (mk(m[0], i, j) == 0) continue;
+                const int lev = lv[0][i + size_t(Nx) * j];
+                const double h = dx * grid[lev].dx;
+                const double r = adv_rhs(Q1, i, j, lev, oobs);
+                q(Q0, i, j)[0] = q(Q1, i, j)[0] - dt / h * r;
+            }
+        oob += oobs;
+        for (int j = 0; j <= Ny - 1; j++)
+            for (int i = 0; i <= Nx - 1; i++)
+                if (mk(m[0], i, j)) q(Q, i, j)[0] = (1.0 / 3.0 * q(Q, i, j)[0] + 2.0 / 3.0 * q(Q0, i, j)[0]);
+        // AWCM_2D::PostProcess -> AWCM_MANAGER_2D::RemoveResidue (AWCM_MANAGER_2D.h:257-266)
+        for (int j = 1; j <= Ny - 2; j++)
+            for (int i = 1; i <= Nx - 2; i++)
+                if (mk(m[0], i, j) == 0) q(Q, i, j)[0] = 0.0;
+    }
     // BOUNDARY_MANAGER_QUAD::SetBoundaryCondition (PROBLEM_QUADRANTS.h:123-203) and
     // BOUNDARY_MANAGER_DMACH::SetBoundaryCondition(t) (PROBLEM_DOUBLEMACH.h:106-245)
     void bc(std::vector<double>& F, double t) {
@@ -1364,7 +1464,9 @@ int orc_ic1(int kind, int res, double xl, double xu, double split, const double*
 
 // QUADRANTS::Initialize (PROBLEM_QUADRANTS.h:15-81) with split point s and
 // states st[4][4] = (rho,u,v,p) for quadrants 1..4; DOUBLEMACH::Initialize
-// (PROBLEM_DOUBLEMACH.h:13-63) with the pre-shock density as a parameter.
+// (PROBLEM_DOUBLEMACH.h:13-63) with the pre-shock density as a parameter;
+// kind 2: BOX_ADVECTION's scalar Gaussian on the cell-centre grid [xl, xu] x
+// [yl, yu] (component 0; the others zero).
 int orc_ic2(int kind, int nx, int ny, double xl, double xu, double yl, double yu, double s,
             const double* st, double gamma, double* q, double* dx_out, double* dy_out) {
     const double dx = (xu - xl) / (nx - 1), dy = (yu - yl) / (ny - 1);
@@ -1380,6 +1482,12 @@ int orc_ic2(int kind, int nx, int ny, double xl, double xu, double yl, double yu
                 else if (x < s && y < s) qd = 2;
                 else qd = 3;
                 rho = st[qd * 4 + 0]; u = st[qd * 4 + 1]; v = st[qd * 4 + 2]; p = st[qd * 4 + 3];
+            } else if (kind == 2) {
+                // BOX_ADVECTION (PROBLEM_BOX_ADVECTION.h:15-47): component 0 only
+                double* o = &q[(size_t(i + GW) + size_t(P) * (j + GW)) * 4];
+                o[0] = std::exp(-50 * SQRd(x - 0.5) - 50 * SQRd(y - 0.5));
+                o[1] = o[2] = o[3] = 0;
+                continue;
             } else {
                 if (y > std::sqrt(3.0) * (x - 1.0 / 6.0)) {
                     rho = 8; u = 8.25 * std::sqrt(3.0) * 0.5; v = -8.25 * 0.5; p = 116.5;
@@ -1390,6 +1498,16 @@ int orc_ic2(int kind, int nx, int ny, double xl, double xu, double yl, double yu
         }
     *dx_out = dx; *dy_out = dy;
     return 0;
+}
+
+// FNV-1a (64-bit) over n bytes, continuing from h: the hash the class-API
+// probes print (tests/native/api_probe*.cpp)
+uint64_t orc_fnv1a(const uint8_t* p, long long n, uint64_t h) {
+    for (long long k = 0; k < n; k++) {
+        h ^= p[k];
+        h *= 1099511628211ull;
+    }
+    return h;
 }
 
 // ---- 1-D solver ---------------------------------------------------------------
@@ -1520,6 +1638,15 @@ void orc2_get_scalar(void* h, double* q) {
     for (size_t k = 0; k < s->m[0].size(); k++) q[k] = s->Q[4 * k];
 }
 void orc2_prepare_scalar(void* h, double tol) { static_cast<Solver2D*>(h)->prepare_scalar(tol); }
+// n iterations of the scalar driver (main_2d_advect.cpp:46-199): Prepare, then
+// the stages, combine and PostProcess; the stages alone with n_prepare = 0
+void orc2_advect(void* h, double tol, double dt, int n, int with_prepare) {
+    auto* s = static_cast<Solver2D*>(h);
+    for (int k = 0; k < n; k++) {
+        if (with_prepare) s->prepare_scalar(tol);
+        s->advect_stages(dt);
+    }
+}
 void orc2_interpolate_scalar(void* h, const uint8_t* mask_from, const uint8_t* mask_to) {
     auto* s = static_cast<Solver2D*>(h);
     std::vector<uint8_t> from(mask_from, mask_from + s->m[0].size()), to(mask_to, mask_to + s->m[0].size());

@@ -98,6 +98,8 @@ def load(prefix="orc"):
         sig["orc1_tape_scalars"] = (None, [V, _dp])
         sig["orc1_replay"] = (None, [V, C.c_int, C.c_int])
         sig["orc2_oob"] = (C.c_long, [V])
+        sig["orc2_advect"] = (None, [V, C.c_double, C.c_double, C.c_int, C.c_int])
+        sig["orc_fnv1a"] = (C.c_uint64, [_u8p, C.c_longlong, C.c_uint64])
     else:
         sig["ref2_create"] = (V, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                   C.c_double, C.c_double, C.c_int])
@@ -148,6 +150,31 @@ def ic2(kind, nx, ny, xl, xu, yl, yu, s, states=None, gamma=1.4):
     st = np.ascontiguousarray(states if states is not None else QUAD_STATES, dtype=np.float64)
     lib.orc_ic2(kind, nx, ny, xl, xu, yl, yu, s, dptr(st), gamma, dptr(q), dptr(dx), dptr(dy))
     return q, float(dx[0]), float(dy[0])
+
+
+def box_grid(res):
+    """BOX_ADVECTION's cell-centre grid (PROBLEM_BOX_ADVECTION.h:26-33): the
+    res+1 face grid on [0, 1] gives dx' = 1/res; the cell grid runs from dx'/2
+    to 1 - dx'/2 in res points.  Returns (x_min, x_max, dx)."""
+    dxx = (1.0 - 0.0) / ((res + 1) - 1)
+    xl, xu = 0.0 + 0.5 * dxx, 1.0 - 0.5 * dxx
+    return xl, xu, (xu - xl) / (res - 1)
+
+
+def ic_box(res):
+    """The scalar box-advection field, (res+6)^2 doubles, and its dx."""
+    xl, xu, _ = box_grid(res)
+    q, dx, _ = ic2(2, res, res, xl, xu, xl, xu, 0.0)
+    return q.reshape(-1, 4)[:, 0].copy(), dx
+
+
+FNV_OFFSET = 1469598103934665603
+
+
+def fnv1a(data, h=FNV_OFFSET):
+    """FNV-1a (64-bit) of the bytes of `data`, continuing from h."""
+    b = np.ascontiguousarray(data).view(np.uint8).reshape(-1)
+    return int(load("orc").orc_fnv1a(u8ptr(b), b.size, h))
 
 
 def print_times(t_final, num_print):
@@ -345,6 +372,12 @@ class Solver2D:
     def prepare_scalar(self, tol):
         """AWCM_2D::Prepare on the scalar field."""
         getattr(self.lib, f"{self.p}2_prepare_scalar")(self.h, tol)
+
+    def advect(self, tol, dt, n=1, with_prepare=True):
+        """n iterations of the box-advection driver (Wavelet/main_2d_advect.cpp)
+        on the scalar field; with_prepare=False runs one iteration's stages,
+        combine and PostProcess without its Prepare."""
+        self.lib.orc2_advect(self.h, tol, dt, n, 1 if with_prepare else 0)
 
     def interpolate_scalar(self, mask_from, mask_to):
         """WAVELET_2D::Interpolate(q, mra, mask_from, mask_to) on the scalar field."""

@@ -58,6 +58,9 @@ def test_host_initial_conditions_match_oracle():
     q, dx = P.ic_double_mach_2d(65, 1.4)
     qo, dxo, _ = O.ic2(1, 257, 65, 0.0, 4.0, 0.0, 1.0, 1.4)
     assert np.array_equal(bits(q), bits(qo)) and dx == dxo
+    q, dx = P.ic_box_advection_2d(449)
+    qo, dxo = O.ic_box(449)
+    assert np.array_equal(bits(q), bits(qo)) and dx == dxo
 
 
 @pytest.mark.skipif(not O.have_ref(), reason="reference build (oracle/_ref) not present")
@@ -102,6 +105,7 @@ def test_invalid_arguments_rejected_before_device_work():
                               None) == -1
     assert lib.awcm_adjoint_record(C.cast(dummy, C.c_void_p), 1e-3, 0.4, 0.2, 1, 1, td, C.byref(cnt), None) == -1
     assert lib.awcm_interpolate_all_grid(None, None) == -1
+    assert lib.awcm_run_scalar(None, 1e-4, 1e-3, 1) == -1
     assert lib.awcm_ic_double_mach_2d(1, 1.4, 1.4, None, None) == -1
 
 

@@ -341,6 +341,32 @@ def test_2d_multistep_run_matches_oracle(tail_cells):
     assert_same(g.download_q(), o.get_q(), mg, mo, lg, lo, "2-D DMR 40 steps")
 
 
+def test_2d_run_past_t_final_keeps_the_last_steps_masks():
+    """A run whose step budget outlasts t_final queues steps that do nothing
+    (t >= t_final): they must leave every mask as the last real step left it
+    -- mask_temp (mask0 before that step's PrepareEuler) and masks 1..3 --
+    not only the field, mask0 and the levels."""
+    P = _pyawcm()
+    L, nxc, nyc = 5, 33, 9
+    q, dx = dmr_setup(L, nxc, nyc)
+    pts = O.print_times(0.2, 21)
+    g = P.Solver(2, L, nxc, 1.4, dx, ny_coarse=nyc, x0=0.0, bc_kind=1)
+    g.set_schedule(pts)
+    g.upload_q(q)
+    o = O.Solver2D("orc", L, nxc, nyc, 1.4, 1, 0.0, 4.0, 0.0, 1.0)
+    o.set_q(q)
+    for k in (1, 2):
+        steps, _ = g.run(1e-3, 0.4, pts[k], 4096)
+        n = 0
+        while o.t[0] < pts[k]:
+            o.step(1e-3, 0.4, pts)
+            n += 1
+        assert steps == n and g.t == o.t[0] and g.cnt == o.cnt.value
+        mg, lg = g.masks()
+        mo, lo = o.masks()
+        assert_same(g.download_q(), o.get_q(), mg, mo, lg, lo, f"2-D DMR run to output time {k} ({n} steps)")
+
+
 def test_2d_prepare_then_run_matches_oracle():
     """awcm_prepare_euler on its own, then steps: the standalone PrepareEuler
     drops points from mask0 without a residue pass, so the next step's

@@ -390,6 +390,29 @@ def test_oracle_scalar_path_matches_reference(case):
         assert np.array_equal(bits(a.get_q()), bits(b.get_q()))
 
 
+@pytest.mark.parametrize("name", ["small", "driver"])
+def test_oracle_advect_matches_reference_driver_text(name):
+    """The box-advection driver (Wavelet/main_2d_advect.cpp:46-199) on the
+    oracle against the reference build's own run of it
+    (tests/golden/api_probe_advect.txt): after every Prepare and every
+    iteration, the hashes of the ghosted field, the five masks and level0..2,
+    over 40 iterations of 65 x 65 and 10 of the driver's 449 x 449."""
+    import advect_lib as A
+    gold = A.golden()
+    L, nxc, eps, steps, res, q, dx, dt = A.setup(name)
+    xl, xu, _ = O.box_grid(res)
+    S = O.Solver2D("orc", L, nxc, nxc, 1.4, 0, xl, xu, xl, xu)
+    S.set_scalar(q)
+    for s in range(steps):
+        S.prepare_scalar(eps)
+        m, lv = S.masks()
+        assert A.report(S.get_scalar(), m, lv) == gold[f"{name}.{s}.prepare"], s
+        S.advect(eps, dt, 1, with_prepare=False)
+        m, lv = S.masks()
+        assert A.report(S.get_scalar(), m, lv) == gold[f"{name}.{s}.step"], s
+    assert O.load("orc").orc2_oob(S.h) == 0
+
+
 @needs_ref
 def test_oracle_scalar_1d_matches_reference():
     """AWCM_1D::Prepare (four rounds) and WAVELET_1D::Interpolate (both forms)
