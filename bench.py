@@ -30,6 +30,8 @@ value = active-point RK3 updates per second = sum over the timed steps of
 
     python bench.py [--steps K] [--warmup W] [--only T] [--no-full] [--no-cpu] [--no-batch] [--no-c2]
     python bench.py --hbm-copy     # a plain device copy of 1 GiB (HBM counter calibration)
+    python bench.py --only 0.1 --save-snapshot s.npz   # then, e.g. under rocprofv3:
+    python bench.py --load-snapshot s.npz --steps 5 --warmup 1 --no-cpu --no-batch --no-c2
 """
 import argparse
 import json
@@ -419,6 +421,10 @@ def main():
                     help="steps per instance of the batched-vs-sequential comparison")
     ap.add_argument("--cpu-sample", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--hbm-copy", action="store_true")
+    ap.add_argument("--save-snapshot", default=None,
+                    help="with --only T: advance to T, save the device state to this .npz and exit")
+    ap.add_argument("--load-snapshot", default=None,
+                    help="start the one window from a state saved by --save-snapshot (for short profiled runs)")
     args = ap.parse_args()
     if args.cpu_sample:
         print(json.dumps(cpu_sample(args.cpu_sample, args.cpu_budget)))
@@ -433,10 +439,15 @@ def main():
 
     P = ensure_built()
     S = dmr_solver(P)
+    if args.load_snapshot:
+        z = np.load(args.load_snapshot)
+        args.only = float(z["t"])
     marks = (args.only,) if args.only is not None else WINDOWS
     headline = args.only if args.only is not None else HEADLINE
     full = None
-    if args.only is None and not args.no_full:
+    if args.load_snapshot:
+        snaps = {args.only: (z["q"], z["m0"], float(z["t"]), int(z["cnt"]))}
+    elif args.only is None and not args.no_full:
         full, snaps = full_run(S, T_FINAL, marks)
     else:
         snaps = {}
@@ -446,6 +457,11 @@ def main():
                 if n == 0:
                     break
             snaps[w] = snapshot(S)
+    if args.save_snapshot:
+        q, m0, t, cnt = snaps[headline]
+        np.savez(args.save_snapshot, q=q, m0=m0, t=t, cnt=cnt)
+        print(json.dumps({"saved": args.save_snapshot, "t": t, "cnt": cnt}))
+        return
     wins, starts = {}, {}
     for w in marks:
         wins[w], starts[w] = window(P, S, snaps[w], args.steps, args.warmup)
