@@ -71,10 +71,10 @@ FLOPS_PER_FACE = 1202
 # neighbouring faces' own reads), its 4 B list entry, and the 32 B flux written.
 BYTES_PER_FACE = 32 + 4 + 32
 # Measured with rocprofv3 on the t ~ 0.1 window (profiles/
-# k2_stage_faces_dmr_pmc.md): HBM bytes per face of a k2_stage_faces launch,
-# FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from `bench.py --hbm-copy`), and
-# issued FP64 lane operations per face.  None until measured.
-TRAFFIC_BYTES_PER_FACE = None
+# k2_stage_faces_dmr_t0.1_pmc.md): HBM bytes per face of a k2_stage_faces
+# launch, FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from `bench.py
+# --hbm-copy`), and issued FP64 lane operations per face.
+TRAFFIC_BYTES_PER_FACE = 180
 ISSUED_FP64_OPS_PER_FACE = 2143
 
 
