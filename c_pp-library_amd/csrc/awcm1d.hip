@@ -43,8 +43,10 @@ static_assert(AWCM_FLUX_WENOZ == kRecWenoZ && AWCM_FLUX_WENO5 == kRecWeno5, "flu
 template <bool kHbm, int FLUX, bool SCALAR>
 __global__ void __launch_bounds__(1024) k1_solve(Inst1D* insts, int n_inst, const int* order, int* qctr, int mode,
                                                  int max_steps, int nxp, uint8_t* hbm) {
-    AWCM_DYNAMIC_LDS uint8_t lds_masks[];
+    AWCM_DYNAMIC_LDS __align__(16) uint8_t lds_masks[];
     uint8_t* const sm = kHbm ? hbm + size_t(blockIdx.x) * 11 * size_t(nxp) : lds_masks;
+    const bool lstate = !kHbm && (mode & kModeLdsState) != 0;
+    mode &= ~kModeLdsState;
     __shared__ double W[4][5];
     __shared__ double red[16];
     __shared__ int s_idx, s_flag;
@@ -62,6 +64,17 @@ __global__ void __launch_bounds__(1024) k1_solve(Inst1D* insts, int n_inst, cons
         c.gamma = in.gamma; c.dx = in.dx;
         c.NP = in.Nx + 2 * G;
         c.Q = in.Q; c.Qc = in.Qc; c.Qt = in.Qt; c.F = in.F;
+        const int nq = 3 * c.NP, nf = 3 * (in.Nx + 1);
+        if (lstate) {
+            double* ls = reinterpret_cast<double*>(lds_masks + lds_state_offset(nxp));
+            for (int k = threadIdx.x; k < nq; k += blockDim.x) {
+                ls[k] = in.Q[k];
+                ls[3 * nxp + k] = in.Qc[k];
+                ls[6 * nxp + k] = in.Qt[k];
+            }
+            for (int k = threadIdx.x; k < nf; k += blockDim.x) ls[9 * nxp + k] = in.F[k];
+            c.Q = ls; c.Qc = ls + 3 * nxp; c.Qt = ls + 6 * nxp; c.F = ls + 9 * nxp;
+        }
         c.mt = sm;
         for (int k = 0; k < 4; k++) c.m[k] = sm + (k + 1) * nxp;
         c.snap = sm + 5 * nxp;
@@ -104,6 +117,14 @@ __global__ void __launch_bounds__(1024) k1_solve(Inst1D* insts, int n_inst, cons
             }
         }
         __syncthreads();
+        if (lstate) {
+            for (int k = threadIdx.x; k < nq; k += blockDim.x) {
+                in.Q[k] = c.Q[k];
+                in.Qc[k] = c.Qc[k];
+                in.Qt[k] = c.Qt[k];
+            }
+            for (int k = threadIdx.x; k < nf; k += blockDim.x) in.F[k] = c.F[k];
+        }
         for (int k = 0; k < 5; k++)
             for (int i = threadIdx.x; i < c.NP; i += blockDim.x) in.masks[k * c.NP + i] = sm[k * nxp + i];
         for (int k = 0; k < 4; k++)

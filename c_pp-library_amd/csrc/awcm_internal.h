@@ -1,9 +1,20 @@
 // Structures shared by the HIP kernels and the host-side runtime of the C ABI.
 #pragma once
 
+#include <cstddef>
 #include <cstdint>
 
 namespace awcm {
+
+// k1_solve mode flag: the instance's state, RK copies and face fluxes live in
+// the workgroup's LDS for the launch, after the masks (single-instance
+// launches whose arrays fit), copied in at the start and out at the end.
+constexpr int kModeLdsState = 0x100;
+// LDS bytes of that state for row pitch nxp: Q, Qc, Qt (3 planes each) and
+// the face fluxes (3 planes), nxp doubles a plane
+constexpr size_t lds_state_bytes(int nxp) { return sizeof(double) * 12 * size_t(nxp); }
+// offset of the state in the dynamic LDS: after the 11 * nxp mask bytes
+constexpr size_t lds_state_offset(int nxp) { return (size_t(11) * nxp + 15) / 16 * 16; }
 
 // One independent 1-D solve.  Device-resident; the kernel updates the
 // time-loop fields (t, cnt, steps, ...) in place.

@@ -37,6 +37,7 @@ struct Batch1D {
     // masks and levels of instances too long for one workgroup's LDS: an
     // 11 * nxp-byte slice per workgroup in HBM (grown on demand, never shrunk)
     size_t lds_cap = 0;           // dynamic LDS one k1_solve workgroup may use
+    bool lds_state = true;        // single-instance launches keep the state in LDS when it fits
     uint8_t* d_hbm_masks = nullptr;
     size_t hbm_masks_bytes = 0;
     // device snapshot of every instance's state (autotuner probes)

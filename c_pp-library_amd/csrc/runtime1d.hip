@@ -23,7 +23,7 @@ __global__ void k1_solve(Inst1D* insts, int n_inst, const int* order, int* qctr,
 // 4 (the scalar path) have their own.
 static void launch_solve(bool hbm_masks, int flux, dim3 grid, dim3 block, size_t lds, hipStream_t stream, Inst1D* insts,
                          int n_inst, const int* order, int* qctr, int mode, int max_steps, int nxp, uint8_t* hbm) {
-    if (mode >= 3) {
+    if ((mode & ~kModeLdsState) >= 3) {
         if (hbm_masks)
             hipLaunchKernelGGL((k1_solve<true, AWCM_FLUX_WENOZ, true>), grid, block, lds, stream, insts, n_inst, order,
                                qctr, mode, max_steps, nxp, hbm);
@@ -190,7 +190,14 @@ int Batch1D::launch(int mode, int max_steps, int first, int count) {
     uint8_t* hbm = nullptr;
     if (int st = mask_space(wgs, nxp, &hbm)) return st;
     if (hipMemsetAsync(d_qctr, 0, sizeof(int), stream) != hipSuccess) return AWCM_EDEVICE;
-    launch_solve(hbm != nullptr, flux, dim3(wgs), dim3(threads), hbm ? 0 : lds_bytes(nxp), stream, d_inst + first, count,
+    // one instance whose state fits beside its masks: the state in LDS too
+    // (its every access is then an LDS access instead of an L2 round trip)
+    size_t lds = hbm ? 0 : lds_bytes(nxp);
+    if (!hbm && count == 1 && lds_state && lds_state_offset(nxp) + lds_state_bytes(nxp) <= lds_cap) {
+        mode |= kModeLdsState;
+        lds = lds_state_offset(nxp) + lds_state_bytes(nxp);
+    }
+    launch_solve(hbm != nullptr, flux, dim3(wgs), dim3(threads), lds, stream, d_inst + first, count,
                  d_order_for(first, count), d_qctr, mode, max_steps, nxp, hbm);
     return hipGetLastError() == hipSuccess ? AWCM_OK : AWCM_EDEVICE;
 }

@@ -15,6 +15,7 @@
 #define __global__
 #define __shared__
 #define __forceinline__ inline
+#define __align__(n) __attribute__((aligned(n)))
 #define __launch_bounds__(...)
 
 struct EmuDim3 {

@@ -27,7 +27,7 @@
 
 namespace awcm {
 uint8_t sm[1 << 21];         // dynamic LDS of the emulated workgroup (k1_adjoint)
-uint8_t lds_masks[1 << 21];  // dynamic LDS of the emulated workgroup (k1_solve<false>)
+__attribute__((aligned(16))) uint8_t lds_masks[1 << 21];  // dynamic LDS of the emulated workgroup (k1_solve<false>)
 }
 
 using namespace awcm;

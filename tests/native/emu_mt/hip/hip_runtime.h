@@ -25,6 +25,7 @@
 #define __global__
 #define __shared__ static
 #define __forceinline__ inline
+#define __align__(n) __attribute__((aligned(n)))
 #define __launch_bounds__(...)
 #define AWCM_DYNAMIC_LDS extern
 
