@@ -375,9 +375,10 @@ def advect_section(P, iters=100, cpu_iters=5, cpu=True):
     dx -- for its 100 iterations in one awcm_run_scalar call (host wall clock
     around the synchronous call, after a warm-up on another handle), with the
     oracle port's host time per iteration from the same start beside it."""
-    import advect_lib as A
-    import oracle_lib as O
-    L, nxc, eps, _, res, q, dx, dt = A.setup("driver")
+    L, nxc, eps = 6, 15, 1e-4  # main_2d_advect.cpp:10-15
+    res = (nxc - 1) * 2 ** (L - 1) + 1
+    q, dx = P.ic_box_advection_2d(res)
+    dt = 0.4 * dx
 
     def fresh():
         S = P.Solver(2, L, nxc, 1.4, dx, ny_coarse=nxc, x0=0.0, bc_kind=P.BC_TRANSMISSIVE)
@@ -396,6 +397,7 @@ def advect_section(P, iters=100, cpu_iters=5, cpu=True):
            "active_points_end": int(m[1, 3:-3, 3:-3].sum())}
     if cpu:
         # the CPU port (oracle) from the same start: a reported baseline only
+        import oracle_lib as O
         xl, xu, _ = O.box_grid(res)
         R = O.Solver2D("orc", L, nxc, nxc, 1.4, 0, xl, xu, xl, xu)
         R.set_scalar(q)
