@@ -8,6 +8,7 @@
 #include <new>
 #include <vector>
 
+#include "awcm/physics.h"
 #include "awcm_capi.h"
 #include "runtime.h"
 
@@ -38,6 +39,13 @@ int flag_status(int flag) {
     if (flag & 2) return AWCM_ENONFINITE;
     if (flag & 1) return AWCM_ERANGE;
     return AWCM_OK;
+}
+
+// q[i] = a[i] / b[i] through the device's shared-divisor quotient
+__global__ void k_divide_shared(const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ q,
+                                long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) q[i] = awcm::Div(b[i])(a[i]);
 }
 
 }  // namespace
@@ -400,6 +408,26 @@ int awcm_set_tuning(awcm_solver* h, int param, int value) {
         return AWCM_OK;
     }
     return AWCM_EINVAL;
+}
+
+int awcm_divide_shared(const double* a, const double* b, double* q, long long n) {
+    if (n < 0 || (n > 0 && (!a || !b || !q))) return AWCM_EINVAL;
+    if (n == 0) return AWCM_OK;
+    if (hipSetDevice(0) != hipSuccess) return AWCM_EDEVICE;
+    double* d = nullptr;
+    const size_t bytes = sizeof(double) * size_t(n);
+    if (hipMalloc(&d, 3 * bytes) != hipSuccess) return AWCM_ENOMEM;
+    int st = AWCM_OK;
+    if (hipMemcpy(d, a, bytes, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d + n, b, bytes, hipMemcpyHostToDevice) != hipSuccess)
+        st = AWCM_EDEVICE;
+    if (st == AWCM_OK) {
+        hipLaunchKernelGGL(k_divide_shared, dim3(unsigned((n + 255) / 256)), dim3(256), 0, 0, d, d + n, d + 2 * n, n);
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
+            hipMemcpy(q, d + 2 * n, bytes, hipMemcpyDeviceToHost) != hipSuccess)
+            st = AWCM_EDEVICE;
+    }
+    return hipFree(d) == hipSuccess ? st : AWCM_EDEVICE;
 }
 
 int awcm_synchronize(awcm_solver* h) {

@@ -27,17 +27,70 @@ namespace awcm {
 AWCM_HD double t_abs(double x) { return x < 0 ? -x : x; }
 AWCM_HD double t_max(double a, double b) { return a > b ? a : b; }
 
+// ---- quotients that share a divisor -------------------------------------------
+// Div(b)(a) is a / b, correctly rounded.  The host divides.  On the device a
+// divisor shared by several quotients gets one reciprocal y = RN(1/b): the
+// hardware reciprocal, two Newton steps, then whichever of the result and its
+// neighbour toward 1/b leaves the smaller exact residual 1 - b*y.  Each quotient
+// is q = a*y corrected once with the exact FMA residual, q + (a - b*q)*y
+// (P. Markstein, IBM J. Res. Develop. 34(1), 1990, the final step of his
+// division): with y = RN(1/b) that is the correctly rounded a/b for operands
+// and quotients in the normal range (checked exhaustively in 5- to 15-bit
+// binary formats by tests/native/markstein_exhaustive.c).  The quotient takes
+// the sign of a*y, which also gives a zero quotient the sign IEEE division
+// gives it.  A divisor of 0, inf or NaN gives NaN quotients, where IEEE gives
+// inf, 0 or NaN: the state is no longer finite in either case.
+struct Div {
+    double b;
+#if defined(__HIP_DEVICE_COMPILE__)
+    double y;
+    __device__ __forceinline__ explicit Div(double b_) : b(b_) {
+        double r = __builtin_amdgcn_rcp(b);
+        double e = __builtin_fma(-b, r, 1.0);
+        r = __builtin_fma(r, e, r);
+        e = __builtin_fma(-b, r, 1.0);
+        r = __builtin_fma(r, e, r);
+        e = __builtin_fma(-b, r, 1.0);  // exact: 1/b = r + e/b
+        // the neighbour toward 1/b is one bit pattern up when e > 0, for either sign
+        const double n = __longlong_as_double(__double_as_longlong(r) + (e > 0 ? 1 : -1));
+        const double en = __builtin_fma(-b, n, 1.0);
+        y = __builtin_fabs(en) < __builtin_fabs(e) ? n : r;
+    }
+    __device__ __forceinline__ double operator()(double a) const {
+        const double q = a * y;
+        const double q1 = __builtin_fma(__builtin_fma(-b, q, a), y, q);
+        return __builtin_copysign(q1, q);
+    }
+    // a / b for a dividend that is never zero (no sign of zero to carry over)
+    __device__ __forceinline__ double nz(double a) const {
+        const double q = a * y;
+        return __builtin_fma(__builtin_fma(-b, q, a), y, q);
+    }
+    // division by 2b: RN(1/(2b)) is RN(1/b)/2 exactly
+    __device__ __forceinline__ Div twice() const { Div d(*this); d.b = 2 * b; d.y = 0.5 * y; return d; }
+#else
+    AWCM_HD explicit Div(double b_) : b(b_) {}
+    AWCM_HD double operator()(double a) const { return a / b; }
+    AWCM_HD double nz(double a) const { return a / b; }
+    AWCM_HD Div twice() const { return Div(2 * b); }
+#endif
+};
+
 // ---- equation of state ------------------------------------------------------
 AWCM_HD double eos_p1(double g, const double* q) { return (g - 1) * (q[2] - 0.5 * (q[1] * q[1]) / q[0]); }
 AWCM_HD double eos_H1(double g, const double* q) { return (q[2] + eos_p1(g, q)) / q[0]; }
 AWCM_HD double eos_c1(double g, const double* q) { return sqrt(g * eos_p1(g, q) / q[0]); }
 AWCM_HD double eos_E1(double g, double rho, double u, double p) { return p / (g - 1) + 0.5 * rho * (u * u); }
 
-AWCM_HD double eos_p2(double g, const double* q) {
-    return (g - 1) * (q[3] - 0.5 * ((q[1] * q[1]) + (q[2] * q[2])) / q[0]);
+// d0 divides by the density q[0]
+AWCM_HD double eos_p2(double g, const double* q, const Div& d0) {
+    return (g - 1) * (q[3] - d0(0.5 * ((q[1] * q[1]) + (q[2] * q[2]))));
 }
-AWCM_HD double eos_H2(double g, const double* q) { return (q[3] + eos_p2(g, q)) / q[0]; }
-AWCM_HD double eos_c2(double g, const double* q) { return sqrt(g * eos_p2(g, q) / q[0]); }
+AWCM_HD double eos_p2(double g, const double* q) { return eos_p2(g, q, Div(q[0])); }
+AWCM_HD double eos_H2(double g, const double* q, const Div& d0) { return d0(q[3] + eos_p2(g, q, d0)); }
+AWCM_HD double eos_H2(double g, const double* q) { return eos_H2(g, q, Div(q[0])); }
+AWCM_HD double eos_c2(double g, const double* q, const Div& d0) { return sqrt(d0(g * eos_p2(g, q, d0))); }
+AWCM_HD double eos_c2(double g, const double* q) { return eos_c2(g, q, Div(q[0])); }
 AWCM_HD double eos_E2(double g, double rho, double u, double v, double p) {
     return p / (g - 1) + 0.5 * rho * (u * u + v * v);
 }
@@ -52,7 +105,8 @@ AWCM_HD void flux_f1(double g, const double* q, double* f) {
 // dir 0: F (x), dir 1: G (y)
 template <int DIR>
 AWCM_HD void flux_f2(double g, const double* q, double* f) {
-    const double rho = q[0], u = q[1] / q[0], v = q[2] / q[0], E = q[3], p = eos_p2(g, q);
+    const Div d0(q[0]);
+    const double rho = q[0], u = d0(q[1]), v = d0(q[2]), E = q[3], p = eos_p2(g, q, d0);
     if (DIR == 0) {
         f[0] = rho * u;
         f[1] = rho * u * u + p;
@@ -71,13 +125,15 @@ AWCM_HD double spec_rad1(double g, const double* q) {
     return t_abs(u) + c;
 }
 AWCM_HD double spec_rad2x(double g, const double* q) {
-    const double u = q[1] / q[0];
-    const double c = eos_c2(g, q);
+    const Div d0(q[0]);
+    const double u = d0(q[1]);
+    const double c = eos_c2(g, q, d0);
     return t_abs(u) + c;
 }
 AWCM_HD double spec_rad2y(double g, const double* q) {
-    const double v = q[2] / q[0];
-    const double c = eos_c2(g, q);
+    const Div d0(q[0]);
+    const double v = d0(q[2]);
+    const double c = eos_c2(g, q, d0);
     return t_abs(v) + c;
 }
 
@@ -109,36 +165,35 @@ AWCM_HD void roe_proj1(const Roe1& a, const double* b, double* x) {
 struct Roe2 { double u, v, H, c; };
 AWCM_HD Roe2 roe_avg2(double g, const double* l, const double* r) {
     const double rl = l[0], rr = r[0];
-    const double Hl = eos_H2(g, l), Hr = eos_H2(g, r);
-    const double ul = l[1] / l[0], ur = r[1] / r[0];
-    const double vl = l[2] / l[0], vr = r[2] / r[0];
-    const double den = sqrt(rl) + sqrt(rr);
+    const Div dl(l[0]), dr(r[0]);
+    const double Hl = eos_H2(g, l, dl), Hr = eos_H2(g, r, dr);
+    const double ul = dl(l[1]), ur = dr(r[1]);
+    const double vl = dl(l[2]), vr = dr(r[2]);
+    const Div den(sqrt(rl) + sqrt(rr));
     Roe2 a;
-    a.u = sqrt(rl) * ul + sqrt(rr) * ur;
-    a.u /= den;
-    a.v = sqrt(rl) * vl + sqrt(rr) * vr;
-    a.v /= den;
-    a.H = sqrt(rl) * Hl + sqrt(rr) * Hr;
-    a.H /= den;
+    a.u = den(sqrt(rl) * ul + sqrt(rr) * ur);
+    a.v = den(sqrt(rl) * vl + sqrt(rr) * vr);
+    a.H = den(sqrt(rl) * Hl + sqrt(rr) * Hr);
     a.c = sqrt(0.4 * (a.H - 0.5 * (a.u * a.u + a.v * a.v)));
     return a;
 }
 template <int DIR>
 AWCM_HD void roe_proj2(const Roe2& a, const double* b, double* x) {
     const double u = a.u, v = a.v, H = a.H, c = a.c;
+    const Div dc(c), d2c = dc.twice();
     double x3, x4, x2;
     if (DIR == 0) {
         x3 = (H - u * u - v * v) * b[0] + u * b[1] + v * b[2] - b[3];
-        x3 *= 0.4 / c / c;
+        x3 *= dc.nz(dc.nz(0.4));
         x2 = b[2] - v * b[0];
         x4 = b[1] + (c - u) * b[0] - c * x3;
     } else {
         x3 = (H - u * u - v * v) * b[0] + v * b[2] + u * b[1] - b[3];
-        x3 *= 0.4 / c / c;
+        x3 *= dc.nz(dc.nz(0.4));
         x2 = b[1] - u * b[0];
         x4 = b[2] + (c - v) * b[0] - c * x3;
     }
-    x4 /= 2 * c;
+    x4 = d2c(x4);
     x[0] = b[0] - x3 - x4;
     x[1] = x2;
     x[2] = x3;
@@ -157,8 +212,8 @@ AWCM_HD double wenoz_left(double ll, double l, double c, double r, double rr) {
     const double a0 = 0.1 * (1 + tau / (b0 + 1e-40));
     const double a1 = 0.6 * (1 + tau / (b1 + 1e-40));
     const double a2 = 0.3 * (1 + tau / (b2 + 1e-40));
-    const double as = a0 + a1 + a2;
-    const double w0 = a0 / as, w1 = a1 / as, w2 = a2 / as;
+    const Div as(a0 + a1 + a2);
+    const double w0 = as.nz(a0), w1 = as.nz(a1), w2 = as.nz(a2);  // a_k >= d_k > 0
     double out = w0 * (11.0 / 6.0 * c + -7.0 / 6.0 * r + 1.0 / 3.0 * rr);
     out += w1 * (1.0 / 3.0 * l + 5.0 / 6.0 * c + -1.0 / 6.0 * r);
     out += w2 * (-1.0 / 6.0 * ll + 5.0 / 6.0 * l + 1.0 / 3.0 * c);

@@ -70,6 +70,7 @@ def lib():
         "awcm_set_schedule": [V, _dp, I],
         "awcm_run": [V, D, D, D, I, _dp, C.POINTER(I), C.POINTER(I), C.POINTER(C.c_longlong)],
         "awcm_synchronize": [V],
+        "awcm_divide_shared": [_dp, _dp, _dp, C.c_longlong],
         "awcm_batch_create_1d": [I, C.POINTER(Instance1D), _dp, I, C.POINTER(V)],
         "awcm_batch_upload_q": [V, I, _dp],
         "awcm_batch_download_q": [V, I, _dp],
@@ -386,6 +387,17 @@ def ic_box_advection_2d(res):
     dx = np.zeros(1)
     _check(lib().awcm_ic_box_advection_2d(res, _d(q), _d(dx)), "awcm_ic_box_advection_2d")
     return q, float(dx[0])
+
+
+def divide_shared(a, b):
+    """a / b elementwise on the device, as the kernels divide by a shared divisor."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    if a.shape != b.shape:
+        raise ValueError("a and b differ in shape")
+    q = np.empty_like(a)
+    _check(lib().awcm_divide_shared(_d(a), _d(b), _d(q), a.size), "awcm_divide_shared")
+    return q
 
 
 def print_times(t_final, num_print):

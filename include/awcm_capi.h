@@ -268,6 +268,14 @@ int awcm_set_tuning(awcm_solver* h, int param, int value);
 /* Block until the handle's queued device work is finished. */
 int awcm_synchronize(awcm_solver* h);
 
+/* Diagnostic, no counterpart in the reference: q[i] = a[i] / b[i] computed on
+ * device 0 the way the kernels form quotients that share a divisor (a
+ * reciprocal RN(1/b) and one exact-residual correction per quotient; struct
+ * Div in awcm/physics.h).  For operands and quotients in the normal range the
+ * result is the correctly rounded quotient, the host's a[i] / b[i] bit for
+ * bit.  Host arrays of n entries; n >= 0. */
+int awcm_divide_shared(const double* a, const double* b, double* q, long long n);
+
 /* ---- initial conditions (host side, reference layout) ------------------- */
 
 /* SHOCKTUBE::Initialize (PROBLEM_SHOCKTUBE.h:14-54) on [xl, xu]: cell-centred

@@ -107,6 +107,9 @@ def test_invalid_arguments_rejected_before_device_work():
     assert lib.awcm_interpolate_all_grid(None, None) == -1
     assert lib.awcm_run_scalar(None, 1e-4, 1e-3, 1) == -1
     assert lib.awcm_ic_double_mach_2d(1, 1.4, 1.4, None, None) == -1
+    assert lib.awcm_divide_shared(None, None, None, -1) == -1
+    assert lib.awcm_divide_shared(None, None, None, 5) == -1
+    assert lib.awcm_divide_shared(None, None, None, 0) == 0             # nothing to do, no device call
 
 
 def test_splitmix64_known_answers():
