@@ -995,10 +995,17 @@ __device__ __forceinline__ void get4(const double* Q, long long plane, long long
 // and turns the maximum into dt with the output-time clip
 // (main_2d_euler(dmach).cpp:76-83).  The other blocks fill the ghosts.
 __global__ void __launch_bounds__(256) k2_bc(Dims2 D, double* Q, int kind, double x0, double dx, int compute_dt,
-                                             const double* pt, AlphaPart P, double* ax, double* ay, int nbc, Scal2* sc) {
+                                             const double* pt, AlphaPart P, double* ax, double* ay, int nbc,
+                                             double* zero_next, Scal2* sc) {
     const int skip = sc->skip;  // tested after the loads, before any store
     if ((int)blockIdx.x >= nbc) {
         const int r = (blockIdx.x - nbc) * blockDim.x + threadIdx.x;
+        // the next two stages' alpha arrays, which their predecessors' stage
+        // updates fill with atomic maxima
+        if (zero_next && !skip && r < D.Nx + D.Ny) {
+            zero_next[r] = 0.0;
+            zero_next[D.Nx + D.Ny + r] = 0.0;
+        }
         if (r < D.Ny) {
             const double m = strided_bits_max(P.px + r, D.Ny, P.ntx);
             if (!skip) ax[r] = m;
@@ -1077,12 +1084,13 @@ __global__ void __launch_bounds__(256) k2_bc(Dims2 D, double* Q, int kind, doubl
 // S = 3: Q0 = Q1 - dt/h r(Q1)  (main_2d_euler(dmach).cpp:162, 242, 321)
 template <int S>
 __device__ __forceinline__ void stage_out(const Dims2& D, long long p, const double* r, double h, double dt,
-                                          const double* Qa, const double* Qs, double* Qo) {
+                                          const double* Qa, const double* Qs, double* Qo, double* out) {
     for (int k = 0; k < 4; k++) {
         const long long o = p + k * D.plane;
-        if (S == 1) Qo[o] = Qs[o] - dt / h * r[k];
-        else if (S == 2) Qo[o] = 0.75 * Qa[o] + 0.25 * (Qs[o] - dt / h * r[k]);
-        else Qo[o] = Qs[o] - dt / h * r[k];
+        if (S == 1) out[k] = Qs[o] - dt / h * r[k];
+        else if (S == 2) out[k] = 0.75 * Qa[o] + 0.25 * (Qs[o] - dt / h * r[k]);
+        else out[k] = Qs[o] - dt / h * r[k];
+        Qo[o] = out[k];
     }
 }
 
@@ -1131,20 +1139,58 @@ __global__ void __launch_bounds__(128) k2_stage_faces(Dims2 D, const uint32_t* _
 // their face fluxes (dF = right - left, dG = top - bottom: main_2d_euler(dmach).cpp
 // :123-128); coarser points take central differences of the physical flux at
 // spacing d with the one-sided variants of AWCM_2D.h:166-187.
+//
+// With `nax` (stages 1 and 2), the kernel also forms the next stage's LF
+// alphas: the points it writes are exactly the next stage's mask (mask2 for
+// stage 1, mask1 for stage 2) and their new states are the next stage's input,
+// so alpha_x(j) = max SRx over row j and alpha_y(i) = max SRy over column i
+// (main_2d_euler(dmach).cpp:167-183, 247-263) are gathered here as bit-pattern
+// maxima of non-negative doubles (exact and order-free) into arrays an earlier
+// kernel of the step zeroed: one atomic per row present in a wavefront, one per
+// point for the columns.  A wavefront's lanes run the loop in lockstep so the
+// row reduction sees every lane.
+__device__ __forceinline__ void next_alpha(bool valid, int i, int j, const double* q, double gamma,
+                                           double* __restrict__ nax, double* __restrict__ nay) {
+    double sx = 0, sy = 0;
+    if (valid) {
+        sx = pos_or_zero(spec_rad2x(gamma, q));
+        sy = pos_or_zero(spec_rad2y(gamma, q));
+    }
+    const int lane = threadIdx.x & 63;
+    unsigned long long live = __ballot(valid);
+    while (live) {
+        const int leader = __builtin_ctzll(live);
+        const int jl = __shfl(j, leader, 64);
+        const bool in = valid && j == jl;
+        const unsigned long long grp = __ballot(in);
+        const double m = wave_max(in ? sx : 0.0);
+        if (lane == leader) atomic_max_pos(reinterpret_cast<unsigned long long*>(nax + jl), m);
+        live &= ~grp;
+    }
+    if (valid) atomic_max_pos(reinterpret_cast<unsigned long long*>(nay + i), sy);
+}
+
 template <int S>
 __global__ void __launch_bounds__(256) k2_stage_update(Dims2 D, const uint32_t* __restrict__ lw,
                                                        const uint32_t* __restrict__ lc, int which,
                                                        const uint8_t* __restrict__ lvl, const double* __restrict__ Qa,
                                                        const double* __restrict__ Qs, double* __restrict__ Qo,
                                                        const double* __restrict__ FX, const double* __restrict__ FY,
-                                                       double gamma, double gdx, Scal2* sc) {
+                                                       double gamma, double gdx, double* __restrict__ nax,
+                                                       double* __restrict__ nay, Scal2* sc) {
     if (sc->skip) return;
     const int nW = sc->cntW[which], nC = sc->cntC[which];
     const double dt = sc->dt;
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nW + nC; t += gridDim.x * blockDim.x) {
-        if (t < nW) {
+    const int stride = gridDim.x * blockDim.x;
+    // wave-uniform trip count: every lane of a wavefront runs every iteration
+    for (int t0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63); t0 < nW + nC; t0 += stride) {
+        const int t = t0 + (threadIdx.x & 63);
+        bool valid = t < nW + nC;
+        int i = 0, j = 0;
+        double qn[4];
+        if (valid && t < nW) {
             const uint32_t e = lw[t];
-            const int i = int(e & 0xffffu), j = int(e >> 16);
+            i = int(e & 0xffffu), j = int(e >> 16);
             const long long p = gidx(D, i, j);
             double a[4], b[4], c[4], d[4], r[4];
             get4(FX, D.plane, p - 1, a);
@@ -1152,59 +1198,64 @@ __global__ void __launch_bounds__(256) k2_stage_update(Dims2 D, const uint32_t* 
             get4(FY, D.plane, p - D.P, c);
             get4(FY, D.plane, p, d);
             for (int k = 0; k < 4; k++) r[k] = (b[k] - a[k]) + (d[k] - c[k]);
-            stage_out<S>(D, p, r, gdx * 1, dt, Qa, Qs, Qo);
-            continue;
+            stage_out<S>(D, p, r, gdx * 1, dt, Qa, Qs, Qo, qn);
+        } else if (valid) {
+            const uint32_t e = lc[t - nW];
+            i = int(e & 0xffffu), j = int(e >> 16);
+            const long long p = gidx(D, i, j);
+            const int lev = lvl[p];
+            const int d = 1 << (D.L - 1 - lev);
+            // a closed mask keeps every used stencil point inside the ghost band
+            const bool okx = (i <= d) ? (i - d >= -G && i + 2 * d <= D.Nx - 1 + G)
+                                      : (i - 2 * d >= -G && i + ((i >= D.Nx - 1 - d) ? d : 2 * d) <= D.Nx - 1 + G);
+            const bool oky = (j <= d) ? (j - d >= -G && j + 2 * d <= D.Ny - 1 + G)
+                                      : (j - 2 * d >= -G && j + ((j >= D.Ny - 1 - d) ? d : 2 * d) <= D.Ny - 1 + G);
+            if (!okx || !oky) {
+                sc->flag |= 1;
+                valid = false;
+            } else {
+                double q[4], fll[4], fl[4], fr[4], frr[4], dF[4], dG[4];
+                const long long sx = d, sy = (long long)d * D.P;
+                if (i <= d) {
+                    get4(Qs, D.plane, p - sx, q); flux_f2<0>(gamma, q, fl);
+                    get4(Qs, D.plane, p + sx, q); flux_f2<0>(gamma, q, fr);
+                    get4(Qs, D.plane, p + 2 * sx, q); flux_f2<0>(gamma, q, frr);
+                    for (int k = 0; k < 4; k++) dF[k] = kD1 * (fr[k] - fl[k]) + kD2 * (frr[k] - fl[k]);
+                } else if (i >= D.Nx - 1 - d) {
+                    get4(Qs, D.plane, p - 2 * sx, q); flux_f2<0>(gamma, q, fll);
+                    get4(Qs, D.plane, p - sx, q); flux_f2<0>(gamma, q, fl);
+                    get4(Qs, D.plane, p + sx, q); flux_f2<0>(gamma, q, fr);
+                    for (int k = 0; k < 4; k++) dF[k] = kD1 * (fr[k] - fl[k]) + kD2 * (fr[k] - fll[k]);
+                } else {
+                    get4(Qs, D.plane, p - 2 * sx, q); flux_f2<0>(gamma, q, fll);
+                    get4(Qs, D.plane, p - sx, q); flux_f2<0>(gamma, q, fl);
+                    get4(Qs, D.plane, p + sx, q); flux_f2<0>(gamma, q, fr);
+                    get4(Qs, D.plane, p + 2 * sx, q); flux_f2<0>(gamma, q, frr);
+                    for (int k = 0; k < 4; k++) dF[k] = kD1 * (fr[k] - fl[k]) + kD2 * (frr[k] - fll[k]);
+                }
+                if (j <= d) {
+                    get4(Qs, D.plane, p - sy, q); flux_f2<1>(gamma, q, fl);
+                    get4(Qs, D.plane, p + sy, q); flux_f2<1>(gamma, q, fr);
+                    get4(Qs, D.plane, p + 2 * sy, q); flux_f2<1>(gamma, q, frr);
+                    for (int k = 0; k < 4; k++) dG[k] = kD1 * (fr[k] - fl[k]) + kD2 * (frr[k] - fl[k]);
+                } else if (j >= D.Ny - 1 - d) {
+                    get4(Qs, D.plane, p - 2 * sy, q); flux_f2<1>(gamma, q, fll);
+                    get4(Qs, D.plane, p - sy, q); flux_f2<1>(gamma, q, fl);
+                    get4(Qs, D.plane, p + sy, q); flux_f2<1>(gamma, q, fr);
+                    for (int k = 0; k < 4; k++) dG[k] = kD1 * (fr[k] - fl[k]) + kD2 * (fr[k] - fll[k]);
+                } else {
+                    get4(Qs, D.plane, p - 2 * sy, q); flux_f2<1>(gamma, q, fll);
+                    get4(Qs, D.plane, p - sy, q); flux_f2<1>(gamma, q, fl);
+                    get4(Qs, D.plane, p + sy, q); flux_f2<1>(gamma, q, fr);
+                    get4(Qs, D.plane, p + 2 * sy, q); flux_f2<1>(gamma, q, frr);
+                    for (int k = 0; k < 4; k++) dG[k] = kD1 * (fr[k] - fl[k]) + kD2 * (frr[k] - fll[k]);
+                }
+                double r[4];
+                for (int k = 0; k < 4; k++) r[k] = dF[k] + dG[k];
+                stage_out<S>(D, p, r, gdx * d, dt, Qa, Qs, Qo, qn);
+            }
         }
-        const uint32_t e = lc[t - nW];
-        const int i = int(e & 0xffffu), j = int(e >> 16);
-        const long long p = gidx(D, i, j);
-        const int lev = lvl[p];
-        const int d = 1 << (D.L - 1 - lev);
-        // a closed mask keeps every used stencil point inside the ghost band
-        const bool okx = (i <= d) ? (i - d >= -G && i + 2 * d <= D.Nx - 1 + G)
-                                  : (i - 2 * d >= -G && i + ((i >= D.Nx - 1 - d) ? d : 2 * d) <= D.Nx - 1 + G);
-        const bool oky = (j <= d) ? (j - d >= -G && j + 2 * d <= D.Ny - 1 + G)
-                                  : (j - 2 * d >= -G && j + ((j >= D.Ny - 1 - d) ? d : 2 * d) <= D.Ny - 1 + G);
-        if (!okx || !oky) { sc->flag |= 1; continue; }
-        double q[4], fll[4], fl[4], fr[4], frr[4], dF[4], dG[4];
-        const long long sx = d, sy = (long long)d * D.P;
-        if (i <= d) {
-            get4(Qs, D.plane, p - sx, q); flux_f2<0>(gamma, q, fl);
-            get4(Qs, D.plane, p + sx, q); flux_f2<0>(gamma, q, fr);
-            get4(Qs, D.plane, p + 2 * sx, q); flux_f2<0>(gamma, q, frr);
-            for (int k = 0; k < 4; k++) dF[k] = kD1 * (fr[k] - fl[k]) + kD2 * (frr[k] - fl[k]);
-        } else if (i >= D.Nx - 1 - d) {
-            get4(Qs, D.plane, p - 2 * sx, q); flux_f2<0>(gamma, q, fll);
-            get4(Qs, D.plane, p - sx, q); flux_f2<0>(gamma, q, fl);
-            get4(Qs, D.plane, p + sx, q); flux_f2<0>(gamma, q, fr);
-            for (int k = 0; k < 4; k++) dF[k] = kD1 * (fr[k] - fl[k]) + kD2 * (fr[k] - fll[k]);
-        } else {
-            get4(Qs, D.plane, p - 2 * sx, q); flux_f2<0>(gamma, q, fll);
-            get4(Qs, D.plane, p - sx, q); flux_f2<0>(gamma, q, fl);
-            get4(Qs, D.plane, p + sx, q); flux_f2<0>(gamma, q, fr);
-            get4(Qs, D.plane, p + 2 * sx, q); flux_f2<0>(gamma, q, frr);
-            for (int k = 0; k < 4; k++) dF[k] = kD1 * (fr[k] - fl[k]) + kD2 * (frr[k] - fll[k]);
-        }
-        if (j <= d) {
-            get4(Qs, D.plane, p - sy, q); flux_f2<1>(gamma, q, fl);
-            get4(Qs, D.plane, p + sy, q); flux_f2<1>(gamma, q, fr);
-            get4(Qs, D.plane, p + 2 * sy, q); flux_f2<1>(gamma, q, frr);
-            for (int k = 0; k < 4; k++) dG[k] = kD1 * (fr[k] - fl[k]) + kD2 * (frr[k] - fl[k]);
-        } else if (j >= D.Ny - 1 - d) {
-            get4(Qs, D.plane, p - 2 * sy, q); flux_f2<1>(gamma, q, fll);
-            get4(Qs, D.plane, p - sy, q); flux_f2<1>(gamma, q, fl);
-            get4(Qs, D.plane, p + sy, q); flux_f2<1>(gamma, q, fr);
-            for (int k = 0; k < 4; k++) dG[k] = kD1 * (fr[k] - fl[k]) + kD2 * (fr[k] - fll[k]);
-        } else {
-            get4(Qs, D.plane, p - 2 * sy, q); flux_f2<1>(gamma, q, fll);
-            get4(Qs, D.plane, p - sy, q); flux_f2<1>(gamma, q, fl);
-            get4(Qs, D.plane, p + sy, q); flux_f2<1>(gamma, q, fr);
-            get4(Qs, D.plane, p + 2 * sy, q); flux_f2<1>(gamma, q, frr);
-            for (int k = 0; k < 4; k++) dG[k] = kD1 * (fr[k] - fl[k]) + kD2 * (frr[k] - fll[k]);
-        }
-        double r[4];
-        for (int k = 0; k < 4; k++) r[k] = dF[k] + dG[k];
-        stage_out<S>(D, p, r, gdx * d, dt, Qa, Qs, Qo);
+        if (nax) next_alpha(valid, i, j, qn, gamma, nax, nay);
     }
 }
 
@@ -1970,10 +2021,10 @@ int Solver2D::enqueue_step() {
         // alpha partials over the stage's outer mask (interior points only),
         // then the ghost fill of Qs, which also reduces the partials (and, in
         // stage 1, turns the CFL maximum into dt)
-        const int mk = s == 0 ? 3 : g.k + 1;
-        AWCM_LAUNCH(k2_alpha, ag, ab, 0, stream, D, m[mk], g.Qs, ap, gamma, sc);
-        AWCM_LAUNCH(k2_bc, dim3(nbc + nred), dim3(256), 0, stream, D, const_cast<double*>(g.Qs), bc_kind, x0, dx,
-                    s == 0 ? 1 : 0, pt, ap, ax(s), ay(s), int(nbc), sc);
+        // (stages 2 and 3 take their alphas from the previous stage's update)
+        if (s == 0) AWCM_LAUNCH(k2_alpha, ag, ab, 0, stream, D, m[3], g.Qs, ap, gamma, sc);
+        AWCM_LAUNCH(k2_bc, dim3(s == 0 ? nbc + nred : nbc), dim3(256), 0, stream, D, const_cast<double*>(g.Qs),
+                    bc_kind, x0, dx, s == 0 ? 1 : 0, pt, ap, ax(s), ay(s), int(nbc), s == 0 ? ax(1) : nullptr, sc);
         if (prof) pe = mark();
         AWCM_LAUNCH(k2_stage_faces, dim3(gw), dim3(128), 0, stream, D, list(g.k, 0), list(g.k, 2), list(g.k, 3), g.k,
                     g.Qs, FX, FY, ax(s), ay(s), sc);
@@ -1981,13 +2032,13 @@ int Solver2D::enqueue_step() {
         if (prof) pe = mark();
         if (s == 0)
             AWCM_LAUNCH(k2_stage_update<1>, dim3(gc), dim3(256), 0, stream, D, list(g.k, 0), list(g.k, 1), g.k, lv[g.k],
-                        g.Qa, g.Qs, g.Qo, FX, FY, gamma, dx, sc);
+                        g.Qa, g.Qs, g.Qo, FX, FY, gamma, dx, ax(1), ay(1), sc);
         else if (s == 1)
             AWCM_LAUNCH(k2_stage_update<2>, dim3(gc), dim3(256), 0, stream, D, list(g.k, 0), list(g.k, 1), g.k, lv[g.k],
-                        g.Qa, g.Qs, g.Qo, FX, FY, gamma, dx, sc);
+                        g.Qa, g.Qs, g.Qo, FX, FY, gamma, dx, ax(2), ay(2), sc);
         else
             AWCM_LAUNCH(k2_stage_update<3>, dim3(gc), dim3(256), 0, stream, D, list(g.k, 0), list(g.k, 1), g.k, lv[g.k],
-                        g.Qa, g.Qs, g.Qo, FX, FY, gamma, dx, sc);
+                        g.Qa, g.Qs, g.Qo, FX, FY, gamma, dx, nullptr, nullptr, sc);
         if (prof) prof_ev[1].push_back({pe, mark()});
     }
     AWCM_LAUNCH(k2_combine, dim3(nblk(D.Nx, 256), D.Ny), dim3(256), 0, stream, D, m[0], m[3], mt, Q, Q0, 1, sc);
