@@ -24,7 +24,8 @@
 //   CFL / alpha reductions (main_2d_euler(dmach).cpp:61-104)     k2_alpha (+ dt in k2_bc)
 //   ghost fill (PROBLEM_QUADRANTS.h:123-203, PROBLEM_DOUBLEMACH.h:106-245)   k2_bc
 //   stage loops (main_2d_euler(dmach).cpp:106-322)               k2_stage_faces, k2_stage_update<S>
-//   RK combine + RemoveResidueEuler (:325-334, AWCM_MANAGER_2D.h:267-280)    k2_combine
+//   RK combine (:325-332)                                        folded into k2_stage_update<3>
+//   RemoveResidueEuler (AWCM_MANAGER_2D.h:267-280)               k2_residue
 //   optional one-workgroup tail of the coarsest levels           k2_tail<...>, k2_check_tail
 // Every kernel returns at once when the step is skipped (t >= t_final), so
 // many steps can be queued or graph-replayed without host round trips.
@@ -988,6 +989,45 @@ __device__ __forceinline__ void get4(const double* Q, long long plane, long long
     v[0] = Q[p]; v[1] = Q[p + plane]; v[2] = Q[p + 2 * plane]; v[3] = Q[p + 3 * plane];
 }
 
+// Ghost state on side 0..3 (left, right, bottom, top) in column i, from the
+// state `src` of the boundary point it faces: a copy (transmissive,
+// PROBLEM_QUADRANTS.h:123-203) or, for the double Mach reflection
+// (BOUNDARY_MANAGER_DMACH::SetBoundaryCondition, PROBLEM_DOUBLEMACH.h:106-245,
+// EOS at gamma = 1.4), the post-shock state on the left, a copy on the right,
+// the reflecting wall (y-momentum negated) or the post-shock state below, and
+// the post- or pre-shock state above as the shock has moved by time T.
+__device__ __forceinline__ void ghost_state(int kind, int side, int i, double x0, double dx, double T,
+                                            const double* src, double* v) {
+    if (kind == AWCM_BC_TRANSMISSIVE) {
+        for (int k = 0; k < 4; k++) v[k] = src[k];
+        return;
+    }
+    const double sqrt3 = sqrt(3.0);
+    if (side == 0) {
+        const double rho = 8, u = 8.25 * sqrt(3.0) * 0.5, vv = -8.25 * 0.5, pr = 116.5;
+        v[0] = rho; v[1] = rho * u; v[2] = rho * vv; v[3] = eos_E2(1.4, rho, u, vv, pr);
+    } else if (side == 1) {
+        for (int k = 0; k < 4; k++) v[k] = src[k];
+    } else if (side == 2) {
+        const double x = x0 + (i - 0) * dx;
+        if (x > 1.0 / 6.0) {
+            for (int k = 0; k < 4; k++) v[k] = src[k];
+            v[2] = -v[2];
+        } else {
+            v[0] = 8; v[1] = 8 * 8.25 * sqrt3 * 0.5; v[2] = -8 * 8.25 * 0.5;
+            v[3] = eos_E2(1.4, 8, 8.25 * sqrt3 * 0.5, -8.25 * 0.5, 116.5);
+        }
+    } else {
+        const double x = x0 + (i - 0) * dx;
+        if (x < 1.0 / 6.0 + (1 + 20 * T) / sqrt3) {
+            v[0] = 8; v[1] = 8 * 8.25 * sqrt3 * 0.5; v[2] = -8 * 8.25 * 0.5;
+            v[3] = eos_E2(1.4, 8, 8.25 * sqrt3 * 0.5, -8.25 * 0.5, 116.5);
+        } else {
+            v[0] = 1.4; v[1] = 1.4 * 0; v[2] = 1.4 * 0; v[3] = eos_E2(1.4, 1.4, 0, 0, 1);
+        }
+    }
+}
+
 // Ghost fill of one field at time t; with compute_dt, block 0 first turns the
 // CFL maximum into dt with the output-time clip (main_2d_euler(dmach).cpp:76-83).
 // Blocks nbc.. of the grid reduce the alpha partials k2_alpha left (one row or
@@ -1042,46 +1082,19 @@ __global__ void __launch_bounds__(256) k2_bc(Dims2 D, double* Q, int kind, doubl
     else if (t < 2 * nl + nb) { side = 2; i = (t - 2 * nl) / G; j = -1 - (t - 2 * nl) % G; }
     else if (t < 2 * nl + 2 * nb) { side = 3; i = (t - 2 * nl - nb) / G; j = D.Ny + (t - 2 * nl - nb) % G; }
     else return;
-    const long long p = gidx(D, i, j);
-    double v[4];
-    if (kind == AWCM_BC_TRANSMISSIVE) {
-        const int si = side == 0 ? 0 : side == 1 ? D.Nx - 1 : i;
-        const int sj = side == 2 ? 0 : side == 3 ? D.Ny - 1 : j;
-        get4(Q, D.plane, gidx(D, si, sj), v);
-        if (!skip) put4(Q, D.plane, p, v);
-        return;
-    }
-    // double Mach reflection (EOS at gamma = 1.4, as BOUNDARY_MANAGER_DMACH)
-    const double sqrt3 = sqrt(3.0);
-    if (side == 0) {
-        const double rho = 8, u = 8.25 * sqrt(3.0) * 0.5, vv = -8.25 * 0.5, pr = 116.5;
-        v[0] = rho; v[1] = rho * u; v[2] = rho * vv; v[3] = eos_E2(1.4, rho, u, vv, pr);
-    } else if (side == 1) {
-        get4(Q, D.plane, gidx(D, D.Nx - 1, j), v);
-    } else if (side == 2) {
-        const double x = x0 + (i - 0) * dx;
-        if (x > 1.0 / 6.0) {
-            get4(Q, D.plane, gidx(D, i, 0), v);
-            v[2] = -v[2];
-        } else {
-            v[0] = 8; v[1] = 8 * 8.25 * sqrt3 * 0.5; v[2] = -8 * 8.25 * 0.5;
-            v[3] = eos_E2(1.4, 8, 8.25 * sqrt3 * 0.5, -8.25 * 0.5, 116.5);
-        }
-    } else {
-        const double x = x0 + (i - 0) * dx;
-        if (x < 1.0 / 6.0 + (1 + 20 * T) / sqrt3) {
-            v[0] = 8; v[1] = 8 * 8.25 * sqrt3 * 0.5; v[2] = -8 * 8.25 * 0.5;
-            v[3] = eos_E2(1.4, 8, 8.25 * sqrt3 * 0.5, -8.25 * 0.5, 116.5);
-        } else {
-            v[0] = 1.4; v[1] = 1.4 * 0; v[2] = 1.4 * 0; v[3] = eos_E2(1.4, 1.4, 0, 0, 1);
-        }
-    }
-    if (!skip) put4(Q, D.plane, p, v);
+    // the boundary point the ghost is filled from
+    const int si = side == 0 ? 0 : side == 1 ? D.Nx - 1 : i;
+    const int sj = side == 2 ? 0 : side == 3 ? D.Ny - 1 : j;
+    double src[4], v[4];
+    get4(Q, D.plane, gidx(D, si, sj), src);
+    ghost_state(kind, side, i, x0, dx, T, src, v);
+    if (!skip) put4(Q, D.plane, gidx(D, i, j), v);
 }
 
 // ------------------------------------------------------------ RK stages ----
 // S = 1: Q2 = Q - dt/h r(Q);  S = 2: Q1 = 0.75 Q + 0.25 (Q2 - dt/h r(Q2));
-// S = 3: Q0 = Q1 - dt/h r(Q1)  (main_2d_euler(dmach).cpp:162, 242, 321)
+// S = 3: Q0 = Q1 - dt/h r(Q1)  (main_2d_euler(dmach).cpp:162, 242, 321),
+// folded at once into the RK combine Q = (Q + 2 Q0) / 3 (:325-332): Qo is Q.
 template <int S>
 __device__ __forceinline__ void stage_out(const Dims2& D, long long p, const double* r, double h, double dt,
                                           const double* Qa, const double* Qs, double* Qo, double* out) {
@@ -1090,7 +1103,32 @@ __device__ __forceinline__ void stage_out(const Dims2& D, long long p, const dou
         if (S == 1) out[k] = Qs[o] - dt / h * r[k];
         else if (S == 2) out[k] = 0.75 * Qa[o] + 0.25 * (Qs[o] - dt / h * r[k]);
         else out[k] = Qs[o] - dt / h * r[k];
-        Qo[o] = out[k];
+        if (S == 3) Qo[o] = (Qo[o] + 2.0 * out[k]) / 3.0;
+        else Qo[o] = out[k];
+    }
+}
+
+// The ghosts a boundary point (i, j) of the new field Qo feeds (k2_bc's fill,
+// done by the thread that writes the point): every boundary point is in every
+// stage mask (the band of k2_band_snap).
+__device__ __forceinline__ void fill_ghosts_from(const Dims2& D, int kind, double x0, double dx, double T, int i, int j,
+                                                 const double* q, double* Qo) {
+    double v[4];
+    if (i == 0) {
+        ghost_state(kind, 0, -1, x0, dx, T, q, v);
+        for (int c = 0; c < G; c++) put4(Qo, D.plane, gidx(D, -1 - c, j), v);
+    }
+    if (i == D.Nx - 1) {
+        ghost_state(kind, 1, D.Nx, x0, dx, T, q, v);
+        for (int c = 0; c < G; c++) put4(Qo, D.plane, gidx(D, D.Nx + c, j), v);
+    }
+    if (j == 0) {
+        ghost_state(kind, 2, i, x0, dx, T, q, v);
+        for (int c = 0; c < G; c++) put4(Qo, D.plane, gidx(D, i, -1 - c), v);
+    }
+    if (j == D.Ny - 1) {
+        ghost_state(kind, 3, i, x0, dx, T, q, v);
+        for (int c = 0; c < G; c++) put4(Qo, D.plane, gidx(D, i, D.Ny + c), v);
     }
 }
 
@@ -1140,6 +1178,10 @@ __global__ void __launch_bounds__(128) k2_stage_faces(Dims2 D, const uint32_t* _
 // :123-128); coarser points take central differences of the physical flux at
 // spacing d with the one-sided variants of AWCM_2D.h:166-187.
 //
+// With gkind >= 0 (stages 1 and 2) the kernel also fills the ghosts of its
+// output, the next stage's input (k2_bc's fill, from the boundary points it
+// writes).  Stage 3 folds the RK combine into its output (stage_out<3>).
+//
 // With `nax` (stages 1 and 2), the kernel also forms the next stage's LF
 // alphas: the points it writes are exactly the next stage's mask (mask2 for
 // stage 1, mask1 for stage 2) and their new states are the next stage's input,
@@ -1177,10 +1219,10 @@ __global__ void __launch_bounds__(256) k2_stage_update(Dims2 D, const uint32_t* 
                                                        const double* __restrict__ Qs, double* __restrict__ Qo,
                                                        const double* __restrict__ FX, const double* __restrict__ FY,
                                                        double gamma, double gdx, double* __restrict__ nax,
-                                                       double* __restrict__ nay, Scal2* sc) {
+                                                       double* __restrict__ nay, int gkind, double x0, Scal2* sc) {
     if (sc->skip) return;
     const int nW = sc->cntW[which], nC = sc->cntC[which];
-    const double dt = sc->dt;
+    const double dt = sc->dt, T = sc->t;
     const int stride = gridDim.x * blockDim.x;
     // wave-uniform trip count: every lane of a wavefront runs every iteration
     for (int t0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63); t0 < nW + nC; t0 += stride) {
@@ -1255,24 +1297,27 @@ __global__ void __launch_bounds__(256) k2_stage_update(Dims2 D, const uint32_t* 
                 stage_out<S>(D, p, r, gdx * d, dt, Qa, Qs, Qo, qn);
             }
         }
+        if (gkind >= 0 && valid && (i == 0 || i == D.Nx - 1 || j == 0 || j == D.Ny - 1))
+            fill_ghosts_from(D, gkind, x0, gdx, T, i, j, qn, Qo);
         if (nax) next_alpha(valid, i, j, qn, gamma, nax, nay);
     }
 }
 
-// RK combine on mask0 and RemoveResidueEuler on the interior; thread 0 advances t.
-// Between steps Q is zero at every interior point outside mask0, and within a
-// step only points of mask3 (inverse transform, interpolation) or of the
-// previous mask0 (mask_temp) can become non-zero, so the residue pass only
-// writes there -- unless the host replaced Q or mask0 (residue_full).
-__global__ void k2_combine(Dims2 D, const uint8_t* __restrict__ m0, const uint8_t* __restrict__ m3,
-                           const uint8_t* __restrict__ mtmp, double* __restrict__ Q, const double* __restrict__ Q0,
-                           int do_combine, Scal2* sc) {
+// RemoveResidueEuler (AWCM_MANAGER_2D.h:267-280) on the interior; with `advance`
+// (the end of a step, whose RK combine stage 3 already did) thread 0 also
+// advances t and the step counters.  Between steps Q is zero at every interior
+// point outside mask0, and within a step only points of mask3 (inverse
+// transform, interpolation) or of the previous mask0 (mask_temp) can become
+// non-zero, so the pass only writes there -- unless the host replaced Q or
+// mask0 (residue_full).
+__global__ void k2_residue(Dims2 D, const uint8_t* __restrict__ m0, const uint8_t* __restrict__ m3,
+                           const uint8_t* __restrict__ mtmp, double* __restrict__ Q, int advance, Scal2* sc) {
     const int skip = sc->skip;  // tested after the mask load
     const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
     const long long p = gidx(D, i < D.Nx ? i : D.Nx - 1, j);
     const uint8_t a0 = m0[p];
     if (skip) return;
-    if (do_combine && i == 0 && j == 0) {
+    if (advance && i == 0 && j == 0) {
         sc->t = sc->t + sc->dt;
         if (sc->do_print) sc->cnt = sc->cnt + 1;
         sc->steps += 1;
@@ -1281,14 +1326,8 @@ __global__ void k2_combine(Dims2 D, const uint8_t* __restrict__ m0, const uint8_
         sc->sumC += (long long)sc->cntC[0] + sc->cntC[1] + sc->cntC[2];
         for (int k = 0; k < 3; k++) sc->sumF += 2LL * sc->cntW[k] + sc->cntXS[k] + sc->cntYS[k];
     }
-    if (i >= D.Nx) return;
-    if (a0) {
-        if (!do_combine) return;
-        for (int q = 0; q < 4; q++) {
-            const long long o = p + q * D.plane;
-            Q[o] = (Q[o] + 2.0 * Q0[o]) / 3.0;
-        }
-    } else if (i >= 1 && i <= D.Nx - 2 && j >= 1 && j <= D.Ny - 2 && (sc->residue_full || m3[p] || mtmp[p])) {
+    if (i >= D.Nx || a0) return;
+    if (i >= 1 && i <= D.Nx - 2 && j >= 1 && j <= D.Ny - 2 && (sc->residue_full || m3[p] || mtmp[p])) {
         for (int q = 0; q < 4; q++) Q[p + q * D.plane] = 0.0;
     }
 }
@@ -1565,7 +1604,7 @@ int Solver2D::init(int L, int nxc_, int nyc_, double g, double dx_, double x0_, 
     const size_t mb = size_t(D.plane);
     const size_t nl = size_t(Nx) * Ny;
     int st;
-    if ((st = dalloc(&Q, pb)) || (st = dalloc(&Q0, pb)) || (st = dalloc(&Q1, pb)) || (st = dalloc(&Q2, pb))) return st;
+    if ((st = dalloc(&Q, pb)) || (st = dalloc(&Q1, pb)) || (st = dalloc(&Q2, pb))) return st;
     if ((st = dalloc(&mt, mb)) || (st = dalloc(&snap, mb)) || (st = dalloc(&tmp, mb)) || (st = dalloc(&work, mb)))
         return st;
     for (int k = 0; k < 4; k++) if ((st = dalloc(&m[k], mb)) || (st = dalloc(&lv[k], mb))) return st;
@@ -1578,7 +1617,7 @@ int Solver2D::init(int L, int nxc_, int nyc_, double g, double dx_, double x0_, 
     if ((st = dalloc(&FX, pb)) || (st = dalloc(&FY, pb))) return st;
     if ((st = dalloc(&pt, sizeof(double) * 4096))) return st;
     if ((st = dalloc(&sc, sizeof(Scal2)))) return st;
-    hipMemset(Q, 0, pb); hipMemset(Q0, 0, pb); hipMemset(Q1, 0, pb); hipMemset(Q2, 0, pb);
+    hipMemset(Q, 0, pb); hipMemset(Q1, 0, pb); hipMemset(Q2, 0, pb);
     // AWCM_2D ctor (AWCM_2D.h:26-41): masks false except mask0; levels = max level
     hipMemset(mt, 0, mb); hipMemset(snap, 0, mb); hipMemset(tmp, 0, mb); hipMemset(work, 0, mb);
     for (int k = 0; k < 4; k++) { hipMemset(m[k], k == 0 ? 1 : 0, mb); hipMemset(lv[k], D.maxlev, mb); }
@@ -2015,16 +2054,19 @@ int Solver2D::enqueue_step() {
     hipEvent_t pe = nullptr;
     // stage s reads Qs (ghosts filled), uses alpha set s and the lists of mask k
     struct StageArgs { int k; const double* Qa; const double* Qs; double* Qo; };
-    const StageArgs sa[3] = {{2, Q, Q, Q2}, {1, Q, Q2, Q1}, {0, Q1, Q1, Q0}};
+    const StageArgs sa[3] = {{2, Q, Q, Q2}, {1, Q, Q2, Q1}, {0, Q1, Q1, Q}};
     for (int s = 0; s < 3; s++) {
         const StageArgs& g = sa[s];
         // alpha partials over the stage's outer mask (interior points only),
         // then the ghost fill of Qs, which also reduces the partials (and, in
         // stage 1, turns the CFL maximum into dt)
-        // (stages 2 and 3 take their alphas from the previous stage's update)
-        if (s == 0) AWCM_LAUNCH(k2_alpha, ag, ab, 0, stream, D, m[3], g.Qs, ap, gamma, sc);
-        AWCM_LAUNCH(k2_bc, dim3(s == 0 ? nbc + nred : nbc), dim3(256), 0, stream, D, const_cast<double*>(g.Qs),
-                    bc_kind, x0, dx, s == 0 ? 1 : 0, pt, ap, ax(s), ay(s), int(nbc), s == 0 ? ax(1) : nullptr, sc);
+        // (stages 2 and 3 take their alphas and ghosts from the previous
+        // stage's update)
+        if (s == 0) {
+            AWCM_LAUNCH(k2_alpha, ag, ab, 0, stream, D, m[3], g.Qs, ap, gamma, sc);
+            AWCM_LAUNCH(k2_bc, dim3(nbc + nred), dim3(256), 0, stream, D, const_cast<double*>(g.Qs), bc_kind, x0, dx,
+                        1, pt, ap, ax(s), ay(s), int(nbc), ax(1), sc);
+        }
         if (prof) pe = mark();
         AWCM_LAUNCH(k2_stage_faces, dim3(gw), dim3(128), 0, stream, D, list(g.k, 0), list(g.k, 2), list(g.k, 3), g.k,
                     g.Qs, FX, FY, ax(s), ay(s), sc);
@@ -2032,16 +2074,16 @@ int Solver2D::enqueue_step() {
         if (prof) pe = mark();
         if (s == 0)
             AWCM_LAUNCH(k2_stage_update<1>, dim3(gc), dim3(256), 0, stream, D, list(g.k, 0), list(g.k, 1), g.k, lv[g.k],
-                        g.Qa, g.Qs, g.Qo, FX, FY, gamma, dx, ax(1), ay(1), sc);
+                        g.Qa, g.Qs, g.Qo, FX, FY, gamma, dx, ax(1), ay(1), bc_kind, x0, sc);
         else if (s == 1)
             AWCM_LAUNCH(k2_stage_update<2>, dim3(gc), dim3(256), 0, stream, D, list(g.k, 0), list(g.k, 1), g.k, lv[g.k],
-                        g.Qa, g.Qs, g.Qo, FX, FY, gamma, dx, ax(2), ay(2), sc);
+                        g.Qa, g.Qs, g.Qo, FX, FY, gamma, dx, ax(2), ay(2), bc_kind, x0, sc);
         else
             AWCM_LAUNCH(k2_stage_update<3>, dim3(gc), dim3(256), 0, stream, D, list(g.k, 0), list(g.k, 1), g.k, lv[g.k],
-                        g.Qa, g.Qs, g.Qo, FX, FY, gamma, dx, nullptr, nullptr, sc);
+                        g.Qa, g.Qs, g.Qo, FX, FY, gamma, dx, nullptr, nullptr, -1, x0, sc);
         if (prof) prof_ev[1].push_back({pe, mark()});
     }
-    AWCM_LAUNCH(k2_combine, dim3(nblk(D.Nx, 256), D.Ny), dim3(256), 0, stream, D, m[0], m[3], mt, Q, Q0, 1, sc);
+    AWCM_LAUNCH(k2_residue, dim3(nblk(D.Nx, 256), D.Ny), dim3(256), 0, stream, D, m[0], m[3], mt, Q, 1, sc);
     return AWCM_OK;
 }
 
@@ -2102,7 +2144,7 @@ int Solver2D::advect(double tol, double dt, int n_steps) {
 
 int Solver2D::enqueue_post() {
     if (int st = enqueue_begin(true, false, false, false)) return st;
-    AWCM_LAUNCH(k2_combine, dim3(nblk(D.Nx, 256), D.Ny), dim3(256), 0, stream, D, m[0], m[3], mt, Q, Q0, 0, sc);
+    AWCM_LAUNCH(k2_residue, dim3(nblk(D.Nx, 256), D.Ny), dim3(256), 0, stream, D, m[0], m[3], mt, Q, 0, sc);
     return AWCM_OK;
 }
 

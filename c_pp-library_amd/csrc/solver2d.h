@@ -54,7 +54,7 @@ struct Solver2D {
     hipEvent_t ev_m3 = nullptr, ev_int = nullptr;
     std::vector<void*> allocs;
 
-    double *Q = nullptr, *Q0 = nullptr, *Q1 = nullptr, *Q2 = nullptr;  // 4 planes each
+    double *Q = nullptr, *Q1 = nullptr, *Q2 = nullptr;  // 4 planes each (stage 3 writes Q directly)
     uint8_t *mt = nullptr, *m[4] = {}, *snap = nullptr;
     uint8_t* tmp = nullptr;      // mask_from of interpolate_all / interpolate
     uint8_t* work = nullptr;     // work mask of interpolate
