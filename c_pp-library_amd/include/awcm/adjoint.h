@@ -187,8 +187,8 @@ AWCM_HD void face_flux1_vjp(double g, const double (*s)[3], double alpha, const 
     }
     double sk[3];
     for (int k = 0; k < 3; k++)
-        sk[k] = wenoz_right(fp[0][k], fp[1][k], fp[2][k], fp[3][k], fp[4][k]) +
-                wenoz_left(fm[1][k], fm[2][k], fm[3][k], fm[4][k], fm[5][k]);
+        sk[k] = wenoz_right<true>(fp[0][k], fp[1][k], fp[2][k], fp[3][k], fp[4][k]) +
+                wenoz_left<true>(fm[1][k], fm[2][k], fm[3][k], fm[4][k], fm[5][k]);
     const double u = a.u, H = a.H, c = a.c;
     // flux = sk0 (1, u-c, H-uc) + sk1 (1, u, u^2/2) + sk2 (1, u+c, H+uc)
     double skb[3];

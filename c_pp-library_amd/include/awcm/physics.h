@@ -77,9 +77,16 @@ struct Div {
 };
 
 // ---- equation of state ------------------------------------------------------
-AWCM_HD double eos_p1(double g, const double* q) { return (g - 1) * (q[2] - 0.5 * (q[1] * q[1]) / q[0]); }
-AWCM_HD double eos_H1(double g, const double* q) { return (q[2] + eos_p1(g, q)) / q[0]; }
-AWCM_HD double eos_c1(double g, const double* q) { return sqrt(g * eos_p1(g, q) / q[0]); }
+// a single quotient a / b, correctly rounded on the device too
+AWCM_HD double rdiv(double a, double b) { return Div(b)(a); }
+
+// d0 divides by the density q[0]
+AWCM_HD double eos_p1(double g, const double* q, const Div& d0) { return (g - 1) * (q[2] - d0(0.5 * (q[1] * q[1]))); }
+AWCM_HD double eos_p1(double g, const double* q) { return eos_p1(g, q, Div(q[0])); }
+AWCM_HD double eos_H1(double g, const double* q, const Div& d0) { return d0(q[2] + eos_p1(g, q, d0)); }
+AWCM_HD double eos_H1(double g, const double* q) { return eos_H1(g, q, Div(q[0])); }
+AWCM_HD double eos_c1(double g, const double* q, const Div& d0) { return sqrt(d0(g * eos_p1(g, q, d0))); }
+AWCM_HD double eos_c1(double g, const double* q) { return eos_c1(g, q, Div(q[0])); }
 AWCM_HD double eos_E1(double g, double rho, double u, double p) { return p / (g - 1) + 0.5 * rho * (u * u); }
 
 // d0 divides by the density q[0]
@@ -97,7 +104,8 @@ AWCM_HD double eos_E2(double g, double rho, double u, double v, double p) {
 
 // ---- physical fluxes and spectral radii --------------------------------------
 AWCM_HD void flux_f1(double g, const double* q, double* f) {
-    const double rho = q[0], u = q[1] / q[0], E = q[2], p = eos_p1(g, q);
+    const Div d0(q[0]);
+    const double rho = q[0], u = d0(q[1]), E = q[2], p = eos_p1(g, q, d0);
     f[0] = rho * u;
     f[1] = rho * u * u + p;
     f[2] = (E + p) * u;
@@ -120,8 +128,9 @@ AWCM_HD void flux_f2(double g, const double* q, double* f) {
     }
 }
 AWCM_HD double spec_rad1(double g, const double* q) {
-    const double u = q[1] / q[0];
-    const double c = eos_c1(g, q);
+    const Div d0(q[0]);
+    const double u = d0(q[1]);
+    const double c = eos_c1(g, q, d0);
     return t_abs(u) + c;
 }
 AWCM_HD double spec_rad2x(double g, const double* q) {
@@ -141,22 +150,21 @@ AWCM_HD double spec_rad2y(double g, const double* q) {
 struct Roe1 { double u, H, c; };
 AWCM_HD Roe1 roe_avg1(double g, const double* l, const double* r) {
     const double rl = l[0], rr = r[0];
-    const double ul = l[1] / l[0], ur = r[1] / r[0];
-    const double Hl = eos_H1(g, l), Hr = eos_H1(g, r);
-    const double den = sqrt(rl) + sqrt(rr);
+    const Div dl(l[0]), dr(r[0]);
+    const double ul = dl(l[1]), ur = dr(r[1]);
+    const double Hl = eos_H1(g, l, dl), Hr = eos_H1(g, r, dr);
+    const Div den(sqrt(rl) + sqrt(rr));
     Roe1 a;
-    a.u = sqrt(rl) * ul + sqrt(rr) * ur;
-    a.u /= den;
-    a.H = sqrt(rl) * Hl + sqrt(rr) * Hr;
-    a.H /= den;
+    a.u = den(sqrt(rl) * ul + sqrt(rr) * ur);
+    a.H = den(sqrt(rl) * Hl + sqrt(rr) * Hr);
     a.c = sqrt((g - 1) * (a.H - 0.5 * a.u * a.u));
     return a;
 }
 AWCM_HD void roe_proj1(const Roe1& a, const double* b, double* x) {
     double x2 = (a.H - a.u * a.u) * b[0] + a.u * b[1] - b[2];
-    x2 *= 0.4 / (a.c * a.c);
+    x2 *= Div(a.c * a.c).nz(0.4);
     double x3 = b[1] + (a.c - a.u) * b[0] - a.c * x2;
-    x3 /= 2 * a.c;
+    x3 = Div(2 * a.c)(x3);
     x[0] = b[0] - x2 - x3;
     x[1] = x2;
     x[2] = x3;
@@ -201,6 +209,10 @@ AWCM_HD void roe_proj2(const Roe2& a, const double* b, double* x) {
 }
 
 // ---- WENO-Z5 (tau5 = |b0-b2|, eps = 1e-40, ideal weights 0.1/0.6/0.3) ---------
+// CR: the three quotients tau / (b_k + eps) correctly rounded on the device as
+// well (the 1-D path); without it they are the hardware divide (the 2-D face
+// kernel, where they cost a quarter of its divide work).
+template <bool CR = false>
 AWCM_HD double wenoz_left(double ll, double l, double c, double r, double rr) {
     const double t0 = c - 2 * r + rr, s0 = 3 * c - 4 * r + rr;
     const double t1 = l - 2 * c + r, s1 = l - r;
@@ -209,9 +221,10 @@ AWCM_HD double wenoz_left(double ll, double l, double c, double r, double rr) {
     const double b1 = 13.0 / 12.0 * (t1 * t1) + 0.25 * (s1 * s1);
     const double b2 = 13.0 / 12.0 * (t2 * t2) + 0.25 * (s2 * s2);
     const double tau = t_abs(b0 - b2);
-    const double a0 = 0.1 * (1 + tau / (b0 + 1e-40));
-    const double a1 = 0.6 * (1 + tau / (b1 + 1e-40));
-    const double a2 = 0.3 * (1 + tau / (b2 + 1e-40));
+    // tau >= +0: a zero quotient needs no sign from the dividend
+    const double a0 = 0.1 * (1 + (CR ? Div(b0 + 1e-40).nz(tau) : tau / (b0 + 1e-40)));
+    const double a1 = 0.6 * (1 + (CR ? Div(b1 + 1e-40).nz(tau) : tau / (b1 + 1e-40)));
+    const double a2 = 0.3 * (1 + (CR ? Div(b2 + 1e-40).nz(tau) : tau / (b2 + 1e-40)));
     const Div as(a0 + a1 + a2);
     const double w0 = as.nz(a0), w1 = as.nz(a1), w2 = as.nz(a2);  // a_k >= d_k > 0
     double out = w0 * (11.0 / 6.0 * c + -7.0 / 6.0 * r + 1.0 / 3.0 * rr);
@@ -219,7 +232,8 @@ AWCM_HD double wenoz_left(double ll, double l, double c, double r, double rr) {
     out += w2 * (-1.0 / 6.0 * ll + 5.0 / 6.0 * l + 1.0 / 3.0 * c);
     return out;
 }
-AWCM_HD double wenoz_right(double ll, double l, double c, double r, double rr) { return wenoz_left(rr, r, c, l, ll); }
+template <bool CR = false>
+AWCM_HD double wenoz_right(double ll, double l, double c, double r, double rr) { return wenoz_left<CR>(rr, r, c, l, ll); }
 
 // ---- WENO5-JS (WENO.h:71-99: a_k = d_k / (1e-6 + b_k)^2, ideal weights 0.1/0.6/0.3) --
 AWCM_HD double wenojs_left(double ll, double l, double c, double r, double rr) {
@@ -230,9 +244,9 @@ AWCM_HD double wenojs_left(double ll, double l, double c, double r, double rr) {
     const double b1 = 13.0 / 12.0 * (t1 * t1) + 0.25 * (s1 * s1);
     const double b2 = 13.0 / 12.0 * (t2 * t2) + 0.25 * (s2 * s2);
     const double e0 = 1e-6 + b0, e1 = 1e-6 + b1, e2 = 1e-6 + b2;
-    const double a0 = 0.1 / (e0 * e0), a1 = 0.6 / (e1 * e1), a2 = 0.3 / (e2 * e2);
-    const double as = a0 + a1 + a2;
-    const double w0 = a0 / as, w1 = a1 / as, w2 = a2 / as;
+    const double a0 = Div(e0 * e0).nz(0.1), a1 = Div(e1 * e1).nz(0.6), a2 = Div(e2 * e2).nz(0.3);
+    const Div as(a0 + a1 + a2);
+    const double w0 = as.nz(a0), w1 = as.nz(a1), w2 = as.nz(a2);  // a_k > 0
     double out = w0 * (11.0 / 6.0 * c + -7.0 / 6.0 * r + 1.0 / 3.0 * rr);
     out += w1 * (1.0 / 3.0 * l + 5.0 / 6.0 * c + -1.0 / 6.0 * r);
     out += w2 * (-1.0 / 6.0 * ll + 5.0 / 6.0 * l + 1.0 / 3.0 * c);
@@ -244,19 +258,20 @@ AWCM_HD double wenojs_right(double ll, double l, double c, double r, double rr) 
 // 2-D drivers and the C1 configuration) or WENO5-JS (WENO.h, the shipped 1-D
 // driver's ComputedF)
 enum { kRecWenoZ = 0, kRecWeno5 = 1 };
-template <int REC>
+template <int REC, bool CR = false>
 AWCM_HD double rec_left(double ll, double l, double c, double r, double rr) {
-    return REC == kRecWeno5 ? wenojs_left(ll, l, c, r, rr) : wenoz_left(ll, l, c, r, rr);
+    return REC == kRecWeno5 ? wenojs_left(ll, l, c, r, rr) : wenoz_left<CR>(ll, l, c, r, rr);
 }
-template <int REC>
+template <int REC, bool CR = false>
 AWCM_HD double rec_right(double ll, double l, double c, double r, double rr) {
-    return REC == kRecWeno5 ? wenojs_right(ll, l, c, r, rr) : wenoz_right(ll, l, c, r, rr);
+    return REC == kRecWeno5 ? wenojs_right(ll, l, c, r, rr) : wenoz_right<CR>(ll, l, c, r, rr);
 }
 
 // ---- Roe-characteristic LF-split WENO numerical flux at one face ----------------
 // s[p] (p = 0..5) are the states lll, ll, l, r, rr, rrr around the face l|r.
 // WENOZ_EULER_SOLVER_1D::ComputeFlux (WENOZ.h:160-205) with REC = kRecWenoZ,
 // WENO_EULER_SOLVER_1D::ComputeFlux (WENO.h:129-174) with REC = kRecWeno5.
+// Every quotient is correctly rounded on the device (Div), as on the host.
 template <int REC = kRecWenoZ>
 AWCM_HD void face_flux1(double g, const double (*s)[3], double alpha, double* flux) {
     double fp[6][3], fm[6][3];
@@ -276,8 +291,8 @@ AWCM_HD void face_flux1(double g, const double (*s)[3], double alpha, double* fl
     double sk[3];
 #pragma unroll
     for (int k = 0; k < 3; k++)
-        sk[k] = rec_right<REC>(fp[0][k], fp[1][k], fp[2][k], fp[3][k], fp[4][k]) +
-                rec_left<REC>(fm[1][k], fm[2][k], fm[3][k], fm[4][k], fm[5][k]);
+        sk[k] = rec_right<REC, true>(fp[0][k], fp[1][k], fp[2][k], fp[3][k], fp[4][k]) +
+                rec_left<REC, true>(fm[1][k], fm[2][k], fm[3][k], fm[4][k], fm[5][k]);
     const double r1[3] = {1, a.u - a.c, a.H - a.u * a.c};
     const double r2[3] = {1, a.u, 0.5 * (a.u * a.u)};
     const double r3[3] = {1, a.u + a.c, a.H + a.u * a.c};

@@ -32,15 +32,6 @@ def assert_same(qg, qo, mg, mo, lg, lo, what):
         f"{what}: fields not bit-identical (max abs diff {np.abs(qg - qo).max()})"
 
 
-def assert_within(qg, qo, mg, mo, lg, lo, what, tol=1e-12):
-    """Masks and levels bit-exact, fields within the stated FP64 tolerance
-    (relative L1 and relative max error <= tol)."""
-    assert np.array_equal(mg, mo), f"{what}: masks differ at {np.argwhere(mg != mo)[:5].tolist()}"
-    assert np.array_equal(lg, lo), f"{what}: levels differ at {np.argwhere(lg != lo)[:5].tolist()}"
-    assert rel_l1(qg, qo) <= tol, f"{what}: rel L1 {rel_l1(qg, qo)}"
-    assert np.abs(qg - qo).max() <= tol * max(1.0, np.abs(qo).max()), f"{what}: max error {np.abs(qg - qo).max()}"
-
-
 def sod_setup(L=7, nxc=17):
     nx = (nxc - 1) * 2 ** (L - 1) + 1
     q, dx, _ = O.ic1(0, nx, 0.0, 1.0, 0.5, O.SOD_L, O.SOD_R)
@@ -105,13 +96,14 @@ def test_1d_weno5_flux_matches_oracle():
     main_1d_euler.cpp:109) on the device, awcm_set_flux(AWCM_FLUX_WENO5): a
     full Sod run (7 levels, 1025 points) and 200 steps in the driver's own
     configuration (SHOCKTUBE on [0, 10], 11 levels, 16 coarse points, 15361
-    points: masks in HBM).  Masks, levels, step counts and t bit-identical to
-    the oracle's WENO5-JS run; fields within 1e-12.  (Step 406 of the Sod run
-    divides E + p = 0x1.6666666666663p-2 by rho = 0x1.ffffffffffffbp-4, a
-    quotient 0.4999999999999999 ulp from the correctly rounded
-    0x1.6666666666667p+1; the device's FP64 division returns the other
-    neighbour there, and the field then differs from the oracle's by ~1e-14.)
-    The adjoint differentiates the WENO-Z pair and refuses a WENO5-JS handle."""
+    points: masks in HBM).  Masks, levels, step counts, t and fields
+    bit-identical to the oracle's WENO5-JS run.  (Step 406 of the Sod run
+    divides E + p = 0x1.6666666666663p-2 by rho = 0x1.ffffffffffffbp-4, whose
+    exact quotient lies 0.5000000000000001 ulp above 0x1.6666666666666p+1: the
+    hardware divide sequence returns 0x1.6666666666666p+1, the 1-D path's
+    correctly rounded quotients (awcm::Div) return 0x1.6666666666667p+1, as
+    the oracle does.)  The adjoint differentiates the WENO-Z pair and
+    refuses a WENO5-JS handle."""
     P = _pyawcm()
     q, dx = sod_setup()
     pts = O.print_times(0.2, 41)
@@ -130,7 +122,7 @@ def test_1d_weno5_flux_matches_oracle():
     assert steps == n and g.t == o.t[0] == 0.2
     mg, lg = g.masks()
     mo, lo = o.masks()
-    assert_within(g.download_q(), o.get_q(), mg, mo, lg, lo, "1-D Sod WENO5-JS t=0.2")
+    assert_same(g.download_q(), o.get_q(), mg, mo, lg, lo, "1-D Sod WENO5-JS t=0.2")
     with pytest.raises(P.AwcmError) as e:
         g.adjoint_record(1e-4, 0.5, 0.3, 4)
     assert e.value.code == -6  # AWCM_ESTATE
@@ -153,7 +145,7 @@ def test_1d_weno5_flux_matches_oracle():
         assert steps == 50 and g.t == o.t[0], f"chunk {chunk}"
         mg, lg = g.masks()
         mo, lo = o.masks()
-        assert_within(g.download_q(), o.get_q(), mg, mo, lg, lo, f"driver configuration, {50 * (chunk + 1)} steps")
+        assert_same(g.download_q(), o.get_q(), mg, mo, lg, lo, f"driver configuration, {50 * (chunk + 1)} steps")
 
 
 def test_1d_largest_instance_matches_oracle():
@@ -186,7 +178,7 @@ def test_1d_largest_instance_matches_oracle():
 def test_batch_weno5_flux_matches_oracle():
     """awcm_batch_set_flux(AWCM_FLUX_WENO5) on a batch of six Riemann
     problems; then back to WENO-Z for a further run (the batch graph is
-    re-captured for the other kernel)."""
+    re-captured for the other kernel).  Bit-identical to the oracle."""
     P = _pyawcm()
     rng = np.random.default_rng(9)
     params, qs = [], []
@@ -222,7 +214,7 @@ def test_batch_weno5_flux_matches_oracle():
         assert st["status"] == 0 and st["t"] == o.t[0], k
         mg, lg = b.masks(k)
         mo, lo = o.masks()
-        assert_within(b.download_q(k), o.get_q(), mg, mo, lg, lo, f"batch instance {k}")
+        assert_same(b.download_q(k), o.get_q(), mg, mo, lg, lo, f"batch instance {k}")
 
 
 def quad_setup(L, nxc):
