@@ -74,8 +74,8 @@ BYTES_PER_FACE = 32 + 4 + 32
 # k2_stage_faces_dmr_t0.1_pmc.md): HBM bytes per face of a k2_stage_faces
 # launch, FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from `bench.py
 # --hbm-copy`), and issued FP64 lane operations per face.
-TRAFFIC_BYTES_PER_FACE = 180
-ISSUED_FP64_OPS_PER_FACE = 2143
+TRAFFIC_BYTES_PER_FACE = 183
+ISSUED_FP64_OPS_PER_FACE = 1803
 
 
 def ensure_built():
