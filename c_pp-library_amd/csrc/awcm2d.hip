@@ -252,18 +252,32 @@ __device__ __forceinline__ void int_d3_pt(const Dims2& D, int i, int j, int d, c
 // finest-grid points), tested as `mask_temp && !mask_from`; the closure only
 // adds points, and the points it can add are all in mask_from, so the test is
 // `!mask_from` alone (WAVELET_2D.h:917-922,943).
+//
+// For the inverse transform of PrepareEuler `aux` is mask_temp (the previous
+// mask0): a point inactive in mask3 gets rho = 0, which only needs a store
+// where rho can be non-zero -- at points of the previous mask0, or anywhere
+// when the host wrote Q or mask0 since the last residue pass (`full`, the
+// step's residue_full) -- because between steps Q is zero outside mask0 and
+// the forward transform writes only active points.  Code 2 marks a point to
+// zero, 1 a point to transform.  Without `aux` (the scalar Prepare, whose
+// mask_temp is not the previous mask0) every inactive point is zeroed.
 template <int OP>
-__device__ __forceinline__ uint8_t op_active(const uint8_t* mask, const uint8_t* aux, long long p) {
+__device__ __forceinline__ uint8_t op_active(const uint8_t* mask, const uint8_t* aux, long long p, int full) {
     if (OP == OP_INT_D12 || OP == OP_INT_D3 || OP == OP_SINT_D12 || OP == OP_SINT_D3)
         return (mask[p] != 0 && aux[p] == 0) ? 1 : 0;
     if (OP == OP_IAG_D12 || OP == OP_IAG_D3) return aux[p] == 0 ? 1 : 0;
+    if (OP == OP_IWT_C || OP == OP_IWT_D12 || OP == OP_IWT_D3) {
+        const uint8_t a = mask[p], was = aux ? aux[p] : 1;  // both loads independent
+        return a ? 1 : (full || was) ? 2 : 0;
+    }
     return mask[p];
 }
 template <int OP>
 __device__ __forceinline__ void op_point(const Dims2& D, int kind, int i, int j, int d, uint8_t act, const Wts& W,
                                          double tol, double* Q, uint8_t* mask) {
     if (OP == OP_IWT_C || OP == OP_IWT_D12 || OP == OP_IWT_D3) {
-        if (!act) { Q[gidx(D, i, j)] = 0; return; }
+        if (act == 2) Q[gidx(D, i, j)] = 0;
+        if (act != 1) return;
     } else if (!act) {
         return;
     }
@@ -298,6 +312,7 @@ __global__ void __launch_bounds__(256) k2_lat(Dims2 D, int lev, Wts W, double to
                                               uint8_t* __restrict__ mask, const uint8_t* __restrict__ aux,
                                               const Scal2* sc) {
     const int skip = sc->skip;  // tested after the mask loads
+    const int full = sc->residue_full;
     const int d = 1 << (D.L - 1 - lev);
     int kind = op_kind1<OP>();
     Lat g = lattice(D, lev, kind);
@@ -314,7 +329,7 @@ __global__ void __launch_bounds__(256) k2_lat(Dims2 D, int lev, Wts W, double to
 #pragma unroll
     for (int k = 0; k < K; k++) {
         const int ii = a * K + k;
-        act[k] = ii < g.ni ? op_active<OP>(mask, aux, gidx(D, g.is + ii * g.dx, j)) : 0;
+        act[k] = ii < g.ni ? op_active<OP>(mask, aux, gidx(D, g.is + ii * g.dx, j), full) : 0;
     }
     if (skip) return;
 #pragma unroll
@@ -339,7 +354,7 @@ __device__ void wg_pass(const Dims2& D, int lev, const Wts& W, double tol, doubl
 #pragma unroll
             for (int k = 0; k < B; k++) {
                 const int t = t0 + k;
-                act[k] = t < n ? op_active<OP>(mask, aux, gidx(D, g.is + (t % g.ni) * g.dx, g.js + (t / g.ni) * g.dy)) : 0;
+                act[k] = t < n ? op_active<OP>(mask, aux, gidx(D, g.is + (t % g.ni) * g.dx, g.js + (t / g.ni) * g.dy), 1) : 0;
             }
 #pragma unroll
             for (int k = 0; k < B; k++) {
@@ -1869,13 +1884,14 @@ int Solver2D::enqueue_prepare(bool force, bool euler) {
         AWCM_LAT_ON(s_int, OP_INT_D3, lev, LAT_D3, false, Q, m[3], mt);
     }
     if (hipEventRecord(ev_int, s_int) != hipSuccess) return AWCM_EDEVICE;
-    // inverse transform, coarsest level first
+    // inverse transform, coarsest level first (mask_temp, the previous mask0,
+    // tells it where an inactive point may hold a non-zero rho)
     if (tail >= 0)
         AWCM_LAUNCH(k2_tail<TAIL_IWT>, dim3(1), dim3(1024), 0, stream, D, tail, W, tol_host, Q, m[3], nullptr, sc);
     for (int lev = tail + 1; lev < D.maxlev; lev++) {
-        AWCM_LAT(OP_IWT_C, lev, LAT_C, false, Q, m[3], nullptr);
-        AWCM_LAT(OP_IWT_D12, lev, LAT_D1, true, Q, m[3], nullptr);
-        AWCM_LAT(OP_IWT_D3, lev, LAT_D3, false, Q, m[3], nullptr);
+        AWCM_LAT(OP_IWT_C, lev, LAT_C, false, Q, m[3], mt);
+        AWCM_LAT(OP_IWT_D12, lev, LAT_D1, true, Q, m[3], mt);
+        AWCM_LAT(OP_IWT_D3, lev, LAT_D3, false, Q, m[3], mt);
     }
     // join the side stream
     if (hipStreamWaitEvent(stream, ev_int, 0) != hipSuccess) return AWCM_EDEVICE;
