@@ -759,6 +759,22 @@ __global__ void k2_level_add(Dims2 D, const uint8_t* __restrict__ src, uint8_t* 
         nbm |= (l | r | up.b[k] | dn.b[k]) ? 1u << k : 0u;
         if (((im & ~vm) >> k) & 1u) L.b[k] = 0;  // inactive interior point
     }
+    if (act == kAll && s.i0 >= 1 && s.i0 + B - 1 <= D.Nx - 2 && s.j >= 1 && s.j <= D.Ny - 2) {
+        // a run of B active interior points: each has an active neighbour in
+        // the run, so each is at the finest level, and the union of their +-3
+        // crosses is the row span i0-3 .. i0+B+2 plus the segment's own
+        // columns in rows j-3 .. j+3 -- aligned B-byte stores of ones where
+        // the loop below would store byte by byte
+        UB<B> top, one;
+#pragma unroll
+        for (int k = 0; k < B; k++) { top.b[k] = uint8_t(D.maxlev); one.b[k] = 1; }
+        stb<B>(lvl + s.p0, top);
+#pragma unroll
+        for (int c = -3; c <= 3; c++) stb<B>(dst + s.p0 + (long long)c * D.P, one);
+#pragma unroll
+        for (int c = 1; c <= 3; c++) { dst[s.p0 - c] = 1; dst[s.p0 + B - 1 + c] = 1; }
+        return;
+    }
     stb<B>(lvl + s.p0, L);  // active points are rewritten below, in program order
     const int j = s.j;
     const int NX = D.Nx - 1, NY = D.Ny - 1;
