@@ -35,11 +35,14 @@ AWCM_HD double t_max(double a, double b) { return a > b ? a : b; }
 // is q = a*y corrected once with the exact FMA residual, q + (a - b*q)*y
 // (P. Markstein, IBM J. Res. Develop. 34(1), 1990, the final step of his
 // division): with y = RN(1/b) that is the correctly rounded a/b for operands
-// and quotients in the normal range (checked exhaustively in 5- to 15-bit
-// binary formats by tests/native/markstein_exhaustive.c).  The quotient takes
-// the sign of a*y, which also gives a zero quotient the sign IEEE division
-// gives it.  A divisor of 0, inf or NaN gives NaN quotients, where IEEE gives
-// inf, 0 or NaN: the state is no longer finite in either case.
+// and quotients in the normal range (checked over every pair of significands
+// of 4- to 12-bit binary formats by tests/native/markstein_exhaustive.c).  The
+// quotient takes the sign of a*y, which also gives a zero quotient the sign
+// IEEE division gives it.  Outside that range the result is not IEEE's: a
+// divisor of 0, inf or NaN gives NaN quotients where IEEE gives inf, 0 or NaN
+// (the state is no longer finite in either case), and subnormal divisors or
+// quotients are not rounded correctly; no physical state of these problems
+// comes near either.
 struct Div {
     double b;
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -77,9 +80,6 @@ struct Div {
 };
 
 // ---- equation of state ------------------------------------------------------
-// a single quotient a / b, correctly rounded on the device too
-AWCM_HD double rdiv(double a, double b) { return Div(b)(a); }
-
 // d0 divides by the density q[0]
 AWCM_HD double eos_p1(double g, const double* q, const Div& d0) { return (g - 1) * (q[2] - d0(0.5 * (q[1] * q[1]))); }
 AWCM_HD double eos_p1(double g, const double* q) { return eos_p1(g, q, Div(q[0])); }
