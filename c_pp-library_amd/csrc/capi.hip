@@ -413,21 +413,23 @@ int awcm_set_tuning(awcm_solver* h, int param, int value) {
 int awcm_divide_shared(const double* a, const double* b, double* q, long long n) {
     if (n < 0 || (n > 0 && (!a || !b || !q))) return AWCM_EINVAL;
     if (n == 0) return AWCM_OK;
-    if (hipSetDevice(0) != hipSuccess) return AWCM_EDEVICE;
+    int dev = 0;  // the caller's current device, restored on return
+    if (hipGetDevice(&dev) != hipSuccess || hipSetDevice(0) != hipSuccess) return AWCM_EDEVICE;
     double* d = nullptr;
     const size_t bytes = sizeof(double) * size_t(n);
-    if (hipMalloc(&d, 3 * bytes) != hipSuccess) return AWCM_ENOMEM;
-    int st = AWCM_OK;
-    if (hipMemcpy(d, a, bytes, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(d + n, b, bytes, hipMemcpyHostToDevice) != hipSuccess)
+    int st = hipMalloc(&d, 3 * bytes) == hipSuccess ? AWCM_OK : AWCM_ENOMEM;
+    if (st == AWCM_OK && (hipMemcpy(d, a, bytes, hipMemcpyHostToDevice) != hipSuccess ||
+                          hipMemcpy(d + n, b, bytes, hipMemcpyHostToDevice) != hipSuccess))
         st = AWCM_EDEVICE;
     if (st == AWCM_OK) {
         hipLaunchKernelGGL(k_divide_shared, dim3(unsigned((n + 255) / 256)), dim3(256), 0, 0, d, d + n, d + 2 * n, n);
-        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(0) != hipSuccess ||
             hipMemcpy(q, d + 2 * n, bytes, hipMemcpyDeviceToHost) != hipSuccess)
             st = AWCM_EDEVICE;
     }
-    return hipFree(d) == hipSuccess ? st : AWCM_EDEVICE;
+    if (d && hipFree(d) != hipSuccess && st == AWCM_OK) st = AWCM_EDEVICE;
+    if (hipSetDevice(dev) != hipSuccess && st == AWCM_OK) st = AWCM_EDEVICE;
+    return st;
 }
 
 int awcm_synchronize(awcm_solver* h) {
