@@ -46,7 +46,8 @@ __global__ void __launch_bounds__(1024) k1_solve(Inst1D* insts, int n_inst, cons
     AWCM_DYNAMIC_LDS __align__(16) uint8_t lds_masks[];
     uint8_t* const sm = kHbm ? hbm + size_t(blockIdx.x) * 11 * size_t(nxp) : lds_masks;
     const bool lstate = !kHbm && (mode & kModeLdsState) != 0;
-    mode &= ~kModeLdsState;
+    const bool lq = !kHbm && !lstate && (mode & kModeLdsQ) != 0;
+    mode &= ~(kModeLdsState | kModeLdsQ);
     __shared__ double W[4][5];
     __shared__ double red[16];
     __shared__ int s_idx, s_flag;
@@ -74,6 +75,10 @@ __global__ void __launch_bounds__(1024) k1_solve(Inst1D* insts, int n_inst, cons
             }
             for (int k = threadIdx.x; k < nf; k += blockDim.x) ls[9 * nxp + k] = in.F[k];
             c.Q = ls; c.Qc = ls + 3 * nxp; c.Qt = ls + 6 * nxp; c.F = ls + 9 * nxp;
+        } else if (lq) {
+            double* ls = reinterpret_cast<double*>(lds_masks + lds_state_offset(nxp));
+            for (int k = threadIdx.x; k < nq; k += blockDim.x) ls[k] = in.Q[k];
+            c.Q = ls;
         }
         c.mt = sm;
         for (int k = 0; k < 4; k++) c.m[k] = sm + (k + 1) * nxp;
@@ -124,6 +129,8 @@ __global__ void __launch_bounds__(1024) k1_solve(Inst1D* insts, int n_inst, cons
                 in.Qt[k] = c.Qt[k];
             }
             for (int k = threadIdx.x; k < nf; k += blockDim.x) in.F[k] = c.F[k];
+        } else if (lq) {
+            for (int k = threadIdx.x; k < nq; k += blockDim.x) in.Q[k] = c.Q[k];
         }
         for (int k = 0; k < 5; k++)
             for (int i = threadIdx.x; i < c.NP; i += blockDim.x) in.masks[k * c.NP + i] = sm[k * nxp + i];

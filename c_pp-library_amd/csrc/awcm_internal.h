@@ -15,6 +15,12 @@ constexpr int kModeLdsState = 0x100;
 constexpr size_t lds_state_bytes(int nxp) { return sizeof(double) * 12 * size_t(nxp); }
 // offset of the state in the dynamic LDS: after the 11 * nxp mask bytes
 constexpr size_t lds_state_offset(int nxp) { return (size_t(11) * nxp + 15) / 16 * 16; }
+// k1_solve mode flag: Q alone lives in LDS after the masks (single-instance
+// launches whose full state does not fit but Q does: the transforms, mask
+// passes and stage updates then touch LDS, the stage copies and face fluxes
+// HBM), copied in at the start and out at the end.
+constexpr int kModeLdsQ = 0x200;
+constexpr size_t lds_q_bytes(int nxp) { return sizeof(double) * 3 * size_t(nxp); }
 
 // One independent 1-D solve.  Device-resident; the kernel updates the
 // time-loop fields (t, cnt, steps, ...) in place.

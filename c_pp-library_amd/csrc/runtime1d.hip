@@ -23,7 +23,7 @@ __global__ void k1_solve(Inst1D* insts, int n_inst, const int* order, int* qctr,
 // 4 (the scalar path) have their own.
 static void launch_solve(bool hbm_masks, int flux, dim3 grid, dim3 block, size_t lds, hipStream_t stream, Inst1D* insts,
                          int n_inst, const int* order, int* qctr, int mode, int max_steps, int nxp, uint8_t* hbm) {
-    if ((mode & ~kModeLdsState) >= 3) {
+    if ((mode & ~(kModeLdsState | kModeLdsQ)) >= 3) {
         if (hbm_masks)
             hipLaunchKernelGGL((k1_solve<true, AWCM_FLUX_WENOZ, true>), grid, block, lds, stream, insts, n_inst, order,
                                qctr, mode, max_steps, nxp, hbm);
@@ -196,6 +196,10 @@ int Batch1D::launch(int mode, int max_steps, int first, int count) {
     if (!hbm && count == 1 && lds_state && lds_state_offset(nxp) + lds_state_bytes(nxp) <= lds_cap) {
         mode |= kModeLdsState;
         lds = lds_state_offset(nxp) + lds_state_bytes(nxp);
+    } else if (!hbm && count == 1 && lds_state && lds_state_offset(nxp) + lds_q_bytes(nxp) <= lds_cap) {
+        // too long for all of it: Q alone (C5's 4097 points: 143 KB)
+        mode |= kModeLdsQ;
+        lds = lds_state_offset(nxp) + lds_q_bytes(nxp);
     }
     launch_solve(hbm != nullptr, flux, dim3(wgs), dim3(threads), lds, stream, d_inst + first, count,
                  d_order_for(first, count), d_qctr, mode, max_steps, nxp, hbm);

@@ -148,6 +148,51 @@ def test_1d_weno5_flux_matches_oracle():
         assert_same(g.download_q(), o.get_q(), mg, mo, lg, lo, f"driver configuration, {50 * (chunk + 1)} steps")
 
 
+def test_1d_state_in_lds_matches_oracle():
+    """Single instances too long for their whole state in one workgroup's LDS
+    but whose masks and Q fit (2049 and 4097 points): Q lives in LDS for the
+    run.  The full Shu-Osher run (9 levels, 4097 points, eps 1e-4, CFL 0.5,
+    t = 1.8, WENO-Z pair, checked at t = 0.9 and 1.8) and a full Sod run at
+    8 levels (2049 points, WENO5-JS): masks, levels, step counts, t and
+    fields bit-identical to the oracle."""
+    P = _pyawcm()
+    q, dx, _ = O.ic1(1, 4097, -5.0, 5.0, -4.0, O.SHU_L, O.SHU_L)
+    pts = O.print_times(1.8, 3)
+    g = P.Solver(1, 9, 17, 1.4, dx)
+    g.set_schedule(pts)
+    g.upload_q(q)
+    o = O.Solver1D("orc", 9, 17, 1.4, dx)
+    o.set_q(q)
+    for tf in (0.9, 1.8):
+        steps, _ = g.run(1e-4, 0.5, tf, 1 << 20)
+        n = 0
+        while o.t[0] < tf:
+            o.step(1e-4, 0.5, pts)
+            n += 1
+        assert steps == n and g.t == o.t[0] == tf
+        mg, lg = g.masks()
+        mo, lo = o.masks()
+        assert_same(g.download_q(), o.get_q(), mg, mo, lg, lo, f"Shu-Osher 4097 points t={tf}")
+    q, dx, _ = O.ic1(0, 2049, 0.0, 1.0, 0.5, O.SOD_L, O.SOD_R)
+    pts = O.print_times(0.2, 41)
+    g = P.Solver(1, 8, 17, 1.4, dx)
+    g.set_flux(P.FLUX_WENO5)
+    g.set_schedule(pts)
+    g.upload_q(q)
+    steps, _ = g.run(1e-4, 0.5, 0.2, 100000)
+    o = O.Solver1D("orc", 8, 17, 1.4, dx)
+    o.set_flux(1)
+    o.set_q(q)
+    n = 0
+    while o.t[0] < 0.2:
+        o.step(1e-4, 0.5, pts)
+        n += 1
+    assert steps == n and g.t == o.t[0] == 0.2
+    mg, lg = g.masks()
+    mo, lo = o.masks()
+    assert_same(g.download_q(), o.get_q(), mg, mo, lg, lo, "Sod 2049 points WENO5-JS t=0.2")
+
+
 def test_1d_largest_instance_matches_oracle():
     """The largest 1-D instance the C ABI accepts with the most levels: 14
     levels over 8 coarse points, 57345 finest points (list entries pack the
