@@ -49,21 +49,30 @@ Adj1DArgs args_of(const Adj1DHost& H) {
 }
 
 // k1_adjoint keeps the masks in LDS only: an instance whose 11 * nxp bytes do
-// not fit one workgroup's LDS next to the kernel's static LDS is refused.
-int adjoint_fits(int nxp) {
+// not fit one workgroup's LDS next to the kernel's static LDS is refused.  When
+// the work state fits beside them as well, it lives there too (*lq set, *lds
+// the dynamic LDS bytes).
+int adjoint_fits(int nxp, int* lq = nullptr, size_t* lds = nullptr) {
     int dev = 0, cap = 0;
     hipFuncAttributes fa;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cap, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
         hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k1_adjoint)) != hipSuccess)
         return AWCM_EDEVICE;
-    return size_t(11) * nxp + fa.sharedSizeBytes <= size_t(cap) ? AWCM_OK : AWCM_EINVAL;
+    if (size_t(11) * nxp + fa.sharedSizeBytes > size_t(cap)) return AWCM_EINVAL;
+    const size_t with_q = lds_state_offset(nxp) + lds_q_bytes(nxp);
+    const bool q = with_q + fa.sharedSizeBytes <= size_t(cap);
+    if (lq) *lq = q ? 1 : 0;
+    if (lds) *lds = q ? with_q : size_t(11) * nxp;
+    return AWCM_OK;
 }
 
 int launch_adjoint(Batch1D& B, const Adj1DHost& H, int mode) {
     const int nxp = nxp_of(B.h[0]);
-    if (int st = adjoint_fits(nxp)) return st;
-    hipLaunchKernelGGL(k1_adjoint, dim3(1), dim3(1024), size_t(11) * nxp, B.stream, B.d_inst, args_of(H), mode, nxp);
+    int lq = 0;
+    size_t lds = 0;
+    if (int st = adjoint_fits(nxp, &lq, &lds)) return st;
+    hipLaunchKernelGGL(k1_adjoint, dim3(1), dim3(1024), lds, B.stream, B.d_inst, args_of(H), mode, nxp, lq);
     if (hipGetLastError() != hipSuccess) return AWCM_EDEVICE;
     int st = B.pull();
     if (!st && B.h[0].status) st = B.h[0].status == -5 ? AWCM_ERANGE : AWCM_EDEVICE;
@@ -135,7 +144,9 @@ int awcm_adjoint(awcm_solver* h, const double* dJ_dQfinal, int checkpoint_every,
     awcm::Inst1D& in = B.h[0];
     const size_t NP = size_t(in.Nx) + 2 * awcm::G;
     const int nxp = awcm::nxp_of(in);
-    int st = awcm::adjoint_fits(nxp);
+    int lq = 0;
+    size_t lds = 0;
+    int st = awcm::adjoint_fits(nxp, &lq, &lds);
     if (st) return st;
     if (!H.Qb) {
         st = H.dalloc(&H.Qb, sizeof(double) * 3 * NP);
@@ -174,7 +185,7 @@ int awcm_adjoint(awcm_solver* h, const double* dJ_dQfinal, int checkpoint_every,
         const awcm::Adj1DArgs A = awcm::args_of(H);
         hipStream_t stream = B.stream;
         if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return AWCM_EDEVICE;
-        hipLaunchKernelGGL(awcm::k1_adjoint, dim3(1), dim3(1024), size_t(11) * nxp, stream, B.d_inst, A, 2, nxp);
+        hipLaunchKernelGGL(awcm::k1_adjoint, dim3(1), dim3(1024), lds, stream, B.d_inst, A, 2, nxp, lq);
         awcm::launch_reverse_step(R, stream);
         hipLaunchKernelGGL(awcm::k1r_next, dim3(1), dim3(64), 0, stream, H.d_step);
         if (hipStreamEndCapture(stream, &H.graph) != hipSuccess) return AWCM_EDEVICE;

@@ -111,8 +111,10 @@ __device__ void replay_step(const Ctx1& c, const Adj1DArgs& A, int n, double* X,
 //         last one), replay the segment from its checkpoint and keep each
 //         step's three stage inputs in A.seg (slot = step - segment start);
 //         otherwise return at once.  Q is used as work space.
-// One workgroup; dynamic LDS 11 * nxp bytes, nxp >= Nx + 6.
-__global__ void __launch_bounds__(1024) k1_adjoint(Inst1D* inst, Adj1DArgs A, int mode, int nxp) {
+// One workgroup; dynamic LDS 11 * nxp bytes, nxp >= Nx + 6, and with `lq` the
+// work state Q after them (lds_state_offset(nxp) + lds_q_bytes(nxp) bytes),
+// copied out to the instance's Q at the end.
+__global__ void __launch_bounds__(1024) k1_adjoint(Inst1D* inst, Adj1DArgs A, int mode, int nxp, int lq) {
     AWCM_DYNAMIC_LDS uint8_t sm[];
     __shared__ double W[4][5];
     __shared__ double red[16];
@@ -131,7 +133,8 @@ __global__ void __launch_bounds__(1024) k1_adjoint(Inst1D* inst, Adj1DArgs A, in
     c.g.L = in.L; c.g.maxlev = in.L - 1; c.g.Nx = in.Nx; c.g.nxc = in.nxc;
     c.gamma = in.gamma; c.dx = in.dx;
     c.NP = in.Nx + 2 * G;
-    c.Q = in.Q; c.Qc = in.Qc; c.Qt = in.Qt; c.F = in.F;
+    c.Q = lq ? reinterpret_cast<double*>(sm + lds_state_offset(nxp)) : in.Q;
+    c.Qc = in.Qc; c.Qt = in.Qt; c.F = in.F;
     c.mt = sm;
     for (int k = 0; k < 4; k++) c.m[k] = sm + (k + 1) * nxp;
     c.snap = sm + 5 * nxp;
@@ -141,6 +144,10 @@ __global__ void __launch_bounds__(1024) k1_adjoint(Inst1D* inst, Adj1DArgs A, in
     c.rec = nullptr;
     __syncthreads();
     if (mode == 0) {
+        if (lq) {
+            copy_planes(c, c.Q, in.Q);
+            __syncthreads();
+        }
         for (int n = 0; n < A.n_steps; n++) replay_step(c, A, n, nullptr, sc);
     } else {
         const long long SX = 9LL * c.NP;  // doubles per step in the segment buffer
@@ -149,6 +156,7 @@ __global__ void __launch_bounds__(1024) k1_adjoint(Inst1D* inst, Adj1DArgs A, in
         for (int n = n0; n < n1; n++) replay_step(c, A, n, A.seg + (n - n0) * SX, sc);
     }
     __syncthreads();
+    if (lq) copy_planes(c, in.Q, c.Q);
     if (threadIdx.x == 0 && s_flag) in.status = -5;
 }
 

@@ -75,10 +75,21 @@ void fill_new(std::vector<T>& v, size_t n, T zero) {
 bool masks_in_hbm = false;
 int stage_flux = kRecWenoZ;  // face fluxes of the stages (kRecWenoZ / kRecWeno5)
 
+// a workgroup's LDS beside the kernels' static LDS (a few hundred bytes; the
+// device runtime asks for the exact figure): the single-instance launches put
+// the whole state, or Q alone, in LDS when it fits, as Batch1D::launch does
+constexpr size_t kEmuLdsCap = 163840 - 512;
+int lds_mode(int nxp) {
+    if (lds_state_offset(nxp) + lds_state_bytes(nxp) <= kEmuLdsCap) return kModeLdsState;
+    if (lds_state_offset(nxp) + lds_q_bytes(nxp) <= kEmuLdsCap) return kModeLdsQ;
+    return 0;
+}
+
 template <int FLUX>
 void launch_solve_f(Emu& e, int mode, int max_steps) {
     int order = 0, qctr = 0;
     poison_lds();
+    if (!masks_in_hbm) mode |= lds_mode(e.nxp());
     if (masks_in_hbm) {
         std::vector<uint8_t> hbm(size_t(11) * e.nxp(), 0xA5);
         emu_launch(EMU_THREADS,
@@ -200,7 +211,8 @@ void emu_replay(void* h) {
     auto* e = static_cast<Emu*>(h);
     const Adj1DArgs A = args(*e, e->K);
     poison_lds();
-    emu_launch(EMU_THREADS, [&] { k1_adjoint(&e->in, A, 0, e->nxp()); });
+    const int lq = lds_state_offset(e->nxp()) + lds_q_bytes(e->nxp()) <= kEmuLdsCap ? 1 : 0;
+    emu_launch(EMU_THREADS, [&] { k1_adjoint(&e->in, A, 0, e->nxp(), lq); });
 }
 
 // awcm_adjoint's sequence: for every step from the last, the segment
@@ -223,9 +235,10 @@ void emu_adjoint(void* h, const double* gf, int seg_steps, double* g0) {
     R.seg = e->seg.data();
     R.Qb = e->Qb.data(); R.Xb = e->Xb.data(); R.Cb = e->Cb.data(); R.Gb = e->Gb.data(); R.FV = e->FV.data();
     R.step = &e->step;
+    const int lq = lds_state_offset(e->nxp()) + lds_q_bytes(e->nxp()) <= kEmuLdsCap ? 1 : 0;
     for (e->step = e->steps - 1; e->step >= 0;) {
         poison_lds();
-        emu_launch(EMU_THREADS, [&] { k1_adjoint(&e->in, A, 2, e->nxp()); });
+        emu_launch(EMU_THREADS, [&] { k1_adjoint(&e->in, A, 2, e->nxp(), lq); });
         launch_reverse_step(R, nullptr);
         AWCM_REV_LAUNCH(k1r_next, 1, &e->step);
     }
