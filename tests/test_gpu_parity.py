@@ -601,14 +601,15 @@ def test_1d_interpolate_euler_matches_oracle():
         assert np.array_equal(mg2, mg) and np.array_equal(lg2, lg), f"InterpolateEuler {name} changed the masks"
 
 
-@pytest.mark.parametrize("L,nxc", [(7, 17), (11, 16)])
+@pytest.mark.parametrize("L,nxc", [(7, 17), (8, 17), (11, 16)])
 def test_1d_scalar_path_matches_oracle(L, nxc):
     """The 1-D scalar path through the C ABI: AWCM_1D::Prepare (awcm_prepare)
     over four rounds with the field moved between them, then
     WAVELET_1D::Interpolate (awcm_interpolate; the one-mask form is `to` =
     every point) from/to five mask pairs; the instance's own masks are the
     same afterwards.  Masks, levels and fields bit-identical to the oracle.
-    The second grid (15361 points) keeps its masks in HBM."""
+    The grids place the state differently: 1025 points keep all of it in LDS,
+    2049 points the field alone, 15361 points keep even the masks in HBM."""
     P = _pyawcm()
     nx = (nxc - 1) * 2 ** (L - 1) + 1
     x = (np.arange(-3, nx + 3) + 0.5) / nx
