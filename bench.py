@@ -23,7 +23,9 @@ graph.  The run:
 4. the batched 1-D form, the adjoint and the scalar box-advection driver's
    loop (main_2d_advect.cpp) on the device;
 5. the reference's CPU path (oracle/_ref, the reference's own headers) on a
-   bounded sample of the headline window, with all host threads and with one.
+   bounded sample of the headline window, with all host threads and with one,
+   and on every 8th instance of the batched workload (one host process per
+   core, one instance at a time each).
 
 value = active-point RK3 updates per second = sum over the timed steps of
 |mask0| / device time.  Prints one JSON line.
@@ -125,6 +127,55 @@ def cpu_baseline(snap, budget_s, threads):
                       f"({res['elapsed_s']:.1f} s wall, OpenMP {threads} thread(s), {res['prefix']} build)"}
 
 
+def c4_cpu_instance(k):
+    """Instance k of the batched workload run to t = 0.15 on the host by the
+    reference build (or the oracle port), the way make_digests.py runs it."""
+    import oracle_lib as O
+    import workloads as W
+    L, rl, rr, pl, pr, u, x0, eps = W.shock_tube_instances(512)[k]
+    nx = 16 * 2 ** (L - 1) + 1
+    q, dx, _ = O.ic1(0, nx, 0.0, 1.0, x0, (rl, u, pl), (rr, u, pr))
+    s = O.Solver1D("ref" if O.have_ref() else "orc", L, 17, 1.4, dx)
+    s.set_q(q)
+    pts = O.print_times(W.C4_T_FINAL, W.C4_NUM_PRINT)
+    steps = upd = 0
+    while s.t[0] < W.C4_T_FINAL and np.isfinite(s.t[0]) and steps < 100000:
+        s.step(eps, W.C4_CFL, pts)
+        steps += 1
+        m, _ = s.masks()
+        upd += int(m[1, 3:-3].sum())
+    return k, steps, upd
+
+
+def c4_cpu_sample(stride, workers):
+    """The sampled instances (every stride-th, from instance 3) on `workers`
+    host processes, one instance at a time each; wall time of the whole pool."""
+    import multiprocessing as mp
+    import oracle_lib as O
+    ks = list(range(3, 512, stride))
+    t0 = time.perf_counter()
+    with mp.get_context("fork").Pool(workers) as pool:
+        res = pool.map(c4_cpu_instance, ks, chunksize=1)
+    wall = time.perf_counter() - t0
+    return {"instances": len(ks), "stride": stride, "wall_s": wall, "steps": sum(r[1] for r in res),
+            "updates": sum(r[2] for r in res), "kind": "reference" if O.have_ref() else "port"}
+
+
+def c4_cpu_baseline(workers, stride=8):
+    """c4_cpu_sample in a child process that never touches the GPU (one OpenMP
+    thread per worker process)."""
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--c4-cpu-sample", str(stride),
+                        "--c4-cpu-workers", str(workers)], capture_output=True, text=True, env=env, timeout=1200)
+    if r.returncode != 0:
+        return {"error": r.stderr[-400:]}
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    return {"value": res["updates"] / res["wall_s"], "unit": "updates/s", "cores": workers, "kind": res["kind"],
+            "sample": f"{res['instances']} of the 512 instances (every {stride}th from instance 3) run to t = 0.15 "
+                      f"on {workers} host processes, one instance each at a time: {res['steps']} steps, "
+                      f"{res['updates']} updates in {res['wall_s']:.1f} s wall"}
+
+
 def hbm_copy():
     """A plain device copy of known size, for scaling the FETCH_SIZE/WRITE_SIZE
     counters: 1 GiB filled, then copied device to device, three times."""
@@ -139,10 +190,11 @@ def hbm_copy():
     print(json.dumps({"hbm_copy_bytes": n * 8, "copies": 3}))
 
 
-def batched_section(P, cmp_steps):
+def batched_section(P, cmp_steps, cpu_workers=0):
     """512 seeded 1-D shock tubes (uneven 513..4097-point grids, eps 1e-3/1e-4,
     CFL 0.5, t = 0.15) in one graph-replayed persistent launch with the
-    autotuned shape, and the same work as one plain launch per instance."""
+    autotuned shape, the same work as one plain launch per instance, and (with
+    cpu_workers) a sample of the instances on the host's cores."""
     import workloads as W
     b, inst, params, _ = W.shock_tube_batch(512)
     threads, wgs, probe = b.autotune(10)
@@ -165,7 +217,7 @@ def batched_section(P, cmp_steps):
     bs.run_sequential(cmp_steps)
     seq_s = time.perf_counter() - t0
     bs.close()
-    return {
+    out = {
         "workload": "512 seeded 1-D shock tubes (splitmix64 seed 0x5EED0C4), 513..4097 finest points, "
                     "eps 1e-3/1e-4, WENO-Z5+Roe, SSP-RK3 CFL 0.5 to t=0.15, FP64",
         "launch": {"threads": threads, "workgroups": wgs or 512, "probe_ms": [round(x, 3) for x in probe]},
@@ -177,6 +229,12 @@ def batched_section(P, cmp_steps):
         "vs_sequential": {"steps_per_instance": cmp_steps, "batched_s": bat_s, "sequential_s": seq_s,
                           "speedup": seq_s / bat_s},
     }
+    if cpu_workers:
+        cb = c4_cpu_baseline(cpu_workers)
+        out["cpu_baseline"] = cb
+        if "value" in cb and cb["value"] > 0:
+            out["gpu_over_cpu"] = out["updates_per_s"] / cb["value"]
+    return out
 
 
 def adjoint_section(P):
@@ -425,6 +483,8 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=40,
                     help="steps per instance of the batched-vs-sequential comparison")
     ap.add_argument("--cpu-sample", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--c4-cpu-sample", type=int, default=0, help=argparse.SUPPRESS)
+    ap.add_argument("--c4-cpu-workers", type=int, default=1, help=argparse.SUPPRESS)
     ap.add_argument("--hbm-copy", action="store_true")
     ap.add_argument("--save-snapshot", default=None,
                     help="with --only T: advance to T, save the device state to this .npz and exit")
@@ -433,6 +493,9 @@ def main():
     args = ap.parse_args()
     if args.cpu_sample:
         print(json.dumps(cpu_sample(args.cpu_sample, args.cpu_budget)))
+        return
+    if args.c4_cpu_sample:
+        print(json.dumps(c4_cpu_sample(args.c4_cpu_sample, args.c4_cpu_workers)))
         return
     if args.hbm_copy:
         hbm_copy()
@@ -504,7 +567,8 @@ def main():
     if not args.no_c2 and args.only is None:
         out["quadrants"] = quadrants_section(P, args.steps, args.warmup)
     if not args.no_batch and args.only is None:
-        out["batched"] = batched_section(P, args.batch_steps)
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        out["batched"] = batched_section(P, args.batch_steps, 0 if args.no_cpu else threads)
         out["adjoint"] = adjoint_section(P)
         out["scalar_advect"] = advect_section(P, cpu=not args.no_cpu)
     if not args.no_cpu:
