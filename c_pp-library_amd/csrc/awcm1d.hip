@@ -4,7 +4,9 @@
 // level fields live in LDS for the whole launch (in HBM, workgroup-private, for
 // instances too long for one workgroup's LDS); the conserved state Q, the RK
 // copies and the face-flux buffer live in HBM (planes of Nx+6 doubles, SoA) and
-// stay L2-resident at these sizes.  A launch runs the reference's time loop
+// stay L2-resident at these sizes.  A launch of a single instance copies them
+// into LDS beside the masks for the launch when they fit (kModeLdsState), or
+// Q alone (kModeLdsQ).  A launch runs the reference's time loop
 // (Wavelet/main_1d_euler.cpp:61-260, with the WENO-Z flux pair of the C1
 // configuration or, FLUX = kRecWeno5, the shipped driver's WENO5-JS ComputedF)
 // to t_final or a step budget without returning to the host,
