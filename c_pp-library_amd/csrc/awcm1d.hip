@@ -6,7 +6,8 @@
 // copies and the face-flux buffer live in HBM (planes of Nx+6 doubles, SoA) and
 // stay L2-resident at these sizes.  A launch of a single instance copies them
 // into LDS beside the masks for the launch when they fit (kModeLdsState), or
-// Q alone (kModeLdsQ).  A launch runs the reference's time loop
+// Q alone (kModeLdsQ); a batched launch of one workgroup per CU does the same
+// instance by instance (kModeLdsAuto).  A launch runs the reference's time loop
 // (Wavelet/main_1d_euler.cpp:61-260, with the WENO-Z flux pair of the C1
 // configuration or, FLUX = kRecWeno5, the shipped driver's WENO5-JS ComputedF)
 // to t_final or a step budget without returning to the host,
@@ -47,9 +48,9 @@ __global__ void __launch_bounds__(1024) k1_solve(Inst1D* insts, int n_inst, cons
                                                  int max_steps, int nxp, uint8_t* hbm) {
     AWCM_DYNAMIC_LDS __align__(16) uint8_t lds_masks[];
     uint8_t* const sm = kHbm ? hbm + size_t(blockIdx.x) * 11 * size_t(nxp) : lds_masks;
-    const bool lstate = !kHbm && (mode & kModeLdsState) != 0;
-    const bool lq = !kHbm && !lstate && (mode & kModeLdsQ) != 0;
-    mode &= ~(kModeLdsState | kModeLdsQ);
+    const int lflags = mode & (kModeLdsState | kModeLdsQ | kModeLdsAuto);
+    const size_t lbudget = size_t(unsigned(mode) >> 16) * 16;
+    mode &= kModeBase;
     __shared__ double W[4][5];
     __shared__ double red[16];
     __shared__ int s_idx, s_flag;
@@ -68,15 +69,26 @@ __global__ void __launch_bounds__(1024) k1_solve(Inst1D* insts, int n_inst, cons
         c.NP = in.Nx + 2 * G;
         c.Q = in.Q; c.Qc = in.Qc; c.Qt = in.Qt; c.F = in.F;
         const int nq = 3 * c.NP, nf = 3 * (in.Nx + 1);
+        // where this instance's state lives for the launch (planes ps apart)
+        bool lstate = false, lq = false;
+        int ps = nxp;
+        if (!kHbm && (lflags & kModeLdsAuto)) {
+            ps = c.NP;
+            lstate = lds_state_offset(nxp) + lds_state_bytes(ps) <= lbudget;
+            lq = !lstate && lds_state_offset(nxp) + lds_q_bytes(ps) <= lbudget;
+        } else if (!kHbm) {
+            lstate = (lflags & kModeLdsState) != 0;
+            lq = !lstate && (lflags & kModeLdsQ) != 0;
+        }
         if (lstate) {
             double* ls = reinterpret_cast<double*>(lds_masks + lds_state_offset(nxp));
             for (int k = threadIdx.x; k < nq; k += blockDim.x) {
                 ls[k] = in.Q[k];
-                ls[3 * nxp + k] = in.Qc[k];
-                ls[6 * nxp + k] = in.Qt[k];
+                ls[3 * ps + k] = in.Qc[k];
+                ls[6 * ps + k] = in.Qt[k];
             }
-            for (int k = threadIdx.x; k < nf; k += blockDim.x) ls[9 * nxp + k] = in.F[k];
-            c.Q = ls; c.Qc = ls + 3 * nxp; c.Qt = ls + 6 * nxp; c.F = ls + 9 * nxp;
+            for (int k = threadIdx.x; k < nf; k += blockDim.x) ls[9 * ps + k] = in.F[k];
+            c.Q = ls; c.Qc = ls + 3 * ps; c.Qt = ls + 6 * ps; c.F = ls + 9 * ps;
         } else if (lq) {
             double* ls = reinterpret_cast<double*>(lds_masks + lds_state_offset(nxp));
             for (int k = threadIdx.x; k < nq; k += blockDim.x) ls[k] = in.Q[k];

@@ -21,6 +21,12 @@ constexpr size_t lds_state_offset(int nxp) { return (size_t(11) * nxp + 15) / 16
 // HBM), copied in at the start and out at the end.
 constexpr int kModeLdsQ = 0x200;
 constexpr size_t lds_q_bytes(int nxp) { return sizeof(double) * 3 * size_t(nxp); }
+// k1_solve mode flag for launches of one workgroup per CU, whose dynamic LDS
+// is the whole budget (mode >> 16, in units of 16 bytes): each instance in
+// turn keeps its state (planes of its own Nx + 6 doubles) or Q alone in the
+// LDS beside the masks when it fits there.  The mode proper is mode & 0xff.
+constexpr int kModeLdsAuto = 0x400;
+constexpr int kModeBase = 0xff;
 
 // One independent 1-D solve.  Device-resident; the kernel updates the
 // time-loop fields (t, cnt, steps, ...) in place.

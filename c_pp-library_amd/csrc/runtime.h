@@ -38,6 +38,7 @@ struct Batch1D {
     // 11 * nxp-byte slice per workgroup in HBM (grown on demand, never shrunk)
     size_t lds_cap = 0;           // dynamic LDS one k1_solve workgroup may use
     bool lds_state = true;        // single-instance launches keep the state in LDS when it fits
+    int ncu = 0;                  // compute units of the device
     uint8_t* d_hbm_masks = nullptr;
     size_t hbm_masks_bytes = 0;
     // device snapshot of every instance's state (autotuner probes)
@@ -52,6 +53,7 @@ struct Batch1D {
     int pull();
     int launch(int mode, int max_steps, int first, int count);
     int launch_graph(int max_steps);
+    int lds_auto(int mode, int wgs, size_t* lds) const;
     const int* d_order_for(int first, int count);
     int mask_space(int wgs, int nxp, uint8_t** hbm);
     int check_status(bool nonfinite_ok = false);

@@ -23,7 +23,7 @@ __global__ void k1_solve(Inst1D* insts, int n_inst, const int* order, int* qctr,
 // 4 (the scalar path) have their own.
 static void launch_solve(bool hbm_masks, int flux, dim3 grid, dim3 block, size_t lds, hipStream_t stream, Inst1D* insts,
                          int n_inst, const int* order, int* qctr, int mode, int max_steps, int nxp, uint8_t* hbm) {
-    if ((mode & ~(kModeLdsState | kModeLdsQ)) >= 3) {
+    if ((mode & kModeBase) >= 3) {
         if (hbm_masks)
             hipLaunchKernelGGL((k1_solve<true, AWCM_FLUX_WENOZ, true>), grid, block, lds, stream, insts, n_inst, order,
                                qctr, mode, max_steps, nxp, hbm);
@@ -87,6 +87,7 @@ int Batch1D::init(int n_, const awcm_instance_1d* params, const double* pt, int 
             hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k1_solve<false, AWCM_FLUX_WENOZ, false>)) != hipSuccess)
             return AWCM_EDEVICE;
         lds_cap = size_t(cap) > fa.sharedSizeBytes ? size_t(cap) - fa.sharedSizeBytes : 0;
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return AWCM_EDEVICE;
     }
     h.resize(n);
     max_nx = 0;
@@ -184,6 +185,16 @@ int Batch1D::mask_space(int wgs, int nxp, uint8_t** hbm) {
     return AWCM_OK;
 }
 
+// A launch of at most one workgroup per CU: its workgroups take the whole LDS
+// budget (a CU holds one of them either way), and each instance keeps its
+// state or Q in the part its masks leave (kModeLdsAuto).  Otherwise the
+// masks alone.
+int Batch1D::lds_auto(int mode, int wgs, size_t* lds) const {
+    if (!lds_state || wgs > ncu || lds_cap <= *lds) return mode;
+    *lds = lds_cap / 16 * 16;
+    return mode | kModeLdsAuto | int((*lds / 16) << 16);
+}
+
 int Batch1D::launch(int mode, int max_steps, int first, int count) {
     const int nxp = ((max_nx + 2 * G) + 15) / 16 * 16;
     const int wgs = workgroups > 0 ? std::min(workgroups, count) : count;
@@ -200,6 +211,8 @@ int Batch1D::launch(int mode, int max_steps, int first, int count) {
         // too long for all of it: Q alone (C5's 4097 points: 143 KB)
         mode |= kModeLdsQ;
         lds = lds_state_offset(nxp) + lds_q_bytes(nxp);
+    } else if (!hbm && count > 1) {
+        mode = lds_auto(mode, wgs, &lds);
     }
     launch_solve(hbm != nullptr, flux, dim3(wgs), dim3(threads), lds, stream, d_inst + first, count,
                  d_order_for(first, count), d_qctr, mode, max_steps, nxp, hbm);
@@ -219,8 +232,10 @@ int Batch1D::launch_graph(int max_steps) {
         if (graph) { hipGraphDestroy(graph); graph = nullptr; }
         if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return AWCM_EDEVICE;
         hipLaunchKernelGGL(k1_reset_counter, dim3(1), dim3(64), 0, stream, d_qctr);
-        launch_solve(hbm != nullptr, flux, dim3(wgs), dim3(threads), hbm ? 0 : lds_bytes(nxp), stream, d_inst, n,
-                     d_order, d_qctr, 1, max_steps, nxp, hbm);
+        size_t lds = hbm ? 0 : lds_bytes(nxp);
+        const int mode = hbm ? 1 : lds_auto(1, wgs, &lds);
+        launch_solve(hbm != nullptr, flux, dim3(wgs), dim3(threads), lds, stream, d_inst, n, d_order, d_qctr, mode,
+                     max_steps, nxp, hbm);
         if (hipStreamEndCapture(stream, &graph) != hipSuccess) return AWCM_EDEVICE;
         if (hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0) != hipSuccess) return AWCM_EDEVICE;
         g_threads = threads;
