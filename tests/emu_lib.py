@@ -36,6 +36,7 @@ def build(out, threads=1, extra=()):
     E.emu_adjoint.argtypes = [V, _dp, C.c_int, _dp]
     E.emu_status.argtypes = [V]
     E.emu_set_masks_in_hbm.argtypes = [C.c_int]
+    E.emu_set_lds_auto.argtypes = [C.c_int]
     E.emu_set_poison.argtypes = [C.c_int]
     E.emu_check_weights.restype = C.c_int
     return E

@@ -73,6 +73,9 @@ void fill_new(std::vector<T>& v, size_t n, T zero) {
 // with masks_in_hbm, k1_solve<true> keeps the masks in a heap slice instead
 // of the emulated LDS (the path of instances too long for one workgroup's LDS)
 bool masks_in_hbm = false;
+// with lds_auto, the launch is a batched one of one workgroup per CU: the whole
+// budget, and the kernel places each instance's state itself (kModeLdsAuto)
+bool lds_auto = false;
 int stage_flux = kRecWenoZ;  // face fluxes of the stages (kRecWenoZ / kRecWeno5)
 
 // a workgroup's LDS beside the kernels' static LDS (a few hundred bytes; the
@@ -89,7 +92,7 @@ template <int FLUX>
 void launch_solve_f(Emu& e, int mode, int max_steps) {
     int order = 0, qctr = 0;
     poison_lds();
-    if (!masks_in_hbm) mode |= lds_mode(e.nxp());
+    if (!masks_in_hbm) mode |= lds_auto ? kModeLdsAuto | int((kEmuLdsCap / 16) << 16) : lds_mode(e.nxp());
     if (masks_in_hbm) {
         std::vector<uint8_t> hbm(size_t(11) * e.nxp(), 0xA5);
         emu_launch(EMU_THREADS,
@@ -109,6 +112,7 @@ void launch_solve(Emu& e, int mode, int max_steps) {
 extern "C" {
 
 void emu_set_masks_in_hbm(int on) { masks_in_hbm = on != 0; }
+void emu_set_lds_auto(int on) { lds_auto = on != 0; }
 void emu_set_flux(int kind) { stage_flux = kind == 1 ? kRecWeno5 : kRecWenoZ; }
 
 void* emu_create(int L, int nxc, double gamma, double dx, const double* pt, int npt) {

@@ -216,6 +216,36 @@ def test_emulated_solve_with_q_in_lds_matches_oracle(emu, tmp_path):
     assert np.array_equal(qe.view(np.uint64), o.get_q().view(np.uint64))
 
 
+@pytest.mark.parametrize("L", [6, 8])
+def test_emulated_solve_with_per_instance_lds_placement(emu, L):
+    """The batched launch of one workgroup per CU (kModeLdsAuto): the kernel
+    places each instance's state itself, in planes of the instance's own
+    length -- all of it for 513 points, Q alone for 2049 -- with every buffer
+    and the LDS poisoned.  12 Sod steps bit-identical to the oracle."""
+    emu.emu_set_poison(1)
+    emu.emu_set_lds_auto(1)
+    T = 0.2
+    nx = 16 * 2 ** (L - 1) + 1
+    q, dx, _ = O.ic1(0, nx, 0.0, 1.0, 0.5, O.SOD_L, O.SOD_R)
+    pts = O.print_times(T, 3)
+    h = emu.emu_create(L, 17, 1.4, dx, _d(pts), len(pts))
+    try:
+        emu.emu_set_q(h, _d(q))
+        n = emu.emu_record(h, 1e-4, 0.5, T, 12, 4)
+        qe = np.zeros_like(q)
+        emu.emu_get_q(h, _d(qe))
+        assert emu.emu_status(h) == 0 and n == 12
+    finally:
+        emu.emu_destroy(h)
+        emu.emu_set_lds_auto(0)
+        emu.emu_set_poison(0)
+    o = O.Solver1D("orc", L, 17, 1.4, dx)
+    o.set_q(q)
+    for _ in range(n):
+        o.step(1e-4, 0.5, pts)
+    assert np.array_equal(qe.view(np.uint64), o.get_q().view(np.uint64))
+
+
 @pytest.mark.parametrize("poison", [0, 1])
 def test_emulated_kernels_forward_replay_and_adjoint(emu, poison):
     """Shu-Osher at 513 points to t = 0.06 (about 30 steps).  With poison, every
