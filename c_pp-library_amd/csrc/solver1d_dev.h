@@ -18,22 +18,25 @@ namespace {
 constexpr int G = kGhost;
 
 // MRA_GRID_INT_1D geometry (MRA_GRID_UNIFORM_1D.h:70-98) for one instance.
+// The reference's integer ranges divide by the point counts of a level; with
+// Nx - 1 = (nxc - 1) 2^(L-1) those quotients are exact powers of two, so the
+// spacing of grid[lev] and grid_x[lev] is 2^(L-1-lev) and a range of n
+// points spans (n - 1) << (L-1-lev): shifts, not integer divisions.
 struct Geo1 {
     int L, maxlev, Nx, nxc;
+    __device__ int shl(int lev) const { return L - 1 - lev; }
     __device__ int dxl(int lev) const { return 1 << (L - 1 - lev); }
     // grid_x[lev]: odd points of level lev+1
     __device__ void gridx(int lev, int& s, int& e, int& d) const {
         s = dxl(lev + 1);
         e = (Nx - 1) - dxl(lev + 1);
-        const int res = (Nx - 1) / (1 << (L - 1 - lev));
-        d = (e - s) / (res - 1);
+        d = dxl(lev);
     }
     // grid[lev]
     __device__ void grid(int lev, int& s, int& e, int& d) const {
         s = 0;
         e = Nx - 1;
-        const int res = (lev == L - 1) ? Nx : (nxc - 1) * (1 << lev) + 1;
-        d = (e - s) / (res - 1);
+        d = dxl(lev);
     }
 };
 
@@ -82,8 +85,8 @@ __device__ void fwt(const Ctx1& c, uint8_t* mask, double tol) {
     for (int lev = c.g.maxlev - 1; lev >= 0; lev--) {
         int s, e, d;
         c.g.gridx(lev, s, e, d);
-        const int dh = d / 2;
-        const int n = (e - s) / d + 1;
+        const int dh = d >> 1;
+        const int n = ((e - s) >> c.g.shl(lev)) + 1;
         for (int t = threadIdx.x; t < n; t += blockDim.x) {
             const int i = s + t * d;
             if (mask[i + G] == 0) continue;
@@ -99,7 +102,7 @@ __device__ void fwt(const Ctx1& c, uint8_t* mask, double tol) {
         __syncthreads();
         int gs, ge, gd;
         c.g.grid(lev, gs, ge, gd);
-        const int nu = (ge - gd - (gs + gd)) / gd + 1;
+        const int nu = ((ge - gd - (gs + gd)) >> c.g.shl(lev)) + 1;
         for (int t = threadIdx.x; t < nu; t += blockDim.x) {
             const int i = gs + gd + t * gd;
             if (mask[i + G] == 0) continue;
@@ -121,10 +124,10 @@ __device__ void iwt(const Ctx1& c, const uint8_t* mask) {
     for (int lev = 0; lev < c.g.maxlev; lev++) {
         int s, e, d;
         c.g.gridx(lev, s, e, d);
-        const int dh = d / 2;
+        const int dh = d >> 1;
         int gs, ge, gd;
         c.g.grid(lev, gs, ge, gd);
-        const int nu = (ge - gd - (gs + gd)) / gd + 1;
+        const int nu = ((ge - gd - (gs + gd)) >> c.g.shl(lev)) + 1;
         for (int t = threadIdx.x; t < nu; t += blockDim.x) {
             const int i = gs + gd + t * gd;
             if (mask[i + G] == 0) { q[i + G] = 0; continue; }
@@ -136,7 +139,7 @@ __device__ void iwt(const Ctx1& c, const uint8_t* mask) {
             q[i + G] = v;
         }
         __syncthreads();
-        const int n = (e - s) / d + 1;
+        const int n = ((e - s) >> c.g.shl(lev)) + 1;
         for (int t = threadIdx.x; t < n; t += blockDim.x) {
             const int i = s + t * d;
             if (mask[i + G] == 0) { q[i + G] = 0; continue; }
@@ -159,8 +162,8 @@ __device__ void check_recon(const Ctx1& c, uint8_t* mask) {
     for (int lev = c.g.maxlev - 1; lev >= 0; lev--) {
         int s, e, d;
         c.g.gridx(lev, s, e, d);
-        const int dh = d / 2;
-        const int n = (e - s) / d + 1;
+        const int dh = d >> 1;
+        const int n = ((e - s) >> c.g.shl(lev)) + 1;
         for (int t = threadIdx.x; t < n; t += blockDim.x) {
             const int i = s + t * d;
             if (mask[i + G] == 0) continue;
@@ -180,7 +183,7 @@ __device__ void adjacent(const Ctx1& c, uint8_t* mask) {
     {
         int s, e, d;
         c.g.gridx(c.g.maxlev - 1, s, e, d);
-        const int n = (e - s) / d + 1;
+        const int n = ((e - s) >> c.g.shl(c.g.maxlev - 1)) + 1;
         for (int t = threadIdx.x; t < n; t += blockDim.x) {
             const int i = s + t * d;
             if (c.snap[i + G]) { set1(c, mask, i - 1); set1(c, mask, i + 1); }
@@ -189,10 +192,10 @@ __device__ void adjacent(const Ctx1& c, uint8_t* mask) {
     for (int lev = c.g.maxlev - 2; lev >= 0; lev--) {
         int gs, ge, gd;
         c.g.grid(lev, gs, ge, gd);
-        const int dq = gd / 4;
+        const int dq = gd >> 2;
         int s, e, d;
         c.g.gridx(lev, s, e, d);
-        const int n = (e - s) / d + 1;
+        const int n = ((e - s) >> c.g.shl(lev)) + 1;
         for (int t = threadIdx.x; t < n; t += blockDim.x) {
             const int i = s + t * d;
             if (c.snap[i + G]) { set1(c, mask, i - dq); set1(c, mask, i + dq); }
@@ -258,8 +261,8 @@ __device__ void interp(const Ctx1& c, const uint8_t* from, const uint8_t* to) {
     for (int lev = 0; lev < c.g.maxlev; lev++) {
         int s, e, d;
         c.g.gridx(lev, s, e, d);
-        const int dh = d / 2;
-        const int n = (e - s) / d + 1;
+        const int dh = d >> 1;
+        const int n = ((e - s) >> c.g.shl(lev)) + 1;
         for (int t = threadIdx.x; t < n; t += blockDim.x) {
             const int i = s + t * d;
             if (c.tmp[i + G] == 0 || from[i + G] == 1) continue;
@@ -394,7 +397,7 @@ __device__ double lf_alpha(const Ctx1& c, const uint8_t* mask, const uint8_t* lv
         if (mask[i + G] == 0 || lvl[i] != c.g.maxlev) continue;
         double q[3];
         load3(c, c.Q, i, q);
-        const double u = q[1] / q[0];
+        const double u = Div(q[0])(q[1]);
         const double cs = eos_c1(c.gamma, q);
         a = t_max(a, t_abs(u) + cs);
     }
@@ -436,12 +439,12 @@ __device__ void rk_stage(const Ctx1& c, int stage, const uint8_t* mask, const ui
         } else {
             coarse_dF(c, i, d, dF);
         }
-        const double h = dt / (c.dx * d);
+        const double h = Div(c.dx * d)(dt);
         for (int k = 0; k < 3; k++) {
             double& q = c.Q[k * NP + i + G];
             if (stage == 0) q = q - h * dF[k];
             else if (stage == 1) q = 0.75 * c.Qc[k * NP + i + G] + 0.25 * (q - h * dF[k]);
-            else q = (c.Qc[k * NP + i + G] + 2.0 * (q - h * dF[k])) / 3.0;
+            else q = Div(3.0)(c.Qc[k * NP + i + G] + 2.0 * (q - h * dF[k]));
         }
     }
     __syncthreads();
@@ -498,7 +501,7 @@ __device__ void step(const Ctx1& c, LoopState& st, const double* print_times) {
         double q[3];
         load3(c, c.Q, i, q);
         const double cs = eos_c1(c.gamma, q);
-        const double u = q[1] / q[0];
+        const double u = Div(q[0])(q[1]);
         const double s = t_abs(u) + cs;
         // the reference's MAX drops a NaN depending on the scan order, so a
         // broken-down state (negative pressure, vacuum) is detected explicitly
@@ -512,7 +515,7 @@ __device__ void step(const Ctx1& c, LoopState& st, const double* print_times) {
         __syncthreads();
         return;
     }
-    double dt = st.cfl * c.dx / vmax;
+    double dt = Div(vmax)(st.cfl * c.dx);
     const double t = st.t;
     const int cnt = st.cnt;
     bool do_print = false;
