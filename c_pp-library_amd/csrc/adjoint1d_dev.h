@@ -51,7 +51,7 @@ __device__ __forceinline__ void even_stencil(int n, int N, int& m0, int& kc) {
     else m0 = n - 2, kc = 2;
 }
 
-// Coefficient of the flux at i + o d in dF at coarse point i (coarse_dF,
+// Coefficient of the flux at i + o d in dF at coarse point i (coarse_dF_at,
 // main_1d_euler.cpp:111-133, AWCM_1D.h:142-189); 0 if the branch skips it.
 __device__ __forceinline__ double pick5(int o, double m2, double m1, double z, double p1, double p2) {
     return o == -2 ? m2 : o == -1 ? m1 : o == 0 ? z : o == 1 ? p1 : p2;

@@ -349,45 +349,65 @@ __device__ void load3(const Ctx1& c, const double* Q, int i, double* o) {
     o[2] = Q[2 * c.NP + i + G];
 }
 
+// Three doubles returned in registers by the out-of-line stage functions
+// below.  Those two hold nearly all of a stage's arithmetic (the WENO face
+// fluxes and the coarse-point flux differences); inlined, their registers
+// push the rest of the solve kernel past the 128 VGPRs a 1024-thread
+// workgroup allows, and every phase then reloads spilled values from scratch
+// memory; called, they spill (if at all) only inside themselves.
+struct F3 { double v[3]; };
+
 // Flux derivative at a coarse point (main_1d_euler.cpp:111-133; AWCM_1D.h:142-189)
-__device__ void coarse_dF(const Ctx1& c, int i, int d, double* out) {
-    const double g = c.gamma;
-    const int ie = c.g.Nx - 1;
+// of the stage input Qt (3 planes of NP doubles); a stencil index outside the
+// ghosted range (which a closed mask never produces) is clamped and reported
+// through *flag.
+__device__ __attribute__((noinline)) F3 coarse_dF_at(const double* Qt, int NP, int Nx, double g, int i, int d,
+                                                     int* flag) {
+    const int ie = Nx - 1;
+    F3 r;
+    double* out = r.v;
+    auto load3 = [&](const double* Q, int k, double* o) {
+        if (k < -G || k > Nx - 1 + G) { *flag = 1; k = k < -G ? -G : Nx - 1 + G; }
+        o[0] = Q[k + G];
+        o[1] = Q[NP + k + G];
+        o[2] = Q[2 * NP + k + G];
+    };
     double q[3], fll[3], fl[3], fc[3], fr[3], frr[3];
     if (i - 2 * d > 0 && i + 2 * d < ie) {
-        load3(c, c.Qt, i - 2 * d, q); flux_f1(g, q, fll);
-        load3(c, c.Qt, i - d, q); flux_f1(g, q, fl);
-        load3(c, c.Qt, i + d, q); flux_f1(g, q, fr);
-        load3(c, c.Qt, i + 2 * d, q); flux_f1(g, q, frr);
+        load3(Qt, i - 2 * d, q); flux_f1(g, q, fll);
+        load3(Qt, i - d, q); flux_f1(g, q, fl);
+        load3(Qt, i + d, q); flux_f1(g, q, fr);
+        load3(Qt, i + 2 * d, q); flux_f1(g, q, frr);
         for (int p = 0; p < 3; p++) out[p] = kD1 * (fr[p] - fl[p]) + kD2 * (frr[p] - fll[p]);
     } else if (i - 2 * d < 0) {
-        load3(c, c.Qt, i - d, q); flux_f1(g, q, fl);
-        load3(c, c.Qt, i, q); flux_f1(g, q, fc);
-        load3(c, c.Qt, i + d, q); flux_f1(g, q, fr);
-        load3(c, c.Qt, i + 2 * d, q); flux_f1(g, q, frr);
+        load3(Qt, i - d, q); flux_f1(g, q, fl);
+        load3(Qt, i, q); flux_f1(g, q, fc);
+        load3(Qt, i + d, q); flux_f1(g, q, fr);
+        load3(Qt, i + 2 * d, q); flux_f1(g, q, frr);
         for (int p = 0; p < 3; p++) out[p] = kD01 * fl[p] + kD11 * fc[p] + kD21 * fr[p] + kD31 * frr[p];
     } else if (i == 2 * d) {
-        load3(c, c.Qt, i - 2 * d, q); flux_f1(g, q, fll);
-        load3(c, c.Qt, i - d, q); flux_f1(g, q, fl);
-        load3(c, c.Qt, i, q); flux_f1(g, q, fc);
-        load3(c, c.Qt, i + d, q); flux_f1(g, q, fr);
-        load3(c, c.Qt, i + 2 * d, q); flux_f1(g, q, frr);
+        load3(Qt, i - 2 * d, q); flux_f1(g, q, fll);
+        load3(Qt, i - d, q); flux_f1(g, q, fl);
+        load3(Qt, i, q); flux_f1(g, q, fc);
+        load3(Qt, i + d, q); flux_f1(g, q, fr);
+        load3(Qt, i + 2 * d, q); flux_f1(g, q, frr);
         for (int p = 0; p < 3; p++) out[p] = kD02 * fll[p] + kD12 * fl[p] + kD22 * fc[p] + kD32 * fr[p] + kD2 * frr[p];
     } else if (i > ie - 2 * d) {
         // AtRightBoundary(ll, l, c, r) = AtLeftBoundary(r, c, l, ll): mirrored, not negated
-        load3(c, c.Qt, i - 2 * d, q); flux_f1(g, q, fll);
-        load3(c, c.Qt, i - d, q); flux_f1(g, q, fl);
-        load3(c, c.Qt, i, q); flux_f1(g, q, fc);
-        load3(c, c.Qt, i + d, q); flux_f1(g, q, fr);
+        load3(Qt, i - 2 * d, q); flux_f1(g, q, fll);
+        load3(Qt, i - d, q); flux_f1(g, q, fl);
+        load3(Qt, i, q); flux_f1(g, q, fc);
+        load3(Qt, i + d, q); flux_f1(g, q, fr);
         for (int p = 0; p < 3; p++) out[p] = kD01 * fr[p] + kD11 * fc[p] + kD21 * fl[p] + kD31 * fll[p];
     } else {
-        load3(c, c.Qt, i - 2 * d, q); flux_f1(g, q, fll);
-        load3(c, c.Qt, i - d, q); flux_f1(g, q, fl);
-        load3(c, c.Qt, i, q); flux_f1(g, q, fc);
-        load3(c, c.Qt, i + d, q); flux_f1(g, q, fr);
-        load3(c, c.Qt, i + 2 * d, q); flux_f1(g, q, frr);
+        load3(Qt, i - 2 * d, q); flux_f1(g, q, fll);
+        load3(Qt, i - d, q); flux_f1(g, q, fl);
+        load3(Qt, i, q); flux_f1(g, q, fc);
+        load3(Qt, i + d, q); flux_f1(g, q, fr);
+        load3(Qt, i + 2 * d, q); flux_f1(g, q, frr);
         for (int p = 0; p < 3; p++) out[p] = kD02 * frr[p] + kD12 * fr[p] + kD22 * fc[p] + kD32 * fl[p] + kD2 * fll[p];
     }
+    return r;
 }
 
 // alpha = max |u|+c over {mask & level == max} of the current Q (main_1d_euler.cpp:86-95)
@@ -402,6 +422,19 @@ __device__ double lf_alpha(const Ctx1& c, const uint8_t* mask, const uint8_t* lv
         a = t_max(a, t_abs(u) + cs);
     }
     return block_max(a, c.red);
+}
+
+// The face flux f (between points f-1 and f) of the stage input Qt, whose six
+// states f-3 .. f+2 lie in the ghosted range for every f in 0..Nx (out of
+// line, as coarse_dF_at).
+template <int FLUX>
+__device__ __attribute__((noinline)) F3 face_at(const double* Qt, int NP, int f, double g, double alpha) {
+    double s[6][3];
+    for (int p = 0; p < 6; p++)
+        for (int k = 0; k < 3; k++) s[p][k] = Qt[k * NP + f - 3 + p + G];
+    F3 r;
+    face_flux1<FLUX>(g, s, alpha, r.v);
+    return r;
 }
 
 // One SSP-RK3 stage over `mask`/`lvl` (main_1d_euler.cpp:97-245).  stage 0:
@@ -422,11 +455,8 @@ __device__ void rk_stage(const Ctx1& c, int stage, const uint8_t* mask, const ui
         const bool lw = (f >= 1) && mask[f - 1 + G] && lvl[f - 1] == maxlev;
         const bool rw = (f <= Nx - 1) && mask[f + G] && lvl[f] == maxlev;
         if (!(lw || rw)) continue;
-        double s[6][3];
-        for (int p = 0; p < 6; p++) load3(c, c.Qt, f - 3 + p, s[p]);
-        double fl[3];
-        face_flux1<FLUX>(c.gamma, s, alpha, fl);
-        for (int k = 0; k < 3; k++) c.F[k * (Nx + 1) + f] = fl[k];
+        const F3 fl = face_at<FLUX>(c.Qt, NP, f, c.gamma, alpha);
+        for (int k = 0; k < 3; k++) c.F[k * (Nx + 1) + f] = fl.v[k];
     }
     __syncthreads();
     for (int i = threadIdx.x; i < Nx; i += blockDim.x) {
@@ -437,7 +467,8 @@ __device__ void rk_stage(const Ctx1& c, int stage, const uint8_t* mask, const ui
         if (lev == maxlev) {
             for (int k = 0; k < 3; k++) dF[k] = c.F[k * (Nx + 1) + i + 1] - c.F[k * (Nx + 1) + i];
         } else {
-            coarse_dF(c, i, d, dF);
+            const F3 r = coarse_dF_at(c.Qt, c.NP, c.g.Nx, c.gamma, i, d, c.flag);
+            for (int k = 0; k < 3; k++) dF[k] = r.v[k];
         }
         const double h = Div(c.dx * d)(dt);
         for (int k = 0; k < 3; k++) {
