@@ -1050,7 +1050,7 @@ __device__ __forceinline__ void ghost_state(int kind, int side, int i, double x0
         }
     } else {
         const double x = x0 + (i - 0) * dx;
-        if (x < 1.0 / 6.0 + (1 + 20 * T) / sqrt3) {
+        if (x < 1.0 / 6.0 + Div(sqrt3)(1 + 20 * T)) {
             v[0] = 8; v[1] = 8 * 8.25 * sqrt3 * 0.5; v[2] = -8 * 8.25 * 0.5;
             v[3] = eos_E2(1.4, 8, 8.25 * sqrt3 * 0.5, -8.25 * 0.5, 116.5);
         } else {
@@ -1096,7 +1096,7 @@ __global__ void __launch_bounds__(256) k2_bc(Dims2 D, double* Q, int kind, doubl
             double vmax = 0.0;
             for (int k = 0; k < (int)blockDim.x; k++) vmax = bits_max(vmax, red[k]);
             sc->vmax = (unsigned long long)__double_as_longlong(vmax);
-            double dt = sc->cfl * dx / vmax;
+            double dt = Div(vmax)(sc->cfl * dx);
             int dp = 0;
             if (sc->cnt < sc->n_print && sc->t + dt > pt[sc->cnt]) { dt = pt[sc->cnt] - sc->t; dp = 1; }
             sc->dt = dt;
@@ -1129,12 +1129,14 @@ __global__ void __launch_bounds__(256) k2_bc(Dims2 D, double* Q, int kind, doubl
 template <int S>
 __device__ __forceinline__ void stage_out(const Dims2& D, long long p, const double* r, double h, double dt,
                                           const double* Qa, const double* Qs, double* Qo, double* out) {
+    const double dth = Div(h)(dt);
+    const Div third(3.0);
     for (int k = 0; k < 4; k++) {
         const long long o = p + k * D.plane;
-        if (S == 1) out[k] = Qs[o] - dt / h * r[k];
-        else if (S == 2) out[k] = 0.75 * Qa[o] + 0.25 * (Qs[o] - dt / h * r[k]);
-        else out[k] = Qs[o] - dt / h * r[k];
-        if (S == 3) Qo[o] = (Qo[o] + 2.0 * out[k]) / 3.0;
+        if (S == 1) out[k] = Qs[o] - dth * r[k];
+        else if (S == 2) out[k] = 0.75 * Qa[o] + 0.25 * (Qs[o] - dth * r[k]);
+        else out[k] = Qs[o] - dth * r[k];
+        if (S == 3) Qo[o] = third(Qo[o] + 2.0 * out[k]);
         else Qo[o] = out[k];
     }
 }
@@ -1433,8 +1435,8 @@ __global__ void __launch_bounds__(256) k2_advect_stage(Dims2 D, const uint8_t* _
         r = dF + dG;
     }
     const double h = gdx * d;
-    if (S == 2) out[p] = 0.75 * Qa[p] + 0.25 * (F[p] - dt / h * r);
-    else out[p] = F[p] - dt / h * r;
+    if (S == 2) out[p] = 0.75 * Qa[p] + 0.25 * (F[p] - Div(h)(dt) * r);
+    else out[p] = F[p] - Div(h)(dt) * r;
 }
 
 // The combine on mask0 (main_2d_advect.cpp:190-197) and PostProcess:

@@ -87,7 +87,7 @@ AWCM_HD double eos_H1(double g, const double* q, const Div& d0) { return d0(q[2]
 AWCM_HD double eos_H1(double g, const double* q) { return eos_H1(g, q, Div(q[0])); }
 AWCM_HD double eos_c1(double g, const double* q, const Div& d0) { return sqrt(d0(g * eos_p1(g, q, d0))); }
 AWCM_HD double eos_c1(double g, const double* q) { return eos_c1(g, q, Div(q[0])); }
-AWCM_HD double eos_E1(double g, double rho, double u, double p) { return p / (g - 1) + 0.5 * rho * (u * u); }
+AWCM_HD double eos_E1(double g, double rho, double u, double p) { return Div(g - 1)(p) + 0.5 * rho * (u * u); }
 
 // d0 divides by the density q[0]
 AWCM_HD double eos_p2(double g, const double* q, const Div& d0) {
@@ -99,7 +99,7 @@ AWCM_HD double eos_H2(double g, const double* q) { return eos_H2(g, q, Div(q[0])
 AWCM_HD double eos_c2(double g, const double* q, const Div& d0) { return sqrt(d0(g * eos_p2(g, q, d0))); }
 AWCM_HD double eos_c2(double g, const double* q) { return eos_c2(g, q, Div(q[0])); }
 AWCM_HD double eos_E2(double g, double rho, double u, double v, double p) {
-    return p / (g - 1) + 0.5 * rho * (u * u + v * v);
+    return Div(g - 1)(p) + 0.5 * rho * (u * u + v * v);
 }
 
 // ---- physical fluxes and spectral radii --------------------------------------
