@@ -230,19 +230,37 @@ __device__ void addpoint(const Ctx1& c, uint8_t* mask, const uint8_t* level) {
 
 // AWCM_MANAGER_1D::ComputeLevel (AWCM_MANAGER_1D.h:109-133).  A point with no
 // active neighbour keeps its previous level, as in the reference.
-__device__ void level(const Ctx1& c, uint8_t* lvl, const uint8_t* mask) {
-    const int Nx = c.g.Nx;
+// The search by spacing d = 1, 2, 4, ... reads every spacing's two mask bytes
+// at once (clamped into the ghosted range, so every read is in the array) and
+// then takes the first spacing that has an active neighbour, or that leaves
+// the ghosted grid (the error flag), as the reference's loop does one
+// spacing after the other.  Out of line, like the stage functions, so its
+// loads in flight do not add to the solve kernel's register count.
+__device__ __attribute__((noinline)) void level_at(int Nx, int maxlev, uint8_t* lvl, const uint8_t* mask, int* flag) {
+    const uint32_t spacings = (2u << maxlev) - 1u;
     for (int i = 1 + threadIdx.x; i <= Nx - 2; i += blockDim.x) {
         if (mask[i + G] == 0) { lvl[i] = 0; continue; }
-        int d = 1;
-        for (int lev = c.g.maxlev; lev >= 0; lev--) {
-            if (i - d < -G || i + d > Nx - 1 + G) { *c.flag = 1; break; }
-            if (mask[i - d + G] || mask[i + d + G]) { lvl[i] = (uint8_t)lev; break; }
-            d *= 2;
+        uint32_t hit = 0, out = 0;
+#pragma unroll
+        for (int q = 0; q < kMaxLevels; q++) {
+            const int d = 1 << q;
+            const bool in = i - d >= -G && i + d <= Nx - 1 + G;
+            const int lo = in ? i - d : -G, hi = in ? i + d : -G;
+            if (!in) out |= 1u << q;
+            else if (mask[lo + G] | mask[hi + G]) hit |= 1u << q;
+        }
+        const uint32_t first = (hit | out) & spacings;
+        if (first) {
+            const int q = __builtin_ctz(first);
+            if ((out >> q) & 1u) *flag = 1;
+            else lvl[i] = (uint8_t)(maxlev - q);
         }
     }
-    if (threadIdx.x == 0) { lvl[0] = (uint8_t)c.g.maxlev; lvl[Nx - 1] = (uint8_t)c.g.maxlev; }
+    if (threadIdx.x == 0) { lvl[0] = (uint8_t)maxlev; lvl[Nx - 1] = (uint8_t)maxlev; }
     __syncthreads();
+}
+__device__ void level(const Ctx1& c, uint8_t* lvl, const uint8_t* mask) {
+    level_at(c.g.Nx, c.g.maxlev, lvl, mask, c.flag);
 }
 
 // WAVELET_1D::InterpolateEuler(q, mra, from, to) (WAVELET_1D.h:393-431): momentum
