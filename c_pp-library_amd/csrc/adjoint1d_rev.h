@@ -227,9 +227,9 @@ __global__ void __launch_bounds__(256) k1r_gather(AdjRev a) {
 // gather from the odd points; then the update's zeroing of even density
 // points outside mask3 (n = 1..N-1; the odd pass below reads only even points
 // inside mask3, so it may come first).
-__global__ void __launch_bounds__(256) k1r_iwt_even(AdjRev a, int lev) {
+__device__ __forceinline__ void k1r_iwt_even_at(const AdjRev& a, int lev, int idx) {
     const int d = 1 << (a.L - 1 - lev), hh = d / 2, N = (a.Nx - 1) / d;
-    const int n = rev_idx();
+    const int n = idx;
     if (n > N) return;
     const StepView v = step_view(a);
     const uint8_t* mt = v.mask(a, 0);
@@ -255,13 +255,14 @@ __global__ void __launch_bounds__(256) k1r_iwt_even(AdjRev a, int lev) {
     B1[e] += a1;
     B2[e] += a2;
 }
+__global__ void __launch_bounds__(256) k1r_iwt_even(AdjRev a, int lev) { k1r_iwt_even_at(a, lev, rev_idx()); }
 
 // Inverse transform, odd targets: the predict's doubling (points outside
 // mask3 end at 0), minus the update^T gathered from the even points inside
 // mask3; interpolated momentum and energy points end at 0.
-__global__ void __launch_bounds__(256) k1r_iwt_odd(AdjRev a, int lev) {
+__device__ __forceinline__ void k1r_iwt_odd_at(const AdjRev& a, int lev, int idx) {
     const int d = 1 << (a.L - 1 - lev), hh = d / 2, N = (a.Nx - 1) / d;
-    const int m = rev_idx();
+    const int m = idx;
     if (m >= N) return;
     const StepView v = step_view(a);
     const uint8_t* mt = v.mask(a, 0);
@@ -285,12 +286,13 @@ __global__ void __launch_bounds__(256) k1r_iwt_odd(AdjRev a, int lev) {
         a.Qb[2LL * a.NP + o] = 0.0;
     }
 }
+__global__ void __launch_bounds__(256) k1r_iwt_odd(AdjRev a, int lev) { k1r_iwt_odd_at(a, lev, rev_idx()); }
 
 // Forward transform, odd targets: update^T gathered from the even points
 // inside mask_temp.
-__global__ void __launch_bounds__(256) k1r_fwt_odd(AdjRev a, int lev) {
+__device__ __forceinline__ void k1r_fwt_odd_at(const AdjRev& a, int lev, int idx) {
     const int d = 1 << (a.L - 1 - lev), hh = d / 2, N = (a.Nx - 1) / d;
-    const int m = rev_idx();
+    const int m = idx;
     if (m >= N) return;
     const StepView v = step_view(a);
     const uint8_t* mt = v.mask(a, 0);
@@ -307,12 +309,13 @@ __global__ void __launch_bounds__(256) k1r_fwt_odd(AdjRev a, int lev) {
     }
     B0[(2 * m + 1) * hh + G] += acc;
 }
+__global__ void __launch_bounds__(256) k1r_fwt_odd(AdjRev a, int lev) { k1r_fwt_odd_at(a, lev, rev_idx()); }
 
 // Forward transform, even targets: predict^T gathered from the odd points
 // inside mask_temp, with the predict's factor 1/2.
-__global__ void __launch_bounds__(256) k1r_fwt_even(AdjRev a, int lev) {
+__device__ __forceinline__ void k1r_fwt_even_at(const AdjRev& a, int lev, int idx) {
     const int d = 1 << (a.L - 1 - lev), hh = d / 2, N = (a.Nx - 1) / d;
-    const int n = rev_idx();
+    const int n = idx;
     if (n > N) return;
     const StepView v = step_view(a);
     const uint8_t* mt = v.mask(a, 0);
@@ -329,20 +332,47 @@ __global__ void __launch_bounds__(256) k1r_fwt_even(AdjRev a, int lev) {
     }
     B0[n * d + G] -= 0.5 * acc;
 }
+__global__ void __launch_bounds__(256) k1r_fwt_even(AdjRev a, int lev) { k1r_fwt_even_at(a, lev, rev_idx()); }
 
 // Forward transform, odd targets inside mask_temp: the predict's factor 1/2.
-__global__ void __launch_bounds__(256) k1r_fwt_half(AdjRev a, int lev) {
+__device__ __forceinline__ void k1r_fwt_half_at(const AdjRev& a, int lev, int idx) {
     const int d = 1 << (a.L - 1 - lev), hh = d / 2, N = (a.Nx - 1) / d;
-    const int m = rev_idx();
+    const int m = idx;
     if (m >= N) return;
     const StepView v = step_view(a);
     const int o = (2 * m + 1) * hh + G;
     if (v.mask(a, 0)[o]) a.Qb[o] *= 0.5;
 }
+__global__ void __launch_bounds__(256) k1r_fwt_half(AdjRev a, int lev) { k1r_fwt_half_at(a, lev, rev_idx()); }
 
 // Step counter: the next replay reverses the step before.
 __global__ void k1r_next(int* step) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *step -= 1;
+}
+
+// The transposed transforms' passes of the coarse levels top..0 (at most
+// kRevWg points each) back to back in one workgroup: the inverse
+// transform's, coarsest last, then the forward transform's, coarsest first --
+// the same gathers as the grid kernels, a barrier after each pass.  One
+// launch where there were five a level, each a launch of a few workgroups.
+constexpr int kRevWg = 1024;
+__global__ void __launch_bounds__(kRevWg) k1r_coarse_levels(AdjRev a, int top) {
+    for (int lev = top; lev >= 0; lev--) {
+        const int N = (a.Nx - 1) >> (a.L - 1 - lev);
+        for (int n = threadIdx.x; n <= N; n += blockDim.x) k1r_iwt_even_at(a, lev, n);
+        __syncthreads();
+        for (int m = threadIdx.x; m < N; m += blockDim.x) k1r_iwt_odd_at(a, lev, m);
+        __syncthreads();
+    }
+    for (int lev = 0; lev <= top; lev++) {
+        const int N = (a.Nx - 1) >> (a.L - 1 - lev);
+        for (int m = threadIdx.x; m < N; m += blockDim.x) k1r_fwt_odd_at(a, lev, m);
+        __syncthreads();
+        for (int n = threadIdx.x; n <= N; n += blockDim.x) k1r_fwt_even_at(a, lev, n);
+        __syncthreads();
+        for (int m = threadIdx.x; m < N; m += blockDim.x) k1r_fwt_half_at(a, lev, m);
+        __syncthreads();
+    }
 }
 
 }  // namespace
@@ -354,6 +384,10 @@ __global__ void k1r_next(int* step) {
 #ifndef AWCM_REV_LAUNCH
 #define AWCM_REV_LAUNCH(kern, items, ...) \
     hipLaunchKernelGGL((kern), dim3(((items) + 255) / 256), dim3(256), 0, stream, __VA_ARGS__)
+#endif
+// AWCM_REV_LAUNCH_WG(kernel, args...): one workgroup of kRevWg threads
+#ifndef AWCM_REV_LAUNCH_WG
+#define AWCM_REV_LAUNCH_WG(kern, ...) hipLaunchKernelGGL((kern), dim3(1), dim3(kRevWg), 0, stream, __VA_ARGS__)
 #endif
 
 inline void launch_reverse_step(const AdjRev& a, hipStream_t stream) {
@@ -368,12 +402,16 @@ inline void launch_reverse_step(const AdjRev& a, hipStream_t stream) {
     AWCM_REV_LAUNCH(k1r_point<0>, a.NP, a);
     AWCM_REV_LAUNCH(k1r_face<0>, a.Nx + 1, a);
     AWCM_REV_LAUNCH(k1r_gather<0>, a.NP, a);
-    for (int lev = maxlev - 1; lev >= 0; lev--) {
+    // levels whose passes fit one workgroup: 0..top
+    int top = -1;
+    while (top + 1 < maxlev && ((a.Nx - 1) >> (a.L - 2 - top)) + 1 <= kRevWg) top++;
+    for (int lev = maxlev - 1; lev > top; lev--) {
         const int N = (a.Nx - 1) >> (a.L - 1 - lev);
         AWCM_REV_LAUNCH(k1r_iwt_even, N + 1, a, lev);
         AWCM_REV_LAUNCH(k1r_iwt_odd, N, a, lev);
     }
-    for (int lev = 0; lev < maxlev; lev++) {
+    if (top >= 0) AWCM_REV_LAUNCH_WG(k1r_coarse_levels, a, top);
+    for (int lev = top + 1; lev < maxlev; lev++) {
         const int N = (a.Nx - 1) >> (a.L - 1 - lev);
         AWCM_REV_LAUNCH(k1r_fwt_odd, N, a, lev);
         AWCM_REV_LAUNCH(k1r_fwt_even, N + 1, a, lev);

@@ -22,6 +22,8 @@
         blockIdx.x = 0;                                                          \
     } while (0)
 
+// the one-workgroup reverse launches run on the emulated workgroup
+#define AWCM_REV_LAUNCH_WG(kern, ...) emu_launch(EMU_THREADS, [&] { kern(__VA_ARGS__); })
 #include "awcm1d.hip"
 #include "adjoint1d_rev.h"
 
