@@ -5,7 +5,7 @@
   the four-quadrant problem, 5 levels, over their whole schedules to t = 0.2
   and t = 0.25);
 * at the first output times of the full-size runs (double Mach 4097 x 1025 to
-  t = 0.03, the quadrants 2049 x 2049 to t = 0.0625);
+  t = 0.05, the quadrants 2049 x 2049 to t = 0.09375);
 * at the end of every instance of the 512-instance batched 1-D workload (and,
   for the instance whose scheme breaks down, where the GPU stops it).
 
@@ -32,8 +32,8 @@ from digest_lib import digest  # noqa: E402
 CASES_2D = {
     "dmr_L5": ("dmr", 5, 65, 17, 0.2, 21, 20),
     "quad_L5": ("quad", 5, 33, 33, 0.25, 9, 8),
-    "dmr_full": ("dmr", 7, 65, 17, 0.2, 21, 3),
-    "quad_full": ("quad", 7, 33, 33, 0.25, 9, 2),
+    "dmr_full": ("dmr", 7, 65, 17, 0.2, 21, 5),
+    "quad_full": ("quad", 7, 33, 33, 0.25, 9, 3),
 }
 TOL_2D, CFL_2D = 1e-3, 0.4
 

@@ -8,8 +8,8 @@ levels, with t, the step count and the print counter:
   levels, 20 output times to t = 0.2, ~2600 steps) and of the four-quadrant
   problem (513 x 513, 5 levels, 8 output times to t = 0.25);
 * at the first output times of the full-size runs: double Mach 4097 x 1025 at
-  t = 0.01, 0.02 and 0.03, the quadrants 2049 x 2049 at t = 0.03125 and
-  0.0625;
+  t = 0.01 .. 0.05 (3331 steps), the quadrants 2049 x 2049 at t = 0.03125,
+  0.0625 and 0.09375 (2216 steps);
 * at the end of every one of the 512 batched shock tubes.
 
 The device runs each case through awcm_run (or the batch) and must reproduce
